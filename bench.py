@@ -1,0 +1,134 @@
+"""Benchmark of the RBSC refinement round on one MI355X.
+
+Workload (default): directed R-MAT (a, b, c, d) = (.57, .19, .19, .05), scale
+27 (134,217,728 nodes), 2^31 draws, uint32 ids, 20% of the nodes cloned,
+initial colour 0 everywhere, built on the device with a fixed seed and
+resident in HBM.  A step is one refinement of that graph to its fixpoint
+(every round of rbsc.cu:354-425); the metric is edge visits per second
+(lines processed per round x rounds / time).
+
+Prints one JSON line (rank 0) with the measured value, a live roofline for
+the round (HIP events on the library's stream) and a single-core CPU
+baseline (the oracle's rsc restatement) on a bounded sample of the same
+R-MAT family.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+CONFIGS = {
+    # name: (scale, log2 draws, clone fraction, seed)
+    "rmat_s27_d31_c20": (27, 31, 0.20, 1003),
+    "rmat_s20_d24_c30": (20, 24, 0.30, 1001),
+    "rmat_s24_d28_c20": (24, 28, 0.20, 1003),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
+    """Single-core rsc restatement (oracle, C) on a bounded R-MAT sample."""
+    import oracle
+    from rbsc_amd import gen
+    g = gen.rmat(scale, 1 << log_draws, clone_frac=clone, seed=seed)
+    src = g.sources()
+    t = time.perf_counter()
+    _, K, R = oracle.rsc_refine(g.n_nodes, src, g.dst)
+    dt = time.perf_counter() - t
+    return {"value": R * g.n_edges / dt, "unit": "edge-visits/s", "cores": 1, "kind": "port",
+            "sample": f"rsc restatement (oracle/rbsc_oracle.c) on R-MAT scale {scale}, "
+                      f"2^{log_draws} draws, {int(clone * 100)}% clones: {g.n_edges} lines, "
+                      f"{R} rounds, {K} blocks, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="rmat_s27_d31_c20", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-scale", type=int, default=22)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: the RBSC round runs on a single GPU in this "
+              "build (no multi-GPU path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import rbsc_amd
+    scale, ldraws, clone, seed = CONFIGS[args.config]
+    ctx = rbsc_amd.Context(0)
+    t0 = time.perf_counter()
+    dg = ctx.gen_rmat(scale, 1 << ldraws, 0.57, 0.19, 0.19, clone, seed)
+    N, E = dg.dims
+    log(f"[bench] generated {args.config}: N={N} E={E} in {time.perf_counter() - t0:.1f}s")
+
+    for i in range(args.warmup):
+        r = dg.refine(partition=False)
+        log(f"[bench] warmup {i}: rounds={r.rounds} blocks={r.nodes} device_ms={r.device_ms:.2f}")
+
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    rounds = 0
+    dev_ms = 0.0
+    t = time.perf_counter()
+    for i in range(args.steps):
+        r = dg.refine(partition=False)
+        rounds += r.rounds
+        dev_ms += r.device_ms
+    wall = time.perf_counter() - t
+    phases = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    log(f"[bench] timed {args.steps} steps: rounds={rounds} wall={wall:.3f}s device={dev_ms:.1f}ms")
+
+    visits = rounds * E
+    value = visits / wall
+    ms_round = dev_ms / rounds
+    b_alg = 8 * E + 32 * N  # unweighted lines
+    round_ms = {k: v[0] / max(v[1], 1) for k, v in phases.items()}
+    per_round_sum = sum(phases[k][0] for k in ("signature", "gather", "sort", "rank")) / rounds
+    achieved = b_alg / (per_round_sum / 1e3)
+    dom = max(("signature", "gather", "sort", "rank"), key=lambda k: phases[k][0])
+    out = {
+        "metric": "edge-visits/s per WL round (% HBM roofline); batched graphs/s, p99 latency",
+        "value": value,
+        "unit": "edge-visits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "rounds_per_step": rounds // args.steps,
+                   "blocks": r.nodes, "ms_per_round": ms_round},
+        "roofline": {"bound": "hbm", "kernel": "refinement round (all kernels of one round)",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "alg_bytes_per_round": b_alg,
+                     "phase_ms_per_round": {k: phases[k][0] / rounds for k in phases if k != "setup"},
+                     "dominant_phase": dom},
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_scale + 4, 0.20, seed)
+    dg.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,147 @@
+/*
+ * rbsc.h -- C ABI of the MI355X (gfx950) RBSC refinement library.
+ *
+ * The reference tool is a single CUDA program whose hot path is the round
+ * loop of src/rbsc/rbsc.cu:354-425, driven by main() (rbsc.cu:182-497), fed
+ * by the stdin parser read_graph (rbsc.cu:168-180) and reporting three lines
+ * (rbsc.cu:492-494).  Its internal seam is a set of CUB device calls
+ * (rbsc.cu:258-261, 366-377, 386, 390, 394, 409).  This header is the seam
+ * that replaces all of it: plain pointers and sizes, int status codes, no
+ * exceptions and no C++ or torch types.
+ *
+ * Status codes: 0 on success, negative on failure; rbsc_last_error() returns
+ * a thread-local message for the last failure of the calling thread.
+ *
+ * Ownership: the caller owns every host buffer passed in or out.  The library
+ * owns device memory behind opaque handles.  One context per host thread (or
+ * per stream); calls on distinct contexts are thread-safe.
+ */
+#ifndef RBSC_H
+#define RBSC_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBSC_OK 0
+#define RBSC_ERR_INVALID -1   /* bad argument */
+#define RBSC_ERR_NOMEM -2     /* host or device allocation failed */
+#define RBSC_ERR_HIP -3       /* HIP runtime error */
+#define RBSC_ERR_BUDGET -4    /* request exceeds the HBM budget */
+#define RBSC_ERR_INPUT -5     /* malformed graph: id >= N, or ungrouped lines */
+#define RBSC_ERR_KERNEL -6    /* a kernel's bounded wait expired (logic error) */
+#define RBSC_ERR_NODEVICE -7  /* no HIP device */
+
+/* Edge-batch modes (opts.mode), used when max_batch_edges < n_edges. */
+#define RBSC_MODE_STREAM 0    /* exact full-graph rounds, edge slices streamed
+                                 from pinned host memory (done = 1) */
+#define RBSC_MODE_QUOTIENT 1  /* reference multi-batch quotient passes,
+                                 rbsc.cu:274-487 (done may be 0) */
+#define RBSC_MODE_QUOTIENT_AGGRESSIVE 2 /* rbsc_aggressive.cu:487 stop rule */
+
+/* A graph in the reference's line form (rbsc.cu:168-180): line e reads
+ * "src[e] weight[e] dst[e]" and adds randomize(colour(dst)) * weight to the
+ * signature of src (rbsc.cu:80-87, 366-377).  Lines must be grouped by src
+ * (equal src contiguous), as the reference's ReduceByKey requires.
+ * Either src (COO) or row_offsets (CSR over node ids, n_nodes + 1 entries)
+ * is given.  weight == NULL means every weight is 1.  init_color == NULL
+ * means the reference's initial colouring (rbsc.cu:51-63: N+1 for nodes with
+ * a line, 0 for the others); otherwise init_color[u] is u's initial colour
+ * (an extension: the reference has no initial-partition input). */
+typedef struct rbsc_graph {
+    uint32_t n_nodes;
+    uint64_t n_edges;
+    const uint64_t* row_offsets; /* CSR, nullable */
+    const uint32_t* src;         /* COO, nullable when row_offsets given */
+    const uint32_t* dst;
+    const uint64_t* weight;      /* nullable */
+    const uint32_t* init_color;  /* nullable */
+} rbsc_graph;
+
+/* Options.  max_batch_edges mirrors argv[1] of the reference (rbsc.cu:204):
+ * 0 or >= n_edges means one batch. */
+typedef struct rbsc_opts {
+    uint64_t max_batch_edges;
+    uint32_t mode;        /* RBSC_MODE_* */
+    uint32_t max_rounds;  /* 0 = until the fixpoint; else stop after this many */
+    uint32_t verify;      /* 1 = exact stability check of the final partition */
+    uint32_t reserved;
+} rbsc_opts;
+
+/* Result.  nodes / done / time_s are the reference's three output lines
+ * (rbsc.cu:492-494). */
+typedef struct rbsc_result {
+    uint64_t nodes;          /* number of blocks */
+    uint32_t done;           /* 1 iff the coarsest reduction was reached */
+    uint32_t rounds;         /* refinement rounds executed (all passes) */
+    double time_s;           /* span after allocation to the final relabel */
+    double device_ms;        /* device time of the rounds (hipEvents) */
+    int32_t verified;        /* 1 stable (exact), 0 not checked, -1 unstable */
+    uint32_t passes;         /* quotient passes (1 when not in quotient mode) */
+    uint64_t edges_per_round;/* edge visits per round (lines) */
+} rbsc_result;
+
+typedef struct rbsc_ctx rbsc_ctx;
+typedef struct rbsc_dgraph rbsc_dgraph;
+
+/* Last error message of the calling thread ("" if none). */
+const char* rbsc_last_error(void);
+
+/* Library version string. */
+const char* rbsc_version(void);
+
+/* Context on HIP device `device`; hbm_budget = 0 means no budget.  Replaces
+ * the per-GPU setup of rbsc.cu:234-270 (streams, scratch, CUB temp). */
+int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out);
+int rbsc_ctx_destroy(rbsc_ctx* ctx);
+
+/* One-shot refinement of a host graph: upload, refine to the fixpoint,
+ * canonical partition out.  Replaces main() of rbsc.cu:182-497 after parsing.
+ * partition_out (nullable, n_nodes entries) receives the first-occurrence
+ * canonical labels (rbsc.cu:431-445). */
+int rbsc_refine(rbsc_ctx* ctx, const rbsc_graph* graph, const rbsc_opts* opts,
+                rbsc_result* result, uint32_t* partition_out);
+
+/* Device-resident graphs: upload once, refine many times (the bench and the
+ * serving loop keep graphs resident).  Upload validates ids and grouping. */
+int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* graph, rbsc_dgraph** out);
+int rbsc_graph_free(rbsc_ctx* ctx, rbsc_dgraph* g);
+/* Bytes of device memory held by a resident graph. */
+uint64_t rbsc_graph_bytes(const rbsc_dgraph* g);
+/* Refine a resident graph.  partition_out: host, nullable, canonical labels.
+ * colors_out: host, nullable, the raw final colours (dense ranks of the last
+ * round, the values rbsc.cu:429 copies back), for per-round parity checks. */
+int rbsc_graph_refine(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
+                      rbsc_result* result, uint32_t* partition_out,
+                      uint32_t* colors_out);
+
+/* Per-phase device timing with hipEvents on the context's stream (off by
+ * default).  Phases: 0 signature kernel + row fix-up, 1 key gather,
+ * 2 radix sort, 3 rank/scan, 4 round-0 set-up.  rbsc_ctx_profile_read fills
+ * ms[5] (summed milliseconds) and launches[5] (phase executions) and resets
+ * the sums when reset != 0.  No counterpart in the reference, which times
+ * only the whole run (rbsc.cu:272, 489-490). */
+int rbsc_ctx_profile(rbsc_ctx* ctx, int enable);
+int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int reset);
+
+/* Seeded R-MAT generated on the device (bench-scale inputs without a host
+ * round trip).  2^scale nodes, n_draws lines drawn with probabilities a, b, c
+ * (d = 1 - a - b - c), node ids permuted by a seeded bijection, then
+ * floor(clone_frac * 2^scale) nodes replace their lines with a copy of a
+ * random non-clone's lines.  Lines are sorted by (src, dst).  Identical to
+ * rbsc_gen_rmat on the host for the same arguments. */
+int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_draws, double a,
+                         double b, double c, double clone_frac, uint64_t seed,
+                         rbsc_dgraph** out);
+/* Copy a resident graph back to host CSR (for checks): row_offsets has
+ * n_nodes + 1 entries, dst n_edges entries.  Sizes via rbsc_graph_dims. */
+int rbsc_graph_dims(const rbsc_dgraph* g, uint32_t* n_nodes, uint64_t* n_edges);
+int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t* row_offsets,
+                        uint32_t* dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBSC_H */

@@ -1,0 +1,255 @@
+"""rbsc_amd -- MI355X-native RBSC refinement (coarsest backward-equivalence /
+1-WL colour refinement with edge multiplicities), a drop-in for the
+reference ``rbsc`` tool (src/rbsc/rbsc.cu of f-biondi/rbsc).
+
+Host-side mirror of the reference's interface:
+
+* ``Context.refine(graph, max_batch_edges=..., mode=...)`` is the reference
+  binary's run: same input (lines ``src weight dst`` grouped by ``src``), the
+  same edge-batch parameter (rbsc.cu:204) and the same three results
+  ``time``, ``done``, ``nodes`` (rbsc.cu:492-494), plus the canonical
+  first-occurrence partition (rbsc.cu:431-445).
+* ``Context.upload`` keeps a graph resident in HBM for repeated refinement.
+
+All arithmetic runs in the HIP library ``librbsc.so``; this module only
+marshals buffers.  It raises when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import (MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM, RbscError,
+                   check, load)
+
+__all__ = ["Context", "DeviceGraph", "HostGraph", "RefineResult", "RbscError", "load",
+           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "gen"]
+
+_MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
+          "aggressive": MODE_QUOTIENT_AGGRESSIVE}
+
+
+def _ptr(a: Optional[np.ndarray], ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class HostGraph:
+    """A graph in the reference's line form.
+
+    ``dst`` holds the head of every line; lines are grouped by source and
+    given either as ``row_offsets`` (CSR, n_nodes + 1 entries) or as ``src``
+    (COO).  ``weight`` None means all weights are 1; ``init_color`` None
+    means the reference's initial colouring (rbsc.cu:51-63).
+    """
+    n_nodes: int
+    dst: np.ndarray
+    row_offsets: Optional[np.ndarray] = None
+    src: Optional[np.ndarray] = None
+    weight: Optional[np.ndarray] = None
+    init_color: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        self.dst = np.ascontiguousarray(self.dst, dtype=np.uint32)
+        if self.row_offsets is not None:
+            self.row_offsets = np.ascontiguousarray(self.row_offsets, dtype=np.uint64)
+        if self.src is not None:
+            self.src = np.ascontiguousarray(self.src, dtype=np.uint32)
+        if self.weight is not None:
+            self.weight = np.ascontiguousarray(self.weight, dtype=np.uint64)
+        if self.init_color is not None:
+            self.init_color = np.ascontiguousarray(self.init_color, dtype=np.uint32)
+        if self.row_offsets is None and self.src is None and len(self.dst):
+            raise ValueError("HostGraph needs src or row_offsets")
+
+    @property
+    def n_edges(self) -> int:
+        return int(len(self.dst))
+
+    def sources(self) -> np.ndarray:
+        """Source of every line (COO view)."""
+        if self.src is not None:
+            return self.src
+        deg = np.diff(self.row_offsets.astype(np.int64))
+        return np.repeat(np.arange(self.n_nodes, dtype=np.uint32), deg)
+
+    def as_struct(self) -> _lib.Graph:
+        g = _lib.Graph()
+        g.n_nodes = self.n_nodes
+        g.n_edges = self.n_edges
+        g.row_offsets = _ptr(self.row_offsets, C.c_uint64)
+        g.src = _ptr(self.src, C.c_uint32)
+        g.dst = _ptr(self.dst, C.c_uint32)
+        g.weight = _ptr(self.weight, C.c_uint64)
+        g.init_color = _ptr(self.init_color, C.c_uint32)
+        return g
+
+    def to_text(self) -> bytes:
+        """The reference's stdin format (rbsc.cu:168-180): N, E, lines "u w v"."""
+        src = self.sources().astype(np.uint64)
+        w = self.weight if self.weight is not None else np.ones(self.n_edges, np.uint64)
+        body = "".join(f"{a} {b} {c}\n" for a, b, c in zip(src.tolist(), w.tolist(), self.dst.tolist()))
+        return f"{self.n_nodes}\n{self.n_edges}\n{body}".encode()
+
+    def to_valmari_text(self) -> bytes:
+        """Valmari's MDP format, 1-indexed (valmari-adapter:9-16; MDP_read.cc:51-64),
+        with the initial colouring as initial blocks (MDP_read.cc:140-150)."""
+        src = self.sources().astype(np.uint64) + 1
+        w = self.weight if self.weight is not None else np.ones(self.n_edges, np.uint64)
+        dst = self.dst.astype(np.uint64) + 1
+        blocks = []
+        if self.init_color is not None:
+            labels = np.unique(self.init_color)
+            for lab in labels[1:]:
+                members = np.nonzero(self.init_color == lab)[0] + 1
+                blocks.append(" ".join(map(str, members.tolist())) + " 0\n")
+        nb = 1 + len(blocks)
+        body = "".join(f"{a} {b} {c}\n" for a, b, c in zip(src.tolist(), w.tolist(), dst.tolist()))
+        return (f"{self.n_nodes} 0 0 {self.n_edges} {nb}\n" + body + "".join(blocks)).encode()
+
+
+@dataclass
+class RefineResult:
+    nodes: int
+    done: int
+    rounds: int
+    time_s: float
+    device_ms: float
+    verified: int
+    passes: int
+    partition: Optional[np.ndarray] = None
+    colors: Optional[np.ndarray] = None
+
+
+def _result(r: _lib.Result, partition=None, colors=None) -> RefineResult:
+    return RefineResult(int(r.nodes), int(r.done), int(r.rounds), float(r.time_s),
+                        float(r.device_ms), int(r.verified), int(r.passes), partition, colors)
+
+
+def _opts(max_batch_edges=0, mode="stream", max_rounds=0, verify=False) -> _lib.Opts:
+    o = _lib.Opts()
+    o.max_batch_edges = int(max_batch_edges)
+    o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
+    o.max_rounds = int(max_rounds)
+    o.verify = 1 if verify else 0
+    return o
+
+
+class DeviceGraph:
+    """A graph resident in HBM (handle to an ``rbsc_dgraph``)."""
+
+    def __init__(self, ctx: "Context", handle: C.c_void_p):
+        self.ctx = ctx
+        self.handle = handle
+
+    @property
+    def dims(self):
+        n = C.c_uint32()
+        e = C.c_uint64()
+        check(load().rbsc_graph_dims(self.handle, C.byref(n), C.byref(e)))
+        return int(n.value), int(e.value)
+
+    @property
+    def nbytes(self) -> int:
+        return int(load().rbsc_graph_bytes(self.handle))
+
+    def refine(self, max_rounds=0, verify=False, partition=True, colors=False, mode="stream",
+               max_batch_edges=0) -> RefineResult:
+        n, _ = self.dims
+        part = np.empty(n, np.uint32) if partition else None
+        col = np.empty(n, np.uint32) if colors else None
+        r = _lib.Result()
+        o = _opts(max_batch_edges, mode, max_rounds, verify)
+        check(load().rbsc_graph_refine(self.ctx.handle, self.handle, C.byref(o), C.byref(r),
+                                       _ptr(part, C.c_uint32), _ptr(col, C.c_uint32)))
+        return _result(r, part, col)
+
+    def download(self) -> HostGraph:
+        n, e = self.dims
+        ro = np.empty(n + 1, np.uint64)
+        dst = np.empty(max(e, 1), np.uint32)
+        check(load().rbsc_graph_download(self.ctx.handle, self.handle, _ptr(ro, C.c_uint64),
+                                         _ptr(dst, C.c_uint32)))
+        return HostGraph(n, dst[:e], row_offsets=ro)
+
+    def free(self):
+        if self.handle:
+            load().rbsc_graph_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device context (``rbsc_ctx``)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.rbsc_ctx_create(device, hbm_budget, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().rbsc_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    PHASES = ("signature", "gather", "sort", "rank", "setup")
+
+    def profile(self, enable: bool = True):
+        check(load().rbsc_ctx_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self, reset: bool = True) -> dict:
+        ms = (C.c_double * 5)()
+        n = (C.c_uint64 * 5)()
+        check(load().rbsc_ctx_profile_read(self.handle, ms, n, 1 if reset else 0))
+        return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(self.PHASES)}
+
+    def refine(self, g: HostGraph, max_batch_edges=0, mode="stream", max_rounds=0,
+               verify=False, partition=True) -> RefineResult:
+        st = g.as_struct()
+        o = _opts(max_batch_edges, mode, max_rounds, verify)
+        part = np.empty(g.n_nodes, np.uint32) if partition else None
+        r = _lib.Result()
+        check(load().rbsc_refine(self.handle, C.byref(st), C.byref(o), C.byref(r),
+                                 _ptr(part, C.c_uint32)))
+        return _result(r, part)
+
+    def upload(self, g: HostGraph) -> DeviceGraph:
+        st = g.as_struct()
+        h = C.c_void_p()
+        check(load().rbsc_graph_upload(self.handle, C.byref(st), C.byref(h)))
+        return DeviceGraph(self, h)
+
+    def gen_rmat(self, scale, n_draws, a=0.57, b=0.19, c=0.19, clone_frac=0.0,
+                 seed=1) -> DeviceGraph:
+        h = C.c_void_p()
+        check(load().rbsc_gen_rmat_device(self.handle, scale, n_draws, a, b, c, clone_frac,
+                                          seed, C.byref(h)))
+        return DeviceGraph(self, h)
+
+
+from . import gen  # noqa: E402  (uses the classes above)

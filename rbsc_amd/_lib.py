@@ -1,0 +1,135 @@
+"""ctypes binding of the gfx950 RBSC library (include/rbsc.h, include/rbsc_gen.h).
+
+The shared library is built in-tree (``rbsc_amd/librbsc.so``) by
+``python -c "import __graft_entry__ as g; g.build()"`` or ``make -C rbsc_amd/csrc``.
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librbsc.so")
+CLI_PATH = os.path.join(_HERE, "bin", "rbsc")
+
+RBSC_OK = 0
+ERRORS = {
+    -1: "RBSC_ERR_INVALID",
+    -2: "RBSC_ERR_NOMEM",
+    -3: "RBSC_ERR_HIP",
+    -4: "RBSC_ERR_BUDGET",
+    -5: "RBSC_ERR_INPUT",
+    -6: "RBSC_ERR_KERNEL",
+    -7: "RBSC_ERR_NODEVICE",
+}
+MODE_STREAM, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE = 0, 1, 2
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+
+class Graph(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32),
+        ("n_edges", C.c_uint64),
+        ("row_offsets", u64p),
+        ("src", u32p),
+        ("dst", u32p),
+        ("weight", u64p),
+        ("init_color", u32p),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("max_batch_edges", C.c_uint64),
+        ("mode", C.c_uint32),
+        ("max_rounds", C.c_uint32),
+        ("verify", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_uint64),
+        ("done", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("time_s", C.c_double),
+        ("device_ms", C.c_double),
+        ("verified", C.c_int32),
+        ("passes", C.c_uint32),
+        ("edges_per_round", C.c_uint64),
+    ]
+
+
+class HostGraph(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32),
+        ("n_edges", C.c_uint64),
+        ("row_offsets", u64p),
+        ("dst", u32p),
+        ("weight", u64p),
+        ("init_color", u32p),
+    ]
+
+
+# Every exported symbol of the C headers, with (restype, argtypes).
+SIGNATURES = {
+    "rbsc_last_error": (C.c_char_p, []),
+    "rbsc_version": (C.c_char_p, []),
+    "rbsc_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "rbsc_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "rbsc_refine": (C.c_int, [C.c_void_p, C.POINTER(Graph), C.POINTER(Opts), C.POINTER(Result), u32p]),
+    "rbsc_graph_upload": (C.c_int, [C.c_void_p, C.POINTER(Graph), C.POINTER(C.c_void_p)]),
+    "rbsc_graph_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "rbsc_graph_bytes": (C.c_uint64, [C.c_void_p]),
+    "rbsc_graph_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result), u32p, u32p]),
+    "rbsc_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "rbsc_ctx_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), u64p, C.c_int]),
+    "rbsc_gen_rmat_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "rbsc_graph_dims": (C.c_int, [C.c_void_p, u32p, u64p]),
+    "rbsc_graph_download": (C.c_int, [C.c_void_p, C.c_void_p, u64p, u32p]),
+    "rbsc_host_graph_free": (None, [C.POINTER(HostGraph)]),
+    "rbsc_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_uint64, C.POINTER(HostGraph)]),
+    "rbsc_gen_ba": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint64,
+                              C.POINTER(HostGraph)]),
+    "rbsc_gen_er_undirected": (C.c_int, [C.c_uint32, C.c_double, C.c_uint64, C.POINTER(HostGraph)]),
+    "rbsc_gen_ba_undirected": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(HostGraph)]),
+    "rbsc_gen_regular_undirected": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(HostGraph)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load librbsc.so (once).  Raises RuntimeError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C rbsc_amd/csrc` "
+            "(there is no CPU fallback for the RBSC path)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class RbscError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def check(code: int) -> None:
+    if code != RBSC_OK:
+        msg = load().rbsc_last_error().decode(errors="replace")
+        raise RbscError(code, msg)

@@ -1,0 +1,851 @@
+// refine.hip -- the RBSC refinement round on gfx950 and the C ABI around it.
+//
+// One round (reference rbsc.cu:354-425), colour c_r, salt r:
+//   sig_r(u) = sum over u's lines of randomize(c_r(v) + r) * w      (set_values
+//              80-87 + ReduceByKey 366-377 + mend_partition 104-116)
+//   c_{r+1}(u) = dense rank of (sig_r(u), c_r(u)), ascending        (384-396)
+//   stop when the number of distinct pairs repeats                  (398-425)
+//
+// gfx950 layout of one round:
+//   sig_kernel    edge-balanced wave tiles over the compressed CSR; each lane
+//                 folds 8 lines, rows are closed with a 64-lane segmented
+//                 scan; lines read coalesced, hashed colours gathered.
+//   sig_fixup     adds the partial sums of rows that span wave tiles.
+//   gather_keys   keys[i] = sig[node of position i], positions kept sorted
+//                 by the current colour from the previous round.
+//   radix sort    stable onesweep sort by the 64-bit signature; stability
+//                 turns colour order into (sig, colour) order, so the
+//                 reference's first sort by colour (rbsc.cu:386) is free.
+//   rank_kernel   head flags + decoupled look-back scan = dense rank; writes
+//                 the packed (rank, node) order for the next round and the
+//                 next round's hashed colour randomize(rank + r + 1) of every
+//                 node, so the signature kernel never re-hashes per line.
+// The reference's collision loop (rbsc.cu:404-419) always exits after one
+// pass because randomize is a bijection; it is not repeated here.
+#include "internal.h"
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+// ---------------------------------------------------------------- errors
+static thread_local char g_err[512] = "";
+extern "C" void rbsc_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+extern "C" const char* rbsc_last_error(void) { return g_err; }
+extern "C" const char* rbsc_version(void) { return "rbsc-gfx950 0.1"; }
+
+namespace {
+
+constexpr int BLK = 256;
+inline unsigned grid_for(u64 n, int blk = BLK) { return (unsigned)((n + blk - 1) / blk); }
+
+// ---------------------------------------------------------------- kernels
+__global__ void fill_u64(u64* p, u64 n, u64 v) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Tile metadata: the row holding each tile's first line, and the row (if
+// any) that continues past the tile's end.
+__global__ void tile_setup_kernel(const u64* __restrict__ row_ptr, u32 U, u64 E, u64 n_tiles,
+                                  u32* __restrict__ tile_row, u32* __restrict__ carry_row) {
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n_tiles) return;
+    if (t == n_tiles) {
+        tile_row[t] = U - 1;
+        return;
+    }
+    const u64 e0 = t * SIG_TW;
+    u32 lo = 0, hi = U - 1;
+    while (lo < hi) {
+        const u32 mid = lo + (hi - lo + 1) / 2;
+        if (row_ptr[mid] <= e0) lo = mid; else hi = mid - 1;
+    }
+    tile_row[t] = lo;
+    const u64 elast = (e0 + SIG_TW < E ? e0 + SIG_TW : E) - 1;
+    hi = U - 1;
+    while (lo < hi) {
+        const u32 mid = lo + (hi - lo + 1) / 2;
+        if (row_ptr[mid] <= elast) lo = mid; else hi = mid - 1;
+    }
+    carry_row[t] = row_ptr[lo + 1] > elast + 1 ? lo : NO_ROW;
+}
+
+// Signature of every row: sig[row_node[k]] = sum of hc[dst] * w over row k.
+template <bool HAS_WT>
+__global__ __launch_bounds__(256) void sig_kernel(
+    const u64* __restrict__ row_ptr, const u32* __restrict__ row_node, u32 U,
+    const u32* __restrict__ dst, const u64* __restrict__ wt, u64 E, const u64* __restrict__ hc,
+    u64* __restrict__ sig, const u32* __restrict__ tile_row, u64* __restrict__ carry_val,
+    u64 n_tiles) {
+    const u64 t = (u64)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (t >= n_tiles) return;  // uniform per wave
+    const int lane = threadIdx.x & 63;
+    const u64 e0 = t * SIG_TW;
+    const u64 e1 = e0 + SIG_TW < E ? e0 + SIG_TW : E;
+    const u64 es = e0 + (u64)lane * SIG_V;
+    const bool active = es < e1;
+    const u64 ee = active ? (es + SIG_V < e1 ? es + SIG_V : e1) : es;
+
+    u64 acc = 0, head_val = 0;
+    bool has_head = false, tail_open = false;
+    u32 head_k = 0;
+    u64 key;
+    if (active) {
+        u32 lo = tile_row[t], hi = tile_row[t + 1];
+        while (lo < hi) {
+            const u32 mid = lo + (hi - lo + 1) / 2;
+            if (row_ptr[mid] <= es) lo = mid; else hi = mid - 1;
+        }
+        u32 k = lo;
+        bool cont = row_ptr[k] < es;
+        u64 rend = row_ptr[k + 1];
+        u64 hv[SIG_V];
+        if (ee - es == SIG_V) {
+            const uint4* p = reinterpret_cast<const uint4*>(dst + es);
+            const uint4 a = p[0], b = p[1];
+            const u32 dv[SIG_V] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int j = 0; j < SIG_V; ++j) hv[j] = hc[dv[j]];
+            if (HAS_WT) {
+#pragma unroll
+                for (int j = 0; j < SIG_V; ++j) hv[j] *= wt[es + j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SIG_V; ++j) {
+                hv[j] = 0;
+                if (es + j < ee) {
+                    hv[j] = hc[dst[es + j]];
+                    if (HAS_WT) hv[j] *= wt[es + j];
+                }
+            }
+        }
+        bool closed_last = false;
+#pragma unroll
+        for (int j = 0; j < SIG_V; ++j) {
+            const u64 e = es + j;
+            if (e < ee) {
+                acc += hv[j];
+                closed_last = false;
+                if (e + 1 == rend) {
+                    if (cont) {
+                        has_head = true;
+                        head_val = acc;
+                        head_k = k;
+                        cont = false;
+                    } else {
+                        sig[row_node[k]] = acc;
+                    }
+                    acc = 0;
+                    closed_last = true;
+                    ++k;
+                    if (k < U) rend = row_ptr[k + 1];
+                }
+            }
+        }
+        tail_open = !closed_last;
+        key = tail_open ? 2 * (u64)k : 2 * (u64)(k - 1) + 1;
+    } else {
+        key = 2 * (u64)U + 1 + 2 * (u64)lane;
+    }
+    // Segmented inclusive scan of open-row partials across the wave.
+    u64 S = tail_open ? acc : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u64 oS = __shfl_up(S, d);
+        const u64 oK = __shfl_up(key, d);
+        if (lane >= d && oK == key) S += oS;
+    }
+    const u64 pS = __shfl_up(S, 1);
+    const u64 pK = __shfl_up(key, 1);
+    if (has_head) {
+        if (lane > 0 && pK == 2 * (u64)head_k) head_val += pS;
+        sig[row_node[head_k]] = head_val;
+    }
+    const int last_lane = (int)((e1 - e0 - 1) / SIG_V);
+    if (lane == last_lane && tail_open) carry_val[t] = S;
+}
+
+__global__ void sig_fixup_kernel(const u32* __restrict__ carry_row,
+                                 const u64* __restrict__ carry_val,
+                                 const u32* __restrict__ row_node, u64* sig, u64 n_tiles) {
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    const u32 r = carry_row[t];
+    if (r != NO_ROW) atomicAdd((unsigned long long*)&sig[row_node[r]], (unsigned long long)carry_val[t]);
+}
+
+// Initial colours: c0 = init[u] when given, else 0 (nodes with a line are
+// raised to N+1 by init_rows_kernel, rbsc.cu:51-63).
+__global__ void init_c0_kernel(const u32* __restrict__ init, u64* __restrict__ c0, u64 N) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < N) c0[u] = init ? (u64)init[u] : 0ULL;
+}
+__global__ void init_rows_kernel(const u32* __restrict__ row_node, u64 U, u64* __restrict__ c0,
+                                 u64 value) {
+    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < U) c0[row_node[k]] = value;
+}
+__global__ void init_pack_kernel(const u64* __restrict__ c0, u64* __restrict__ P,
+                                 u64* __restrict__ hc, u64 N) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < N) {
+        const u64 c = c0[u];
+        P[u] = (c << 32) | u;
+        hc[u] = rbsc_randomize(c + 0);
+    }
+}
+
+__global__ void gather_keys_kernel(const u64* __restrict__ P, const u64* __restrict__ sig,
+                                   u64* __restrict__ keys, u64 n) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = sig[(u32)P[i]];
+}
+
+constexpr int RANK_ITEMS = 16;
+constexpr u64 RANK_TILE = 256 * RANK_ITEMS;
+
+// Dense rank of (sig, colour) over the sorted sequence (rbsc.cu:392-396):
+// flag = position starts a new distinct pair, rank = inclusive scan of flags.
+// Writes Pout[i] = (rank << 32) | node and hc[node] = randomize(rank + salt).
+__global__ __launch_bounds__(256) void rank_kernel(
+    const u64* __restrict__ keys, const u64* __restrict__ Pin, u64* __restrict__ Pout,
+    u64* __restrict__ hc, u64 n, u64 salt_next, u64* status, u64* ticket, u64 ticket_base,
+    u32 epoch, u32* K_out, u32* err) {
+    __shared__ u64 s_wave[4];
+    __shared__ u64 s_tile, s_prefix;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd((unsigned long long*)ticket, 1ULL) - ticket_base;
+    __syncthreads();
+    const u64 tile = s_tile;
+    const u64 base = tile * RANK_TILE + (u64)tid * RANK_ITEMS;
+    u64 k[RANK_ITEMS], p[RANK_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RANK_ITEMS; ++j) {
+        const u64 i = base + j;
+        k[j] = i < n ? keys[i] : 0;
+        p[j] = i < n ? Pin[i] : 0;
+    }
+    u64 pk = 0, pp = 0;
+    if (base > 0 && base < n) {
+        pk = keys[base - 1];
+        pp = Pin[base - 1];
+    }
+    u32 flags = 0;  // bit j set: item j starts a new pair
+    u32 cnt = 0;
+#pragma unroll
+    for (int j = 0; j < RANK_ITEMS; ++j) {
+        const u64 i = base + j;
+        const u64 prevk = j ? k[j - 1] : pk;
+        const u64 prevc = (j ? p[j - 1] : pp) >> 32;
+        const bool f = i < n && i > 0 && (k[j] != prevk || (p[j] >> 32) != prevc);
+        flags |= (u32)f << j;
+        cnt += f;
+    }
+    // Block exclusive scan of per-thread counts.
+    u64 x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u64 y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    u64 woff = 0, agg = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        if (w < wave) woff += s_wave[w];
+        agg += s_wave[w];
+    }
+    const u64 texcl = woff + x - cnt;
+    if (tid == 0) {
+        u64* st = status + tile;
+        u64 excl = 0;
+        if (tile == 0) {
+            lb_store(st, lb_pack(LB_PREFIX, epoch, agg));
+        } else {
+            lb_store(st, lb_pack(LB_AGG, epoch, agg));
+            long long q = (long long)tile - 1;
+            u32 spins = 0;
+            while (q >= 0) {
+                const u64 s = lb_load(status + q);
+                const u64 state = s >> LB_STATE_SHIFT;
+                const u32 ep = (u32)((s >> LB_EPOCH_SHIFT) & LB_EPOCH_MASK);
+                if (state == 0 || ep != (epoch & LB_EPOCH_MASK)) {
+                    if (++spins > LB_SPIN_LIMIT) {
+                        atomicOr(err, KERR_SPIN);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & LB_COUNT_MASK;
+                if (state == LB_PREFIX) break;
+                --q;
+            }
+            lb_store(st, lb_pack(LB_PREFIX, epoch, excl + agg));
+        }
+        s_prefix = excl;
+    }
+    __syncthreads();
+    u64 r = s_prefix + texcl;
+#pragma unroll
+    for (int j = 0; j < RANK_ITEMS; ++j) {
+        const u64 i = base + j;
+        if (i < n) {
+            r += (flags >> j) & 1u;
+            const u32 node = (u32)p[j];
+            Pout[i] = (r << 32) | node;
+            hc[node] = rbsc_randomize(r + salt_next);
+            if (i == n - 1) *K_out = (u32)(r + 1);
+        }
+    }
+}
+
+// Raw colours by node from the packed order.
+__global__ void colors_kernel(const u64* __restrict__ P, u32* __restrict__ col, u64 n) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) col[(u32)P[i]] = (u32)(P[i] >> 32);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+u32 canonicalize_host(u32 N, const u32* colors, u32* out) {
+    std::vector<u32> map;
+    u32 next = 0;
+    u32 maxc = 0;
+    for (u32 i = 0; i < N; ++i) maxc = std::max(maxc, colors[i]);
+    map.assign((size_t)maxc + 1, UINT32_MAX);
+    for (u32 i = 0; i < N; ++i) {
+        u32& m = map[colors[i]];
+        if (m == UINT32_MAX) m = next++;
+        out[i] = m;
+    }
+    return next;
+}
+
+void prof_begin(rbsc_ctx* ctx, int phase) {
+    if (!ctx->prof) return;
+    if (ctx->prof_used + 2 > ctx->prof_ev.size()) {
+        for (int i = 0; i < 64; ++i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            ctx->prof_ev.push_back(e);
+        }
+    }
+    (void)hipEventRecord(ctx->prof_ev[ctx->prof_used], ctx->stream);
+    ctx->prof_phase.push_back(phase);
+    ctx->prof_used += 1;
+}
+
+void prof_end(rbsc_ctx* ctx) {
+    if (!ctx->prof || ctx->prof_used % 2 == 0) return;
+    (void)hipEventRecord(ctx->prof_ev[ctx->prof_used], ctx->stream);
+    ctx->prof_used += 1;
+}
+
+int prof_collect(rbsc_ctx* ctx) {
+    if (!ctx->prof) return RBSC_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i + 1 < ctx->prof_used; i += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_ev[i], ctx->prof_ev[i + 1]));
+        const int ph = ctx->prof_phase[i / 2];
+        ctx->prof_ms[ph] += ms;
+        ctx->prof_n[ph] += 1;
+    }
+    ctx->prof_used = 0;
+    ctx->prof_phase.clear();
+    return RBSC_OK;
+}
+
+int check_kernel_errors(rbsc_ctx* ctx) {
+    u32 e = 0;
+    HIP_TRY(hipMemcpy(&e, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost));
+    if (e) {
+        HIP_TRY(hipMemset(ctx->d_err, 0, sizeof(u32)));
+        rbsc_set_error("kernel error flags 0x%x (look-back wait expired)", e);
+        return RBSC_ERR_KERNEL;
+    }
+    return RBSC_OK;
+}
+
+int ws_reserve(rbsc_ctx* ctx, u64 n) {
+    Workspace& w = ctx->ws;
+    if (n <= w.cap_n && w.keys[0]) return RBSC_OK;
+    for (int i = 0; i < 2; ++i) {
+        if (w.keys[i]) (void)hipFree(w.keys[i]);
+        if (w.P[i]) (void)hipFree(w.P[i]);
+        w.keys[i] = w.P[i] = nullptr;
+    }
+    if (w.sig) (void)hipFree(w.sig);
+    if (w.hc) (void)hipFree(w.hc);
+    if (w.rank_status) (void)hipFree(w.rank_status);
+    w = Workspace();
+    const u64 cap = std::max<u64>(n, 4096);
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(hipMalloc(&w.keys[i], sizeof(u64) * cap));
+        HIP_TRY(hipMalloc(&w.P[i], sizeof(u64) * cap));
+    }
+    HIP_TRY(hipMalloc(&w.sig, sizeof(u64) * cap));
+    HIP_TRY(hipMalloc(&w.hc, sizeof(u64) * cap));
+    w.rank_tiles = (cap + RANK_TILE - 1) / RANK_TILE + 1;
+    HIP_TRY(hipMalloc(&w.rank_status, sizeof(u64) * w.rank_tiles));
+    HIP_TRY(hipMemsetAsync(w.rank_status, 0, sizeof(u64) * w.rank_tiles, ctx->stream));
+    w.cap_n = cap;
+    int rc = sort_ws_init(&ctx->sort, cap, ctx->d_err, ctx->stream);
+    if (rc) return rc;
+    return RBSC_OK;
+}
+
+int dgraph_alloc(rbsc_dgraph* g, u32 N, u64 E, u32 U, bool weighted, bool has_init) {
+    g->N = N;
+    g->E = E;
+    g->U = U;
+    g->n_tiles = (E + SIG_TW - 1) / SIG_TW;
+    u64 bytes = 0;
+    HIP_TRY(hipMalloc(&g->row_ptr, sizeof(u64) * ((u64)U + 1)));
+    bytes += sizeof(u64) * ((u64)U + 1);
+    HIP_TRY(hipMalloc(&g->row_node, sizeof(u32) * std::max<u64>(U, 1)));
+    bytes += sizeof(u32) * std::max<u64>(U, 1);
+    HIP_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
+    bytes += sizeof(u32) * std::max<u64>(E, 1);
+    if (weighted) {
+        HIP_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
+        bytes += sizeof(u64) * E;
+    }
+    if (has_init) {
+        HIP_TRY(hipMalloc(&g->init, sizeof(u32) * std::max<u32>(N, 1)));
+        bytes += sizeof(u32) * N;
+    }
+    HIP_TRY(hipMalloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1)));
+    HIP_TRY(hipMalloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1)));
+    HIP_TRY(hipMalloc(&g->carry_val, sizeof(u64) * std::max<u64>(g->n_tiles, 1)));
+    bytes += 16 * (g->n_tiles + 1);
+    g->bytes = bytes;
+    return RBSC_OK;
+}
+
+void dgraph_release(rbsc_dgraph* g) {
+    if (!g) return;
+    if (g->owns) {
+        if (g->row_ptr) (void)hipFree(g->row_ptr);
+        if (g->row_node) (void)hipFree(g->row_node);
+        if (g->dst) (void)hipFree(g->dst);
+        if (g->wt) (void)hipFree(g->wt);
+        if (g->init) (void)hipFree(g->init);
+    }
+    if (g->tile_row) (void)hipFree(g->tile_row);
+    if (g->carry_row) (void)hipFree(g->carry_row);
+    if (g->carry_val) (void)hipFree(g->carry_val);
+    *g = rbsc_dgraph();
+}
+
+int dgraph_build_tiles(rbsc_ctx* ctx, rbsc_dgraph* g) {
+    if (g->n_tiles == 0) return RBSC_OK;
+    tile_setup_kernel<<<grid_for(g->n_tiles + 1), BLK, 0, ctx->stream>>>(
+        g->row_ptr, g->U, g->E, g->n_tiles, g->tile_row, g->carry_row);
+    HIP_TRY(hipGetLastError());
+    return RBSC_OK;
+}
+
+// Host-side compression of a graph into rows (validates ids and grouping).
+static int compress_host(const rbsc_graph* gr, std::vector<u64>& row_ptr,
+                         std::vector<u32>& row_node) {
+    const u32 N = gr->n_nodes;
+    const u64 E = gr->n_edges;
+    row_ptr.clear();
+    row_node.clear();
+    if (gr->row_offsets) {
+        const u64* ro = gr->row_offsets;
+        if (ro[0] != 0 || ro[N] != E) {
+            rbsc_set_error("row_offsets must start at 0 and end at n_edges");
+            return RBSC_ERR_INPUT;
+        }
+        for (u32 u = 0; u < N; ++u) {
+            if (ro[u + 1] < ro[u]) {
+                rbsc_set_error("row_offsets not monotone at node %u", u);
+                return RBSC_ERR_INPUT;
+            }
+            if (ro[u + 1] > ro[u]) {
+                row_ptr.push_back(ro[u]);
+                row_node.push_back(u);
+            }
+        }
+    } else {
+        if (!gr->src && E) {
+            rbsc_set_error("graph needs src or row_offsets");
+            return RBSC_ERR_INVALID;
+        }
+        std::vector<unsigned char> seen(N, 0);
+        for (u64 e = 0; e < E; ++e) {
+            const u32 s = gr->src[e];
+            if (s >= N) {
+                rbsc_set_error("line %llu: source %u >= N=%u", (unsigned long long)e, s, N);
+                return RBSC_ERR_INPUT;
+            }
+            if (e == 0 || gr->src[e - 1] != s) {
+                if (seen[s]) {
+                    rbsc_set_error("line %llu: lines of source %u are not grouped",
+                                   (unsigned long long)e, s);
+                    return RBSC_ERR_INPUT;
+                }
+                seen[s] = 1;
+                row_ptr.push_back(e);
+                row_node.push_back(s);
+            }
+        }
+    }
+    row_ptr.push_back(E);
+    for (u64 e = 0; e < E; ++e) {
+        if (gr->dst[e] >= N) {
+            rbsc_set_error("line %llu: destination %u >= N=%u", (unsigned long long)e, gr->dst[e], N);
+            return RBSC_ERR_INPUT;
+        }
+    }
+    return RBSC_OK;
+}
+
+// ---------------------------------------------------------------- refine
+static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
+                           rbsc_result* res, uint32_t* partition_out, uint32_t* colors_out) {
+    hipStream_t s = ctx->stream;
+    const u64 N = g->N;
+    rbsc_result r{};
+    r.passes = 1;
+    r.edges_per_round = g->E;
+    const u32 max_rounds = opts ? opts->max_rounds : 0;
+    if (N == 0) {
+        r.done = 1;
+        if (res) *res = r;
+        return RBSC_OK;
+    }
+    int rc = ws_reserve(ctx, N);
+    if (rc) return rc;
+    Workspace& w = ctx->ws;
+    auto t0 = std::chrono::steady_clock::now();
+
+    // Round-0 state: P sorted by c0, hc = randomize(c0 + 0).
+    prof_begin(ctx, 4);
+    init_c0_kernel<<<grid_for(N), BLK, 0, s>>>(g->init, w.keys[0], N);
+    if (!g->init && g->U)
+        init_rows_kernel<<<grid_for(g->U), BLK, 0, s>>>(g->row_node, g->U, w.keys[0], N + 1);
+    init_pack_kernel<<<grid_for(N), BLK, 0, s>>>(w.keys[0], w.P[0], w.hc, N);
+    fill_u64<<<grid_for(N), BLK, 0, s>>>(w.sig, N, 0);
+    int which = 0;
+    {
+        u64* kk[2] = {w.keys[0], w.keys[1]};
+        u64* pp[2] = {w.P[0], w.P[1]};
+        rc = radix_sort_u64(&ctx->sort, kk, pp, N, 0, 32, &which, s);
+        if (rc) return rc;
+    }
+    int pc = which;  // P[pc] holds the current packed order
+    prof_end(ctx);
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    u32 K = 1, Knew = 1;
+    u32 it = 0;
+    const u64 n_tiles = g->n_tiles;
+    do {
+        K = Knew;
+        prof_begin(ctx, 0);
+        if (n_tiles) {
+            if (g->wt)
+                sig_kernel<true><<<grid_for(n_tiles * 64), BLK, 0, s>>>(
+                    g->row_ptr, g->row_node, g->U, g->dst, g->wt, g->E, w.hc, w.sig, g->tile_row,
+                    g->carry_val, n_tiles);
+            else
+                sig_kernel<false><<<grid_for(n_tiles * 64), BLK, 0, s>>>(
+                    g->row_ptr, g->row_node, g->U, g->dst, nullptr, g->E, w.hc, w.sig,
+                    g->tile_row, g->carry_val, n_tiles);
+            sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, g->carry_val,
+                                                               g->row_node, w.sig, n_tiles);
+        }
+        prof_end(ctx);
+        prof_begin(ctx, 1);
+        gather_keys_kernel<<<grid_for(N), BLK, 0, s>>>(w.P[pc], w.sig, w.keys[0], N);
+        prof_end(ctx);
+        {
+            u64* kk[2] = {w.keys[0], w.keys[1]};
+            u64* pp[2] = {w.P[pc], w.P[pc ^ 1]};
+            prof_begin(ctx, 2);
+            rc = radix_sort_u64(&ctx->sort, kk, pp, N, 0, 64, &which, s);
+            prof_end(ctx);
+            if (rc) return rc;
+            // sorted data in kk[which], pp[which]; rank into the other P buffer
+            u64* sorted_keys = kk[which];
+            u64* sorted_P = pp[which];
+            u64* out_P = pp[which ^ 1];
+            const u64 tiles = (N + RANK_TILE - 1) / RANK_TILE;
+            const u32 ep = lb_next_epoch(&ctx->sort, s);
+            prof_begin(ctx, 3);
+            rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
+                sorted_keys, sorted_P, out_P, w.hc, N, (u64)it + 1, w.rank_status,
+                ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar, ctx->d_err);
+            prof_end(ctx);
+            ctx->sort.ticket_total += tiles;
+            pc = (out_P == w.P[0]) ? 0 : 1;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(u32), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        Knew = ctx->h_scalar[0];
+        ++it;
+        if (max_rounds && it >= max_rounds) break;
+    } while (K != Knew);
+    HIP_TRY(hipEventRecord(ctx->ev1, s));
+    rc = check_kernel_errors(ctx);
+    if (rc) return rc;
+    rc = prof_collect(ctx);
+    if (rc) return rc;
+
+    if (partition_out || colors_out) {
+        u32* d_col = reinterpret_cast<u32*>(w.keys[0]);
+        colors_kernel<<<grid_for(N), BLK, 0, s>>>(w.P[pc], d_col, N);
+        std::vector<u32> col;
+        u32* hcol = colors_out;
+        if (!hcol) {
+            col.resize(N);
+            hcol = col.data();
+        }
+        HIP_TRY(hipMemcpyAsync(hcol, d_col, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (partition_out) canonicalize_host((u32)N, hcol, partition_out);
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    r.nodes = Knew;
+    r.done = 1;
+    r.rounds = it;
+    r.device_ms = ms;
+    r.time_s = std::chrono::duration<double>(t1 - t0).count();
+    r.verified = 0;
+    if (res) *res = r;
+    return RBSC_OK;
+}
+
+// ---------------------------------------------------------------- C ABI
+extern "C" int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out) {
+    if (!out) {
+        rbsc_set_error("rbsc_ctx_create: out is null");
+        return RBSC_ERR_INVALID;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        rbsc_set_error("no HIP device available");
+        return RBSC_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        rbsc_set_error("device %d out of range (%d devices)", device, ndev);
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(device));
+    rbsc_ctx* c = new rbsc_ctx();
+    c->device = device;
+    c->budget = hbm_budget;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&c->d_err, sizeof(u32)));
+    HIP_TRY(hipMemset(c->d_err, 0, sizeof(u32)));
+    HIP_TRY(hipMalloc(&c->d_scalar, sizeof(u32) * 16));
+    HIP_TRY(hipHostMalloc(&c->h_scalar, sizeof(u32) * 16, hipHostMallocDefault));
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+    *out = c;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
+    if (!c) return RBSC_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    Workspace& w = c->ws;
+    for (int i = 0; i < 2; ++i) {
+        if (w.keys[i]) (void)hipFree(w.keys[i]);
+        if (w.P[i]) (void)hipFree(w.P[i]);
+    }
+    if (w.sig) (void)hipFree(w.sig);
+    if (w.hc) (void)hipFree(w.hc);
+    if (w.rank_status) (void)hipFree(w.rank_status);
+    sort_ws_free(&c->sort);
+    if (c->d_err) (void)hipFree(c->d_err);
+    if (c->d_scalar) (void)hipFree(c->d_scalar);
+    if (c->h_scalar) (void)hipHostFree(c->h_scalar);
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    delete c;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgraph** out) {
+    if (!ctx || !gr || !out) {
+        rbsc_set_error("rbsc_graph_upload: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (gr->n_nodes == 0xFFFFFFFFu) {
+        rbsc_set_error("n_nodes must be below 2^32-1 (colour N+1 must fit 32 bits)");
+        return RBSC_ERR_INVALID;
+    }
+    if (gr->n_edges && !gr->dst) {
+        rbsc_set_error("rbsc_graph_upload: dst is null");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<u64> row_ptr;
+    std::vector<u32> row_node;
+    int rc = compress_host(gr, row_ptr, row_node);
+    if (rc) return rc;
+    const u32 U = (u32)row_node.size();
+    const u64 need = sizeof(u64) * (U + 1) + 4ull * U + 4ull * gr->n_edges +
+                     (gr->weight ? 8ull * gr->n_edges : 0) + 48ull * gr->n_nodes;
+    if (ctx->budget && need > ctx->budget) {
+        rbsc_set_error("graph needs %llu bytes, budget is %llu", (unsigned long long)need,
+                       (unsigned long long)ctx->budget);
+        return RBSC_ERR_BUDGET;
+    }
+    rbsc_dgraph* g = new rbsc_dgraph();
+    rc = dgraph_alloc(g, gr->n_nodes, gr->n_edges, U, gr->weight != nullptr,
+                      gr->init_color != nullptr);
+    if (rc) {
+        dgraph_release(g);
+        delete g;
+        return rc;
+    }
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(g->row_ptr, row_ptr.data(), sizeof(u64) * (U + 1), hipMemcpyHostToDevice, s));
+    if (U) HIP_TRY(hipMemcpyAsync(g->row_node, row_node.data(), sizeof(u32) * U, hipMemcpyHostToDevice, s));
+    if (gr->n_edges) {
+        HIP_TRY(hipMemcpyAsync(g->dst, gr->dst, sizeof(u32) * gr->n_edges, hipMemcpyHostToDevice, s));
+        if (gr->weight)
+            HIP_TRY(hipMemcpyAsync(g->wt, gr->weight, sizeof(u64) * gr->n_edges, hipMemcpyHostToDevice, s));
+    }
+    if (gr->init_color && gr->n_nodes)
+        HIP_TRY(hipMemcpyAsync(g->init, gr->init_color, sizeof(u32) * gr->n_nodes, hipMemcpyHostToDevice, s));
+    rc = dgraph_build_tiles(ctx, g);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    *out = g;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_graph_free(rbsc_ctx* ctx, rbsc_dgraph* g) {
+    if (!g) return RBSC_OK;
+    if (ctx) (void)hipSetDevice(ctx->device);
+    dgraph_release(g);
+    delete g;
+    return RBSC_OK;
+}
+
+extern "C" uint64_t rbsc_graph_bytes(const rbsc_dgraph* g) { return g ? g->bytes : 0; }
+
+extern "C" int rbsc_graph_dims(const rbsc_dgraph* g, uint32_t* n_nodes, uint64_t* n_edges) {
+    if (!g) {
+        rbsc_set_error("rbsc_graph_dims: null graph");
+        return RBSC_ERR_INVALID;
+    }
+    if (n_nodes) *n_nodes = g->N;
+    if (n_edges) *n_edges = g->E;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t* row_offsets,
+                                   uint32_t* dst) {
+    if (!ctx || !g) {
+        rbsc_set_error("rbsc_graph_download: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<u64> rp(g->U + 1);
+    std::vector<u32> rn(g->U);
+    HIP_TRY(hipMemcpy(rp.data(), g->row_ptr, sizeof(u64) * (g->U + 1), hipMemcpyDeviceToHost));
+    if (g->U) HIP_TRY(hipMemcpy(rn.data(), g->row_node, sizeof(u32) * g->U, hipMemcpyDeviceToHost));
+    if (row_offsets) {
+        // Expand to CSR over all nodes; rows of a resident graph are stored in
+        // node order only when the input was sorted by source.
+        std::vector<u64> deg(g->N, 0);
+        for (u32 k = 0; k < g->U; ++k) deg[rn[k]] = rp[k + 1] - rp[k];
+        row_offsets[0] = 0;
+        for (u32 u = 0; u < g->N; ++u) row_offsets[u + 1] = row_offsets[u] + deg[u];
+        for (u32 k = 0; k + 1 < g->U; ++k)
+            if (rn[k] >= rn[k + 1]) {
+                rbsc_set_error("resident graph rows are not in node order");
+                return RBSC_ERR_INVALID;
+            }
+    }
+    if (dst && g->E) HIP_TRY(hipMemcpy(dst, g->dst, sizeof(u32) * g->E, hipMemcpyDeviceToHost));
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_graph_refine(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
+                                 rbsc_result* result, uint32_t* partition_out,
+                                 uint32_t* colors_out) {
+    if (!ctx || !g) {
+        rbsc_set_error("rbsc_graph_refine: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    return refine_resident(ctx, g, opts, result, partition_out, colors_out);
+}
+
+extern "C" int rbsc_refine(rbsc_ctx* ctx, const rbsc_graph* graph, const rbsc_opts* opts,
+                           rbsc_result* result, uint32_t* partition_out) {
+    if (!ctx || !graph) {
+        rbsc_set_error("rbsc_refine: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    rbsc_dgraph* g = nullptr;
+    int rc = rbsc_graph_upload(ctx, graph, &g);
+    if (rc) return rc;
+    rbsc_result r{};
+    rc = rbsc_graph_refine(ctx, g, opts, &r, partition_out, nullptr);
+    rbsc_graph_free(ctx, g);
+    if (rc) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    r.time_s = std::chrono::duration<double>(t1 - t0).count();
+    if (result) *result = r;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_ctx_profile(rbsc_ctx* ctx, int enable) {
+    if (!ctx) {
+        rbsc_set_error("rbsc_ctx_profile: null context");
+        return RBSC_ERR_INVALID;
+    }
+    ctx->prof = enable != 0;
+    ctx->prof_used = 0;
+    ctx->prof_phase.clear();
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int reset) {
+    if (!ctx) {
+        rbsc_set_error("rbsc_ctx_profile_read: null context");
+        return RBSC_ERR_INVALID;
+    }
+    for (int i = 0; i < rbsc_ctx::NPHASE; ++i) {
+        if (ms) ms[i] = ctx->prof_ms[i];
+        if (launches) launches[i] = ctx->prof_n[i];
+        if (reset) {
+            ctx->prof_ms[i] = 0;
+            ctx->prof_n[i] = 0;
+        }
+    }
+    return RBSC_OK;
+}

@@ -1,0 +1,184 @@
+"""HIP path vs the oracle, through the C ABI (librbsc.so).
+
+Bit-exact per-round colours (the dense ranks rbsc.cu:384-396 produces each
+round), final canonical partitions, block counts and round counts, on seeded
+graphs, golden fixtures, closed forms and edge cases; plus the device R-MAT
+generator against the host one and the rbsc command end to end.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import rbsc_amd
+from rbsc_amd import HostGraph, gen
+from rbsc_amd._lib import CLI_PATH
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(__file__)
+
+
+def _oracle_full(g, trace_rounds=0):
+    return oracle.rbsc_refine(g.n_nodes, g.sources(), g.dst, g.weight, g.init_color,
+                              trace_rounds=trace_rounds)
+
+
+CASES = {
+    "rmat_s10_c30": lambda: gen.rmat(10, 1 << 14, clone_frac=0.3, seed=5),
+    "rmat_s14_c20": lambda: gen.rmat(14, 1 << 18, clone_frac=0.2, seed=11),
+    "rmat_s12_w": lambda: _weighted(gen.rmat(12, 1 << 16, clone_frac=0.3, seed=3), 7),
+    "ba_init3": lambda: gen.ba(20000, 4, 0.1, 3, seed=9),
+    "er_und": lambda: gen.er_undirected(5000, 5.0, seed=2),
+    "reg3": lambda: gen.regular_undirected(3000, 3, seed=4),
+}
+
+
+def _weighted(g, wmax):
+    g.weight = np.random.default_rng(1).integers(1, wmax + 1, g.n_edges).astype(np.uint64)
+    return g
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_per_round_colors_bit_exact(ctx, name):
+    g = CASES[name]()
+    c, K, R, tr = _oracle_full(g, trace_rounds=64)
+    dg = ctx.upload(g)
+    for r in range(1, R + 1):
+        res = dg.refine(max_rounds=r, partition=False, colors=True)
+        assert res.rounds == r
+        assert np.array_equal(res.colors, tr[r - 1]), f"round {r} colours differ"
+    full = dg.refine()
+    assert (full.nodes, full.rounds, full.done) == (K, R, 1)
+    assert np.array_equal(full.partition, oracle.canonical(c))
+    dg.free()
+
+
+def _golden_specs():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden",
+                                                  os.path.join(HERE, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    with open(os.path.join(HERE, "golden", "fixtures.json")) as f:
+        recs = {r["name"]: r for r in json.load(f)["fixtures"]}
+    return [(n, fn, recs[n]) for n, fn in mg.SPECS]
+
+
+@pytest.mark.parametrize("spec", _golden_specs(), ids=lambda s: s[0])
+def test_golden_fixture_block_counts(ctx, spec):
+    name, fn, rec = spec
+    g = fn()
+    r = ctx.refine(g)
+    assert r.nodes == rec["valmari_nodes"]
+    if "rsc_nodes" in rec:
+        assert r.nodes == rec["rsc_nodes"]
+    c, K, R, _ = _oracle_full(g)
+    assert r.rounds == R
+    assert np.array_equal(r.partition, oracle.canonical(c))
+
+
+def _coo_graph(n, pairs, weight=None):
+    pairs = sorted(pairs)
+    src = np.array([p[0] for p in pairs], np.uint32)
+    dst = np.array([p[1] for p in pairs], np.uint32)
+    return HostGraph(n, dst, src=src, weight=weight)
+
+
+def test_edge_cases(ctx):
+    # no lines at all: one block after one round
+    r = ctx.refine(HostGraph(7, np.zeros(0, np.uint32), src=np.zeros(0, np.uint32)))
+    assert (r.nodes, r.rounds) == (1, 1)
+    assert np.array_equal(r.partition, np.zeros(7, np.uint32))
+    # one node with a self loop
+    r = ctx.refine(_coo_graph(1, [(0, 0)]))
+    assert r.nodes == 1
+    # a star whose centre row spans many wave tiles (carry fix-up path)
+    n = 40000
+    g = _coo_graph(n, [(0, i) for i in range(1, n)] + [(i, 0) for i in range(1, n, 2)])
+    c, K, R, _ = _oracle_full(g)
+    r = ctx.refine(g)
+    assert (r.nodes, r.rounds) == (K, R)
+    assert np.array_equal(r.partition, oracle.canonical(c))
+    # directed cycle and path
+    assert ctx.refine(_coo_graph(50, [(i, (i + 1) % 50) for i in range(50)])).nodes == 1
+    assert ctx.refine(_coo_graph(50, [(i, i + 1) for i in range(49)])).nodes == 50
+
+
+def test_long_rows_mixed_with_short(ctx):
+    rng = np.random.default_rng(5)
+    n = 6000
+    deg = rng.integers(0, 4, n)
+    deg[rng.integers(0, n, 12)] = rng.integers(600, 5000, 12)
+    src = np.repeat(np.arange(n), deg).astype(np.uint32)
+    dst = rng.integers(0, n, len(src)).astype(np.uint32)
+    g = HostGraph(n, dst, src=src)
+    c, K, R, tr = _oracle_full(g, trace_rounds=64)
+    dg = ctx.upload(g)
+    for r in range(1, R + 1):
+        assert np.array_equal(dg.refine(max_rounds=r, partition=False, colors=True).colors, tr[r - 1])
+    dg.free()
+
+
+def test_grouped_but_unsorted_sources(ctx):
+    # groups must be contiguous, not ascending (ReduceByKey contiguity)
+    g = _coo_graph(5, [(0, 1), (1, 2), (2, 0)])
+    src = np.array([3, 3, 1, 0, 4], np.uint32)
+    dst = np.array([1, 2, 0, 3, 3], np.uint32)
+    g = HostGraph(5, dst, src=src)
+    c, K, R, _ = oracle.rbsc_refine(5, src, dst)
+    r = ctx.refine(g)
+    assert r.nodes == K and np.array_equal(r.partition, oracle.canonical(c))
+
+
+def test_input_errors(ctx):
+    with pytest.raises(rbsc_amd.RbscError) as e:
+        ctx.refine(HostGraph(3, np.array([1, 2, 0], np.uint32), src=np.array([0, 1, 0], np.uint32)))
+    assert e.value.code == -5  # ungrouped
+    with pytest.raises(rbsc_amd.RbscError) as e:
+        ctx.refine(HostGraph(3, np.array([7], np.uint32), src=np.array([0], np.uint32)))
+    assert e.value.code == -5  # id out of range
+
+
+def test_device_rmat_equals_host_rmat(ctx):
+    h = gen.rmat(13, 1 << 17, clone_frac=0.3, seed=21)
+    d = ctx.gen_rmat(13, 1 << 17, clone_frac=0.3, seed=21)
+    back = d.download()
+    assert back.n_nodes == h.n_nodes
+    assert np.array_equal(back.row_offsets, h.row_offsets)
+    assert np.array_equal(back.dst, h.dst)
+    c, K, R, _ = _oracle_full(h)
+    r = d.refine()
+    assert (r.nodes, r.rounds) == (K, R)
+    assert np.array_equal(r.partition, oracle.canonical(c))
+    d.free()
+
+
+def test_rmat_2p20_clone30_bit_exact(ctx):
+    """The 2^20-node / 2^24-draw R-MAT with 30% clones, checked bit-exact
+    against the oracle (final colours and partition)."""
+    g = gen.rmat_2p20_clone30()
+    c, K, R, _ = _oracle_full(g)
+    dg = ctx.upload(g)
+    res = dg.refine(colors=True)
+    assert (res.nodes, res.rounds) == (K, R)
+    assert np.array_equal(res.colors, c)
+    assert np.array_equal(res.partition, oracle.canonical(c))
+    dg.free()
+
+
+def test_cli_end_to_end(tmp_path):
+    g = gen.rmat(10, 1 << 13, clone_frac=0.3, seed=6)
+    c, K, R, _ = _oracle_full(g)
+    part = tmp_path / "part.txt"
+    p = subprocess.run([CLI_PATH, str(g.n_edges), "1", "--partition-out", str(part)],
+                       input=g.to_text(), stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr.decode()
+    lines = p.stdout.decode().split("\n")
+    float(lines[0])
+    assert int(lines[1]) == 1 and int(lines[2]) == K
+    got = np.loadtxt(part, dtype=np.uint32)
+    assert np.array_equal(got, oracle.canonical(c))
