@@ -117,6 +117,14 @@ int rbsc_graph_refine(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
                       rbsc_result* result, uint32_t* partition_out,
                       uint32_t* colors_out);
 
+/* Move a resident graph's line arrays (dst, weights) to pinned host memory;
+ * later refinements stream them in slices of opts.max_batch_edges lines with
+ * hipMemcpyAsync on a side stream, double-buffered against the signature
+ * kernel, and compute the exact full-graph rounds (done = 1).  Replaces the
+ * reference's per-batch host->device copies (rbsc.cu:341-344) for graphs
+ * larger than HBM. */
+int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g);
+
 /* Per-phase device timing with hipEvents on the context's stream (off by
  * default).  Phases: 0 signature kernel + row fix-up, 1 key gather,
  * 2 radix sort, 3 rank/scan, 4 round-0 set-up.  rbsc_ctx_profile_read fills

@@ -170,6 +170,11 @@ class DeviceGraph:
                                        _ptr(part, C.c_uint32), _ptr(col, C.c_uint32)))
         return _result(r, part, col)
 
+    def lines_to_host(self):
+        """Keep the line arrays in pinned host memory; refinements then stream
+        them in slices of ``max_batch_edges`` lines."""
+        check(load().rbsc_graph_lines_to_host(self.ctx.handle, self.handle))
+
     def download(self) -> HostGraph:
         n, e = self.dims
         ro = np.empty(n + 1, np.uint64)

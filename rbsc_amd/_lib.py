@@ -86,6 +86,7 @@ SIGNATURES = {
     "rbsc_graph_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_graph_bytes": (C.c_uint64, [C.c_void_p]),
     "rbsc_graph_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result), u32p, u32p]),
+    "rbsc_graph_lines_to_host": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_ctx_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), u64p, C.c_int]),
     "rbsc_gen_rmat_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double,

@@ -84,9 +84,11 @@ __global__ __launch_bounds__(256) void sig_kernel(
     const u64* __restrict__ row_ptr, const u32* __restrict__ row_node, u32 U,
     const u32* __restrict__ dst, const u64* __restrict__ wt, u64 E, const u64* __restrict__ hc,
     u64* __restrict__ sig, const u32* __restrict__ tile_row, u64* __restrict__ carry_val,
-    u64 n_tiles) {
-    const u64 t = (u64)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (t >= n_tiles) return;  // uniform per wave
+    u64 t_begin, u64 t_end, u64 line_base) {
+    // Tiles [t_begin, t_end); dst/wt hold lines [line_base, ...) (a streamed
+    // slice, or the whole array when line_base == 0).
+    const u64 t = t_begin + (u64)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (t >= t_end) return;  // uniform per wave
     const int lane = threadIdx.x & 63;
     const u64 e0 = t * SIG_TW;
     const u64 e1 = e0 + SIG_TW < E ? e0 + SIG_TW : E;
@@ -109,22 +111,22 @@ __global__ __launch_bounds__(256) void sig_kernel(
         u64 rend = row_ptr[k + 1];
         u64 hv[SIG_V];
         if (ee - es == SIG_V) {
-            const uint4* p = reinterpret_cast<const uint4*>(dst + es);
+            const uint4* p = reinterpret_cast<const uint4*>(dst + (es - line_base));
             const uint4 a = p[0], b = p[1];
             const u32 dv[SIG_V] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
             for (int j = 0; j < SIG_V; ++j) hv[j] = hc[dv[j]];
             if (HAS_WT) {
 #pragma unroll
-                for (int j = 0; j < SIG_V; ++j) hv[j] *= wt[es + j];
+                for (int j = 0; j < SIG_V; ++j) hv[j] *= wt[es - line_base + j];
             }
         } else {
 #pragma unroll
             for (int j = 0; j < SIG_V; ++j) {
                 hv[j] = 0;
                 if (es + j < ee) {
-                    hv[j] = hc[dst[es + j]];
-                    if (HAS_WT) hv[j] *= wt[es + j];
+                    hv[j] = hc[dst[es - line_base + j]];
+                    if (HAS_WT) hv[j] *= wt[es - line_base + j];
                 }
             }
         }
@@ -408,7 +410,8 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     return RBSC_OK;
 }
 
-int dgraph_alloc(rbsc_dgraph* g, u32 N, u64 E, u32 U, bool weighted, bool has_init) {
+int dgraph_alloc(rbsc_dgraph* g, u32 N, u64 E, u32 U, bool weighted, bool has_init,
+                 bool lines_on_device) {
     g->N = N;
     g->E = E;
     g->U = U;
@@ -418,11 +421,17 @@ int dgraph_alloc(rbsc_dgraph* g, u32 N, u64 E, u32 U, bool weighted, bool has_in
     bytes += sizeof(u64) * ((u64)U + 1);
     HIP_TRY(hipMalloc(&g->row_node, sizeof(u32) * std::max<u64>(U, 1)));
     bytes += sizeof(u32) * std::max<u64>(U, 1);
-    HIP_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
-    bytes += sizeof(u32) * std::max<u64>(E, 1);
-    if (weighted) {
-        HIP_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
-        bytes += sizeof(u64) * E;
+    if (lines_on_device) {
+        HIP_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
+        bytes += sizeof(u32) * std::max<u64>(E, 1);
+        if (weighted) {
+            HIP_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
+            bytes += sizeof(u64) * E;
+        }
+    } else {
+        HIP_TRY(hipHostMalloc(&g->dst_host, sizeof(u32) * std::max<u64>(E, 1), hipHostMallocDefault));
+        if (weighted)
+            HIP_TRY(hipHostMalloc(&g->wt_host, sizeof(u64) * std::max<u64>(E, 1), hipHostMallocDefault));
     }
     if (has_init) {
         HIP_TRY(hipMalloc(&g->init, sizeof(u32) * std::max<u32>(N, 1)));
@@ -445,6 +454,8 @@ void dgraph_release(rbsc_dgraph* g) {
         if (g->wt) (void)hipFree(g->wt);
         if (g->init) (void)hipFree(g->init);
     }
+    if (g->dst_host) (void)hipHostFree(g->dst_host);
+    if (g->wt_host) (void)hipHostFree(g->wt_host);
     if (g->tile_row) (void)hipFree(g->tile_row);
     if (g->carry_row) (void)hipFree(g->carry_row);
     if (g->carry_val) (void)hipFree(g->carry_val);
@@ -517,31 +528,58 @@ static int compress_host(const rbsc_graph* gr, std::vector<u64>& row_ptr,
 }
 
 // ---------------------------------------------------------------- refine
-static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
-                           rbsc_result* res, uint32_t* partition_out, uint32_t* colors_out) {
+static void launch_sig(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* dst, const u64* wt,
+                       u64 t_begin, u64 t_end, u64 line_base) {
+    if (t_end <= t_begin) return;
+    Workspace& w = ctx->ws;
+    const unsigned grid = grid_for((t_end - t_begin) * 64);
+    if (wt)
+        sig_kernel<true><<<grid, BLK, 0, ctx->stream>>>(g->row_ptr, g->row_node, g->U, dst, wt, g->E,
+                                                        w.hc, w.sig, g->tile_row, g->carry_val,
+                                                        t_begin, t_end, line_base);
+    else
+        sig_kernel<false><<<grid, BLK, 0, ctx->stream>>>(g->row_ptr, g->row_node, g->U, dst,
+                                                         nullptr, g->E, w.hc, w.sig, g->tile_row,
+                                                         g->carry_val, t_begin, t_end, line_base);
+}
+
+static int stream_reserve(rbsc_ctx* ctx, u64 lines, bool weighted) {
+    if (ctx->sbuf_lines >= lines && (!weighted || ctx->swbuf[0])) return RBSC_OK;
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->sbuf[b]) (void)hipFree(ctx->sbuf[b]);
+        if (ctx->swbuf[b]) (void)hipFree(ctx->swbuf[b]);
+        ctx->sbuf[b] = nullptr;
+        ctx->swbuf[b] = nullptr;
+    }
+    ctx->sbuf_lines = 0;
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipMalloc(&ctx->sbuf[b], sizeof(u32) * lines + 32));
+        if (weighted) HIP_TRY(hipMalloc(&ctx->swbuf[b], sizeof(u64) * lines));
+    }
+    ctx->sbuf_lines = lines;
+    return RBSC_OK;
+}
+
+int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOut* out) {
     hipStream_t s = ctx->stream;
     const u64 N = g->N;
-    rbsc_result r{};
-    r.passes = 1;
-    r.edges_per_round = g->E;
-    const u32 max_rounds = opts ? opts->max_rounds : 0;
-    if (N == 0) {
-        r.done = 1;
-        if (res) *res = r;
-        return RBSC_OK;
-    }
+    *out = RefineOut();
+    if (N == 0) return RBSC_OK;
     int rc = ws_reserve(ctx, N);
     if (rc) return rc;
     Workspace& w = ctx->ws;
-    auto t0 = std::chrono::steady_clock::now();
+    const u32* init = prm.d_init ? prm.d_init : g->init;
 
     // Round-0 state: P sorted by c0, hc = randomize(c0 + 0).
     prof_begin(ctx, 4);
-    init_c0_kernel<<<grid_for(N), BLK, 0, s>>>(g->init, w.keys[0], N);
-    if (!g->init && g->U)
+    init_c0_kernel<<<grid_for(N), BLK, 0, s>>>(init, w.keys[0], N);
+    if (!init && g->U)
         init_rows_kernel<<<grid_for(g->U), BLK, 0, s>>>(g->row_node, g->U, w.keys[0], N + 1);
     init_pack_kernel<<<grid_for(N), BLK, 0, s>>>(w.keys[0], w.P[0], w.hc, N);
-    fill_u64<<<grid_for(N), BLK, 0, s>>>(w.sig, N, 0);
+    if (prm.d_sig_fill)
+        HIP_TRY(hipMemcpyAsync(w.sig, prm.d_sig_fill, sizeof(u64) * N, hipMemcpyDeviceToDevice, s));
+    else
+        fill_u64<<<grid_for(N), BLK, 0, s>>>(w.sig, N, 0);
     int which = 0;
     {
         u64* kk[2] = {w.keys[0], w.keys[1]};
@@ -551,25 +589,78 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
     }
     int pc = which;  // P[pc] holds the current packed order
     prof_end(ctx);
+
+    // Streaming set-up: slices of whole wave tiles, double-buffered.
+    const bool streaming = g->dst == nullptr && g->E > 0;
+    u64 Bs = 0, n_slices = 0;
+    const bool weighted = streaming ? g->wt_host != nullptr : g->wt != nullptr;
+    if (streaming) {
+        if (!g->dst_host) {
+            rbsc_set_error("graph has no line array");
+            return RBSC_ERR_INVALID;
+        }
+        const u64 want = prm.slice_lines ? prm.slice_lines : g->E;
+        Bs = std::max<u64>(SIG_TW, (want / SIG_TW) * SIG_TW);
+        n_slices = (g->E + Bs - 1) / Bs;
+        if (n_slices <= 2) Bs = std::min<u64>(Bs, ((g->E + SIG_TW - 1) / SIG_TW) * SIG_TW);
+        rc = stream_reserve(ctx, Bs, weighted);
+        if (rc) return rc;
+        for (int b = 0; b < 2; ++b) HIP_TRY(hipEventRecord(ctx->ev_done[b], s));
+    }
+    bool prefetched[2] = {false, false};
+    auto issue_copy = [&](u64 sl) -> int {
+        const int b = (int)(sl & 1);
+        const u64 lo = sl * Bs, hi = std::min<u64>(g->E, lo + Bs);
+        HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->ev_done[b], 0));
+        HIP_TRY(hipMemcpyAsync(ctx->sbuf[b], g->dst_host + lo, sizeof(u32) * (hi - lo),
+                               hipMemcpyHostToDevice, ctx->copy_stream));
+        if (weighted)
+            HIP_TRY(hipMemcpyAsync(ctx->swbuf[b], g->wt_host + lo, sizeof(u64) * (hi - lo),
+                                   hipMemcpyHostToDevice, ctx->copy_stream));
+        HIP_TRY(hipEventRecord(ctx->ev_copy[b], ctx->copy_stream));
+        out->pcie_bytes += (double)(hi - lo) * (weighted ? 12.0 : 4.0);
+        return RBSC_OK;
+    };
+
     HIP_TRY(hipEventRecord(ctx->ev0, s));
-    u32 K = 1, Knew = 1;
+    u32 K = prm.k0, Knew = prm.k0;
     u32 it = 0;
     const u64 n_tiles = g->n_tiles;
     do {
         K = Knew;
         prof_begin(ctx, 0);
-        if (n_tiles) {
-            if (g->wt)
-                sig_kernel<true><<<grid_for(n_tiles * 64), BLK, 0, s>>>(
-                    g->row_ptr, g->row_node, g->U, g->dst, g->wt, g->E, w.hc, w.sig, g->tile_row,
-                    g->carry_val, n_tiles);
-            else
-                sig_kernel<false><<<grid_for(n_tiles * 64), BLK, 0, s>>>(
-                    g->row_ptr, g->row_node, g->U, g->dst, nullptr, g->E, w.hc, w.sig,
-                    g->tile_row, g->carry_val, n_tiles);
+        if (n_tiles && !streaming) {
+            launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0);
+        } else if (n_tiles) {
+            const bool resident = n_slices <= 2 && it > 0;  // both slices stay in the buffers
+            for (u64 sl = 0; sl < n_slices; ++sl) {
+                const int b = (int)(sl & 1);
+                if (!resident && sl < 2 && !prefetched[b]) {  // later slices were issued
+                    rc = issue_copy(sl);                         // two iterations earlier
+                    if (rc) return rc;
+                }
+                if (!resident) HIP_TRY(hipStreamWaitEvent(s, ctx->ev_copy[b], 0));
+                const u64 lo = sl * Bs, hi = std::min<u64>(g->E, lo + Bs);
+                launch_sig(ctx, g, ctx->sbuf[b], weighted ? ctx->swbuf[b] : nullptr, lo / SIG_TW,
+                           (hi + SIG_TW - 1) / SIG_TW, lo);
+                HIP_TRY(hipEventRecord(ctx->ev_done[b], s));
+                if (!resident && sl + 2 < n_slices) {
+                    rc = issue_copy(sl + 2);
+                    if (rc) return rc;
+                }
+            }
+            prefetched[0] = prefetched[1] = false;
+            if (n_slices > 2) {  // next round's first slices overlap this round's node phase
+                for (u64 sl = 0; sl < 2; ++sl) {
+                    rc = issue_copy(sl);
+                    if (rc) return rc;
+                    prefetched[sl] = true;
+                }
+            }
+        }
+        if (n_tiles)
             sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, g->carry_val,
                                                                g->row_node, w.sig, n_tiles);
-        }
         prof_end(ctx);
         prof_begin(ctx, 1);
         gather_keys_kernel<<<grid_for(N), BLK, 0, s>>>(w.P[pc], w.sig, w.keys[0], N);
@@ -581,7 +672,6 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
             rc = radix_sort_u64(&ctx->sort, kk, pp, N, 0, 64, &which, s);
             prof_end(ctx);
             if (rc) return rc;
-            // sorted data in kk[which], pp[which]; rank into the other P buffer
             u64* sorted_keys = kk[which];
             u64* sorted_P = pp[which];
             u64* out_P = pp[which ^ 1];
@@ -600,35 +690,64 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
         HIP_TRY(hipStreamSynchronize(s));
         Knew = ctx->h_scalar[0];
         ++it;
-        if (max_rounds && it >= max_rounds) break;
+        if (prm.max_rounds && it >= prm.max_rounds) break;
     } while (K != Knew);
     HIP_TRY(hipEventRecord(ctx->ev1, s));
+    if (streaming) HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
     rc = check_kernel_errors(ctx);
     if (rc) return rc;
     rc = prof_collect(ctx);
     if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    w.pc = pc;
+    out->K = Knew;
+    out->rounds = it;
+    out->device_ms = ms;
+    return RBSC_OK;
+}
 
-    if (partition_out || colors_out) {
-        u32* d_col = reinterpret_cast<u32*>(w.keys[0]);
-        colors_kernel<<<grid_for(N), BLK, 0, s>>>(w.P[pc], d_col, N);
+int download_colors(rbsc_ctx* ctx, u64 N, u32* host) {
+    if (N == 0) return RBSC_OK;
+    Workspace& w = ctx->ws;
+    u32* d_col = reinterpret_cast<u32*>(w.keys[0]);
+    colors_kernel<<<grid_for(N), BLK, 0, ctx->stream>>>(w.P[w.pc], d_col, N);
+    HIP_TRY(hipMemcpyAsync(host, d_col, sizeof(u32) * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RBSC_OK;
+}
+
+static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
+                           rbsc_result* res, uint32_t* partition_out, uint32_t* colors_out) {
+    const u64 N = g->N;
+    auto t0 = std::chrono::steady_clock::now();
+    RefineParams prm;
+    prm.max_rounds = opts ? opts->max_rounds : 0;
+    if (opts && opts->max_batch_edges && opts->max_batch_edges < g->E)
+        prm.slice_lines = opts->max_batch_edges;
+    RefineOut o;
+    int rc = refine_core(ctx, g, prm, &o);
+    if (rc) return rc;
+    if (N && (partition_out || colors_out)) {
         std::vector<u32> col;
         u32* hcol = colors_out;
         if (!hcol) {
             col.resize(N);
             hcol = col.data();
         }
-        HIP_TRY(hipMemcpyAsync(hcol, d_col, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        rc = download_colors(ctx, N, hcol);
+        if (rc) return rc;
         if (partition_out) canonicalize_host((u32)N, hcol, partition_out);
     }
-    HIP_TRY(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    r.nodes = Knew;
+    rbsc_result r{};
+    r.passes = 1;
+    r.edges_per_round = g->E;
+    r.nodes = N ? o.K : 0;
     r.done = 1;
-    r.rounds = it;
-    r.device_ms = ms;
+    r.rounds = o.rounds;
+    r.device_ms = o.device_ms;
     r.time_s = std::chrono::duration<double>(t1 - t0).count();
     r.verified = 0;
     if (res) *res = r;
@@ -663,6 +782,10 @@ extern "C" int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out) 
     HIP_TRY(hipHostMalloc(&c->h_scalar, sizeof(u32) * 16, hipHostMallocDefault));
     HIP_TRY(hipEventCreate(&c->ev0));
     HIP_TRY(hipEventCreate(&c->ev1));
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_copy[b], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_done[b], hipEventDisableTiming));
+    }
     *out = c;
     return RBSC_OK;
 }
@@ -684,6 +807,13 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
     if (c->d_scalar) (void)hipFree(c->d_scalar);
     if (c->h_scalar) (void)hipHostFree(c->h_scalar);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(c->copy_stream);
+    for (int b = 0; b < 2; ++b) {
+        if (c->sbuf[b]) (void)hipFree(c->sbuf[b]);
+        if (c->swbuf[b]) (void)hipFree(c->swbuf[b]);
+        if (c->ev_copy[b]) (void)hipEventDestroy(c->ev_copy[b]);
+        if (c->ev_done[b]) (void)hipEventDestroy(c->ev_done[b]);
+    }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -692,7 +822,7 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
     return RBSC_OK;
 }
 
-extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgraph** out) {
+int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out) {
     if (!ctx || !gr || !out) {
         rbsc_set_error("rbsc_graph_upload: null argument");
         return RBSC_ERR_INVALID;
@@ -712,16 +842,17 @@ extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgrap
     int rc = compress_host(gr, row_ptr, row_node);
     if (rc) return rc;
     const u32 U = (u32)row_node.size();
-    const u64 need = sizeof(u64) * (U + 1) + 4ull * U + 4ull * gr->n_edges +
-                     (gr->weight ? 8ull * gr->n_edges : 0) + 48ull * gr->n_nodes;
+    const u64 E = gr->n_edges;
+    const u64 line_bytes = lines_on_host ? 0 : 4ull * E + (gr->weight ? 8ull * E : 0);
+    const u64 need = sizeof(u64) * (U + 1) + 4ull * U + line_bytes + 48ull * gr->n_nodes;
     if (ctx->budget && need > ctx->budget) {
         rbsc_set_error("graph needs %llu bytes, budget is %llu", (unsigned long long)need,
                        (unsigned long long)ctx->budget);
         return RBSC_ERR_BUDGET;
     }
     rbsc_dgraph* g = new rbsc_dgraph();
-    rc = dgraph_alloc(g, gr->n_nodes, gr->n_edges, U, gr->weight != nullptr,
-                      gr->init_color != nullptr);
+    rc = dgraph_alloc(g, gr->n_nodes, E, U, gr->weight != nullptr, gr->init_color != nullptr,
+                      !(lines_on_host && E));
     if (rc) {
         dgraph_release(g);
         delete g;
@@ -730,10 +861,13 @@ extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgrap
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(g->row_ptr, row_ptr.data(), sizeof(u64) * (U + 1), hipMemcpyHostToDevice, s));
     if (U) HIP_TRY(hipMemcpyAsync(g->row_node, row_node.data(), sizeof(u32) * U, hipMemcpyHostToDevice, s));
-    if (gr->n_edges) {
-        HIP_TRY(hipMemcpyAsync(g->dst, gr->dst, sizeof(u32) * gr->n_edges, hipMemcpyHostToDevice, s));
+    if (E && g->dst) {
+        HIP_TRY(hipMemcpyAsync(g->dst, gr->dst, sizeof(u32) * E, hipMemcpyHostToDevice, s));
         if (gr->weight)
-            HIP_TRY(hipMemcpyAsync(g->wt, gr->weight, sizeof(u64) * gr->n_edges, hipMemcpyHostToDevice, s));
+            HIP_TRY(hipMemcpyAsync(g->wt, gr->weight, sizeof(u64) * E, hipMemcpyHostToDevice, s));
+    } else if (E) {
+        memcpy(g->dst_host, gr->dst, sizeof(u32) * E);
+        if (gr->weight) memcpy(g->wt_host, gr->weight, sizeof(u64) * E);
     }
     if (gr->init_color && gr->n_nodes)
         HIP_TRY(hipMemcpyAsync(g->init, gr->init_color, sizeof(u32) * gr->n_nodes, hipMemcpyHostToDevice, s));
@@ -741,6 +875,33 @@ extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgrap
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     *out = g;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgraph** out) {
+    return graph_upload_host(ctx, gr, false, out);
+}
+
+extern "C" int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g) {
+    if (!ctx || !g) {
+        rbsc_set_error("rbsc_graph_lines_to_host: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    if (!g->dst) return RBSC_OK;  // already on the host (or no lines)
+    HIP_TRY(hipSetDevice(ctx->device));
+    const u64 E = g->E;
+    HIP_TRY(hipHostMalloc(&g->dst_host, sizeof(u32) * std::max<u64>(E, 1), hipHostMallocDefault));
+    HIP_TRY(hipMemcpy(g->dst_host, g->dst, sizeof(u32) * E, hipMemcpyDeviceToHost));
+    if (g->wt) {
+        HIP_TRY(hipHostMalloc(&g->wt_host, sizeof(u64) * std::max<u64>(E, 1), hipHostMallocDefault));
+        HIP_TRY(hipMemcpy(g->wt_host, g->wt, sizeof(u64) * E, hipMemcpyDeviceToHost));
+        HIP_TRY(hipFree(g->wt));
+        g->wt = nullptr;
+        g->bytes -= sizeof(u64) * E;
+    }
+    HIP_TRY(hipFree(g->dst));
+    g->dst = nullptr;
+    g->bytes -= sizeof(u32) * E;
     return RBSC_OK;
 }
 
@@ -788,7 +949,10 @@ extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t
                 return RBSC_ERR_INVALID;
             }
     }
-    if (dst && g->E) HIP_TRY(hipMemcpy(dst, g->dst, sizeof(u32) * g->E, hipMemcpyDeviceToHost));
+    if (dst && g->E) {
+        if (g->dst) HIP_TRY(hipMemcpy(dst, g->dst, sizeof(u32) * g->E, hipMemcpyDeviceToHost));
+        else memcpy(dst, g->dst_host, sizeof(u32) * g->E);
+    }
     return RBSC_OK;
 }
 
@@ -809,12 +973,23 @@ extern "C" int rbsc_refine(rbsc_ctx* ctx, const rbsc_graph* graph, const rbsc_op
         rbsc_set_error("rbsc_refine: null argument");
         return RBSC_ERR_INVALID;
     }
+    const u64 B = opts ? opts->max_batch_edges : 0;
+    const bool batched = B && B < graph->n_edges;
+    const u32 mode = opts ? opts->mode : RBSC_MODE_STREAM;
+    if (mode != RBSC_MODE_STREAM && mode != RBSC_MODE_QUOTIENT &&
+        mode != RBSC_MODE_QUOTIENT_AGGRESSIVE) {
+        rbsc_set_error("rbsc_refine: unknown mode %u", mode);
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (batched && mode != RBSC_MODE_STREAM)
+        return quotient_refine(ctx, graph, opts, result, partition_out);
     auto t0 = std::chrono::steady_clock::now();
     rbsc_dgraph* g = nullptr;
-    int rc = rbsc_graph_upload(ctx, graph, &g);
+    int rc = graph_upload_host(ctx, graph, batched, &g);
     if (rc) return rc;
     rbsc_result r{};
-    rc = rbsc_graph_refine(ctx, g, opts, &r, partition_out, nullptr);
+    rc = refine_resident(ctx, g, opts, &r, partition_out, nullptr);
     rbsc_graph_free(ctx, g);
     if (rc) return rc;
     auto t1 = std::chrono::steady_clock::now();

@@ -182,3 +182,54 @@ def test_cli_end_to_end(tmp_path):
     assert int(lines[1]) == 1 and int(lines[2]) == K
     got = np.loadtxt(part, dtype=np.uint32)
     assert np.array_equal(got, oracle.canonical(c))
+
+
+@pytest.mark.parametrize("frac", [0.5, 0.25, 0.1])
+def test_streaming_equals_resident(ctx, frac):
+    """Edge-batch streaming (lines in pinned host memory, slices of
+    max_batch_edges lines) computes the exact full-graph rounds."""
+    g = _weighted(gen.rmat(14, 1 << 18, clone_frac=0.2, seed=15), 3) if frac == 0.1 else \
+        gen.rmat(14, 1 << 18, clone_frac=0.2, seed=15)
+    c, K, R, tr = _oracle_full(g, trace_rounds=64)
+    B = int(g.n_edges * frac)
+    r = ctx.refine(g, max_batch_edges=B, mode="stream")
+    assert (r.nodes, r.rounds, r.done) == (K, R, 1)
+    assert np.array_equal(r.partition, oracle.canonical(c))
+    # the same through a resident graph whose lines were moved to the host
+    dg = ctx.upload(g)
+    dg.lines_to_host()
+    for rr in range(1, R + 1):
+        res = dg.refine(max_rounds=rr, partition=False, colors=True, max_batch_edges=B)
+        assert np.array_equal(res.colors, tr[rr - 1])
+    dg.free()
+
+
+@pytest.mark.parametrize("frac,aggr", [(0.5, False), (0.25, False), (0.25, True), (0.1, False)])
+def test_quotient_mode_matches_oracle(ctx, frac, aggr):
+    """Reference multi-batch quotient passes (rbsc.cu:274-487): nodes, done and
+    passes equal the oracle's restatement."""
+    g = gen.rmat(12, 1 << 16, clone_frac=0.3, seed=17)
+    src = g.sources()
+    B = int(g.n_edges * frac)
+    want = oracle.rbsc_multibatch(g.n_nodes, src, g.dst, None, B, aggressive=aggr)
+    r = ctx.refine(g, max_batch_edges=B, mode="aggressive" if aggr else "quotient")
+    assert (r.nodes, r.done, r.passes) == want
+    assert len(np.unique(r.partition)) == r.nodes
+
+
+def test_cli_batch_modes(tmp_path):
+    g = gen.rmat(10, 1 << 13, clone_frac=0.3, seed=6)
+    src = g.sources()
+    c, K, R, _ = _oracle_full(g)
+    B = g.n_edges // 4
+    p = subprocess.run([CLI_PATH, str(B), "1"], input=g.to_text(), stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, timeout=120)
+    assert p.returncode == 0, p.stderr.decode()
+    lines = p.stdout.decode().split("\n")
+    assert int(lines[1]) == 1 and int(lines[2]) == K
+    want = oracle.rbsc_multibatch(g.n_nodes, src, g.dst, None, B)
+    p = subprocess.run([CLI_PATH, str(B), "1", "--mode", "quotient"], input=g.to_text(),
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=120)
+    assert p.returncode == 0, p.stderr.decode()
+    lines = p.stdout.decode().split("\n")
+    assert (int(lines[2]), int(lines[1])) == want[:2]
