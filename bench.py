@@ -52,6 +52,22 @@ def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
                       f"{R} rounds, {K} blocks, {dt:.2f} s"}
 
 
+def copy_probe(nbytes: int) -> None:
+    """One plain device-to-device copy of nbytes through the HIP runtime, so a
+    counter run can read FETCH_SIZE / WRITE_SIZE of a kernel of known size."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy probe: {nbytes} bytes device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +76,14 @@ def main():
     ap.add_argument("--config", default="rmat_s27_d31_c20", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-scale", type=int, default=22)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--parts", type=int, default=1,
+                    help="destination partitions of the line layout (power of two)")
+    ap.add_argument("--hashed64", action="store_true",
+                    help="gather per-node hashed colours instead of raw u32 colours")
+    ap.add_argument("--xcd", action="store_true",
+                    help="with --parts 8: one launch, partition b mod 8 per workgroup b")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the RBSC round runs on a single GPU in this "
@@ -67,15 +91,22 @@ def main():
         sys.exit(2)
 
     import rbsc_amd
+    if args.calib_copy_mib:
+        copy_probe(args.calib_copy_mib << 20)
     scale, ldraws, clone, seed = CONFIGS[args.config]
     ctx = rbsc_amd.Context(0)
     t0 = time.perf_counter()
     dg = ctx.gen_rmat(scale, 1 << ldraws, 0.57, 0.19, 0.19, clone, seed)
     N, E = dg.dims
     log(f"[bench] generated {args.config}: N={N} E={E} in {time.perf_counter() - t0:.1f}s")
+    if args.parts > 1:
+        t0 = time.perf_counter()
+        dg.relayout(args.parts)
+        log(f"[bench] relayout into {args.parts} destination partitions in {time.perf_counter() - t0:.1f}s")
 
+    flags = (rbsc_amd.FLAG_HASHED64 if args.hashed64 else 0) | (rbsc_amd.FLAG_XCD if args.xcd else 0)
     for i in range(args.warmup):
-        r = dg.refine(partition=False)
+        r = dg.refine(partition=False, flags=flags)
         log(f"[bench] warmup {i}: rounds={r.rounds} blocks={r.nodes} device_ms={r.device_ms:.2f}")
 
     ctx.profile(True)
@@ -84,7 +115,7 @@ def main():
     dev_ms = 0.0
     t = time.perf_counter()
     for i in range(args.steps):
-        r = dg.refine(partition=False)
+        r = dg.refine(partition=False, flags=flags)
         rounds += r.rounds
         dev_ms += r.device_ms
     wall = time.perf_counter() - t
@@ -113,7 +144,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "rounds_per_step": rounds // args.steps,
+        "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "parts": args.parts, "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
+                   "rounds_per_step": rounds // args.steps,
                    "blocks": r.nodes, "ms_per_round": ms_round},
         "roofline": {"bound": "hbm", "kernel": "refinement round (all kernels of one round)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

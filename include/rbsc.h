@@ -67,8 +67,28 @@ typedef struct rbsc_opts {
     uint32_t mode;        /* RBSC_MODE_* */
     uint32_t max_rounds;  /* 0 = until the fixpoint; else stop after this many */
     uint32_t verify;      /* 1 = exact stability check of the final partition */
-    uint32_t reserved;
+    uint32_t flags;       /* RBSC_FLAG_* implementation choices (results unchanged) */
 } rbsc_opts;
+
+/* opts.flags: gather per-node hashed colours randomize(colour + r) (8 bytes)
+ * instead of the default raw 32-bit colours hashed on every line
+ * (rbsc.cu:85): fewer multiplies, twice the gathered bytes. */
+#define RBSC_FLAG_HASHED64 1u
+/* opts.flags: on a graph relaid out into 8 destination partitions, run all
+ * partitions in one launch with workgroup b serving partition b mod 8, so
+ * each of the 8 XCDs' L2 caches its own eighth of the colours. */
+#define RBSC_FLAG_XCD 2u
+/* opts.flags: sort the top 32 signature bits (4 passes instead of 8), then
+ * order each run of equal top halves by the low halves; a run longer than
+ * 4096 entries makes the round redo a full 64-bit sort.  Measured slower
+ * than the full sort on R-MAT (large classes of equal signatures make long
+ * runs), so it is off by default. */
+#define RBSC_FLAG_SPLIT 4u
+/* Verification hooks for RBSC_FLAG_SPLIT (results unchanged, slower): split
+ * at 16 bits instead, so runs are common; cap runs at 2 entries, so the
+ * full-sort fallback runs. */
+#define RBSC_FLAG_SPLIT16 8u
+#define RBSC_FLAG_TIESMALL 16u
 
 /* Result.  nodes / done / time_s are the reference's three output lines
  * (rbsc.cu:492-494). */
@@ -124,6 +144,14 @@ int rbsc_graph_refine(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
  * reference's per-batch host->device copies (rbsc.cu:341-344) for graphs
  * larger than HBM. */
 int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g);
+
+/* Regroup a resident graph's lines by destination (head id mod n_parts,
+ * n_parts a power of two <= 16; 1 keeps the plain layout).  Each round then
+ * runs the signature kernel once per partition, so the hashed colours one
+ * launch gathers are 1/n_parts of all of them and stay in L2.  Results are
+ * unchanged.  No counterpart in the reference (its set_values gathers over
+ * the whole colour array, rbsc.cu:80-87). */
+int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts);
 
 /* Per-phase device timing with hipEvents on the context's stream (off by
  * default).  Phases: 0 signature kernel + row fix-up, 1 key gather,

@@ -23,11 +23,11 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM, RbscError,
-                   check, load)
+from ._lib import (FLAG_HASHED64, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
+                   RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "RefineResult", "RbscError", "load",
-           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "gen"]
+           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "gen"]
 
 _MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
           "aggressive": MODE_QUOTIENT_AGGRESSIVE}
@@ -132,8 +132,9 @@ def _result(r: _lib.Result, partition=None, colors=None) -> RefineResult:
                         float(r.device_ms), int(r.verified), int(r.passes), partition, colors)
 
 
-def _opts(max_batch_edges=0, mode="stream", max_rounds=0, verify=False) -> _lib.Opts:
+def _opts(max_batch_edges=0, mode="stream", max_rounds=0, verify=False, flags=0) -> _lib.Opts:
     o = _lib.Opts()
+    o.flags = int(flags)
     o.max_batch_edges = int(max_batch_edges)
     o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
     o.max_rounds = int(max_rounds)
@@ -160,15 +161,20 @@ class DeviceGraph:
         return int(load().rbsc_graph_bytes(self.handle))
 
     def refine(self, max_rounds=0, verify=False, partition=True, colors=False, mode="stream",
-               max_batch_edges=0) -> RefineResult:
+               max_batch_edges=0, flags=0) -> RefineResult:
         n, _ = self.dims
         part = np.empty(n, np.uint32) if partition else None
         col = np.empty(n, np.uint32) if colors else None
         r = _lib.Result()
-        o = _opts(max_batch_edges, mode, max_rounds, verify)
+        o = _opts(max_batch_edges, mode, max_rounds, verify, flags)
         check(load().rbsc_graph_refine(self.ctx.handle, self.handle, C.byref(o), C.byref(r),
                                        _ptr(part, C.c_uint32), _ptr(col, C.c_uint32)))
         return _result(r, part, col)
+
+    def relayout(self, n_parts: int):
+        """Group lines by destination (head mod n_parts) for cache-resident
+        gathers; results are unchanged."""
+        check(load().rbsc_graph_relayout(self.ctx.handle, self.handle, n_parts))
 
     def lines_to_host(self):
         """Keep the line arrays in pinned host memory; refinements then stream
@@ -234,9 +240,9 @@ class Context:
         return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(self.PHASES)}
 
     def refine(self, g: HostGraph, max_batch_edges=0, mode="stream", max_rounds=0,
-               verify=False, partition=True) -> RefineResult:
+               verify=False, partition=True, flags=0) -> RefineResult:
         st = g.as_struct()
-        o = _opts(max_batch_edges, mode, max_rounds, verify)
+        o = _opts(max_batch_edges, mode, max_rounds, verify, flags)
         part = np.empty(g.n_nodes, np.uint32) if partition else None
         r = _lib.Result()
         check(load().rbsc_refine(self.handle, C.byref(st), C.byref(o), C.byref(r),

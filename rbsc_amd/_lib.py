@@ -24,6 +24,11 @@ ERRORS = {
     -7: "RBSC_ERR_NODEVICE",
 }
 MODE_STREAM, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE = 0, 1, 2
+FLAG_HASHED64 = 1
+FLAG_XCD = 2
+FLAG_SPLIT = 4
+FLAG_SPLIT16 = 8
+FLAG_TIESMALL = 16
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -47,7 +52,7 @@ class Opts(C.Structure):
         ("mode", C.c_uint32),
         ("max_rounds", C.c_uint32),
         ("verify", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -86,6 +91,7 @@ SIGNATURES = {
     "rbsc_graph_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_graph_bytes": (C.c_uint64, [C.c_void_p]),
     "rbsc_graph_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result), u32p, u32p]),
+    "rbsc_graph_relayout": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "rbsc_graph_lines_to_host": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_ctx_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), u64p, C.c_int]),

@@ -76,6 +76,7 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
         *d_tot = nullptr;
     int rc = RBSC_OK;
     rbsc_dgraph* g = nullptr;
+    RawGraph raw;
     u64 h_tot[2] = {0, 0};
     auto cleanup = [&]() {
         for (int i = 0; i < 2; ++i)
@@ -93,7 +94,7 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
         if (_e != hipSuccess) {                                                         \
             rbsc_set_error("%s failed: %s", #expr, hipGetErrorString(_e));              \
             cleanup();                                                                  \
-            if (g) { dgraph_release(g); delete g; }                                     \
+            raw_release(&raw);                                                          \
             return _e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;           \
         }                                                                               \
     } while (0)
@@ -123,19 +124,28 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
     const u64 E = h_tot[0];
     const u64 U = h_tot[1];
     set_u64<<<1, 1, 0, s>>>(ro + N, E);
-    g = new rbsc_dgraph();
-    rc = dgraph_alloc(g, (u32)N, E, (u32)U, false, false);
-    if (rc) { cleanup(); dgraph_release(g); delete g; return rc; }
-    build_rows_kernel<<<grid_for(N), BLK, 0, s>>>(ro, flag, deg, N, g->row_ptr, g->row_node);
-    set_u64<<<1, 1, 0, s>>>(g->row_ptr + U, E);
-    copy_lines_kernel<<<grid_for(N), BLK, 0, s>>>(spec, rp, ro, sk, N, g->dst);
+    raw.N = N;
+    raw.E = E;
+    raw.U = U;
+    GEN_TRY(hipMalloc(&raw.row_ptr, sizeof(u64) * (U + 1)));
+    GEN_TRY(hipMalloc(&raw.row_node, sizeof(u32) * std::max<u64>(U, 1)));
+    GEN_TRY(hipMalloc(&raw.dst, sizeof(u32) * std::max<u64>(E, 1)));
+    build_rows_kernel<<<grid_for(N), BLK, 0, s>>>(ro, flag, deg, N, raw.row_ptr, raw.row_node);
+    set_u64<<<1, 1, 0, s>>>(raw.row_ptr + U, E);
+    copy_lines_kernel<<<grid_for(N), BLK, 0, s>>>(spec, rp, ro, sk, N, raw.dst);
     GEN_TRY(hipGetLastError());
-    rc = dgraph_build_tiles(ctx, g);
-    if (rc) { cleanup(); dgraph_release(g); delete g; return rc; }
     GEN_TRY(hipStreamSynchronize(s));
-    rc = check_kernel_errors(ctx);
-    cleanup();
-    if (rc) { dgraph_release(g); delete g; return rc; }
+    cleanup();  // the sorted draws are no longer needed
+    for (int i = 0; i < 2; ++i) keys[i] = nullptr;
+    rp = deg = flag = ro = tmp = d_tot = nullptr;
+    g = new rbsc_dgraph();
+    rc = dgraph_build(ctx, &raw, g);
+    raw_release(&raw);
+    if (rc) {
+        dgraph_release(g);
+        delete g;
+        return rc;
+    }
     *out = g;
     return RBSC_OK;
 #undef GEN_TRY

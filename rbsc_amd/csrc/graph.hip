@@ -1,0 +1,432 @@
+// graph.hip -- building the resident graph layout in HBM.
+//
+// A graph arrives "raw": rows with >= 1 line in input order (row_node_raw[k]
+// is the node of row k, lines row_ptr_raw[k] .. row_ptr_raw[k+1]) and line
+// heads as original node ids.  dgraph_build turns it into the layout the
+// round kernels read:
+//
+//   * internal node ids: nodes with lines first, by in-degree descending
+//     (ties by original id), then the nodes without lines in original order.
+//     Colours depend only on (signature, colour) pairs, never on node ids, so
+//     every round's colour of every node is unchanged by the renumbering; it
+//     makes the hashed colours of the most-gathered heads a compact prefix
+//     that stays in L2 / Infinity Cache, and lets the node-side kernels skip
+//     nodes without lines entirely.
+//   * rows in internal order (row j = node j), so signature writes are
+//     coalesced and no row -> node indirection is read per round.
+//   * a row-end bitmap (bit p = line p is the last of its row): the signature
+//     kernel finds row boundaries from one byte per 8 lines instead of
+//     searching row offsets.
+//   * pi[orig] = internal id, to map colours back.
+#include "internal.h"
+#include "scan.h"
+#include <algorithm>
+
+namespace {
+constexpr int BLK = 256;
+inline unsigned grid_for(u64 n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+__global__ void zero_u32(u32* p, u64 n) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void fill_ones_kernel(u64* p, u64 n) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 1;
+}
+
+__global__ void indeg_kernel(const u32* __restrict__ dst, u64 E, u32* __restrict__ indeg) {
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) atomicAdd(&indeg[dst[e]], 1u);
+}
+
+// keys: (~in-degree, original id) so an ascending sort gives in-degree
+// descending; vals: the raw row index.  Also marks nodes that have rows.
+__global__ void order_keys_kernel(const u32* __restrict__ row_node, u64 U,
+                                  const u32* __restrict__ indeg, u64* __restrict__ keys,
+                                  u64* __restrict__ vals, u64* __restrict__ dangling) {
+    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= U) return;
+    const u32 v = row_node[k];
+    keys[k] = ((u64)(0xFFFFFFFFu - indeg[v]) << 32) | v;
+    vals[k] = k;
+    dangling[v] = 0;
+}
+
+__global__ void place_rows_kernel(const u64* __restrict__ keys, const u64* __restrict__ vals,
+                                  u64 U, const u64* __restrict__ row_ptr_raw,
+                                  u32* __restrict__ pi, u64* __restrict__ deg_new,
+                                  u32* __restrict__ src_row) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= U) return;
+    const u32 v = (u32)keys[j];
+    const u64 k = vals[j];
+    pi[v] = (u32)j;
+    deg_new[j] = row_ptr_raw[k + 1] - row_ptr_raw[k];
+    src_row[j] = (u32)k;
+}
+
+__global__ void place_dangling_kernel(const u64* __restrict__ dangling,
+                                      const u64* __restrict__ dscan, u64 N, u64 U,
+                                      u32* __restrict__ pi) {
+    const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N && dangling[v]) pi[v] = (u32)(U + dscan[v]);
+}
+
+// One thread per new row: copy its lines with heads renumbered.
+template <bool HAS_WT>
+__global__ void copy_rows_kernel(const u64* __restrict__ row_ptr_new, const u32* __restrict__ src_row,
+                                 const u64* __restrict__ row_ptr_raw, const u32* __restrict__ dst_raw,
+                                 const u64* __restrict__ wt_raw, const u32* __restrict__ pi, u64 U,
+                                 u32* __restrict__ dst_new, u64* __restrict__ wt_new,
+                                 u32* __restrict__ end_bits) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= U) return;
+    const u64 k = src_row[j];
+    const u64 a = row_ptr_raw[k], b = row_ptr_raw[k + 1];
+    u64 o = row_ptr_new[j];
+    for (u64 p = a; p < b; ++p, ++o) {
+        dst_new[o] = pi[dst_raw[p]];
+        if (HAS_WT) wt_new[o] = wt_raw[p];
+    }
+    const u64 last = row_ptr_new[j + 1] - 1;
+    atomicOr(&end_bits[last >> 5], 1u << (last & 31));
+}
+
+__global__ void permute_init_kernel(const u32* __restrict__ init_raw, const u32* __restrict__ pi,
+                                    u64 N, u32* __restrict__ init_new) {
+    const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N) init_new[pi[v]] = init_raw[v];
+}
+
+__global__ void set_last_kernel(u64* p, u64 idx, u64 v) { p[idx] = v; }
+
+// Tile metadata over internal rows.
+__global__ void tile_setup_kernel(const u64* __restrict__ row_ptr, u32 U, u64 E, u64 n_tiles,
+                                  u32* __restrict__ tile_row, u32* __restrict__ carry_row) {
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n_tiles) return;
+    if (t == n_tiles) {
+        tile_row[t] = U - 1;
+        return;
+    }
+    const u64 e0 = t * SIG_TW;
+    u32 lo = 0, hi = U - 1;
+    while (lo < hi) {
+        const u32 mid = lo + (hi - lo + 1) / 2;
+        if (row_ptr[mid] <= e0) lo = mid; else hi = mid - 1;
+    }
+    tile_row[t] = lo;
+    const u64 elast = (e0 + SIG_TW < E ? e0 + SIG_TW : E) - 1;
+    hi = U - 1;
+    while (lo < hi) {
+        const u32 mid = lo + (hi - lo + 1) / 2;
+        if (row_ptr[mid] <= elast) lo = mid; else hi = mid - 1;
+    }
+    carry_row[t] = row_ptr[lo + 1] > elast + 1 ? lo : NO_ROW;
+}
+}  // namespace
+
+void raw_release(RawGraph* r) {
+    if (r->row_ptr) (void)hipFree(r->row_ptr);
+    if (r->row_node) (void)hipFree(r->row_node);
+    if (r->dst) (void)hipFree(r->dst);
+    if (r->wt) (void)hipFree(r->wt);
+    if (r->init) (void)hipFree(r->init);
+    *r = RawGraph();
+}
+
+int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
+    hipStream_t s = ctx->stream;
+    const u64 N = raw->N, E = raw->E, U = raw->U;
+    *g = rbsc_dgraph();
+    g->N = (u32)N;
+    g->E = E;
+    g->U = (u32)U;
+    g->n_tiles = (E + SIG_TW - 1) / SIG_TW;
+    u64 *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *dang = nullptr,
+        *deg = nullptr, *tmp = nullptr;
+    u32 *indeg = nullptr, *src_row = nullptr;
+    int rc = RBSC_OK;
+#define B_TRY(expr)                                                                  \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            rbsc_set_error("%s failed: %s", #expr, hipGetErrorString(_e));           \
+            rc = _e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;          \
+            goto done;                                                               \
+        }                                                                            \
+    } while (0)
+    B_TRY(hipMalloc(&g->row_ptr, sizeof(u64) * (U + 1)));
+    B_TRY(hipMalloc(&g->pi, sizeof(u32) * std::max<u64>(N, 1)));
+    B_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
+    if (raw->wt) B_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
+    if (raw->init) B_TRY(hipMalloc(&g->init, sizeof(u32) * std::max<u64>(N, 1)));
+    g->end_words = (E + 31) / 32 + 64;  // padding covers whole-tile byte reads
+    B_TRY(hipMalloc(&g->end_bits, sizeof(u32) * g->end_words));
+    B_TRY(hipMalloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1)));
+    B_TRY(hipMalloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1)));
+    B_TRY(hipMalloc(&g->carry_val, sizeof(u64) * std::max<u64>(g->n_tiles, 1)));
+    g->bytes = sizeof(u64) * (U + 1) + 4 * N + 4 * E + (raw->wt ? 8 * E : 0) +
+               (raw->init ? 4 * N : 0) + 4 * g->end_words + 16 * (g->n_tiles + 1);
+
+    B_TRY(hipMalloc(&indeg, sizeof(u32) * std::max<u64>(N, 1)));
+    B_TRY(hipMalloc(&dang, sizeof(u64) * (2 * N + 1)));
+    B_TRY(hipMalloc(&deg, sizeof(u64) * (U + 1)));
+    B_TRY(hipMalloc(&src_row, sizeof(u32) * std::max<u64>(U, 1)));
+    B_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(std::max(N, U) + 1)));
+    for (int i = 0; i < 2; ++i) {
+        B_TRY(hipMalloc(&keys[i], sizeof(u64) * std::max<u64>(U, 1)));
+        B_TRY(hipMalloc(&vals[i], sizeof(u64) * std::max<u64>(U, 1)));
+    }
+    zero_u32<<<grid_for(N), BLK, 0, s>>>(indeg, N);
+    if (E) indeg_kernel<<<grid_for(E), BLK, 0, s>>>(raw->dst, E, indeg);
+    fill_ones_kernel<<<grid_for(N), BLK, 0, s>>>(dang, N);
+    if (U) {
+        order_keys_kernel<<<grid_for(U), BLK, 0, s>>>(raw->row_node, U, indeg, keys[0], vals[0], dang);
+        rc = sort_ws_init(&ctx->sort, U, ctx->d_err, s);
+        if (rc) goto done;
+        int which = 0;
+        rc = radix_sort_u64(&ctx->sort, keys, vals, U, 0, 64, &which, s);
+        if (rc) goto done;
+        place_rows_kernel<<<grid_for(U), BLK, 0, s>>>(keys[which], vals[which], U, raw->row_ptr,
+                                                       g->pi, deg, src_row);
+    }
+    rc = scan_excl_u64(dang, dang + N, N, nullptr, tmp, s);  // scan into the upper half
+    if (rc) goto done;
+    place_dangling_kernel<<<grid_for(N), BLK, 0, s>>>(dang, dang + N, N, U, g->pi);
+    rc = scan_excl_u64(deg, g->row_ptr, U, nullptr, tmp, s);
+    if (rc) goto done;
+    set_last_kernel<<<1, 1, 0, s>>>(g->row_ptr, U, E);
+    zero_u32<<<grid_for(g->end_words), BLK, 0, s>>>(g->end_bits, g->end_words);
+    if (U) {
+        if (raw->wt)
+            copy_rows_kernel<true><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
+                                                               raw->dst, raw->wt, g->pi, U, g->dst,
+                                                               g->wt, g->end_bits);
+        else
+            copy_rows_kernel<false><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
+                                                                raw->dst, nullptr, g->pi, U, g->dst,
+                                                                nullptr, g->end_bits);
+    }
+    if (raw->init) permute_init_kernel<<<grid_for(N), BLK, 0, s>>>(raw->init, g->pi, N, g->init);
+    if (g->n_tiles)
+        tile_setup_kernel<<<grid_for(g->n_tiles + 1), BLK, 0, s>>>(g->row_ptr, (u32)U, E, g->n_tiles,
+                                                                   g->tile_row, g->carry_row);
+    B_TRY(hipGetLastError());
+    B_TRY(hipStreamSynchronize(s));
+    rc = check_kernel_errors(ctx);
+done:
+    for (int i = 0; i < 2; ++i) {
+        if (keys[i]) (void)hipFree(keys[i]);
+        if (vals[i]) (void)hipFree(vals[i]);
+    }
+    if (dang) (void)hipFree(dang);
+    if (deg) (void)hipFree(deg);
+    if (tmp) (void)hipFree(tmp);
+    if (indeg) (void)hipFree(indeg);
+    if (src_row) (void)hipFree(src_row);
+    return rc;
+#undef B_TRY
+}
+
+// ------------------------------------------------------------ partitioning
+namespace {
+// counts[p * U + j] = lines of row j whose head is in partition p.
+__global__ void part_count_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
+                                  u64 U, u32 P, u64* __restrict__ counts) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= U) return;
+    u64 c[16];
+    for (u32 p = 0; p < P; ++p) c[p] = 0;
+    for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) c[dst[e] & (P - 1)]++;
+    for (u32 p = 0; p < P; ++p) counts[(u64)p * U + j] = c[p];
+}
+__global__ void flag_nonzero_kernel(const u64* __restrict__ a, u64 n, u64* __restrict__ f) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = a[i] != 0;
+}
+// Scatter each row's lines into its partition runs; write segment starts,
+// segment rows and row-end bits.
+template <bool HAS_WT>
+__global__ void part_fill_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
+                                 const u64* __restrict__ wt, u64 U, u32 P,
+                                 const u64* __restrict__ line_off, const u64* __restrict__ seg_idx,
+                                 const u64* __restrict__ counts, const u64* __restrict__ adj_line,
+                                 const u64* __restrict__ adj_seg, u32* __restrict__ ndst,
+                                 u64* __restrict__ nwt, u64* __restrict__ seg_ptr,
+                                 u32* __restrict__ seg_row, u32* __restrict__ end_bits) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= U) return;
+    u64 pos[16];
+    for (u32 p = 0; p < P; ++p) {
+        const u64 i = (u64)p * U + j;
+        pos[p] = line_off[i] + adj_line[p];
+        if (counts[i]) {
+            const u64 k = seg_idx[i] + adj_seg[p];
+            seg_ptr[k] = pos[p];
+            seg_row[k] = (u32)j;
+            const u64 last = pos[p] + counts[i] - 1;
+            atomicOr(&end_bits[last >> 5], 1u << (last & 31));
+        }
+    }
+    for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) {
+        const u32 d = dst[e];
+        const u32 p = d & (P - 1);
+        ndst[pos[p]] = d;
+        if (HAS_WT) nwt[pos[p]] = wt[e];
+        pos[p]++;
+    }
+}
+__global__ void pad_fill_kernel(u32* __restrict__ ndst, u64 a, u64 b) {
+    const u64 i = a + (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < b) ndst[i] = PAD_DST;
+}
+__global__ void pad_seg_kernel(u64* __restrict__ seg_ptr, u32* __restrict__ seg_row, u64 k,
+                               u64 start, u32 slot, u32* __restrict__ end_bits, u64 last) {
+    seg_ptr[k] = start;
+    seg_row[k] = slot;
+    atomicOr(&end_bits[last >> 5], 1u << (last & 31));
+}
+}  // namespace
+
+int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
+    if (P == 1 && g->n_parts == 1) return RBSC_OK;
+    if (g->n_parts != 1 || P == 0 || P > 16 || (P & (P - 1))) {
+        rbsc_set_error("relayout: partitions must be a power of two <= 16, from the plain layout");
+        return RBSC_ERR_INVALID;
+    }
+    if (!g->dst) {
+        rbsc_set_error("relayout: lines are streamed from the host; partitioning needs them resident");
+        return RBSC_ERR_INVALID;
+    }
+    hipStream_t s = ctx->stream;
+    const u64 U = g->U, E = g->E, N = g->N;
+    if (U == 0 || E == 0) return RBSC_OK;
+    const u64 PU = (u64)P * U;
+    u64 *counts = nullptr, *line_off = nullptr, *flags = nullptr, *tmp = nullptr, *tot = nullptr,
+        *adj = nullptr, *seg_ptr = nullptr, *nwt = nullptr;
+    u32 *ndst = nullptr, *seg_row = nullptr, *nbits = nullptr, *tile_row = nullptr,
+        *carry_row = nullptr;
+    u64* carry_val = nullptr;
+    std::vector<u64> Ep(P), Rp(P), adj_line(P), adj_seg(P), h(2 * P);
+    int rc = RBSC_OK;
+    u64 E_pad = 0, R = 0, n_tiles = 0, end_words = 0;
+#define R_TRY(expr)                                                                  \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            rbsc_set_error("%s failed: %s", #expr, hipGetErrorString(_e));           \
+            rc = _e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;          \
+            goto done;                                                               \
+        }                                                                            \
+    } while (0)
+    R_TRY(hipMalloc(&counts, sizeof(u64) * PU));
+    R_TRY(hipMalloc(&line_off, sizeof(u64) * PU));
+    R_TRY(hipMalloc(&flags, sizeof(u64) * PU));
+    R_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(PU + 1)));
+    R_TRY(hipMalloc(&tot, sizeof(u64) * 2 * P));
+    R_TRY(hipMalloc(&adj, sizeof(u64) * 2 * P));
+    part_count_kernel<<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, U, P, counts);
+    flag_nonzero_kernel<<<grid_for(PU), BLK, 0, s>>>(counts, PU, flags);
+    // per-partition totals of lines and segments
+    for (u32 p = 0; p < P; ++p) {
+        rc = scan_excl_u64(counts + (u64)p * U, line_off + (u64)p * U, U, tot + p, tmp, s);
+        if (rc) goto done;
+    }
+    for (u32 p = 0; p < P; ++p) {
+        rc = scan_excl_u64(flags + (u64)p * U, flags + (u64)p * U, U, tot + P + p, tmp, s);
+        if (rc) goto done;
+    }
+    R_TRY(hipMemcpyAsync(h.data(), tot, sizeof(u64) * 2 * P, hipMemcpyDeviceToHost, s));
+    R_TRY(hipStreamSynchronize(s));
+    {
+        u64 line_base = 0, seg_base = 0;
+        for (u32 p = 0; p < P; ++p) {
+            Ep[p] = h[p];
+            Rp[p] = h[P + p];
+            adj_line[p] = line_base;  // partition p starts at this padded line
+            adj_seg[p] = seg_base;
+            const u64 padded = (Ep[p] + SIG_TW - 1) / SIG_TW * SIG_TW;
+            line_base += padded;
+            seg_base += Rp[p] + (padded > Ep[p] ? 1 : 0);
+        }
+        E_pad = line_base;
+        R = seg_base;
+    }
+    n_tiles = E_pad / SIG_TW;
+    end_words = (E_pad + 31) / 32 + 64;
+    R_TRY(hipMalloc(&ndst, sizeof(u32) * E_pad + 32));
+    if (g->wt) R_TRY(hipMalloc(&nwt, sizeof(u64) * E_pad));
+    R_TRY(hipMalloc(&seg_ptr, sizeof(u64) * (R + 1)));
+    R_TRY(hipMalloc(&seg_row, sizeof(u32) * R));
+    R_TRY(hipMalloc(&nbits, sizeof(u32) * end_words));
+    R_TRY(hipMalloc(&tile_row, sizeof(u32) * (n_tiles + 1)));
+    R_TRY(hipMalloc(&carry_row, sizeof(u32) * n_tiles));
+    R_TRY(hipMalloc(&carry_val, sizeof(u64) * n_tiles));
+    R_TRY(hipMemcpyAsync(adj, adj_line.data(), sizeof(u64) * P, hipMemcpyHostToDevice, s));
+    R_TRY(hipMemcpyAsync(adj + P, adj_seg.data(), sizeof(u64) * P, hipMemcpyHostToDevice, s));
+    zero_u32<<<grid_for(end_words), BLK, 0, s>>>(nbits, end_words);
+    if (nwt) R_TRY(hipMemsetAsync(nwt, 0, sizeof(u64) * E_pad, s));
+    if (g->wt)
+        part_fill_kernel<true><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, g->wt, U, P, line_off,
+                                                           flags, counts, adj, adj + P, ndst, nwt,
+                                                           seg_ptr, seg_row, nbits);
+    else
+        part_fill_kernel<false><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, nullptr, U, P,
+                                                            line_off, flags, counts, adj, adj + P,
+                                                            ndst, nullptr, seg_ptr, seg_row, nbits);
+    for (u32 p = 0; p < P; ++p) {
+        const u64 a = adj_line[p] + Ep[p];
+        const u64 b = (p + 1 < P) ? adj_line[p + 1] : E_pad;
+        if (b > a) {
+            pad_fill_kernel<<<grid_for(b - a), BLK, 0, s>>>(ndst, a, b);
+            pad_seg_kernel<<<1, 1, 0, s>>>(seg_ptr, seg_row, adj_seg[p] + Rp[p], a, (u32)N, nbits,
+                                           b - 1);
+        }
+    }
+    set_last_kernel<<<1, 1, 0, s>>>(seg_ptr, R, E_pad);
+    tile_setup_kernel<<<grid_for(n_tiles + 1), BLK, 0, s>>>(seg_ptr, (u32)R, E_pad, n_tiles,
+                                                            tile_row, carry_row);
+    R_TRY(hipGetLastError());
+    R_TRY(hipStreamSynchronize(s));
+    // swap in the partitioned arrays
+    (void)hipFree(g->dst);
+    if (g->wt) (void)hipFree(g->wt);
+    (void)hipFree(g->end_bits);
+    (void)hipFree(g->tile_row);
+    (void)hipFree(g->carry_row);
+    (void)hipFree(g->carry_val);
+    g->dst = ndst;
+    g->wt = nwt;
+    g->end_bits = nbits;
+    g->end_words = end_words;
+    g->tile_row = tile_row;
+    g->carry_row = carry_row;
+    g->carry_val = carry_val;
+    g->n_tiles = n_tiles;
+    g->seg_row = seg_row;
+    g->n_segs = R;
+    g->E_pad = E_pad;
+    g->n_parts = P;
+    g->part_tile.assign(P + 1, 0);
+    for (u32 p = 0; p < P; ++p) g->part_tile[p] = adj_line[p] / SIG_TW;
+    g->part_tile[P] = n_tiles;
+    R_TRY(hipMalloc(&g->d_part_tile, sizeof(u64) * (P + 1)));
+    R_TRY(hipMemcpy(g->d_part_tile, g->part_tile.data(), sizeof(u64) * (P + 1), hipMemcpyHostToDevice));
+    g->bytes += 4 * (E_pad - E) + 4 * R;
+    ndst = nullptr;
+    nwt = nullptr;
+    nbits = nullptr;
+    tile_row = carry_row = nullptr;
+    carry_val = nullptr;
+    seg_row = nullptr;
+done:
+    for (u64* p : {counts, line_off, flags, tmp, tot, adj, seg_ptr, nwt, carry_val})
+        if (p) (void)hipFree(p);
+    for (u32* p : {ndst, seg_row, nbits, tile_row, carry_row})
+        if (p) (void)hipFree(p);
+    return rc;
+#undef R_TRY
+}

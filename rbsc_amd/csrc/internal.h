@@ -10,29 +10,53 @@
 constexpr int SIG_V = 8;
 constexpr u64 SIG_TW = 64 * SIG_V;
 constexpr u32 NO_ROW = 0xFFFFFFFFu;
+constexpr u32 PAD_DST = 0xFFFFFFFFu;  // head of a padding line (contributes to a spare slot)
 
-// A graph resident in HBM, in compressed CSR form: only rows with at least
-// one line are stored (row k is node row_node[k], its lines are
-// dst[row_ptr[k] .. row_ptr[k+1])), in the order the lines were given.
-// Nodes without lines keep signature 0 (rbsc.cu:110-111).
-struct rbsc_dgraph {
-    u32 N = 0;
-    u64 E = 0;
-    u32 U = 0;                  // rows with >= 1 line
+// A graph as it arrives, on the device: rows with >= 1 line in input order,
+// line heads as original node ids.
+struct RawGraph {
+    u64 N = 0, E = 0, U = 0;
     u64* row_ptr = nullptr;     // U + 1
     u32* row_node = nullptr;    // U
     u32* dst = nullptr;         // E
-    u64* wt = nullptr;          // E or null (all 1)
+    u64* wt = nullptr;          // E or null
     u32* init = nullptr;        // N or null
+};
+
+// A graph resident in HBM in the round's layout (graph.hip): internal ids
+// put nodes with lines first (in-degree descending) and nodes without lines
+// last; row j is node j; lines are grouped by row.  Nodes without lines keep
+// signature 0 (rbsc.cu:110-111).
+struct rbsc_dgraph {
+    u32 N = 0;
+    u64 E = 0;
+    u32 U = 0;                  // nodes with >= 1 line = rows
+    u64* row_ptr = nullptr;     // U + 1
+    u32* pi = nullptr;          // N: original id -> internal id
+    u32* dst = nullptr;         // E, internal ids (null when streamed)
+    u64* wt = nullptr;          // E or null (all 1)
+    u32* init = nullptr;        // N (internal order) or null
+    u32* end_bits = nullptr;    // bit p: line p is the last line of its row
+    u64 end_words = 0;
     u64 n_tiles = 0;            // wave tiles of SIG_TW lines
     u32* tile_row = nullptr;    // n_tiles + 1: row holding each tile's first line
     u32* carry_row = nullptr;   // n_tiles: row continuing past the tile, or NO_ROW
     u64* carry_val = nullptr;   // n_tiles: that row's partial signature
     u64 bytes = 0;
-    bool owns = true;
     // Streaming: lines kept in pinned host memory (dst == nullptr on device).
     u32* dst_host = nullptr;    // E, pinned
     u64* wt_host = nullptr;     // E, pinned, or null
+    // Destination-partitioned layout (rbsc_graph_relayout, n_parts > 1): lines
+    // regrouped by (dst mod n_parts), each partition padded to whole tiles;
+    // "segments" are the (partition, row) runs, seg_row[k] the row of segment k
+    // (padding segments point at the spare signature slot N).  Tiles, the
+    // row-end bitmap and carries then count segments instead of rows.
+    u32 n_parts = 1;
+    u64 E_pad = 0;              // lines including padding
+    u32* seg_row = nullptr;     // segments
+    u64 n_segs = 0;
+    std::vector<u64> part_tile;  // n_parts + 1: first tile of each partition
+    u64* d_part_tile = nullptr;  // the same on the device
 };
 
 // Per-context device workspace for one refinement of up to cap_n nodes.
@@ -44,16 +68,20 @@ struct Workspace {
     u64* hc = nullptr;                 // randomize(colour + salt) per node
     u64* rank_status = nullptr;
     u64 rank_tiles = 0;
+    u64* tie_runs = nullptr;           // (start, end) of runs to fix after a 32-bit sort
+    u64 tie_cap = 0;
     int pc = 0;                        // P[pc] holds the last refinement's order
+    u64 n_rank = 0;                    // nodes ranked by the last refinement
 };
 
 // Inputs of one refinement beyond the graph itself.
 struct RefineParams {
-    const u32* d_init = nullptr;      // initial colours on the device (else graph's / default)
-    const u64* d_sig_fill = nullptr;  // per-node signature of nodes without lines (else 0)
+    const u64* d_sig_fill = nullptr;  // per-node signature of nodes without lines, by
+                                      // original id (else 0); needs initial colours
     u32 k0 = 1;                       // K before round 0 (rbsc.cu:339)
     u32 max_rounds = 0;               // 0 = to the fixpoint
     u64 slice_lines = 0;              // streaming slice size; 0 = lines resident
+    u32 flags = 0;                    // RBSC_FLAG_*
 };
 struct RefineOut {
     u32 K = 0;
@@ -70,7 +98,9 @@ struct rbsc_ctx {
     SortWorkspace sort;
     Workspace ws;
     u32* d_err = nullptr;
-    u32* d_scalar = nullptr;   // [0] = K of the last rank pass
+    u32* d_scalar = nullptr;   // [0] = K of the last rank pass, [1] = tie fix-up overflow,
+                               // [2] = tie runs listed
+    u64 full_sort_redos = 0;   // rounds whose tie fix-up overflowed
     u32* h_scalar = nullptr;   // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // streaming double buffer
@@ -94,16 +124,18 @@ void prof_end(rbsc_ctx* ctx);
 int prof_collect(rbsc_ctx* ctx);
 
 int ws_reserve(rbsc_ctx* ctx, u64 n);
+void raw_release(RawGraph* r);
+// Builds g from raw (raw arrays are left for the caller to release).
+int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g);
+int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 n_parts);
 int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& p, RefineOut* out);
-// Raw colours (dense ranks of the last round) of the last refine_core, to host.
-int download_colors(rbsc_ctx* ctx, u64 N, u32* host);
+// Raw colours (dense ranks of the last round) of the last refine_core on g,
+// by original node id, to host.
+int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host);
 int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out);
 int quotient_refine(rbsc_ctx* ctx, const rbsc_graph* gr, const rbsc_opts* opts, rbsc_result* res,
                     uint32_t* partition_out);
 int check_kernel_errors(rbsc_ctx* ctx);
-int dgraph_build_tiles(rbsc_ctx* ctx, rbsc_dgraph* g);
-int dgraph_alloc(rbsc_dgraph* g, u32 N, u64 E, u32 U, bool weighted, bool has_init,
-                 bool lines_on_device = true);
 void dgraph_release(rbsc_dgraph* g);
 // Canonical first-occurrence labels (rbsc.cu:431-445) of a colour vector.
 u32 canonicalize_host(u32 N, const u32* colors, u32* out);

@@ -148,7 +148,7 @@ int quotient_refine(rbsc_ctx* ctx, const rbsc_graph* gr, const rbsc_opts* opts, 
                 rc = refine_core(ctx, g, prm, &o);
                 if (!rc) {
                     colors.resize(node_n);
-                    rc = download_colors(ctx, node_n, colors.data());
+                    rc = download_colors(ctx, g, colors.data());
                 }
                 rbsc_graph_free(ctx, g);
                 if (rc) goto out;

@@ -52,112 +52,131 @@ __global__ void fill_u64(u64* p, u64 n, u64 v) {
     if (i < n) p[i] = v;
 }
 
-// Tile metadata: the row holding each tile's first line, and the row (if
-// any) that continues past the tile's end.
-__global__ void tile_setup_kernel(const u64* __restrict__ row_ptr, u32 U, u64 E, u64 n_tiles,
-                                  u32* __restrict__ tile_row, u32* __restrict__ carry_row) {
-    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > n_tiles) return;
-    if (t == n_tiles) {
-        tile_row[t] = U - 1;
-        return;
-    }
-    const u64 e0 = t * SIG_TW;
-    u32 lo = 0, hi = U - 1;
-    while (lo < hi) {
-        const u32 mid = lo + (hi - lo + 1) / 2;
-        if (row_ptr[mid] <= e0) lo = mid; else hi = mid - 1;
-    }
-    tile_row[t] = lo;
-    const u64 elast = (e0 + SIG_TW < E ? e0 + SIG_TW : E) - 1;
-    hi = U - 1;
-    while (lo < hi) {
-        const u32 mid = lo + (hi - lo + 1) / 2;
-        if (row_ptr[mid] <= elast) lo = mid; else hi = mid - 1;
-    }
-    carry_row[t] = row_ptr[lo + 1] > elast + 1 ? lo : NO_ROW;
-}
-
-// Signature of every row: sig[row_node[k]] = sum of hc[dst] * w over row k.
-template <bool HAS_WT>
+// Signature of every row over the internal layout: sig[j] = sum over row
+// j's lines of hc[dst] * w (hc[v] = randomize(c(v) + r)).  Heads at or
+// beyond gather_limit are nodes without lines whose hashed colour is the
+// per-round constant dang_h, so they are never gathered.
+// One wave per tile of 512 lines, 8 lines per lane.  Row ends come from the
+// row-end bitmap (one byte per lane); the lane's first row is the tile's
+// first row plus the row ends before it (wave prefix sum).  Rows spanning
+// lanes close with a segmented scan; rows spanning tiles get the tile's
+// partial sum in carry_val, added by sig_fixup_kernel.
+// With PART (destination-partitioned layout) the runs are segments: the
+// partial sum of segment k is added to sig[seg_row[k]] (one segment per row
+// per partition, partitions in separate launches, so no two lanes add to the
+// same row concurrently).
+// With C32 the per-node array holds raw u32 colours and each line hashes
+// randomize(colour + salt) itself (half the gathered bytes, four 64-bit
+// multiplies per line); otherwise it holds the hashed colours.
+// XCD (PART == 2): one launch covers all partitions; workgroup b serves
+// partition b mod 8 (workgroups are dealt round-robin to the 8 XCDs, so each
+// XCD's L2 caches a disjoint eighth of the colours) and partial sums are
+// added atomically because partitions now run concurrently.
+template <bool HAS_WT, int PART, bool C32>
 __global__ __launch_bounds__(256) void sig_kernel(
-    const u64* __restrict__ row_ptr, const u32* __restrict__ row_node, u32 U,
-    const u32* __restrict__ dst, const u64* __restrict__ wt, u64 E, const u64* __restrict__ hc,
-    u64* __restrict__ sig, const u32* __restrict__ tile_row, u64* __restrict__ carry_val,
-    u64 t_begin, u64 t_end, u64 line_base) {
+    const u32* __restrict__ dst, const u64* __restrict__ wt, u64 E,
+    const unsigned char* __restrict__ end_bytes, const u64* __restrict__ hc, u32 gather_limit,
+    u64 dang_h, u64 salt, u64* __restrict__ sig, const u32* __restrict__ seg_row,
+    const u32* __restrict__ tile_row, u64* __restrict__ carry_val, u64 t_begin, u64 t_end,
+    u64 line_base, const u64* __restrict__ part_tile) {
+    const u32* __restrict__ col = reinterpret_cast<const u32*>(hc);
     // Tiles [t_begin, t_end); dst/wt hold lines [line_base, ...) (a streamed
     // slice, or the whole array when line_base == 0).
-    const u64 t = t_begin + (u64)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (t >= t_end) return;  // uniform per wave
+    u64 t;
+    if (PART == 2) {
+        const u32 p = blockIdx.x & 7;
+        t = part_tile[p] + (u64)(blockIdx.x >> 3) * (blockDim.x / 64) + (threadIdx.x >> 6);
+        if (t >= part_tile[p + 1]) return;  // uniform per wave
+    } else {
+        t = t_begin + (u64)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+        if (t >= t_end) return;  // uniform per wave
+    }
     const int lane = threadIdx.x & 63;
     const u64 e0 = t * SIG_TW;
     const u64 e1 = e0 + SIG_TW < E ? e0 + SIG_TW : E;
     const u64 es = e0 + (u64)lane * SIG_V;
     const bool active = es < e1;
-    const u64 ee = active ? (es + SIG_V < e1 ? es + SIG_V : e1) : es;
+    const u32 nv = active ? (u32)(e1 - es < SIG_V ? e1 - es : SIG_V) : 0;
+    // Row-end flags of this lane's lines (bits beyond E are zero).
+    const u32 flags = active ? (u32)end_bytes[t * 64 + lane] : 0u;
+    // Row of the lane's first line.
+    u32 cnt = (u32)__popc(flags);
+    u32 x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    u32 k = tile_row[t] + (x - cnt);
+    // Does the lane start inside a row that began earlier?
+    const u32 pflags = __shfl_up(flags, 1);
+    u32 prev_end;
+    if (lane > 0) prev_end = (pflags >> 7) & 1u;
+    else prev_end = t == 0 ? 1u : ((u32)end_bytes[t * 64 - 1] >> 7) & 1u;
+    bool cont = prev_end == 0;
 
-    u64 acc = 0, head_val = 0;
-    bool has_head = false, tail_open = false;
-    u32 head_k = 0;
-    u64 key;
-    if (active) {
-        u32 lo = tile_row[t], hi = tile_row[t + 1];
-        while (lo < hi) {
-            const u32 mid = lo + (hi - lo + 1) / 2;
-            if (row_ptr[mid] <= es) lo = mid; else hi = mid - 1;
-        }
-        u32 k = lo;
-        bool cont = row_ptr[k] < es;
-        u64 rend = row_ptr[k + 1];
-        u64 hv[SIG_V];
-        if (ee - es == SIG_V) {
-            const uint4* p = reinterpret_cast<const uint4*>(dst + (es - line_base));
-            const uint4 a = p[0], b = p[1];
-            const u32 dv[SIG_V] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    u64 hv[SIG_V];
+    if (nv == SIG_V) {
+        typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4* p = reinterpret_cast<const u32x4*>(dst + (es - line_base));
+        const u32x4 a = __builtin_nontemporal_load(p);
+        const u32x4 b = __builtin_nontemporal_load(p + 1);
+        const u32 dv[SIG_V] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        if (C32) {
+            u32 cv[SIG_V];
 #pragma unroll
-            for (int j = 0; j < SIG_V; ++j) hv[j] = hc[dv[j]];
-            if (HAS_WT) {
+            for (int j = 0; j < SIG_V; ++j) cv[j] = dv[j] < gather_limit ? col[dv[j]] : 0u;
 #pragma unroll
-                for (int j = 0; j < SIG_V; ++j) hv[j] *= wt[es - line_base + j];
-            }
+            for (int j = 0; j < SIG_V; ++j)
+                hv[j] = dv[j] < gather_limit ? rbsc_randomize((u64)cv[j] + salt) : dang_h;
         } else {
 #pragma unroll
-            for (int j = 0; j < SIG_V; ++j) {
-                hv[j] = 0;
-                if (es + j < ee) {
-                    hv[j] = hc[dst[es - line_base + j]];
-                    if (HAS_WT) hv[j] *= wt[es - line_base + j];
-                }
-            }
+            for (int j = 0; j < SIG_V; ++j) hv[j] = dv[j] < gather_limit ? hc[dv[j]] : dang_h;
         }
-        bool closed_last = false;
+        if (HAS_WT) {
+#pragma unroll
+            for (int j = 0; j < SIG_V; ++j) hv[j] *= wt[es - line_base + j];
+        }
+    } else {
 #pragma unroll
         for (int j = 0; j < SIG_V; ++j) {
-            const u64 e = es + j;
-            if (e < ee) {
-                acc += hv[j];
-                closed_last = false;
-                if (e + 1 == rend) {
-                    if (cont) {
-                        has_head = true;
-                        head_val = acc;
-                        head_k = k;
-                        cont = false;
-                    } else {
-                        sig[row_node[k]] = acc;
-                    }
-                    acc = 0;
-                    closed_last = true;
-                    ++k;
-                    if (k < U) rend = row_ptr[k + 1];
-                }
+            hv[j] = 0;
+            if ((u32)j < nv) {
+                const u32 d = dst[es - line_base + j];
+                if (C32) hv[j] = d < gather_limit ? rbsc_randomize((u64)col[d] + salt) : dang_h;
+                else hv[j] = d < gather_limit ? hc[d] : dang_h;
+                if (HAS_WT) hv[j] *= wt[es - line_base + j];
             }
         }
-        tail_open = !closed_last;
-        key = tail_open ? 2 * (u64)k : 2 * (u64)(k - 1) + 1;
-    } else {
-        key = 2 * (u64)U + 1 + 2 * (u64)lane;
     }
+    u64 acc = 0, head_val = 0;
+    bool has_head = false;
+    u32 head_k = 0;
+#pragma unroll
+    for (int j = 0; j < SIG_V; ++j) {
+        if ((u32)j < nv) {
+            acc += hv[j];
+            if ((flags >> j) & 1u) {
+                if (cont) {
+                    has_head = true;
+                    head_val = acc;
+                    head_k = k;
+                    cont = false;
+                } else if (PART == 2) {
+                    atomicAdd((unsigned long long*)&sig[seg_row[k]], (unsigned long long)acc);
+                } else if (PART) {
+                    sig[seg_row[k]] += acc;
+                } else {
+                    sig[k] = acc;
+                }
+                acc = 0;
+                ++k;
+            }
+        }
+    }
+    const bool tail_open = nv > 0 && !((flags >> (nv - 1)) & 1u);
+    u64 key = active ? (tail_open ? 2 * (u64)k : 2 * (u64)k - 1)
+                     : (1ULL << 40) + 2 * (u64)lane + 1;
     // Segmented inclusive scan of open-row partials across the wave.
     u64 S = tail_open ? acc : 0;
 #pragma unroll
@@ -170,7 +189,10 @@ __global__ __launch_bounds__(256) void sig_kernel(
     const u64 pK = __shfl_up(key, 1);
     if (has_head) {
         if (lane > 0 && pK == 2 * (u64)head_k) head_val += pS;
-        sig[row_node[head_k]] = head_val;
+        if (PART == 2)
+            atomicAdd((unsigned long long*)&sig[seg_row[head_k]], (unsigned long long)head_val);
+        else if (PART) sig[seg_row[head_k]] += head_val;
+        else sig[head_k] = head_val;
     }
     const int last_lane = (int)((e1 - e0 - 1) / SIG_V);
     if (lane == last_lane && tail_open) carry_val[t] = S;
@@ -178,38 +200,105 @@ __global__ __launch_bounds__(256) void sig_kernel(
 
 __global__ void sig_fixup_kernel(const u32* __restrict__ carry_row,
                                  const u64* __restrict__ carry_val,
-                                 const u32* __restrict__ row_node, u64* sig, u64 n_tiles) {
+                                 const u32* __restrict__ seg_row, u64* sig, u64 n_tiles) {
     const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_tiles) return;
     const u32 r = carry_row[t];
-    if (r != NO_ROW) atomicAdd((unsigned long long*)&sig[row_node[r]], (unsigned long long)carry_val[t]);
+    if (r != NO_ROW)
+        atomicAdd((unsigned long long*)&sig[seg_row ? seg_row[r] : r],
+                  (unsigned long long)carry_val[t]);
 }
 
-// Initial colours: c0 = init[u] when given, else 0 (nodes with a line are
-// raised to N+1 by init_rows_kernel, rbsc.cu:51-63).
-__global__ void init_c0_kernel(const u32* __restrict__ init, u64* __restrict__ c0, u64 N) {
+// Round-0 state when every node with a line has colour N+1 (rbsc.cu:51-63)
+// and nodes without lines are left out of the node side: positions are
+// already sorted by colour.
+__global__ void init_uniform_kernel(u64* __restrict__ P, u64* __restrict__ hc, u64 U, u64 c,
+                                    int c32) {
     const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < N) c0[u] = init ? (u64)init[u] : 0ULL;
+    if (u < U) {
+        P[u] = (c << 32) | u;
+        if (c32) reinterpret_cast<u32*>(hc)[u] = (u32)c;
+        else hc[u] = rbsc_randomize(c + 0);
+    }
 }
-__global__ void init_rows_kernel(const u32* __restrict__ row_node, u64 U, u64* __restrict__ c0,
-                                 u64 value) {
-    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < U) c0[row_node[k]] = value;
-}
-__global__ void init_pack_kernel(const u64* __restrict__ c0, u64* __restrict__ P,
-                                 u64* __restrict__ hc, u64 N) {
+// Round-0 state from explicit initial colours (internal order); sorted by
+// colour afterwards.
+__global__ void init_given_kernel(const u32* __restrict__ init, u64* __restrict__ keys,
+                                  u64* __restrict__ P, u64* __restrict__ hc, u64 N, int c32) {
     const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (u < N) {
-        const u64 c = c0[u];
+        const u64 c = init[u];
+        keys[u] = c;
         P[u] = (c << 32) | u;
-        hc[u] = rbsc_randomize(c + 0);
+        if (c32) reinterpret_cast<u32*>(hc)[u] = (u32)c;
+        else hc[u] = rbsc_randomize(c + 0);
     }
+}
+__global__ void permute_fill_kernel(const u64* __restrict__ fill, const u32* __restrict__ pi,
+                                    u64 N, u64* __restrict__ sig) {
+    const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N) sig[pi[v]] = fill[v];
 }
 
 __global__ void gather_keys_kernel(const u64* __restrict__ P, const u64* __restrict__ sig,
                                    u64* __restrict__ keys, u64 n) {
     const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) keys[i] = sig[(u32)P[i]];
+}
+
+// After a stable sort by the top 32 signature bits (colour order kept inside
+// equal keys), every run of equal top halves must be ordered by the low 32
+// bits, stably.  Such runs come from distinct signatures colliding in 32
+// bits (rare, short) or from equal signatures (already ordered: no "break"
+// between neighbours).  tie_find_kernel (read-only) lists each run that has a
+// break once, from its first break; tie_sort_kernel insertion-sorts the listed
+// runs (disjoint).  A run longer than TIE_MAX, or more runs than the list
+// holds, raise *overflow and the caller redoes a full 64-bit sort.
+// The split is at bit `lo` (32 normally; smaller splits are verification
+// hooks that make ties frequent).
+constexpr u64 TIE_MAX = 4096;
+__global__ void tie_find_kernel(const u64* __restrict__ keys, u64 n, u32 lo, u64 tie_max,
+                                u64* __restrict__ runs, u64 cap, u32* __restrict__ count,
+                                u32* __restrict__ overflow) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 || i >= n) return;
+    const u64 k = keys[i], pk = keys[i - 1];
+    if ((k >> lo) != (pk >> lo) || k == pk) return;  // not a break
+    u64 s = i - 1;
+    while (s > 0 && (keys[s - 1] >> lo) == (k >> lo)) {
+        if (keys[s - 1] != keys[s]) return;  // an earlier break lists the run
+        if (i - s > tie_max) { atomicOr(overflow, 1u); return; }
+        --s;
+    }
+    u64 e = i + 1;
+    while (e < n && (keys[e] >> lo) == (k >> lo)) {
+        if (e - s > tie_max) { atomicOr(overflow, 1u); return; }
+        ++e;
+    }
+    const u32 slot = atomicAdd(count, 1u);
+    if (slot >= cap) { atomicOr(overflow, 1u); return; }
+    runs[2 * (u64)slot] = s;
+    runs[2 * (u64)slot + 1] = e;
+}
+__global__ void tie_sort_kernel(u64* __restrict__ keys, u64* __restrict__ P,
+                                const u64* __restrict__ runs, const u32* __restrict__ count,
+                                u64 cap, u32 lo) {
+    const u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const u64 nr = *count;
+    if (r >= nr || r >= cap) return;
+    const u64 s = runs[2 * r], e = runs[2 * r + 1];
+    const u64 mask = (1ULL << lo) - 1;
+    for (u64 a = s + 1; a < e; ++a) {  // stable insertion sort by the low bits
+        const u64 ka = keys[a], pa = P[a];
+        u64 b = a;
+        while (b > s && (keys[b - 1] & mask) > (ka & mask)) {
+            keys[b] = keys[b - 1];
+            P[b] = P[b - 1];
+            --b;
+        }
+        keys[b] = ka;
+        P[b] = pa;
+    }
 }
 
 constexpr int RANK_ITEMS = 16;
@@ -220,8 +309,8 @@ constexpr u64 RANK_TILE = 256 * RANK_ITEMS;
 // Writes Pout[i] = (rank << 32) | node and hc[node] = randomize(rank + salt).
 __global__ __launch_bounds__(256) void rank_kernel(
     const u64* __restrict__ keys, const u64* __restrict__ Pin, u64* __restrict__ Pout,
-    u64* __restrict__ hc, u64 n, u64 salt_next, u64* status, u64* ticket, u64 ticket_base,
-    u32 epoch, u32* K_out, u32* err) {
+    u64* __restrict__ hc, u64 n, u64 rank_offset, u64 salt_next, int c32, u64* status,
+    u64* ticket, u64 ticket_base, u32 epoch, u32* K_out, u32* err) {
     __shared__ u64 s_wave[4];
     __shared__ u64 s_tile, s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -298,7 +387,7 @@ __global__ __launch_bounds__(256) void rank_kernel(
         s_prefix = excl;
     }
     __syncthreads();
-    u64 r = s_prefix + texcl;
+    u64 r = s_prefix + texcl + rank_offset;
 #pragma unroll
     for (int j = 0; j < RANK_ITEMS; ++j) {
         const u64 i = base + j;
@@ -306,16 +395,32 @@ __global__ __launch_bounds__(256) void rank_kernel(
             r += (flags >> j) & 1u;
             const u32 node = (u32)p[j];
             Pout[i] = (r << 32) | node;
-            hc[node] = rbsc_randomize(r + salt_next);
+            if (c32) reinterpret_cast<u32*>(hc)[node] = (u32)r;
+            else hc[node] = rbsc_randomize(r + salt_next);
             if (i == n - 1) *K_out = (u32)(r + 1);
         }
     }
 }
 
-// Raw colours by node from the packed order.
+// Raw colours by internal node from the packed order (nodes beyond n keep
+// the colour they were filled with).
 __global__ void colors_kernel(const u64* __restrict__ P, u32* __restrict__ col, u64 n) {
     const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) col[(u32)P[i]] = (u32)(P[i] >> 32);
+}
+__global__ void fill_u32(u32* p, u64 n, u32 v) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void fill_u32_kernel(u32* p, u64 n, u32 v) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+// Colours by original id.
+__global__ void map_colors_kernel(const u32* __restrict__ col_int, const u32* __restrict__ pi,
+                                  u64 N, u32* __restrict__ out) {
+    const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N) out[v] = col_int[pi[v]];
 }
 
 }  // namespace
@@ -393,14 +498,17 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     if (w.sig) (void)hipFree(w.sig);
     if (w.hc) (void)hipFree(w.hc);
     if (w.rank_status) (void)hipFree(w.rank_status);
+    if (w.tie_runs) (void)hipFree(w.tie_runs);
     w = Workspace();
     const u64 cap = std::max<u64>(n, 4096);
     for (int i = 0; i < 2; ++i) {
         HIP_TRY(hipMalloc(&w.keys[i], sizeof(u64) * cap));
         HIP_TRY(hipMalloc(&w.P[i], sizeof(u64) * cap));
     }
-    HIP_TRY(hipMalloc(&w.sig, sizeof(u64) * cap));
+    HIP_TRY(hipMalloc(&w.sig, sizeof(u64) * (cap + 1)));  // + spare slot for padding
     HIP_TRY(hipMalloc(&w.hc, sizeof(u64) * cap));
+    w.tie_cap = cap / 16 + 1024;
+    HIP_TRY(hipMalloc(&w.tie_runs, sizeof(u64) * 2 * w.tie_cap));
     w.rank_tiles = (cap + RANK_TILE - 1) / RANK_TILE + 1;
     HIP_TRY(hipMalloc(&w.rank_status, sizeof(u64) * w.rank_tiles));
     HIP_TRY(hipMemsetAsync(w.rank_status, 0, sizeof(u64) * w.rank_tiles, ctx->stream));
@@ -410,64 +518,22 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     return RBSC_OK;
 }
 
-int dgraph_alloc(rbsc_dgraph* g, u32 N, u64 E, u32 U, bool weighted, bool has_init,
-                 bool lines_on_device) {
-    g->N = N;
-    g->E = E;
-    g->U = U;
-    g->n_tiles = (E + SIG_TW - 1) / SIG_TW;
-    u64 bytes = 0;
-    HIP_TRY(hipMalloc(&g->row_ptr, sizeof(u64) * ((u64)U + 1)));
-    bytes += sizeof(u64) * ((u64)U + 1);
-    HIP_TRY(hipMalloc(&g->row_node, sizeof(u32) * std::max<u64>(U, 1)));
-    bytes += sizeof(u32) * std::max<u64>(U, 1);
-    if (lines_on_device) {
-        HIP_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
-        bytes += sizeof(u32) * std::max<u64>(E, 1);
-        if (weighted) {
-            HIP_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
-            bytes += sizeof(u64) * E;
-        }
-    } else {
-        HIP_TRY(hipHostMalloc(&g->dst_host, sizeof(u32) * std::max<u64>(E, 1), hipHostMallocDefault));
-        if (weighted)
-            HIP_TRY(hipHostMalloc(&g->wt_host, sizeof(u64) * std::max<u64>(E, 1), hipHostMallocDefault));
-    }
-    if (has_init) {
-        HIP_TRY(hipMalloc(&g->init, sizeof(u32) * std::max<u32>(N, 1)));
-        bytes += sizeof(u32) * N;
-    }
-    HIP_TRY(hipMalloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1)));
-    HIP_TRY(hipMalloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1)));
-    HIP_TRY(hipMalloc(&g->carry_val, sizeof(u64) * std::max<u64>(g->n_tiles, 1)));
-    bytes += 16 * (g->n_tiles + 1);
-    g->bytes = bytes;
-    return RBSC_OK;
-}
-
 void dgraph_release(rbsc_dgraph* g) {
     if (!g) return;
-    if (g->owns) {
-        if (g->row_ptr) (void)hipFree(g->row_ptr);
-        if (g->row_node) (void)hipFree(g->row_node);
-        if (g->dst) (void)hipFree(g->dst);
-        if (g->wt) (void)hipFree(g->wt);
-        if (g->init) (void)hipFree(g->init);
-    }
+    if (g->row_ptr) (void)hipFree(g->row_ptr);
+    if (g->pi) (void)hipFree(g->pi);
+    if (g->dst) (void)hipFree(g->dst);
+    if (g->wt) (void)hipFree(g->wt);
+    if (g->init) (void)hipFree(g->init);
+    if (g->end_bits) (void)hipFree(g->end_bits);
+    if (g->seg_row) (void)hipFree(g->seg_row);
+    if (g->d_part_tile) (void)hipFree(g->d_part_tile);
     if (g->dst_host) (void)hipHostFree(g->dst_host);
     if (g->wt_host) (void)hipHostFree(g->wt_host);
     if (g->tile_row) (void)hipFree(g->tile_row);
     if (g->carry_row) (void)hipFree(g->carry_row);
     if (g->carry_val) (void)hipFree(g->carry_val);
     *g = rbsc_dgraph();
-}
-
-int dgraph_build_tiles(rbsc_ctx* ctx, rbsc_dgraph* g) {
-    if (g->n_tiles == 0) return RBSC_OK;
-    tile_setup_kernel<<<grid_for(g->n_tiles + 1), BLK, 0, ctx->stream>>>(
-        g->row_ptr, g->U, g->E, g->n_tiles, g->tile_row, g->carry_row);
-    HIP_TRY(hipGetLastError());
-    return RBSC_OK;
 }
 
 // Host-side compression of a graph into rows (validates ids and grouping).
@@ -529,18 +595,38 @@ static int compress_host(const rbsc_graph* gr, std::vector<u64>& row_ptr,
 
 // ---------------------------------------------------------------- refine
 static void launch_sig(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* dst, const u64* wt,
-                       u64 t_begin, u64 t_end, u64 line_base) {
+                       u64 t_begin, u64 t_end, u64 line_base, u32 gather_limit, u64 dang_h,
+                       u64 salt, bool c32, bool xcd = false) {
     if (t_end <= t_begin) return;
     Workspace& w = ctx->ws;
-    const unsigned grid = grid_for((t_end - t_begin) * 64);
-    if (wt)
-        sig_kernel<true><<<grid, BLK, 0, ctx->stream>>>(g->row_ptr, g->row_node, g->U, dst, wt, g->E,
-                                                        w.hc, w.sig, g->tile_row, g->carry_val,
-                                                        t_begin, t_end, line_base);
-    else
-        sig_kernel<false><<<grid, BLK, 0, ctx->stream>>>(g->row_ptr, g->row_node, g->U, dst,
-                                                         nullptr, g->E, w.hc, w.sig, g->tile_row,
-                                                         g->carry_val, t_begin, t_end, line_base);
+    unsigned grid = grid_for((t_end - t_begin) * 64);
+    if (xcd) {  // 8 partitions, each on its own XCD
+        u64 most = 0;
+        for (u32 p = 0; p < 8; ++p) most = std::max(most, g->part_tile[p + 1] - g->part_tile[p]);
+        grid = (unsigned)(8 * ((most + 3) / 4));
+    }
+    const unsigned char* eb = reinterpret_cast<const unsigned char*>(g->end_bits);
+    const u64 E = g->n_parts > 1 ? g->E_pad : g->E;
+#define SIG_LAUNCH(W, PT, C)                                                                   \
+    sig_kernel<W, PT, C><<<grid, BLK, 0, ctx->stream>>>(dst, W ? wt : nullptr, E, eb, w.hc,    \
+                                                        gather_limit, dang_h, salt, w.sig,     \
+                                                        g->seg_row, g->tile_row, g->carry_val, \
+                                                        t_begin, t_end, line_base, g->d_part_tile)
+#define SIG_C(W, PT)                   \
+    if (c32) SIG_LAUNCH(W, PT, true);  \
+    else SIG_LAUNCH(W, PT, false)
+    if (xcd) {
+        if (wt) { SIG_C(true, 2); }
+        else { SIG_C(false, 2); }
+    } else if (g->n_parts > 1) {
+        if (wt) { SIG_C(true, 1); }
+        else { SIG_C(false, 1); }
+    } else {
+        if (wt) { SIG_C(true, 0); }
+        else { SIG_C(false, 0); }
+    }
+#undef SIG_C
+#undef SIG_LAUNCH
 }
 
 static int stream_reserve(rbsc_ctx* ctx, u64 lines, bool weighted) {
@@ -568,26 +654,40 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     int rc = ws_reserve(ctx, N);
     if (rc) return rc;
     Workspace& w = ctx->ws;
-    const u32* init = prm.d_init ? prm.d_init : g->init;
+    // Without initial colours, nodes without lines are always colour 0: their
+    // pair (0, 0) is the least of every round (others have colour >= 1 from
+    // c0 = N+1 on), so they are left out of the node side, ranks of the rest
+    // are shifted by one, and their hashed colour is the constant
+    // randomize(0 + r).  With initial colours every node takes part.
+    const bool excluded = g->init == nullptr;
+    if (prm.d_sig_fill && excluded) {
+        rbsc_set_error("signature defaults need initial colours");
+        return RBSC_ERR_INVALID;
+    }
+    const u64 n_rank = excluded ? g->U : N;
+    const u64 rank_offset = excluded && N > g->U ? 1 : 0;
+    const u32 gather_limit = excluded ? g->U : (u32)N;
+    const int c32 = (prm.flags & RBSC_FLAG_HASHED64) ? 0 : 1;
 
     // Round-0 state: P sorted by c0, hc = randomize(c0 + 0).
     prof_begin(ctx, 4);
-    init_c0_kernel<<<grid_for(N), BLK, 0, s>>>(init, w.keys[0], N);
-    if (!init && g->U)
-        init_rows_kernel<<<grid_for(g->U), BLK, 0, s>>>(g->row_node, g->U, w.keys[0], N + 1);
-    init_pack_kernel<<<grid_for(N), BLK, 0, s>>>(w.keys[0], w.P[0], w.hc, N);
-    if (prm.d_sig_fill)
-        HIP_TRY(hipMemcpyAsync(w.sig, prm.d_sig_fill, sizeof(u64) * N, hipMemcpyDeviceToDevice, s));
-    else
-        fill_u64<<<grid_for(N), BLK, 0, s>>>(w.sig, N, 0);
     int which = 0;
-    {
+    int pc = 0;
+    if (excluded) {
+        if (n_rank)
+            init_uniform_kernel<<<grid_for(n_rank), BLK, 0, s>>>(w.P[0], w.hc, n_rank, N + 1, c32);
+    } else {
+        init_given_kernel<<<grid_for(N), BLK, 0, s>>>(g->init, w.keys[0], w.P[0], w.hc, N, c32);
         u64* kk[2] = {w.keys[0], w.keys[1]};
         u64* pp[2] = {w.P[0], w.P[1]};
         rc = radix_sort_u64(&ctx->sort, kk, pp, N, 0, 32, &which, s);
         if (rc) return rc;
+        pc = which;
     }
-    int pc = which;  // P[pc] holds the current packed order
+    if (prm.d_sig_fill)
+        permute_fill_kernel<<<grid_for(N), BLK, 0, s>>>(prm.d_sig_fill, g->pi, N, w.sig);
+    else
+        fill_u64<<<grid_for(N), BLK, 0, s>>>(w.sig, N, 0);
     prof_end(ctx);
 
     // Streaming set-up: slices of whole wave tiles, double-buffered.
@@ -629,8 +729,17 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     do {
         K = Knew;
         prof_begin(ctx, 0);
-        if (n_tiles && !streaming) {
-            launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0);
+        const u64 dang_h = rbsc_randomize(0 + (u64)it);
+        if (n_tiles && !streaming && g->n_parts == 8 && (prm.flags & RBSC_FLAG_XCD)) {
+            fill_u64<<<grid_for(g->U), BLK, 0, s>>>(w.sig, g->U, 0);
+            launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0, gather_limit, dang_h, it, c32, true);
+        } else if (n_tiles && !streaming && g->n_parts > 1) {
+            fill_u64<<<grid_for(g->U), BLK, 0, s>>>(w.sig, g->U, 0);
+            for (u32 p = 0; p < g->n_parts; ++p)
+                launch_sig(ctx, g, g->dst, g->wt, g->part_tile[p], g->part_tile[p + 1], 0,
+                           gather_limit, dang_h, it, c32);
+        } else if (n_tiles && !streaming) {
+            launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0, gather_limit, dang_h, it, c32);
         } else if (n_tiles) {
             const bool resident = n_slices <= 2 && it > 0;  // both slices stay in the buffers
             for (u64 sl = 0; sl < n_slices; ++sl) {
@@ -642,7 +751,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                 if (!resident) HIP_TRY(hipStreamWaitEvent(s, ctx->ev_copy[b], 0));
                 const u64 lo = sl * Bs, hi = std::min<u64>(g->E, lo + Bs);
                 launch_sig(ctx, g, ctx->sbuf[b], weighted ? ctx->swbuf[b] : nullptr, lo / SIG_TW,
-                           (hi + SIG_TW - 1) / SIG_TW, lo);
+                           (hi + SIG_TW - 1) / SIG_TW, lo, gather_limit, dang_h, it, c32);
                 HIP_TRY(hipEventRecord(ctx->ev_done[b], s));
                 if (!resident && sl + 2 < n_slices) {
                     rc = issue_copy(sl + 2);
@@ -660,35 +769,74 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         }
         if (n_tiles)
             sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, g->carry_val,
-                                                               g->row_node, w.sig, n_tiles);
+                                                               g->n_parts > 1 ? g->seg_row : nullptr,
+                                                               w.sig, n_tiles);
         prof_end(ctx);
-        prof_begin(ctx, 1);
-        gather_keys_kernel<<<grid_for(N), BLK, 0, s>>>(w.P[pc], w.sig, w.keys[0], N);
-        prof_end(ctx);
-        {
+        // Node side: keys in colour order, stable sort by signature, dense rank.
+        // sorted_pc is the P buffer holding the sorted order (input of rank).
+        int sorted_pc = pc;
+        auto sort_and_rank = [&](bool full, bool gather) -> int {
+            if (gather) {
+                prof_begin(ctx, 1);
+                gather_keys_kernel<<<grid_for(n_rank), BLK, 0, s>>>(w.P[pc], w.sig, w.keys[0],
+                                                                    n_rank);
+                prof_end(ctx);
+            }
             u64* kk[2] = {w.keys[0], w.keys[1]};
             u64* pp[2] = {w.P[pc], w.P[pc ^ 1]};
             prof_begin(ctx, 2);
-            rc = radix_sort_u64(&ctx->sort, kk, pp, N, 0, 64, &which, s);
+            const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
+            int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, full ? 0 : lo, 64, &which, s);
+            if (r2) return r2;
+            if (!full) {
+                const u64 cap = w.tie_cap;
+                fill_u32<<<1, 1, 0, s>>>(ctx->d_scalar + 2, 1, 0);
+                const u64 tie_max = (prm.flags & RBSC_FLAG_TIESMALL) ? 2 : TIE_MAX;
+                tie_find_kernel<<<grid_for(n_rank), BLK, 0, s>>>(kk[which], n_rank, lo, tie_max,
+                                                                 w.tie_runs, cap, ctx->d_scalar + 2,
+                                                                 ctx->d_scalar + 1);
+                tie_sort_kernel<<<grid_for(cap), BLK, 0, s>>>(kk[which], pp[which], w.tie_runs,
+                                                              ctx->d_scalar + 2, cap, lo);
+            }
             prof_end(ctx);
-            if (rc) return rc;
-            u64* sorted_keys = kk[which];
-            u64* sorted_P = pp[which];
-            u64* out_P = pp[which ^ 1];
-            const u64 tiles = (N + RANK_TILE - 1) / RANK_TILE;
+            sorted_pc = (pp[which] == w.P[0]) ? 0 : 1;
+            const u64 tiles = (n_rank + RANK_TILE - 1) / RANK_TILE;
             const u32 ep = lb_next_epoch(&ctx->sort, s);
             prof_begin(ctx, 3);
             rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
-                sorted_keys, sorted_P, out_P, w.hc, N, (u64)it + 1, w.rank_status,
-                ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar, ctx->d_err);
+                kk[which], pp[which], pp[which ^ 1], w.hc, n_rank, rank_offset, (u64)it + 1, c32,
+                w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
+                ctx->d_err);
             prof_end(ctx);
             ctx->sort.ticket_total += tiles;
-            pc = (out_P == w.P[0]) ? 0 : 1;
+            return RBSC_OK;
+        };
+        const bool full_sort = (prm.flags & RBSC_FLAG_SPLIT) == 0;
+        if (n_rank) {
+            rc = sort_and_rank(full_sort, true);
+            if (rc) return rc;
         }
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(u32), hipMemcpyDeviceToHost, s));
+        if (n_rank)
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 2 * sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        Knew = ctx->h_scalar[0];
+        if (n_rank && ctx->h_scalar[1]) {
+            // A run of signatures equal in their top 32 bits was too long for
+            // the fix-up: sort this round's (signature, P) again on all 64 bits.
+            // Inside equal signatures the arrays still hold colour order, so
+            // the full stable sort gives (signature, colour) order.
+            ++ctx->full_sort_redos;
+            fill_u32<<<1, 1, 0, s>>>(ctx->d_scalar + 1, 1, 0);
+            pc = sorted_pc;
+            rc = sort_and_rank(true, false);
+            if (rc) return rc;
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        if (n_rank) pc = sorted_pc ^ 1;
+        Knew = n_rank ? ctx->h_scalar[0] : (u32)rank_offset;
         ++it;
         if (prm.max_rounds && it >= prm.max_rounds) break;
     } while (K != Knew);
@@ -702,19 +850,25 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     w.pc = pc;
+    w.n_rank = n_rank;
     out->K = Knew;
     out->rounds = it;
     out->device_ms = ms;
     return RBSC_OK;
 }
 
-int download_colors(rbsc_ctx* ctx, u64 N, u32* host) {
+int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host) {
+    const u64 N = g->N;
     if (N == 0) return RBSC_OK;
     Workspace& w = ctx->ws;
-    u32* d_col = reinterpret_cast<u32*>(w.keys[0]);
-    colors_kernel<<<grid_for(N), BLK, 0, ctx->stream>>>(w.P[w.pc], d_col, N);
-    HIP_TRY(hipMemcpyAsync(host, d_col, sizeof(u32) * N, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    hipStream_t s = ctx->stream;
+    u32* d_int = reinterpret_cast<u32*>(w.keys[0]);
+    u32* d_out = reinterpret_cast<u32*>(w.keys[1]);
+    if (w.n_rank < N) fill_u32_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, N, 0);
+    if (w.n_rank) colors_kernel<<<grid_for(w.n_rank), BLK, 0, s>>>(w.P[w.pc], d_int, w.n_rank);
+    map_colors_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, g->pi, N, d_out);
+    HIP_TRY(hipMemcpyAsync(host, d_out, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return RBSC_OK;
 }
 
@@ -724,6 +878,7 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
     auto t0 = std::chrono::steady_clock::now();
     RefineParams prm;
     prm.max_rounds = opts ? opts->max_rounds : 0;
+    prm.flags = opts ? opts->flags : 0;
     if (opts && opts->max_batch_edges && opts->max_batch_edges < g->E)
         prm.slice_lines = opts->max_batch_edges;
     RefineOut o;
@@ -736,7 +891,7 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
             col.resize(N);
             hcol = col.data();
         }
-        rc = download_colors(ctx, N, hcol);
+        rc = download_colors(ctx, g, hcol);
         if (rc) return rc;
         if (partition_out) canonicalize_host((u32)N, hcol, partition_out);
     }
@@ -779,6 +934,7 @@ extern "C" int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out) 
     HIP_TRY(hipMalloc(&c->d_err, sizeof(u32)));
     HIP_TRY(hipMemset(c->d_err, 0, sizeof(u32)));
     HIP_TRY(hipMalloc(&c->d_scalar, sizeof(u32) * 16));
+    HIP_TRY(hipMemset(c->d_scalar, 0, sizeof(u32) * 16));
     HIP_TRY(hipHostMalloc(&c->h_scalar, sizeof(u32) * 16, hipHostMallocDefault));
     HIP_TRY(hipEventCreate(&c->ev0));
     HIP_TRY(hipEventCreate(&c->ev1));
@@ -802,6 +958,7 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
     if (w.sig) (void)hipFree(w.sig);
     if (w.hc) (void)hipFree(w.hc);
     if (w.rank_status) (void)hipFree(w.rank_status);
+    if (w.tie_runs) (void)hipFree(w.tie_runs);
     sort_ws_free(&c->sort);
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->d_scalar) (void)hipFree(c->d_scalar);
@@ -841,39 +998,55 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
     std::vector<u32> row_node;
     int rc = compress_host(gr, row_ptr, row_node);
     if (rc) return rc;
-    const u32 U = (u32)row_node.size();
-    const u64 E = gr->n_edges;
+    const u64 U = row_node.size();
+    const u64 N = gr->n_nodes, E = gr->n_edges;
     const u64 line_bytes = lines_on_host ? 0 : 4ull * E + (gr->weight ? 8ull * E : 0);
-    const u64 need = sizeof(u64) * (U + 1) + 4ull * U + line_bytes + 48ull * gr->n_nodes;
+    const u64 need = 8 * (U + 1) + 8 * N + line_bytes + E / 8 + 48 * N;
     if (ctx->budget && need > ctx->budget) {
         rbsc_set_error("graph needs %llu bytes, budget is %llu", (unsigned long long)need,
                        (unsigned long long)ctx->budget);
         return RBSC_ERR_BUDGET;
     }
+    hipStream_t s = ctx->stream;
+    RawGraph raw;
+    raw.N = N;
+    raw.E = E;
+    raw.U = U;
+    auto fail = [&](hipError_t e, const char* what) {
+        rbsc_set_error("%s failed: %s", what, hipGetErrorString(e));
+        raw_release(&raw);
+        return e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&raw.row_ptr, sizeof(u64) * (U + 1))) != hipSuccess) return fail(e, "hipMalloc");
+    if ((e = hipMalloc(&raw.row_node, sizeof(u32) * std::max<u64>(U, 1))) != hipSuccess) return fail(e, "hipMalloc");
+    if ((e = hipMalloc(&raw.dst, sizeof(u32) * std::max<u64>(E, 1))) != hipSuccess) return fail(e, "hipMalloc");
+    if (gr->weight && (e = hipMalloc(&raw.wt, sizeof(u64) * std::max<u64>(E, 1))) != hipSuccess)
+        return fail(e, "hipMalloc");
+    if (gr->init_color && (e = hipMalloc(&raw.init, sizeof(u32) * std::max<u64>(N, 1))) != hipSuccess)
+        return fail(e, "hipMalloc");
+    if ((e = hipMemcpyAsync(raw.row_ptr, row_ptr.data(), 8 * (U + 1), hipMemcpyHostToDevice, s)) ||
+        (U && (e = hipMemcpyAsync(raw.row_node, row_node.data(), 4 * U, hipMemcpyHostToDevice, s))) ||
+        (E && (e = hipMemcpyAsync(raw.dst, gr->dst, 4 * E, hipMemcpyHostToDevice, s))) ||
+        (E && gr->weight && (e = hipMemcpyAsync(raw.wt, gr->weight, 8 * E, hipMemcpyHostToDevice, s))) ||
+        (N && gr->init_color && (e = hipMemcpyAsync(raw.init, gr->init_color, 4 * N, hipMemcpyHostToDevice, s))))
+        return fail(e, "hipMemcpyAsync");
     rbsc_dgraph* g = new rbsc_dgraph();
-    rc = dgraph_alloc(g, gr->n_nodes, E, U, gr->weight != nullptr, gr->init_color != nullptr,
-                      !(lines_on_host && E));
+    rc = dgraph_build(ctx, &raw, g);
+    raw_release(&raw);
     if (rc) {
         dgraph_release(g);
         delete g;
         return rc;
     }
-    hipStream_t s = ctx->stream;
-    HIP_TRY(hipMemcpyAsync(g->row_ptr, row_ptr.data(), sizeof(u64) * (U + 1), hipMemcpyHostToDevice, s));
-    if (U) HIP_TRY(hipMemcpyAsync(g->row_node, row_node.data(), sizeof(u32) * U, hipMemcpyHostToDevice, s));
-    if (E && g->dst) {
-        HIP_TRY(hipMemcpyAsync(g->dst, gr->dst, sizeof(u32) * E, hipMemcpyHostToDevice, s));
-        if (gr->weight)
-            HIP_TRY(hipMemcpyAsync(g->wt, gr->weight, sizeof(u64) * E, hipMemcpyHostToDevice, s));
-    } else if (E) {
-        memcpy(g->dst_host, gr->dst, sizeof(u32) * E);
-        if (gr->weight) memcpy(g->wt_host, gr->weight, sizeof(u64) * E);
+    if (lines_on_host && E) {
+        rc = rbsc_graph_lines_to_host(ctx, g);
+        if (rc) {
+            dgraph_release(g);
+            delete g;
+            return rc;
+        }
     }
-    if (gr->init_color && gr->n_nodes)
-        HIP_TRY(hipMemcpyAsync(g->init, gr->init_color, sizeof(u32) * gr->n_nodes, hipMemcpyHostToDevice, s));
-    rc = dgraph_build_tiles(ctx, g);
-    if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(s));
     *out = g;
     return RBSC_OK;
 }
@@ -882,9 +1055,22 @@ extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgrap
     return graph_upload_host(ctx, gr, false, out);
 }
 
+extern "C" int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts) {
+    if (!ctx || !g) {
+        rbsc_set_error("rbsc_graph_relayout: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    return dgraph_relayout(ctx, g, n_parts);
+}
+
 extern "C" int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g) {
     if (!ctx || !g) {
         rbsc_set_error("rbsc_graph_lines_to_host: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    if (g->n_parts > 1) {
+        rbsc_set_error("rbsc_graph_lines_to_host: graph has a partitioned line layout");
         return RBSC_ERR_INVALID;
     }
     if (!g->dst) return RBSC_OK;  // already on the host (or no lines)
@@ -931,27 +1117,32 @@ extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t
         rbsc_set_error("rbsc_graph_download: null argument");
         return RBSC_ERR_INVALID;
     }
-    HIP_TRY(hipSetDevice(ctx->device));
-    std::vector<u64> rp(g->U + 1);
-    std::vector<u32> rn(g->U);
-    HIP_TRY(hipMemcpy(rp.data(), g->row_ptr, sizeof(u64) * (g->U + 1), hipMemcpyDeviceToHost));
-    if (g->U) HIP_TRY(hipMemcpy(rn.data(), g->row_node, sizeof(u32) * g->U, hipMemcpyDeviceToHost));
-    if (row_offsets) {
-        // Expand to CSR over all nodes; rows of a resident graph are stored in
-        // node order only when the input was sorted by source.
-        std::vector<u64> deg(g->N, 0);
-        for (u32 k = 0; k < g->U; ++k) deg[rn[k]] = rp[k + 1] - rp[k];
-        row_offsets[0] = 0;
-        for (u32 u = 0; u < g->N; ++u) row_offsets[u + 1] = row_offsets[u] + deg[u];
-        for (u32 k = 0; k + 1 < g->U; ++k)
-            if (rn[k] >= rn[k + 1]) {
-                rbsc_set_error("resident graph rows are not in node order");
-                return RBSC_ERR_INVALID;
-            }
+    if (g->n_parts > 1) {
+        rbsc_set_error("rbsc_graph_download: graph has a partitioned line layout");
+        return RBSC_ERR_INVALID;
     }
-    if (dst && g->E) {
-        if (g->dst) HIP_TRY(hipMemcpy(dst, g->dst, sizeof(u32) * g->E, hipMemcpyDeviceToHost));
-        else memcpy(dst, g->dst_host, sizeof(u32) * g->E);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const u64 N = g->N, U = g->U, E = g->E;
+    std::vector<u64> rp(U + 1);
+    std::vector<u32> pi(N), inv(N), d(E);
+    HIP_TRY(hipMemcpy(rp.data(), g->row_ptr, sizeof(u64) * (U + 1), hipMemcpyDeviceToHost));
+    if (N) HIP_TRY(hipMemcpy(pi.data(), g->pi, sizeof(u32) * N, hipMemcpyDeviceToHost));
+    if (E) {
+        if (g->dst) HIP_TRY(hipMemcpy(d.data(), g->dst, sizeof(u32) * E, hipMemcpyDeviceToHost));
+        else memcpy(d.data(), g->dst_host, sizeof(u32) * E);
+    }
+    for (u64 v = 0; v < N; ++v) inv[pi[v]] = (u32)v;
+    // Rows come back in original node order (the input is assumed sorted by
+    // source, as the generators produce it).
+    u64 o = 0;
+    if (row_offsets) row_offsets[0] = 0;
+    for (u64 v = 0; v < N; ++v) {
+        const u64 j = pi[v];
+        if (j < U) {
+            for (u64 p = rp[j]; p < rp[j + 1]; ++p, ++o)
+                if (dst) dst[o] = inv[d[p]];
+        }
+        if (row_offsets) row_offsets[v + 1] = o;
     }
     return RBSC_OK;
 }

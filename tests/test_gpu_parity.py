@@ -233,3 +233,73 @@ def test_cli_batch_modes(tmp_path):
     assert p.returncode == 0, p.stderr.decode()
     lines = p.stdout.decode().split("\n")
     assert (int(lines[2]), int(lines[1])) == want[:2]
+
+
+@pytest.mark.parametrize("parts", [2, 4, 8, 16])
+def test_destination_partitioned_layout(ctx, parts):
+    """Lines regrouped by head mod P (padding lines, segment rows, carries)
+    give the same per-round colours."""
+    for g in (gen.rmat(14, 1 << 18, clone_frac=0.2, seed=15),
+              _weighted(gen.rmat(12, 1 << 16, clone_frac=0.3, seed=3), 5),
+              gen.ba(20000, 4, 0.1, 3, seed=9)):
+        c, K, R, tr = _oracle_full(g, trace_rounds=64)
+        dg = ctx.upload(g)
+        dg.relayout(parts)
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True)
+            assert np.array_equal(res.colors, tr[r - 1]), f"P={parts} round {r}"
+        full = dg.refine()
+        assert (full.nodes, full.rounds) == (K, R)
+        assert np.array_equal(full.partition, oracle.canonical(c))
+        dg.free()
+
+
+@pytest.mark.parametrize("parts", [1, 4])
+def test_hashed64_per_node_hashing(ctx, parts):
+    """Per-node hashed colours (instead of per-line hashing of raw colours)
+    give the same colours."""
+    for g in (gen.rmat(14, 1 << 18, clone_frac=0.2, seed=15),
+              _weighted(gen.rmat(12, 1 << 16, clone_frac=0.3, seed=3), 5),
+              gen.ba(20000, 4, 0.1, 3, seed=9)):
+        c, K, R, tr = _oracle_full(g, trace_rounds=64)
+        dg = ctx.upload(g)
+        if parts > 1:
+            dg.relayout(parts)
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True, flags=rbsc_amd.FLAG_HASHED64)
+            assert np.array_equal(res.colors, tr[r - 1]), f"round {r}"
+        dg.free()
+    r = ctx.refine(gen.rmat(13, 1 << 17, clone_frac=0.2, seed=8), max_batch_edges=30000,
+                   flags=rbsc_amd.FLAG_HASHED64)
+    g = gen.rmat(13, 1 << 17, clone_frac=0.2, seed=8)
+    c, K, R, _ = _oracle_full(g)
+    assert (r.nodes, r.rounds) == (K, R) and np.array_equal(r.partition, oracle.canonical(c))
+
+
+@pytest.mark.parametrize("flags", [rbsc_amd.FLAG_XCD, rbsc_amd.FLAG_XCD | rbsc_amd.FLAG_HASHED64])
+def test_xcd_partitioned_single_launch(ctx, flags):
+    """Eight destination partitions served concurrently (atomic row sums)."""
+    for g in (gen.rmat(14, 1 << 18, clone_frac=0.2, seed=15),
+              _weighted(gen.rmat(12, 1 << 16, clone_frac=0.3, seed=3), 5),
+              gen.ba(20000, 4, 0.1, 3, seed=9)):
+        c, K, R, tr = _oracle_full(g, trace_rounds=64)
+        dg = ctx.upload(g)
+        dg.relayout(8)
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True, flags=flags)
+            assert np.array_equal(res.colors, tr[r - 1]), f"round {r}"
+        dg.free()
+
+
+@pytest.mark.parametrize("flags", [0, 4, 4 | 8, 4 | 8 | 16, 4 | 16])
+def test_sort_split_and_tie_fixup(ctx, flags):
+    """Full sort (default), top-half sort + run fix-up (4), frequent ties via a
+    16-bit split (8), and the full-sort fallback forced by tiny runs (16)."""
+    for g in (gen.rmat(15, 1 << 19, clone_frac=0.2, seed=15),
+              gen.ba(30000, 3, 0.1, 3, seed=9)):
+        c, K, R, tr = _oracle_full(g, trace_rounds=64)
+        dg = ctx.upload(g)
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True, flags=flags)
+            assert np.array_equal(res.colors, tr[r - 1]), f"flags={flags} round {r}"
+        dg.free()
