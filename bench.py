@@ -52,6 +52,51 @@ def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
                       f"{R} rounds, {K} blocks, {dt:.2f} s"}
 
 
+def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
+    """Many small graphs: one persistent launch over a resident batch vs the
+    single-graph path called once per graph (on a sample of the batch)."""
+    import numpy as np
+    import rbsc_amd
+    from rbsc_amd import gen
+    t0 = time.perf_counter()
+    hb = gen.batch_mixed(n_graphs, 10, 5000, seed=seed)
+    db = ctx.upload_batch(hb)
+    log(f"[bench] batch: {n_graphs} graphs, {hb.n_nodes} nodes, {len(hb.dst)} lines, "
+        f"built and uploaded in {time.perf_counter() - t0:.1f}s")
+    threads, wgs, tune_ms = db.autotune()
+    log(f"[bench] batch autotune: {threads} threads x {wgs} workgroups ({tune_ms:.1f} ms)")
+    db.refine(labels=True)
+    reps = 5
+    t = time.perf_counter()
+    dev = 0.0
+    for _ in range(reps):
+        r = db.refine(labels=True)
+        dev += r.total.device_ms
+    wall = (time.perf_counter() - t) / reps
+    blocks = int(r.blocks.sum())
+    db.free()
+    # per-graph loop on the first loop_sample graphs (resident single-graph path)
+    idx = np.linspace(0, n_graphs - 1, loop_sample).astype(int)
+    dgs = [ctx.upload(hb.graph(int(i))) for i in idx]
+    for g in dgs[:4]:
+        g.refine(partition=True)
+    t = time.perf_counter()
+    for g in dgs:
+        g.refine(partition=True)
+    loop_s = time.perf_counter() - t
+    for g in dgs:
+        g.free()
+    loop_rate = loop_sample / loop_s
+    return {"workload": f"batch_mixed_{n_graphs}_n10_5000",
+            "graphs": n_graphs, "nodes": hb.n_nodes, "lines": int(len(hb.dst)),
+            "graphs_per_s": n_graphs / wall, "ms_per_batch": wall * 1e3,
+            "device_ms_per_batch": dev / reps, "blocks": blocks,
+            "launch": {"threads": threads, "workgroups": wgs},
+            "per_graph_loop_graphs_per_s": loop_rate,
+            "per_graph_loop_sample": loop_sample,
+            "speedup_vs_per_graph_loop": (n_graphs / wall) / loop_rate}
+
+
 def copy_probe(nbytes: int) -> None:
     """One plain device-to-device copy of nbytes through the HIP runtime, so a
     counter run can read FETCH_SIZE / WRITE_SIZE of a kernel of known size."""
@@ -82,6 +127,8 @@ def main():
                     help="gather per-node hashed colours instead of raw u32 colours")
     ap.add_argument("--xcd", action="store_true",
                     help="with --parts 8: one launch, partition b mod 8 per workgroup b")
+    ap.add_argument("--batch-graphs", type=int, default=50000,
+                    help="graphs in the many-small-graphs leg (0 skips it)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -154,6 +201,10 @@ def main():
                      "phase_ms_per_round": {k: phases[k][0] / rounds for k in phases if k != "setup"},
                      "dominant_phase": dom},
     }
+    if args.batch_graphs:
+        out["batched"] = batch_leg(ctx, args.batch_graphs, 1004, 500)
+        log(f"[bench] batched: {out['batched']['graphs_per_s']:.0f} graphs/s "
+            f"(per-graph loop {out['batched']['per_graph_loop_graphs_per_s']:.0f} graphs/s)")
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_scale + 4, 0.20, seed)

@@ -162,6 +162,44 @@ int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts);
 int rbsc_ctx_profile(rbsc_ctx* ctx, int enable);
 int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int reset);
 
+/* ---- Batches of independent graphs (no counterpart in the reference, which
+ * refines one graph per process).  Graph g owns nodes
+ * [node_offsets[g], node_offsets[g+1]) and lines
+ * [line_offsets[g], line_offsets[g+1]); src and dst are local node ids
+ * (0 .. n_g - 1) and a graph's lines are grouped by source, as for
+ * rbsc_graph.  init_color (nullable) is indexed by batch node.  Graphs of at
+ * most 5120 nodes are refined inside one workgroup's LDS, many graphs per
+ * persistent launch; larger ones go through the single-graph path.  Each
+ * graph gets exactly the colours, rounds and blocks it gets alone. */
+typedef struct rbsc_batch {
+    uint32_t n_graphs;
+    const uint64_t* node_offsets;
+    const uint64_t* line_offsets;
+    const uint32_t* src;
+    const uint32_t* dst;
+    const uint64_t* weight;
+    const uint32_t* init_color;
+} rbsc_batch;
+typedef struct rbsc_dbatch rbsc_dbatch;
+
+int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* batch, rbsc_dbatch** out);
+int rbsc_batch_free(rbsc_ctx* ctx, rbsc_dbatch* b);
+/* blocks_out, rounds_out: per graph; labels_out: first-occurrence labels per
+ * batch node, canonical within each graph; colors_out: raw final colours per
+ * batch node.  All nullable.  total: sums blocks, max rounds, times. */
+int rbsc_batch_refine(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, uint32_t* blocks_out,
+                      uint32_t* rounds_out, uint32_t* labels_out, uint32_t* colors_out,
+                      rbsc_result* total);
+/* Times the persistent launch at workgroup widths 256/512/1024 and 1/2/4
+ * workgroups per CU on this batch and keeps the fastest for later refines. */
+int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* threads_out, uint32_t* wgs_out,
+                        double* best_ms_out);
+int rbsc_batch_set_launch(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t threads, uint32_t wgs);
+/* One-shot: upload, refine, free. */
+int rbsc_refine_batch(rbsc_ctx* ctx, const rbsc_batch* batch, const rbsc_opts* opts,
+                      uint32_t* blocks_out, uint32_t* rounds_out, uint32_t* labels_out,
+                      rbsc_result* total);
+
 /* Seeded R-MAT generated on the device (bench-scale inputs without a host
  * round trip).  2^scale nodes, n_draws lines drawn with probabilities a, b, c
  * (d = 1 - a - b - c), node ids permuted by a seeded bijection, then

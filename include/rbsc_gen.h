@@ -50,6 +50,23 @@ int rbsc_gen_er_undirected(uint32_t n, double avg_deg, uint64_t seed, rbsc_host_
 int rbsc_gen_ba_undirected(uint32_t n, uint32_t m, uint64_t seed, rbsc_host_graph* out);
 int rbsc_gen_regular_undirected(uint32_t n, uint32_t k, uint64_t seed, rbsc_host_graph* out);
 
+/* A host batch allocated by the library (see rbsc_batch in rbsc.h). */
+typedef struct rbsc_host_batch {
+    uint32_t n_graphs;
+    uint64_t* node_offsets;  /* n_graphs + 1 */
+    uint64_t* line_offsets;  /* n_graphs + 1 */
+    uint32_t* src;           /* local ids */
+    uint32_t* dst;           /* local ids */
+} rbsc_host_batch;
+void rbsc_host_batch_free(rbsc_host_batch* b);
+
+/* n_graphs undirected graphs, node counts log-uniform in [n_min, n_max];
+ * graph i is Erdos-Renyi (average degree uniform in [2, 8]) when i % 3 == 0,
+ * Barabasi-Albert (m uniform in {1..4}) when i % 3 == 1, random k-regular
+ * (k in {3, 4}) when i % 3 == 2.  Each from its own seed. */
+int rbsc_gen_batch_mixed(uint32_t n_graphs, uint32_t n_min, uint32_t n_max, uint64_t seed,
+                         rbsc_host_batch* out);
+
 #ifdef __cplusplus
 }
 #endif

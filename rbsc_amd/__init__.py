@@ -26,7 +26,8 @@ from . import _lib
 from ._lib import (FLAG_HASHED64, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
                    RbscError, check, load)
 
-__all__ = ["Context", "DeviceGraph", "HostGraph", "RefineResult", "RbscError", "load",
+__all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
+           "RefineResult", "RbscError", "load",
            "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "gen"]
 
 _MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
@@ -202,6 +203,130 @@ class DeviceGraph:
             pass
 
 
+@dataclass
+class HostBatch:
+    """Independent graphs in one set of arrays: graph g owns nodes
+    [node_offsets[g], node_offsets[g+1]) and lines [line_offsets[g],
+    line_offsets[g+1]); src/dst are local ids; lines grouped by source."""
+    node_offsets: np.ndarray
+    line_offsets: np.ndarray
+    src: np.ndarray
+    dst: np.ndarray
+    weight: Optional[np.ndarray] = None
+    init_color: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        self.node_offsets = np.ascontiguousarray(self.node_offsets, np.uint64)
+        self.line_offsets = np.ascontiguousarray(self.line_offsets, np.uint64)
+        self.src = np.ascontiguousarray(self.src, np.uint32)
+        self.dst = np.ascontiguousarray(self.dst, np.uint32)
+        if self.weight is not None:
+            self.weight = np.ascontiguousarray(self.weight, np.uint64)
+        if self.init_color is not None:
+            self.init_color = np.ascontiguousarray(self.init_color, np.uint32)
+
+    @property
+    def n_graphs(self) -> int:
+        return len(self.node_offsets) - 1
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.node_offsets[-1])
+
+    def graph(self, g: int) -> HostGraph:
+        a, b = int(self.line_offsets[g]), int(self.line_offsets[g + 1])
+        na, nb = int(self.node_offsets[g]), int(self.node_offsets[g + 1])
+        return HostGraph(nb - na, self.dst[a:b], src=self.src[a:b],
+                         weight=None if self.weight is None else self.weight[a:b],
+                         init_color=None if self.init_color is None else self.init_color[na:nb])
+
+    @staticmethod
+    def from_graphs(graphs) -> "HostBatch":
+        no, lo, src, dst, w, ic = [0], [0], [], [], [], []
+        for g in graphs:
+            src.append(g.sources())
+            dst.append(g.dst)
+            w.append(g.weight)
+            ic.append(g.init_color)
+            no.append(no[-1] + g.n_nodes)
+            lo.append(lo[-1] + g.n_edges)
+        weight = None if all(x is None for x in w) else np.concatenate(
+            [x if x is not None else np.ones(len(d), np.uint64) for x, d in zip(w, dst)])
+        if any(x is None for x in ic) and not all(x is None for x in ic):
+            raise ValueError("initial colours must be given for every graph of a batch or none")
+        init = None if not ic or ic[0] is None else np.concatenate(ic)
+        return HostBatch(np.array(no), np.array(lo),
+                         np.concatenate(src) if src else np.zeros(0, np.uint32),
+                         np.concatenate(dst) if dst else np.zeros(0, np.uint32), weight, init)
+
+    def as_struct(self) -> _lib.Batch:
+        b = _lib.Batch()
+        b.n_graphs = self.n_graphs
+        b.node_offsets = _ptr(self.node_offsets, C.c_uint64)
+        b.line_offsets = _ptr(self.line_offsets, C.c_uint64)
+        b.src = _ptr(self.src, C.c_uint32)
+        b.dst = _ptr(self.dst, C.c_uint32)
+        b.weight = _ptr(self.weight, C.c_uint64)
+        b.init_color = _ptr(self.init_color, C.c_uint32)
+        return b
+
+
+@dataclass
+class BatchResult:
+    blocks: np.ndarray
+    rounds: np.ndarray
+    labels: Optional[np.ndarray]
+    colors: Optional[np.ndarray]
+    total: RefineResult
+
+
+class DeviceBatch:
+    """A batch resident in HBM (``rbsc_dbatch``)."""
+
+    def __init__(self, ctx: "Context", handle, host: HostBatch):
+        self.ctx = ctx
+        self.handle = handle
+        self.n_graphs = host.n_graphs
+        self.n_nodes = host.n_nodes
+
+    def refine(self, max_rounds=0, labels=True, colors=False) -> BatchResult:
+        blocks = np.empty(max(self.n_graphs, 1), np.uint32)
+        rounds = np.empty(max(self.n_graphs, 1), np.uint32)
+        lab = np.empty(max(self.n_nodes, 1), np.uint32) if labels else None
+        col = np.empty(max(self.n_nodes, 1), np.uint32) if colors else None
+        r = _lib.Result()
+        o = _opts(max_rounds=max_rounds)
+        check(load().rbsc_batch_refine(self.ctx.handle, self.handle, C.byref(o),
+                                       _ptr(blocks, C.c_uint32), _ptr(rounds, C.c_uint32),
+                                       _ptr(lab, C.c_uint32), _ptr(col, C.c_uint32), C.byref(r)))
+        n, g = self.n_nodes, self.n_graphs
+        return BatchResult(blocks[:g], rounds[:g], None if lab is None else lab[:n],
+                           None if col is None else col[:n], _result(r))
+
+    def autotune(self):
+        t = C.c_uint32()
+        w = C.c_uint32()
+        ms = C.c_double()
+        check(load().rbsc_batch_autotune(self.ctx.handle, self.handle, C.byref(t), C.byref(w),
+                                         C.byref(ms)))
+        return int(t.value), int(w.value), float(ms.value)
+
+    def set_launch(self, threads: int, wgs: int):
+        check(load().rbsc_batch_set_launch(self.ctx.handle, self.handle, threads, wgs))
+
+    def free(self):
+        if self.handle:
+            load().rbsc_batch_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """One HIP device context (``rbsc_ctx``)."""
 
@@ -254,6 +379,12 @@ class Context:
         h = C.c_void_p()
         check(load().rbsc_graph_upload(self.handle, C.byref(st), C.byref(h)))
         return DeviceGraph(self, h)
+
+    def upload_batch(self, b: HostBatch) -> DeviceBatch:
+        st = b.as_struct()
+        h = C.c_void_p()
+        check(load().rbsc_batch_upload(self.handle, C.byref(st), C.byref(h)))
+        return DeviceBatch(self, h, b)
 
     def gen_rmat(self, scale, n_draws, a=0.57, b=0.19, c=0.19, clone_frac=0.0,
                  seed=1) -> DeviceGraph:

@@ -80,6 +80,28 @@ class HostGraph(C.Structure):
     ]
 
 
+class Batch(C.Structure):
+    _fields_ = [
+        ("n_graphs", C.c_uint32),
+        ("node_offsets", u64p),
+        ("line_offsets", u64p),
+        ("src", u32p),
+        ("dst", u32p),
+        ("weight", u64p),
+        ("init_color", u32p),
+    ]
+
+
+class HostBatch(C.Structure):
+    _fields_ = [
+        ("n_graphs", C.c_uint32),
+        ("node_offsets", u64p),
+        ("line_offsets", u64p),
+        ("src", u32p),
+        ("dst", u32p),
+    ]
+
+
 # Every exported symbol of the C headers, with (restype, argtypes).
 SIGNATURES = {
     "rbsc_last_error": (C.c_char_p, []),
@@ -95,6 +117,17 @@ SIGNATURES = {
     "rbsc_graph_lines_to_host": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_ctx_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), u64p, C.c_int]),
+    "rbsc_batch_upload": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(C.c_void_p)]),
+    "rbsc_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "rbsc_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), u32p, u32p, u32p, u32p,
+                                    C.POINTER(Result)]),
+    "rbsc_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, u32p, u32p, C.POINTER(C.c_double)]),
+    "rbsc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "rbsc_refine_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Opts), u32p, u32p, u32p,
+                                    C.POINTER(Result)]),
+    "rbsc_host_batch_free": (None, [C.POINTER(HostBatch)]),
+    "rbsc_gen_batch_mixed": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                       C.POINTER(HostBatch)]),
     "rbsc_gen_rmat_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double,
                                        C.c_double, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "rbsc_graph_dims": (C.c_int, [C.c_void_p, u32p, u64p]),

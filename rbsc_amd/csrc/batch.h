@@ -1,0 +1,28 @@
+// batch.h -- device layout of a batch of small graphs (batch.hip).
+#pragma once
+#include "common.h"
+
+// Largest graph (nodes) refined inside one workgroup's LDS.
+constexpr u32 BATCH_NMAX = 5120;
+
+struct BatchDesc {
+    u64 node_base;  // first node of the graph in the batch's node numbering
+    u64 line_base;  // first line in the small-graph line arrays
+    u64 rp_base;    // first local row offset (n + 1 entries)
+    u32 n;          // nodes
+    u32 pad;
+};
+
+struct BatchDevice {
+    BatchDesc* desc = nullptr;      // per small graph
+    u32* order = nullptr;           // small graphs, largest first
+    u32 n_small = 0;
+    u32* rp = nullptr;              // local row offsets
+    unsigned short* dst = nullptr;  // local heads
+    u64* wt = nullptr;              // or null
+    u32* init = nullptr;            // per node (batch numbering) or null
+};
+
+struct rbsc_ctx;
+int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
+                 u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg);

@@ -1,0 +1,664 @@
+// batch.hip -- many small graphs refined in one persistent launch.
+//
+// Each workgroup takes graphs from an atomic queue (largest first) and runs
+// the whole refinement of one graph inside LDS: colours, signatures, a
+// stable LDS radix sort of (signature) over positions kept in colour order,
+// the dense rank and the fixpoint test, then the first-occurrence canonical
+// labels.  Every graph gets exactly the colours the single-graph path gives
+// it alone (same round function, same dense ranks, same stop rule), so a
+// result is bit-identical alone and in a batch.  Graphs with more nodes than
+// fit in LDS go through the single-graph path instead (host side below).
+#include "internal.h"
+#include "batch.h"
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// Workgroup width is a template parameter (picked by the autotuner).
+template <int BT>
+struct Cfg {
+    static constexpr int BW = BT / 64;                           // waves
+    static constexpr int IPT = (BATCH_NMAX + BT - 1) / BT;       // items per thread (blocked)
+    static constexpr int IPW = (BATCH_NMAX + BW - 1) / BW;       // items per wave chunk
+    static constexpr int WIT = (IPW + 63) / 64;                  // 64-item iterations per wave
+};
+
+template <int BT>
+struct Lds {
+    u64 key[2][BATCH_NMAX];
+    u32 val[2][BATCH_NMAX];   // (dense colour << 16) | node
+    u32 col[BATCH_NMAX];      // colour used in the hash (exact c0 in round 0)
+    u32 whist[Cfg<BT>::BW][256];
+    u32 wsum[Cfg<BT>::BW];
+    u32 scal[8];
+};
+
+// Exclusive scan of one u32 per thread over the workgroup; returns the
+// exclusive prefix, total in *tot.  Uses s->wsum.
+template <int BT>
+__device__ u32 block_scan_excl(u32 v, Lds<BT>* s, u32* tot) {
+    constexpr int BW = Cfg<BT>::BW;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    u32 x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s->wsum[wave] = x;
+    __syncthreads();
+    u32 off = 0, t = 0;
+#pragma unroll
+    for (int w = 0; w < BW; ++w) {
+        if (w < wave) off += s->wsum[w];
+        t += s->wsum[w];
+    }
+    __syncthreads();
+    *tot = t;
+    return off + x - v;
+}
+
+// One stable LSD pass over n items by digit (key >> shift) & 255, from
+// buffer `in` to buffer 1 - in.  Wave w owns items [w*IPW, (w+1)*IPW).
+template <int BT>
+__device__ void lds_radix_pass(Lds<BT>* s, int in, int n, int shift) {
+    constexpr int BW = Cfg<BT>::BW, IPW = Cfg<BT>::IPW, WIT = Cfg<BT>::WIT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < BW * 256; i += BT) (&s->whist[0][0])[i] = 0;
+    __syncthreads();
+    const u64 below = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+    u32 rank[WIT];
+#pragma unroll
+    for (int it = 0; it < WIT; ++it) {
+        const int idx = wave * IPW + it * 64 + lane;
+        const bool valid = idx < n && (it * 64 + lane) < IPW;
+        const u32 d = valid ? (u32)(s->key[in][idx] >> shift) & 255u : 0u;
+        u64 peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const u64 bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const int leader = valid ? (__ffsll((long long)peers) - 1) : lane;
+        u32 old = 0;
+        if (valid && lane == leader) old = s->whist[wave][d];
+        old = __shfl(old, leader);
+        if (valid && lane == leader) s->whist[wave][d] = old + (u32)__popcll(peers);
+        rank[it] = old + (u32)__popcll(peers & below);
+    }
+    __syncthreads();
+    // digit-major offsets: for digit d, waves in order
+    u32 tot_d = 0;
+    if (threadIdx.x < 256) {
+        const int d = threadIdx.x;
+#pragma unroll
+        for (int w = 0; w < BW; ++w) {
+            const u32 c = s->whist[w][d];
+            s->whist[w][d] = tot_d;
+            tot_d += c;
+        }
+    }
+    u32 total;
+    const u32 base = block_scan_excl(threadIdx.x < 256 ? tot_d : 0u, s, &total);
+    if (threadIdx.x < 256) {
+        const int d = threadIdx.x;
+#pragma unroll
+        for (int w = 0; w < BW; ++w) s->whist[w][d] += base;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < WIT; ++it) {
+        const int idx = wave * IPW + it * 64 + lane;
+        if (idx < n && (it * 64 + lane) < IPW) {
+            const u64 k = s->key[in][idx];
+            const u32 d = (u32)(k >> shift) & 255u;
+            const u32 pos = s->whist[wave][d] + rank[it];
+            s->key[1 - in][pos] = k;
+            s->val[1 - in][pos] = s->val[in][idx];
+        }
+    }
+    __syncthreads();
+}
+
+template <int BT>
+__device__ void lds_sort(Lds<BT>* s, int n, int bit_begin, int bit_end) {
+    int in = 0;
+    for (int sh = bit_begin; sh < bit_end; sh += 8) {
+        lds_radix_pass(s, in, n, sh);
+        in ^= 1;
+    }
+    // callers use an even number of passes: result is back in buffer 0
+}
+
+template <int BT, bool HAS_WT, bool HAS_INIT>
+__global__ __launch_bounds__(BT) void batch_refine_kernel(
+    const BatchDesc* __restrict__ desc, const u32* __restrict__ order, u32 n_jobs,
+    const u32* __restrict__ rp_all, const unsigned short* __restrict__ dst_all,
+    const u64* __restrict__ wt_all, const u32* __restrict__ init_all, u32 max_rounds,
+    u32* __restrict__ labels_out, u32* __restrict__ colors_out, u32* __restrict__ blocks_out,
+    u32* __restrict__ rounds_out, u32* __restrict__ queue) {
+    constexpr int IPT = Cfg<BT>::IPT;
+    __shared__ Lds<BT> s;
+    // Every job runs the whole body: no `continue` back to the thread-0 queue
+    // fetch.  A branch that skips the body lets the compiler put the barrier
+    // after the fetch at the head of an inner loop that only thread 0 leaves,
+    // and the rest of wave 0 then waits there for ever.  Graphs without nodes
+    // are therefore answered on the host and never queued.
+    for (;;) {
+        if (threadIdx.x == 0) s.scal[0] = atomicAdd(queue, 1u);
+        __syncthreads();
+        const u32 job = s.scal[0];
+        __syncthreads();
+        if (job >= n_jobs) break;
+        const u32 g = order[job];
+        const BatchDesc d = desc[g];
+        const int n = (int)d.n;
+        const u32* rp = rp_all + d.rp_base;
+        const unsigned short* dst = dst_all + d.line_base;
+        const u64* wt = HAS_WT ? wt_all + d.line_base : nullptr;
+
+        // ---- round-0 colours, positions sorted by colour
+        if (HAS_INIT) {
+            const u32* ic = init_all + d.node_base;
+            for (int u = threadIdx.x; u < n; u += BT) {
+                s.col[u] = ic[u];
+                s.key[0][u] = ic[u];
+                s.val[0][u] = (u32)u;
+            }
+            __syncthreads();
+            lds_sort(&s, n, 0, 32);
+            // dense rank of the initial colours, in position order
+            u32 f[IPT];
+            u32 cnt = 0;
+            const int b0 = threadIdx.x * IPT;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = b0 + j;
+                f[j] = (i > 0 && i < n && s.key[0][i] != s.key[0][i - 1]) ? 1u : 0u;
+                cnt += f[j];
+            }
+            u32 tot;
+            u32 r = block_scan_excl(cnt, &s, &tot);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = b0 + j;
+                r += f[j];
+                if (i < n) s.val[0][i] = (r << 16) | (s.val[0][i] & 0xFFFFu);
+            }
+            __syncthreads();
+        } else {
+            // nodes without lines first (colour 0), then the rest (colour n+1)
+            u32 f[IPT];
+            u32 cnt = 0;
+            const int b0 = threadIdx.x * IPT;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int u = b0 + j;
+                f[j] = (u < n && rp[u + 1] == rp[u]) ? 1u : 0u;
+                cnt += f[j];
+            }
+            u32 ndang;
+            const u32 pd = block_scan_excl(cnt, &s, &ndang);
+            u32 qd = pd;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int u = b0 + j;
+                if (u < n) {
+                    const u32 pos = f[j] ? qd : ndang + ((u32)u - qd);
+                    qd += f[j];
+                    s.val[0][pos] = ((f[j] ? 0u : 1u) << 16) | (u32)u;
+                    s.col[u] = f[j] ? 0u : (u32)n + 1u;
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- rounds
+        u32 K = 1, Knew = 1, it = 0;
+        do {
+            K = Knew;
+            // signatures by node into key[1]
+            for (int u = threadIdx.x; u < n; u += BT) {
+                u64 acc = 0;
+                for (u32 e = rp[u]; e < rp[u + 1]; ++e) {
+                    u64 h = rbsc_randomize((u64)s.col[dst[e]] + it);
+                    if (HAS_WT) h *= wt[e];
+                    acc += h;
+                }
+                s.key[1][u] = acc;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += BT) s.key[0][i] = s.key[1][s.val[0][i] & 0xFFFFu];
+            __syncthreads();
+            lds_sort(&s, n, 0, 64);
+            // dense rank of (signature, colour)
+            u32 f[IPT];
+            u32 cnt = 0;
+            const int b0 = threadIdx.x * IPT;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = b0 + j;
+                f[j] = (i > 0 && i < n &&
+                        (s.key[0][i] != s.key[0][i - 1] || (s.val[0][i] >> 16) != (s.val[0][i - 1] >> 16)))
+                           ? 1u : 0u;
+                cnt += f[j];
+            }
+            u32 tot;
+            u32 r = block_scan_excl(cnt, &s, &tot);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = b0 + j;
+                r += f[j];
+                if (i < n) {
+                    const u32 node = s.val[0][i] & 0xFFFFu;
+                    s.val[0][i] = (r << 16) | node;
+                    s.col[node] = r;
+                }
+            }
+            Knew = n > 0 ? tot + 1 : 0;
+            ++it;
+            __syncthreads();
+        } while (K != Knew && !(max_rounds && it >= max_rounds));
+
+        // ---- outputs: raw colours, first-occurrence labels, counts
+        u32* first = reinterpret_cast<u32*>(&s.key[1][0]);
+        u32* lab = first + BATCH_NMAX;
+        for (int c = threadIdx.x; c < n; c += BT) first[c] = 0xFFFFFFFFu;
+        __syncthreads();
+        for (int u = threadIdx.x; u < n; u += BT) atomicMin(&first[s.col[u]], (u32)u);
+        __syncthreads();
+        {
+            u32 f[IPT];
+            u32 cnt = 0;
+            const int b0 = threadIdx.x * IPT;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int u = b0 + j;
+                f[j] = (u < n && first[s.col[u]] == (u32)u) ? 1u : 0u;
+                cnt += f[j];
+            }
+            u32 tot;
+            u32 r = block_scan_excl(cnt, &s, &tot);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int u = b0 + j;
+                if (u < n && f[j]) lab[s.col[u]] = r;
+                r += f[j];
+            }
+        }
+        __syncthreads();
+        for (int u = threadIdx.x; u < n; u += BT) {
+            if (labels_out) labels_out[d.node_base + u] = lab[s.col[u]];
+            if (colors_out) colors_out[d.node_base + u] = s.col[u];
+        }
+        if (threadIdx.x == 0) {
+            blocks_out[g] = Knew;
+            rounds_out[g] = it;
+        }
+        __syncthreads();
+    }
+}
+static_assert(sizeof(Lds<1024>) <= 160 * 1024, "batch LDS layout exceeds 160 KB");
+
+__global__ void zero_queue(u32* q) { *q = 0; }
+
+template <int BT>
+static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 max_rounds,
+                      u32* d_labels, u32* d_colors, u32* d_blocks, u32* d_rounds, u32* d_queue) {
+#define BL(W, I)                                                                        \
+    batch_refine_kernel<BT, W, I><<<grid, BT, 0, s>>>(b.desc, b.order, b.n_small, b.rp, b.dst, \
+                                                      b.wt, b.init, max_rounds, d_labels,       \
+                                                      d_colors, d_blocks, d_rounds, d_queue)
+    if (b.wt) {
+        if (b.init) BL(true, true); else BL(true, false);
+    } else {
+        if (b.init) BL(false, true); else BL(false, false);
+    }
+#undef BL
+}
+
+}  // namespace
+
+int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
+                 u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg) {
+    hipStream_t s = ctx->stream;
+    zero_queue<<<1, 1, 0, s>>>(d_queue);
+    if (b.n_small == 0) return RBSC_OK;
+    const unsigned grid = (unsigned)std::max(1, std::min<int>(n_wg, (int)b.n_small));
+    switch (threads) {
+        case 256: launch_bt<256>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue); break;
+        case 1024: launch_bt<1024>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue); break;
+        default: launch_bt<512>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return RBSC_OK;
+}
+
+// ------------------------------------------------------------------ host
+struct rbsc_dbatch {
+    u32 n_graphs = 0;
+    u64 total_nodes = 0, total_lines = 0;
+    std::vector<u64> node_off;          // n_graphs + 1
+    std::vector<u32> small, big, empty; // graph ids (empty: no nodes, never launched)
+    std::vector<rbsc_dgraph*> big_g;    // resident big graphs (same order as big)
+    BatchDevice dev;
+    u32 *d_labels = nullptr, *d_colors = nullptr, *d_blocks = nullptr, *d_rounds = nullptr,
+        *d_queue = nullptr;
+    int threads = 512, wgs = 0;
+    u64 bytes = 0;
+};
+
+static void dbatch_release(rbsc_ctx* ctx, rbsc_dbatch* b) {
+    for (rbsc_dgraph* g : b->big_g) rbsc_graph_free(ctx, g);
+    for (void* p : {(void*)b->dev.desc, (void*)b->dev.order, (void*)b->dev.rp, (void*)b->dev.dst,
+                    (void*)b->dev.wt, (void*)b->dev.init, (void*)b->d_labels, (void*)b->d_colors,
+                    (void*)b->d_blocks, (void*)b->d_rounds, (void*)b->d_queue})
+        if (p) (void)hipFree(p);
+}
+
+extern "C" int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* in, rbsc_dbatch** out) {
+    if (!ctx || !in || !out || !in->node_offsets || !in->line_offsets) {
+        rbsc_set_error("rbsc_batch_upload: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const u32 G = in->n_graphs;
+    const u64 NT = in->node_offsets[G], LT = in->line_offsets[G];
+    if (in->node_offsets[0] != 0 || in->line_offsets[0] != 0 || (LT && (!in->src || !in->dst))) {
+        rbsc_set_error("rbsc_batch_upload: offsets must start at 0 and lines need src and dst");
+        return RBSC_ERR_INVALID;
+    }
+    for (u32 g = 0; g < G; ++g)
+        if (in->node_offsets[g + 1] < in->node_offsets[g] || in->line_offsets[g + 1] < in->line_offsets[g] ||
+            in->node_offsets[g + 1] - in->node_offsets[g] >= 0xFFFFFFFFull) {
+            rbsc_set_error("rbsc_batch_upload: offsets of graph %u not monotone", g);
+            return RBSC_ERR_INVALID;
+        }
+    rbsc_dbatch* b = new rbsc_dbatch();
+    b->n_graphs = G;
+    b->total_nodes = NT;
+    b->total_lines = LT;
+    b->node_off.assign(in->node_offsets, in->node_offsets + G + 1);
+    // Small graphs get compacted row-offset and line arrays (line_base and
+    // rp_base count small graphs only); big graphs keep their own uploads.
+    std::vector<BatchDesc> desc(G);
+    u64 small_rp = 0, small_lines = 0;
+    for (u32 g = 0; g < G; ++g) {
+        const u64 n = in->node_offsets[g + 1] - in->node_offsets[g];
+        desc[g].node_base = in->node_offsets[g];
+        desc[g].n = (u32)n;
+        if (n == 0) {
+            b->empty.push_back(g);  // no nodes: 0 blocks, 0 rounds, set on the host
+        } else if (n <= BATCH_NMAX) {
+            desc[g].line_base = small_lines;
+            desc[g].rp_base = small_rp;
+            small_lines += in->line_offsets[g + 1] - in->line_offsets[g];
+            small_rp += n + 1;
+            b->small.push_back(g);
+        } else {
+            b->big.push_back(g);
+        }
+    }
+    std::vector<u32> rp(std::max<u64>(small_rp, 1), 0);
+    std::vector<unsigned short> dst16(std::max<u64>(small_lines, 1));
+    std::vector<u64> wt(in->weight ? small_lines : 0);
+    // Validate and build local CSR of every small graph (parallel over graphs).
+    std::vector<int> bad(G, 0);
+    {
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                std::vector<u32> cnt;
+                std::vector<unsigned char> seen;
+                for (u64 gi = t; gi < G; gi += nt) {
+                    const u32 g = (u32)gi;
+                    const u32 n = desc[g].n;
+                    if (n > BATCH_NMAX) continue;  // validated by rbsc_graph_upload
+                    const u64 a = in->line_offsets[g], z = in->line_offsets[g + 1];
+                    cnt.assign(n + 1, 0);
+                    seen.assign(n, 0);
+                    for (u64 e = a; e < z; ++e) {
+                        const u32 sv = in->src[e], dv = in->dst[e];
+                        if (sv >= n || dv >= n) { bad[g] = 1; break; }
+                        if (e == a || in->src[e - 1] != sv) {
+                            if (seen[sv]) { bad[g] = 2; break; }
+                            seen[sv] = 1;
+                        }
+                        cnt[sv + 1]++;
+                    }
+                    if (bad[g]) continue;
+                    u32* r = rp.data() + desc[g].rp_base;
+                    r[0] = 0;
+                    for (u32 u = 0; u < n; ++u) r[u + 1] = r[u] + cnt[u + 1];
+                    const u64 lb = desc[g].line_base;
+                    std::vector<u32> pos(r, r + n);
+                    for (u64 e = a; e < z; ++e) {
+                        const u32 o = pos[in->src[e]]++;
+                        dst16[lb + o] = (unsigned short)in->dst[e];
+                        if (in->weight) wt[lb + o] = in->weight[e];
+                    }
+                }
+            });
+        for (auto& x : th) x.join();
+    }
+    for (u32 g = 0; g < G; ++g)
+        if (bad[g]) {
+            rbsc_set_error("graph %u of the batch: %s", g,
+                           bad[g] == 1 ? "node id out of range" : "lines of a source are not grouped");
+            delete b;
+            return RBSC_ERR_INPUT;
+        }
+    std::sort(b->small.begin(), b->small.end(),
+              [&](u32 x, u32 y) { return desc[x].n > desc[y].n || (desc[x].n == desc[y].n && x < y); });
+    // Big graphs go through the single-graph path.
+    for (u32 g : b->big) {
+        rbsc_graph one{};
+        one.n_nodes = desc[g].n;
+        one.n_edges = in->line_offsets[g + 1] - in->line_offsets[g];
+        one.src = in->src + in->line_offsets[g];
+        one.dst = in->dst + in->line_offsets[g];
+        one.weight = in->weight ? in->weight + in->line_offsets[g] : nullptr;
+        one.init_color = in->init_color ? in->init_color + in->node_offsets[g] : nullptr;
+        rbsc_dgraph* dg = nullptr;
+        const int rc = rbsc_graph_upload(ctx, &one, &dg);
+        if (rc) {
+            dbatch_release(ctx, b);
+            delete b;
+            return rc;
+        }
+        b->big_g.push_back(dg);
+    }
+    hipStream_t s = ctx->stream;
+    auto up = [&](void** dptr, const void* src, size_t bytes) -> int {
+        HIP_TRY(hipMalloc(dptr, std::max<size_t>(bytes, 8)));
+        if (bytes && src) HIP_TRY(hipMemcpyAsync(*dptr, src, bytes, hipMemcpyHostToDevice, s));
+        b->bytes += bytes;
+        return RBSC_OK;
+    };
+    int rc = RBSC_OK;
+    if (!rc) rc = up((void**)&b->dev.desc, desc.data(), sizeof(BatchDesc) * G);
+    if (!rc) rc = up((void**)&b->dev.order, b->small.data(), sizeof(u32) * b->small.size());
+    if (!rc) rc = up((void**)&b->dev.rp, rp.data(), sizeof(u32) * small_rp);
+    if (!rc) rc = up((void**)&b->dev.dst, dst16.data(), sizeof(unsigned short) * small_lines);
+    if (!rc && in->weight) rc = up((void**)&b->dev.wt, wt.data(), sizeof(u64) * small_lines);
+    if (!rc && in->init_color) rc = up((void**)&b->dev.init, in->init_color, sizeof(u32) * NT);
+    b->dev.n_small = (u32)b->small.size();
+    if (!rc) rc = up((void**)&b->d_labels, nullptr, sizeof(u32) * NT);
+    if (!rc) rc = up((void**)&b->d_colors, nullptr, sizeof(u32) * NT);
+    if (!rc) rc = up((void**)&b->d_blocks, nullptr, sizeof(u32) * G);
+    if (!rc) rc = up((void**)&b->d_rounds, nullptr, sizeof(u32) * G);
+    if (!rc) rc = up((void**)&b->d_queue, nullptr, sizeof(u32));
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) {
+        rbsc_set_error("batch upload failed");
+        rc = RBSC_ERR_HIP;
+    }
+    if (rc) {
+        dbatch_release(ctx, b);
+        delete b;
+        return rc;
+    }
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    b->wgs = cus;
+    *out = b;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_batch_free(rbsc_ctx* ctx, rbsc_dbatch* b) {
+    if (!b) return RBSC_OK;
+    if (ctx) (void)hipSetDevice(ctx->device);
+    dbatch_release(ctx, b);
+    delete b;
+    return RBSC_OK;
+}
+
+static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* blocks_out,
+                     u32* rounds_out, u32* labels_out, u32* colors_out, rbsc_result* total,
+                     int threads, int wgs) {
+    hipStream_t s = ctx->stream;
+    const u32 max_rounds = opts ? opts->max_rounds : 0;
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    int rc = batch_launch(ctx, b->dev, max_rounds, b->d_labels, b->d_colors, b->d_blocks,
+                          b->d_rounds, b->d_queue, threads, wgs);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev1, s));
+    const u32 G = b->n_graphs;
+    std::vector<u32> blocks(G, 0), rounds(G, 0);
+    HIP_TRY(hipMemcpyAsync(blocks.data(), b->d_blocks, sizeof(u32) * G, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(rounds.data(), b->d_rounds, sizeof(u32) * G, hipMemcpyDeviceToHost, s));
+    if (labels_out)
+        HIP_TRY(hipMemcpyAsync(labels_out, b->d_labels, sizeof(u32) * b->total_nodes,
+                               hipMemcpyDeviceToHost, s));
+    if (colors_out)
+        HIP_TRY(hipMemcpyAsync(colors_out, b->d_colors, sizeof(u32) * b->total_nodes,
+                               hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    double dev_ms = ms;
+    for (u32 g : b->empty) blocks[g] = rounds[g] = 0;
+    // graphs too large for LDS: the single-graph path, one by one
+    for (size_t i = 0; i < b->big.size(); ++i) {
+        const u32 g = b->big[i];
+        rbsc_dgraph* dg = b->big_g[i];
+        RefineParams prm;
+        prm.max_rounds = max_rounds;
+        RefineOut o;
+        rc = refine_core(ctx, dg, prm, &o);
+        if (rc) return rc;
+        dev_ms += o.device_ms;
+        blocks[g] = o.K;
+        rounds[g] = o.rounds;
+        if (labels_out || colors_out) {
+            std::vector<u32> col(dg->N);
+            rc = download_colors(ctx, dg, col.data());
+            if (rc) return rc;
+            if (colors_out) memcpy(colors_out + b->node_off[g], col.data(), sizeof(u32) * dg->N);
+            if (labels_out) canonicalize_host(dg->N, col.data(), labels_out + b->node_off[g]);
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (blocks_out) memcpy(blocks_out, blocks.data(), sizeof(u32) * G);
+    if (rounds_out) memcpy(rounds_out, rounds.data(), sizeof(u32) * G);
+    if (total) {
+        rbsc_result r{};
+        for (u32 g = 0; g < G; ++g) {
+            r.nodes += blocks[g];
+            r.rounds = std::max(r.rounds, rounds[g]);
+        }
+        r.done = 1;
+        r.passes = 1;
+        r.device_ms = dev_ms;
+        r.time_s = std::chrono::duration<double>(t1 - t0).count();
+        r.edges_per_round = b->total_lines;
+        *total = r;
+    }
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_batch_refine(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts,
+                                 uint32_t* blocks_out, uint32_t* rounds_out,
+                                 uint32_t* labels_out, uint32_t* colors_out,
+                                 rbsc_result* total) {
+    if (!ctx || !b) {
+        rbsc_set_error("rbsc_batch_refine: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    return batch_run(ctx, b, opts, blocks_out, rounds_out, labels_out, colors_out, total,
+                     b->threads, b->wgs);
+}
+
+extern "C" int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* threads_out,
+                                   uint32_t* wgs_out, double* best_ms_out) {
+    if (!ctx || !b) {
+        rbsc_set_error("rbsc_batch_autotune: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    double best = 1e30;
+    int bt = b->threads, bw = b->wgs;
+    for (int threads : {256, 512, 1024}) {
+        for (int mult : {1, 2, 4}) {
+            const int wgs = cus * mult;
+            double ms_sum = 0;
+            for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
+                HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+                int rc = batch_launch(ctx, b->dev, 0, b->d_labels, nullptr, b->d_blocks,
+                                      b->d_rounds, b->d_queue, threads, wgs);
+                if (rc) return rc;
+                HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+                HIP_TRY(hipEventSynchronize(ctx->ev1));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                if (rep) ms_sum += ms;
+            }
+            if (ms_sum < best) {
+                best = ms_sum;
+                bt = threads;
+                bw = wgs;
+            }
+        }
+    }
+    b->threads = bt;
+    b->wgs = bw;
+    if (threads_out) *threads_out = (uint32_t)bt;
+    if (wgs_out) *wgs_out = (uint32_t)bw;
+    if (best_ms_out) *best_ms_out = best;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_batch_set_launch(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t threads,
+                                     uint32_t wgs) {
+    if (!ctx || !b || (threads != 256 && threads != 512 && threads != 1024) || wgs == 0) {
+        rbsc_set_error("rbsc_batch_set_launch: threads must be 256, 512 or 1024 and wgs > 0");
+        return RBSC_ERR_INVALID;
+    }
+    b->threads = (int)threads;
+    b->wgs = (int)wgs;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_refine_batch(rbsc_ctx* ctx, const rbsc_batch* batch, const rbsc_opts* opts,
+                                 uint32_t* blocks_out, uint32_t* rounds_out,
+                                 uint32_t* labels_out, rbsc_result* total) {
+    rbsc_dbatch* b = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = rbsc_batch_upload(ctx, batch, &b);
+    if (rc) return rc;
+    rc = rbsc_batch_refine(ctx, b, opts, blocks_out, rounds_out, labels_out, nullptr, total);
+    rbsc_batch_free(ctx, b);
+    if (!rc && total)
+        total->time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}

@@ -2,6 +2,7 @@
 #include "gen.h"
 #include "../../include/rbsc_gen.h"
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -305,4 +306,86 @@ extern "C" int rbsc_gen_regular_undirected(uint32_t n, uint32_t k, uint64_t seed
     }
     sort_keys(keys);
     return csr_from_keys(n, keys, out);
+}
+
+extern "C" void rbsc_host_batch_free(rbsc_host_batch* b) {
+    if (!b) return;
+    free(b->node_offsets);
+    free(b->line_offsets);
+    free(b->src);
+    free(b->dst);
+    memset(b, 0, sizeof(*b));
+}
+
+extern "C" int rbsc_gen_batch_mixed(uint32_t n_graphs, uint32_t n_min, uint32_t n_max,
+                                    uint64_t seed, rbsc_host_batch* out) {
+    if (!out || n_min < 5 || n_max < n_min) {
+        rbsc_set_error("rbsc_gen_batch_mixed: need 5 <= n_min <= n_max");
+        return RBSC_ERR_INVALID;
+    }
+    std::vector<rbsc_host_graph> gs(n_graphs);
+    std::vector<int> rcs(n_graphs, 0);
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (uint64_t i = t; i < n_graphs; i += nt) {
+                Rng r(seed * 0x9e3779b97f4a7c15ULL + i);
+                const double lo = log((double)n_min), hi = log((double)n_max);
+                uint32_t n = (uint32_t)floor(exp(lo + (hi - lo) * r.unit()) + 0.5);
+                n = std::min(std::max(n, n_min), n_max);
+                const uint64_t gseed = r.next();
+                memset(&gs[i], 0, sizeof(gs[i]));
+                if (i % 3 == 0) {
+                    rcs[i] = rbsc_gen_er_undirected(n, 2.0 + 6.0 * r.unit(), gseed, &gs[i]);
+                } else if (i % 3 == 1) {
+                    const uint32_t m = 1 + (uint32_t)r.below(4);
+                    rcs[i] = rbsc_gen_ba_undirected(std::max(n, m + 1), m, gseed, &gs[i]);
+                } else {
+                    uint32_t k = 3 + (uint32_t)r.below(2);
+                    if ((uint64_t)n * k % 2) k = 4;
+                    rcs[i] = rbsc_gen_regular_undirected(n, k, gseed, &gs[i]);
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    for (uint32_t i = 0; i < n_graphs; ++i)
+        if (rcs[i]) {
+            for (auto& g : gs) rbsc_host_graph_free(&g);
+            return rcs[i];
+        }
+    memset(out, 0, sizeof(*out));
+    out->n_graphs = n_graphs;
+    out->node_offsets = (uint64_t*)malloc(sizeof(uint64_t) * (n_graphs + 1));
+    out->line_offsets = (uint64_t*)malloc(sizeof(uint64_t) * (n_graphs + 1));
+    out->node_offsets[0] = out->line_offsets[0] = 0;
+    for (uint32_t i = 0; i < n_graphs; ++i) {
+        out->node_offsets[i + 1] = out->node_offsets[i] + gs[i].n_nodes;
+        out->line_offsets[i + 1] = out->line_offsets[i] + gs[i].n_edges;
+    }
+    const uint64_t L = out->line_offsets[n_graphs];
+    out->src = (uint32_t*)malloc(sizeof(uint32_t) * (L ? L : 1));
+    out->dst = (uint32_t*)malloc(sizeof(uint32_t) * (L ? L : 1));
+    if (!out->node_offsets || !out->line_offsets || !out->src || !out->dst) {
+        for (auto& g : gs) rbsc_host_graph_free(&g);
+        rbsc_host_batch_free(out);
+        rbsc_set_error("host allocation failed");
+        return RBSC_ERR_NOMEM;
+    }
+    th.clear();
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (uint64_t i = t; i < n_graphs; i += nt) {
+                const rbsc_host_graph& g = gs[i];
+                uint64_t o = out->line_offsets[i];
+                for (uint32_t u = 0; u < g.n_nodes; ++u)
+                    for (uint64_t e = g.row_offsets[u]; e < g.row_offsets[u + 1]; ++e, ++o) {
+                        out->src[o] = u;
+                        out->dst[o] = g.dst[e];
+                    }
+                rbsc_host_graph_free(&gs[i]);
+            }
+        });
+    for (auto& x : th) x.join();
+    return RBSC_OK;
 }

@@ -820,7 +820,9 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         if (n_rank)
             HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 2 * sizeof(u32),
                                    hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 4, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        if (ctx->h_scalar[4]) return check_kernel_errors(ctx);  // K of a failed round is void
         if (n_rank && ctx->h_scalar[1]) {
             // A run of signatures equal in their top 32 bits was too long for
             // the fix-up: sort this round's (signature, P) again on all 64 bits.
@@ -838,6 +840,12 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         if (n_rank) pc = sorted_pc ^ 1;
         Knew = n_rank ? ctx->h_scalar[0] : (u32)rank_offset;
         ++it;
+        // K grows every round that is not the last and cannot pass N.
+        if (it > N + 2) {
+            rbsc_set_error("refinement passed %llu rounds without a fixpoint",
+                           (unsigned long long)it);
+            return RBSC_ERR_KERNEL;
+        }
         if (prm.max_rounds && it >= prm.max_rounds) break;
     } while (K != Knew);
     HIP_TRY(hipEventRecord(ctx->ev1, s));

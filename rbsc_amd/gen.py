@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import HostGraph, _lib
+from . import HostBatch, HostGraph, _lib
 from ._lib import check, load
 
 
@@ -67,3 +67,20 @@ def rmat_2p20_clone30(seed=1001) -> HostGraph:
 def ba_8m_m6_init3(seed=1002) -> HostGraph:
     """BA growth, 8M nodes, m = 6, 10% lines with multiplicity 2-4, 3 initial colours."""
     return ba(8_000_000, 6, 0.10, 3, seed)
+
+
+def batch_mixed(n_graphs: int, n_min=10, n_max=5000, seed=1004) -> HostBatch:
+    """Undirected ER / BA / random-regular graphs (i mod 3), node counts
+    log-uniform in [n_min, n_max]."""
+    hb = _lib.HostBatch()
+    check(load().rbsc_gen_batch_mixed(n_graphs, n_min, n_max, seed, C.byref(hb)))
+    try:
+        g = int(hb.n_graphs)
+        no = np.ctypeslib.as_array(hb.node_offsets, shape=(g + 1,)).copy()
+        lo = np.ctypeslib.as_array(hb.line_offsets, shape=(g + 1,)).copy()
+        L = int(lo[-1])
+        src = np.ctypeslib.as_array(hb.src, shape=(max(L, 1),))[:L].copy()
+        dst = np.ctypeslib.as_array(hb.dst, shape=(max(L, 1),))[:L].copy()
+    finally:
+        load().rbsc_host_batch_free(C.byref(hb))
+    return HostBatch(no, lo, src, dst)

@@ -118,3 +118,32 @@ def test_cli_rejects_bad_input(text):
                        timeout=60)
     assert p.returncode != 0
     assert p.stdout == b""
+
+
+def test_batch_generator_shapes():
+    from rbsc_amd import gen
+    b = gen.batch_mixed(200, 10, 5000, seed=5)
+    b2 = gen.batch_mixed(200, 10, 5000, seed=5)
+    assert np.array_equal(b.dst, b2.dst) and np.array_equal(b.node_offsets, b2.node_offsets)
+    n = np.diff(b.node_offsets.astype(np.int64))
+    assert b.n_graphs == 200 and n.min() >= 10 and n.max() <= 5000
+    for g in range(0, 200, 13):
+        hg = b.graph(g)
+        s = hg.sources()
+        assert np.all(np.diff(s.astype(np.int64)) >= 0)
+        assert hg.dst.max() < hg.n_nodes
+        fwd = set(zip(s.tolist(), hg.dst.tolist()))
+        assert fwd == set(zip(hg.dst.tolist(), s.tolist()))  # undirected: both directions
+
+
+def test_host_batch_from_graphs_roundtrip():
+    from rbsc_amd import HostBatch, gen
+    gs = [gen.er_undirected(30 + i, 3.0, seed=i) for i in range(5)]
+    b = HostBatch.from_graphs(gs)
+    for i, g in enumerate(gs):
+        h = b.graph(i)
+        assert h.n_nodes == g.n_nodes
+        assert np.array_equal(h.dst, g.dst) and np.array_equal(h.sources(), g.sources())
+    gs[1].init_color = np.zeros(gs[1].n_nodes, np.uint32)
+    with pytest.raises(ValueError):
+        HostBatch.from_graphs(gs)
