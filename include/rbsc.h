@@ -162,6 +162,17 @@ int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts);
 int rbsc_ctx_profile(rbsc_ctx* ctx, int enable);
 int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int reset);
 
+/* Exact, hash-free stability check of a colouring of g (colors: one u32
+ * label per node id, any values).  *bad_out = number of violations: 0 iff
+ * every node has the same multiset of (block of head, summed weight) over
+ * its lines as every other node of its block and no block mixes initial
+ * colours -- then the colouring is the coarsest stable partition the round
+ * function converges to, with no 64-bit signature collision.  No
+ * counterpart in the reference (its collision loop rbsc.cu:404-419 checks
+ * only the colour hash); rbsc_opts.verify runs it on the final colours. */
+int rbsc_graph_check_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const uint32_t* colors,
+                               uint64_t* bad_out);
+
 /* ---- Batches of independent graphs (no counterpart in the reference, which
  * refines one graph per process).  Graph g owns nodes
  * [node_offsets[g], node_offsets[g+1]) and lines

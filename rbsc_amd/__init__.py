@@ -172,6 +172,18 @@ class DeviceGraph:
                                        _ptr(part, C.c_uint32), _ptr(col, C.c_uint32)))
         return _result(r, part, col)
 
+    def check_partition(self, colors: np.ndarray) -> int:
+        """Exact, hash-free stability check of a colouring (one label per
+        node).  Returns the number of violations: 0 iff stable."""
+        col = np.ascontiguousarray(colors, np.uint32)
+        n, _ = self.dims
+        if len(col) != n:
+            raise ValueError("one colour per node expected")
+        bad = C.c_uint64()
+        check(load().rbsc_graph_check_partition(self.ctx.handle, self.handle,
+                                                _ptr(col, C.c_uint32), C.byref(bad)))
+        return int(bad.value)
+
     def relayout(self, n_parts: int):
         """Group lines by destination (head mod n_parts) for cache-resident
         gathers; results are unchanged."""

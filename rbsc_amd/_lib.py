@@ -117,6 +117,7 @@ SIGNATURES = {
     "rbsc_graph_lines_to_host": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_ctx_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), u64p, C.c_int]),
+    "rbsc_graph_check_partition": (C.c_int, [C.c_void_p, C.c_void_p, u32p, C.POINTER(C.c_uint64)]),
     "rbsc_batch_upload": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(C.c_void_p)]),
     "rbsc_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), u32p, u32p, u32p, u32p,

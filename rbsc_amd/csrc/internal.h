@@ -132,6 +132,11 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& p, RefineOut*
 // Raw colours (dense ranks of the last round) of the last refine_core on g,
 // by original node id, to host.
 int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host);
+// The same colours by internal id into a device buffer of N words.
+void device_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* d_int);
+// Exact stability check (verify.hip) of internal colours d_col (< K);
+// *bad_out = number of violations, 0 iff stable.
+int verify_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* d_col, u32 K, u64* bad_out);
 int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out);
 int quotient_refine(rbsc_ctx* ctx, const rbsc_graph* gr, const rbsc_opts* opts, rbsc_result* res,
                     uint32_t* partition_out);

@@ -865,6 +865,14 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     return RBSC_OK;
 }
 
+void device_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* d_int) {
+    const u64 N = g->N;
+    Workspace& w = ctx->ws;
+    hipStream_t s = ctx->stream;
+    if (w.n_rank < N) fill_u32_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, N, 0);
+    if (w.n_rank) colors_kernel<<<grid_for(w.n_rank), BLK, 0, s>>>(w.P[w.pc], d_int, w.n_rank);
+}
+
 int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host) {
     const u64 N = g->N;
     if (N == 0) return RBSC_OK;
@@ -872,8 +880,7 @@ int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host) {
     hipStream_t s = ctx->stream;
     u32* d_int = reinterpret_cast<u32*>(w.keys[0]);
     u32* d_out = reinterpret_cast<u32*>(w.keys[1]);
-    if (w.n_rank < N) fill_u32_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, N, 0);
-    if (w.n_rank) colors_kernel<<<grid_for(w.n_rank), BLK, 0, s>>>(w.P[w.pc], d_int, w.n_rank);
+    device_colors(ctx, g, d_int);
     map_colors_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, g->pi, N, d_out);
     HIP_TRY(hipMemcpyAsync(host, d_out, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -903,6 +910,15 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
         if (rc) return rc;
         if (partition_out) canonicalize_host((u32)N, hcol, partition_out);
     }
+    int32_t verified = 0;
+    if (opts && opts->verify && N) {
+        u32* d_int = reinterpret_cast<u32*>(ctx->ws.keys[0]);
+        device_colors(ctx, g, d_int);
+        u64 bad = 0;
+        rc = verify_partition(ctx, g, d_int, o.K, &bad);
+        if (rc) return rc;
+        verified = bad ? -1 : 1;
+    }
     auto t1 = std::chrono::steady_clock::now();
     rbsc_result r{};
     r.passes = 1;
@@ -912,7 +928,7 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
     r.rounds = o.rounds;
     r.device_ms = o.device_ms;
     r.time_s = std::chrono::duration<double>(t1 - t0).count();
-    r.verified = 0;
+    r.verified = verified;
     if (res) *res = r;
     return RBSC_OK;
 }
