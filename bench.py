@@ -97,6 +97,81 @@ def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
             "speedup_vs_per_graph_loop": (n_graphs / wall) / loop_rate}
 
 
+def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int) -> dict:
+    """Poisson arrivals of refinement requests over a pool of R-MAT graphs
+    (E log-uniform in [1e3, 5e8], N = E/16, colour 0): small graphs as host
+    graphs (batched), larger ones resident in HBM.  Reports p50/p99 latency
+    (arrival -> result) and graphs/s per rate, against a plain loop that
+    refines the same requests one after another, and checks that every
+    served partition equals the loop's."""
+    import numpy as np
+    import rbsc_amd
+    from rbsc_amd import gen
+    rng = np.random.default_rng(seed)
+    E = np.exp(rng.uniform(np.log(1e3), np.log(5e8), pool_size)).astype(np.int64)
+    pool, pool_bytes = [], 0
+    t0 = time.perf_counter()
+    for k, e in enumerate(E):
+        scale = int(max(3, round(np.log2(e / 16))))
+        if (1 << scale) <= 5120:
+            pool.append(("host", gen.rmat(scale, int(e), seed=seed + k)))
+        else:
+            dg = ctx.gen_rmat(scale, int(e), 0.57, 0.19, 0.19, 0.0, seed + k)
+            pool_bytes += dg.nbytes
+            pool.append(("resident", dg))
+    log(f"[bench] serving pool: {pool_size} graphs, {int(E.sum())} lines, "
+        f"{pool_bytes / 2**30:.1f} GiB resident, built in {time.perf_counter() - t0:.1f}s")
+
+    def submit(srv, item, delay):
+        kind, g = item
+        return srv.submit(g, delay_s=delay) if kind == "host" else srv.submit_resident(g, delay_s=delay)
+
+    # plain loop: the same kind of requests, one at a time, no overlap
+    picks0 = rng.integers(0, pool_size, n_req)
+    loop_part = {}
+    t = time.perf_counter()
+    for i in picks0:
+        kind, g = pool[i]
+        r = ctx.refine(g) if kind == "host" else g.refine()
+        loop_part[int(i)] = r.partition
+    loop_s = time.perf_counter() - t
+    out = {"workload": f"rmat_pool{pool_size}_E1e3_5e8_poisson", "requests_per_rate": n_req,
+           "pool_lines": int(E.sum()), "plain_loop_graphs_per_s": n_req / loop_s, "rates": []}
+    budget = (250 << 30) - pool_bytes
+    for lam in rates:
+        gaps = rng.exponential(1.0 / lam, n_req)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        picks = rng.integers(0, pool_size, n_req)
+        with rbsc_amd.Server(workers=3, hbm_budget=budget, keep_partition=True) as srv:
+            t = time.perf_counter()
+            ts = [submit(srv, pool[i], float(a)) for i, a in zip(picks, arrivals)]
+            rs = [srv.wait(tk) for tk in ts]
+            wall = time.perf_counter() - t
+            st = srv.stats()
+        lat = np.array([r.latency_s for r in rs])
+        first = min(r.arrival_s for r in rs)
+        last = max(r.done_s for r in rs)
+        mism = 0
+        for i, r in zip(picks, rs):
+            i = int(i)
+            if i in loop_part and not np.array_equal(r.partition, loop_part[i]):
+                mism += 1
+        rec = {"rate_per_s": lam, "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+               "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+               "graphs_per_s": n_req / (last - first), "wall_s": wall,
+               "batches": st["batches"], "batched_graphs": st["batched_graphs"],
+               "peak_admitted_gib": st["peak_admitted_bytes"] / 2**30,
+               "partition_mismatches_vs_loop": mism}
+        log(f"[bench] serving rate {lam}/s: p50 {rec['p50_latency_ms']:.1f} ms, "
+            f"p99 {rec['p99_latency_ms']:.1f} ms, {rec['graphs_per_s']:.0f} graphs/s, "
+            f"mismatches {mism}")
+        out["rates"].append(rec)
+    for kind, g in pool:
+        if kind == "resident":
+            g.free()
+    return out
+
+
 def copy_probe(nbytes: int) -> None:
     """One plain device-to-device copy of nbytes through the HIP runtime, so a
     counter run can read FETCH_SIZE / WRITE_SIZE of a kernel of known size."""
@@ -129,6 +204,9 @@ def main():
                     help="with --parts 8: one launch, partition b mod 8 per workgroup b")
     ap.add_argument("--batch-graphs", type=int, default=50000,
                     help="graphs in the many-small-graphs leg (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests per arrival rate in the serving leg (0 skips it)")
+    ap.add_argument("--serve-rates", default="50,200,1000,2000")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -205,6 +283,9 @@ def main():
         out["batched"] = batch_leg(ctx, args.batch_graphs, 1004, 500)
         log(f"[bench] batched: {out['batched']['graphs_per_s']:.0f} graphs/s "
             f"(per-graph loop {out['batched']['per_graph_loop_graphs_per_s']:.0f} graphs/s)")
+    if args.serve_requests:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["serving"] = serving_leg(ctx, rates, args.serve_requests, 64, 1005)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_scale + 4, 0.20, seed)

@@ -211,6 +211,62 @@ int rbsc_refine_batch(rbsc_ctx* ctx, const rbsc_batch* batch, const rbsc_opts* o
                       uint32_t* blocks_out, uint32_t* rounds_out, uint32_t* labels_out,
                       rbsc_result* total);
 
+/* ---- Serving: a request loop on one device (no counterpart in the
+ * reference, which refines one graph per process).  Requests arrive at
+ * their arrival time (submit time + delay_s), wait in FIFO order until their
+ * HBM estimate fits hbm_budget, then run: graphs of at most 5120 nodes are
+ * gathered into batch launches (every small request queued when the batcher
+ * is free goes into its next launch), larger graphs run on `workers`
+ * concurrent contexts (one stream and workspace each).  Results are those
+ * of rbsc_graph_refine on each graph alone. */
+typedef struct rbsc_server rbsc_server;
+typedef struct rbsc_server_opts {
+    int device;
+    uint64_t hbm_budget;      /* bytes for requests in flight; 0 = 250 GiB */
+    uint32_t workers;         /* concurrent single-graph refinements; 0 = 2, at most 8 */
+    uint32_t batch_max;       /* most small graphs per batch launch; 0 = 4096 */
+    uint32_t keep_partition;  /* 1: results carry the canonical partition */
+} rbsc_server_opts;
+typedef struct rbsc_request_result {
+    uint64_t ticket;
+    int32_t status;           /* RBSC_OK or the request's error code */
+    uint32_t nodes;           /* blocks of the final partition */
+    uint32_t rounds;
+    uint32_t batched;         /* 1: refined in a batch launch */
+    uint32_t n_nodes;
+    uint64_t n_edges;
+    double latency_s;         /* arrival -> result */
+    double queue_s;           /* arrival -> admission */
+    double service_s;         /* start of work -> result */
+    double arrival_s, done_s; /* steady-clock seconds */
+} rbsc_request_result;
+typedef struct rbsc_server_stats {
+    uint64_t submitted, completed, failed, batches, batched_graphs, in_flight;
+    uint64_t admitted_bytes, peak_admitted_bytes, budget;
+} rbsc_server_stats;
+#define RBSC_PENDING 1        /* rbsc_server_wait: not finished within the timeout */
+
+int rbsc_server_create(const rbsc_server_opts* opts, rbsc_server** out);
+/* Waits until every submitted request has finished, then stops. */
+int rbsc_server_destroy(rbsc_server* s);
+/* A host graph (copied; uploaded at admission, freed after). */
+int rbsc_server_submit(rbsc_server* s, const rbsc_graph* g, double delay_s, uint64_t* ticket);
+/* An R-MAT graph as rbsc_gen_rmat_device / rbsc_gen_rmat build it, made at
+ * admission (small graphs on the host, others on the device). */
+int rbsc_server_submit_rmat(rbsc_server* s, uint32_t scale, uint64_t n_draws, double a, double b,
+                            double c, double clone_frac, uint64_t seed, double delay_s,
+                            uint64_t* ticket);
+/* A graph already resident on the server's device (owned by the caller,
+ * read only; it must outlive the request). */
+int rbsc_server_submit_resident(rbsc_server* s, rbsc_dgraph* g, double delay_s, uint64_t* ticket);
+/* Waits up to timeout_s (< 0: for ever).  Returns the request's status
+ * (RBSC_OK or its error) and releases it, or RBSC_PENDING.  partition_out
+ * (nullable, n_nodes words) receives the canonical partition when
+ * keep_partition is set. */
+int rbsc_server_wait(rbsc_server* s, uint64_t ticket, double timeout_s, rbsc_request_result* out,
+                     uint32_t* partition_out);
+int rbsc_server_get_stats(rbsc_server* s, rbsc_server_stats* out);
+
 /* Seeded R-MAT generated on the device (bench-scale inputs without a host
  * round trip).  2^scale nodes, n_draws lines drawn with probabilities a, b, c
  * (d = 1 - a - b - c), node ids permuted by a seeded bijection, then

@@ -27,6 +27,7 @@ from ._lib import (FLAG_HASHED64, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOT
                    RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
+           "Server", "RequestResult",
            "RefineResult", "RbscError", "load",
            "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "gen"]
 
@@ -407,3 +408,93 @@ class Context:
 
 
 from . import gen  # noqa: E402  (uses the classes above)
+
+
+@dataclass
+class RequestResult:
+    ticket: int
+    status: int
+    nodes: int
+    rounds: int
+    batched: bool
+    n_nodes: int
+    n_edges: int
+    latency_s: float
+    queue_s: float
+    service_s: float
+    arrival_s: float
+    done_s: float
+    partition: Optional[np.ndarray] = None
+
+
+class Server:
+    """Continuous-batching request loop on one device (``rbsc_server``):
+    FIFO admission under an HBM budget, small graphs batched into persistent
+    launches, larger graphs on concurrent worker contexts."""
+
+    def __init__(self, device=0, hbm_budget=0, workers=0, batch_max=0, keep_partition=True):
+        o = _lib.ServerOpts()
+        o.device = device
+        o.hbm_budget = int(hbm_budget)
+        o.workers = int(workers)
+        o.batch_max = int(batch_max)
+        o.keep_partition = 1 if keep_partition else 0
+        self.keep_partition = keep_partition
+        self.handle = C.c_void_p()
+        check(load().rbsc_server_create(C.byref(o), C.byref(self.handle)))
+        self._nodes = {}
+
+    def submit(self, g: HostGraph, delay_s=0.0) -> int:
+        st = g.as_struct()
+        t = C.c_uint64()
+        check(load().rbsc_server_submit(self.handle, C.byref(st), float(delay_s), C.byref(t)))
+        self._nodes[t.value] = g.n_nodes
+        return int(t.value)
+
+    def submit_rmat(self, scale, n_draws, a=0.57, b=0.19, c=0.19, clone_frac=0.0, seed=1,
+                    delay_s=0.0) -> int:
+        t = C.c_uint64()
+        check(load().rbsc_server_submit_rmat(self.handle, scale, n_draws, a, b, c, clone_frac, seed,
+                                             float(delay_s), C.byref(t)))
+        self._nodes[t.value] = 1 << scale
+        return int(t.value)
+
+    def submit_resident(self, dg: DeviceGraph, delay_s=0.0) -> int:
+        t = C.c_uint64()
+        check(load().rbsc_server_submit_resident(self.handle, dg.handle, float(delay_s),
+                                                 C.byref(t)))
+        self._nodes[t.value] = dg.dims[0]
+        return int(t.value)
+
+    def wait(self, ticket: int, timeout_s=-1.0) -> Optional[RequestResult]:
+        """The request's result (raises on its error), or None if it is not
+        finished within timeout_s."""
+        r = _lib.RequestResult()
+        n = self._nodes.get(ticket, 0)
+        part = np.empty(max(n, 1), np.uint32) if self.keep_partition else None
+        rc = load().rbsc_server_wait(self.handle, ticket, float(timeout_s), C.byref(r),
+                                     _ptr(part, C.c_uint32))
+        if rc == _lib.PENDING:
+            return None
+        self._nodes.pop(ticket, None)
+        check(rc)
+        return RequestResult(int(r.ticket), int(r.status), int(r.nodes), int(r.rounds),
+                             bool(r.batched), int(r.n_nodes), int(r.n_edges), float(r.latency_s),
+                             float(r.queue_s), float(r.service_s), float(r.arrival_s),
+                             float(r.done_s), None if part is None else part[:n])
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        check(load().rbsc_server_get_stats(self.handle, C.byref(st)))
+        return {k: int(getattr(st, k)) for k, _ in _lib.ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            load().rbsc_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

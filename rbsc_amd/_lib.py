@@ -102,6 +102,41 @@ class HostBatch(C.Structure):
     ]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("hbm_budget", C.c_uint64),
+        ("workers", C.c_uint32),
+        ("batch_max", C.c_uint32),
+        ("keep_partition", C.c_uint32),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("status", C.c_int32),
+        ("nodes", C.c_uint32),
+        ("rounds", C.c_uint32),
+        ("batched", C.c_uint32),
+        ("n_nodes", C.c_uint32),
+        ("n_edges", C.c_uint64),
+        ("latency_s", C.c_double),
+        ("queue_s", C.c_double),
+        ("service_s", C.c_double),
+        ("arrival_s", C.c_double),
+        ("done_s", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in (
+        "submitted", "completed", "failed", "batches", "batched_graphs", "in_flight",
+        "admitted_bytes", "peak_admitted_bytes", "budget")]
+
+
+PENDING = 1
+
 # Every exported symbol of the C headers, with (restype, argtypes).
 SIGNATURES = {
     "rbsc_last_error": (C.c_char_p, []),
@@ -118,6 +153,18 @@ SIGNATURES = {
     "rbsc_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_ctx_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), u64p, C.c_int]),
     "rbsc_graph_check_partition": (C.c_int, [C.c_void_p, C.c_void_p, u32p, C.POINTER(C.c_uint64)]),
+    "rbsc_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    "rbsc_server_destroy": (C.c_int, [C.c_void_p]),
+    "rbsc_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Graph), C.c_double,
+                                     C.POINTER(C.c_uint64)]),
+    "rbsc_server_submit_rmat": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                          C.c_double, C.POINTER(C.c_uint64)]),
+    "rbsc_server_submit_resident": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double,
+                                              C.POINTER(C.c_uint64)]),
+    "rbsc_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double,
+                                   C.POINTER(RequestResult), u32p]),
+    "rbsc_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "rbsc_batch_upload": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(C.c_void_p)]),
     "rbsc_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), u32p, u32p, u32p, u32p,

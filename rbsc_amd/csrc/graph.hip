@@ -167,7 +167,6 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     B_TRY(hipMalloc(&g->end_bits, sizeof(u32) * g->end_words));
     B_TRY(hipMalloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1)));
     B_TRY(hipMalloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1)));
-    B_TRY(hipMalloc(&g->carry_val, sizeof(u64) * std::max<u64>(g->n_tiles, 1)));
     g->bytes = sizeof(u64) * (U + 1) + 4 * N + 4 * E + (raw->wt ? 8 * E : 0) +
                (raw->init ? 4 * N : 0) + 4 * g->end_words + 16 * (g->n_tiles + 1);
 
@@ -309,7 +308,6 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
         *adj = nullptr, *seg_ptr = nullptr, *nwt = nullptr;
     u32 *ndst = nullptr, *seg_row = nullptr, *nbits = nullptr, *tile_row = nullptr,
         *carry_row = nullptr;
-    u64* carry_val = nullptr;
     std::vector<u64> Ep(P), Rp(P), adj_line(P), adj_seg(P), h(2 * P);
     int rc = RBSC_OK;
     u64 E_pad = 0, R = 0, n_tiles = 0, end_words = 0;
@@ -364,7 +362,6 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
     R_TRY(hipMalloc(&nbits, sizeof(u32) * end_words));
     R_TRY(hipMalloc(&tile_row, sizeof(u32) * (n_tiles + 1)));
     R_TRY(hipMalloc(&carry_row, sizeof(u32) * n_tiles));
-    R_TRY(hipMalloc(&carry_val, sizeof(u64) * n_tiles));
     R_TRY(hipMemcpyAsync(adj, adj_line.data(), sizeof(u64) * P, hipMemcpyHostToDevice, s));
     R_TRY(hipMemcpyAsync(adj + P, adj_seg.data(), sizeof(u64) * P, hipMemcpyHostToDevice, s));
     zero_u32<<<grid_for(end_words), BLK, 0, s>>>(nbits, end_words);
@@ -397,14 +394,12 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
     (void)hipFree(g->end_bits);
     (void)hipFree(g->tile_row);
     (void)hipFree(g->carry_row);
-    (void)hipFree(g->carry_val);
     g->dst = ndst;
     g->wt = nwt;
     g->end_bits = nbits;
     g->end_words = end_words;
     g->tile_row = tile_row;
     g->carry_row = carry_row;
-    g->carry_val = carry_val;
     g->n_tiles = n_tiles;
     g->seg_row = seg_row;
     g->n_segs = R;
@@ -420,10 +415,9 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
     nwt = nullptr;
     nbits = nullptr;
     tile_row = carry_row = nullptr;
-    carry_val = nullptr;
     seg_row = nullptr;
 done:
-    for (u64* p : {counts, line_off, flags, tmp, tot, adj, seg_ptr, nwt, carry_val})
+    for (u64* p : {counts, line_off, flags, tmp, tot, adj, seg_ptr, nwt})
         if (p) (void)hipFree(p);
     for (u32* p : {ndst, seg_row, nbits, tile_row, carry_row})
         if (p) (void)hipFree(p);

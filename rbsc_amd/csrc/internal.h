@@ -23,7 +23,9 @@ struct RawGraph {
     u32* init = nullptr;        // N or null
 };
 
-// A graph resident in HBM in the round's layout (graph.hip): internal ids
+// A graph resident in HBM in the round's layout (graph.hip).  Refinement
+// only reads it (all per-refinement state lives in the context), so several
+// contexts may refine one resident graph at the same time.  Internal ids
 // put nodes with lines first (in-degree descending) and nodes without lines
 // last; row j is node j; lines are grouped by row.  Nodes without lines keep
 // signature 0 (rbsc.cu:110-111).
@@ -41,7 +43,6 @@ struct rbsc_dgraph {
     u64 n_tiles = 0;            // wave tiles of SIG_TW lines
     u32* tile_row = nullptr;    // n_tiles + 1: row holding each tile's first line
     u32* carry_row = nullptr;   // n_tiles: row continuing past the tile, or NO_ROW
-    u64* carry_val = nullptr;   // n_tiles: that row's partial signature
     u64 bytes = 0;
     // Streaming: lines kept in pinned host memory (dst == nullptr on device).
     u32* dst_host = nullptr;    // E, pinned
@@ -68,6 +69,8 @@ struct Workspace {
     u64* hc = nullptr;                 // randomize(colour + salt) per node
     u64* rank_status = nullptr;
     u64 rank_tiles = 0;
+    u64* carry = nullptr;              // per tile: partial signature of its carry row
+    u64 carry_cap = 0;
     u64* tie_runs = nullptr;           // (start, end) of runs to fix after a 32-bit sort
     u64 tie_cap = 0;
     int pc = 0;                        // P[pc] holds the last refinement's order
@@ -124,6 +127,8 @@ void prof_end(rbsc_ctx* ctx);
 int prof_collect(rbsc_ctx* ctx);
 
 int ws_reserve(rbsc_ctx* ctx, u64 n);
+// Frees the refinement workspace (it is allocated again on demand).
+void ws_release(rbsc_ctx* ctx);
 void raw_release(RawGraph* r);
 // Builds g from raw (raw arrays are left for the caller to release).
 int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g);

@@ -487,9 +487,9 @@ int check_kernel_errors(rbsc_ctx* ctx) {
     return RBSC_OK;
 }
 
-int ws_reserve(rbsc_ctx* ctx, u64 n) {
+void ws_release(rbsc_ctx* ctx) {
     Workspace& w = ctx->ws;
-    if (n <= w.cap_n && w.keys[0]) return RBSC_OK;
+    if (w.keys[0] || w.sig) (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < 2; ++i) {
         if (w.keys[i]) (void)hipFree(w.keys[i]);
         if (w.P[i]) (void)hipFree(w.P[i]);
@@ -499,7 +499,16 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     if (w.hc) (void)hipFree(w.hc);
     if (w.rank_status) (void)hipFree(w.rank_status);
     if (w.tie_runs) (void)hipFree(w.tie_runs);
+    if (w.carry) (void)hipFree(w.carry);
     w = Workspace();
+    ctx->sort.extra_status = nullptr;
+    ctx->sort.extra_words = 0;
+}
+
+int ws_reserve(rbsc_ctx* ctx, u64 n) {
+    Workspace& w = ctx->ws;
+    if (n <= w.cap_n && w.keys[0]) return RBSC_OK;
+    ws_release(ctx);
     const u64 cap = std::max<u64>(n, 4096);
     for (int i = 0; i < 2; ++i) {
         HIP_TRY(hipMalloc(&w.keys[i], sizeof(u64) * cap));
@@ -513,6 +522,8 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     HIP_TRY(hipMalloc(&w.rank_status, sizeof(u64) * w.rank_tiles));
     HIP_TRY(hipMemsetAsync(w.rank_status, 0, sizeof(u64) * w.rank_tiles, ctx->stream));
     w.cap_n = cap;
+    ctx->sort.extra_status = w.rank_status;
+    ctx->sort.extra_words = w.rank_tiles;
     int rc = sort_ws_init(&ctx->sort, cap, ctx->d_err, ctx->stream);
     if (rc) return rc;
     return RBSC_OK;
@@ -532,7 +543,6 @@ void dgraph_release(rbsc_dgraph* g) {
     if (g->wt_host) (void)hipHostFree(g->wt_host);
     if (g->tile_row) (void)hipFree(g->tile_row);
     if (g->carry_row) (void)hipFree(g->carry_row);
-    if (g->carry_val) (void)hipFree(g->carry_val);
     *g = rbsc_dgraph();
 }
 
@@ -610,7 +620,7 @@ static void launch_sig(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* dst, const u64*
 #define SIG_LAUNCH(W, PT, C)                                                                   \
     sig_kernel<W, PT, C><<<grid, BLK, 0, ctx->stream>>>(dst, W ? wt : nullptr, E, eb, w.hc,    \
                                                         gather_limit, dang_h, salt, w.sig,     \
-                                                        g->seg_row, g->tile_row, g->carry_val, \
+                                                        g->seg_row, g->tile_row, w.carry,      \
                                                         t_begin, t_end, line_base, g->d_part_tile)
 #define SIG_C(W, PT)                   \
     if (c32) SIG_LAUNCH(W, PT, true);  \
@@ -654,6 +664,13 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     int rc = ws_reserve(ctx, N);
     if (rc) return rc;
     Workspace& w = ctx->ws;
+    if (g->n_tiles > w.carry_cap || !w.carry) {
+        if (w.carry) HIP_TRY(hipFree(w.carry));
+        w.carry = nullptr;
+        w.carry_cap = 0;
+        HIP_TRY(hipMalloc(&w.carry, sizeof(u64) * std::max<u64>(g->n_tiles, 1)));
+        w.carry_cap = std::max<u64>(g->n_tiles, 1);
+    }
     // Without initial colours, nodes without lines are always colour 0: their
     // pair (0, 0) is the least of every round (others have colour >= 1 from
     // c0 = N+1 on), so they are left out of the node side, ranks of the rest
@@ -768,7 +785,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             }
         }
         if (n_tiles)
-            sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, g->carry_val,
+            sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, w.carry,
                                                                g->n_parts > 1 ? g->seg_row : nullptr,
                                                                w.sig, n_tiles);
         prof_end(ctx);
@@ -974,15 +991,7 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
     if (!c) return RBSC_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    Workspace& w = c->ws;
-    for (int i = 0; i < 2; ++i) {
-        if (w.keys[i]) (void)hipFree(w.keys[i]);
-        if (w.P[i]) (void)hipFree(w.P[i]);
-    }
-    if (w.sig) (void)hipFree(w.sig);
-    if (w.hc) (void)hipFree(w.hc);
-    if (w.rank_status) (void)hipFree(w.rank_status);
-    if (w.tie_runs) (void)hipFree(w.tie_runs);
+    ws_release(c);
     sort_ws_free(&c->sort);
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->d_scalar) (void)hipFree(c->d_scalar);

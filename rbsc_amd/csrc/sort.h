@@ -14,6 +14,10 @@ struct SortWorkspace {
     u64 ticket_total = 0;       // tickets handed out so far (host mirror)
     u32 epoch = 0;              // look-back epoch (host mirror)
     u32* err = nullptr;         // kernel error flags
+    // Another look-back status array stamped with this epoch counter (the
+    // dense-rank pass); cleared with the sort's own on epoch wrap-around.
+    u64* extra_status = nullptr;
+    u64 extra_words = 0;
 };
 
 // Number of keys per onesweep tile.

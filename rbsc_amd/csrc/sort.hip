@@ -209,6 +209,9 @@ u32 lb_next_epoch(SortWorkspace* ws, hipStream_t s) {
         // Wrapped: clear every status word so no stale word matches.
         const u64 n = ws->status_tiles * RADIX;
         if (n) fill_u64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ws->status, n, 0);
+        const u64 m = ws->extra_words;
+        if (m && ws->extra_status)
+            fill_u64_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(ws->extra_status, m, 0);
         ws->epoch = 1;
     }
     return ws->epoch;
