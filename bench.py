@@ -172,6 +172,21 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int) -> dict:
     return out
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (FETCH_SIZE and WRITE_SIZE runs, calibrated against a
+    copy of known size), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(nbytes: int) -> None:
     """One plain device-to-device copy of nbytes through the HIP runtime, so a
     counter run can read FETCH_SIZE / WRITE_SIZE of a kernel of known size."""
@@ -251,11 +266,18 @@ def main():
     visits = rounds * E
     value = visits / wall
     ms_round = dev_ms / rounds
-    b_alg = 8 * E + 32 * N  # unweighted lines
-    round_ms = {k: v[0] / max(v[1], 1) for k, v in phases.items()}
-    per_round_sum = sum(phases[k][0] for k in ("signature", "gather", "sort", "rank")) / rounds
-    achieved = b_alg / (per_round_sum / 1e3)
-    dom = max(("signature", "gather", "sort", "rank"), key=lambda k: phases[k][0])
+    U = dg.rows
+    # Whole round: 8 B per line (head id + neighbour colour) and 32 B per
+    # node (row offset, signature write and read, old and new colour).
+    b_round = 8 * E + 32 * N
+    # Dominant kernel, the signature pass: 8 B per line plus 16 B per row
+    # (row offset, signature write); one launch per round.
+    b_sig = 8 * E + 16 * U
+    per_round = {k: phases[k][0] / rounds for k in phases if k != "setup"}
+    round_ms = sum(per_round[k] for k in ("signature", "gather", "sort", "rank"))
+    dom = max(("signature", "gather", "sort", "rank"), key=lambda k: per_round[k])
+    sig_ms = per_round["signature"]
+    traffic = measured_traffic(args.config, "sig_kernel") if args.parts == 1 else None
     out = {
         "metric": "edge-visits/s per WL round (% HBM roofline); batched graphs/s, p99 latency",
         "value": value,
@@ -269,15 +291,20 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "parts": args.parts, "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
+        "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "rows": U,
+                   "parts": args.parts, "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
                    "rounds_per_step": rounds // args.steps,
                    "blocks": r.nodes, "ms_per_round": ms_round},
-        "roofline": {"bound": "hbm", "kernel": "refinement round (all kernels of one round)",
-                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "alg_bytes_per_round": b_alg,
-                     "phase_ms_per_round": {k: phases[k][0] / rounds for k in phases if k != "setup"},
-                     "dominant_phase": dom},
+        "roofline": {"bound": "hbm", "kernel": "sig_kernel",
+                     "achieved": b_sig / (sig_ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": b_sig / (sig_ms / 1e3) / HBM_PEAK,
+                     "traffic": traffic,
+                     "alg_bytes_per_launch": b_sig, "launch_ms": sig_ms,
+                     "dominant_phase": dom,
+                     "round": {"alg_bytes": b_round, "ms": round_ms,
+                               "achieved": b_round / (round_ms / 1e3) / 1e9,
+                               "frac": b_round / (round_ms / 1e3) / HBM_PEAK,
+                               "phase_ms": per_round}},
     }
     if args.batch_graphs:
         out["batched"] = batch_leg(ctx, args.batch_graphs, 1004, 500)

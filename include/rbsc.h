@@ -279,6 +279,8 @@ int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_draws, double
 /* Copy a resident graph back to host CSR (for checks): row_offsets has
  * n_nodes + 1 entries, dst n_edges entries.  Sizes via rbsc_graph_dims. */
 int rbsc_graph_dims(const rbsc_dgraph* g, uint32_t* n_nodes, uint64_t* n_edges);
+/* Nodes with at least one line (the rows the signature pass visits). */
+uint32_t rbsc_graph_rows(const rbsc_dgraph* g);
 int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t* row_offsets,
                         uint32_t* dst);
 

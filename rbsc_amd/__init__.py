@@ -159,6 +159,11 @@ class DeviceGraph:
         return int(n.value), int(e.value)
 
     @property
+    def rows(self) -> int:
+        """Nodes with at least one line."""
+        return int(load().rbsc_graph_rows(self.handle))
+
+    @property
     def nbytes(self) -> int:
         return int(load().rbsc_graph_bytes(self.handle))
 

@@ -179,6 +179,7 @@ SIGNATURES = {
     "rbsc_gen_rmat_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_double,
                                        C.c_double, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "rbsc_graph_dims": (C.c_int, [C.c_void_p, u32p, u64p]),
+    "rbsc_graph_rows": (C.c_uint32, [C.c_void_p]),
     "rbsc_graph_download": (C.c_int, [C.c_void_p, C.c_void_p, u64p, u32p]),
     "rbsc_host_graph_free": (None, [C.POINTER(HostGraph)]),
     "rbsc_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,

@@ -1144,6 +1144,8 @@ extern "C" int rbsc_graph_dims(const rbsc_dgraph* g, uint32_t* n_nodes, uint64_t
     return RBSC_OK;
 }
 
+extern "C" uint32_t rbsc_graph_rows(const rbsc_dgraph* g) { return g ? g->U : 0; }
+
 extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t* row_offsets,
                                    uint32_t* dst) {
     if (!ctx || !g) {
