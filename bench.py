@@ -213,6 +213,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--parts", type=int, default=1,
                     help="destination partitions of the line layout (power of two)")
+    ap.add_argument("--hot", type=int, default=0,
+                    help="two-pass layout: lines into the HOT most-gathered heads first")
+    ap.add_argument("--fullsort", action="store_true",
+                    help="sort all 64 signature bits (default: top 32 bits + colliding runs)")
     ap.add_argument("--hashed64", action="store_true",
                     help="gather per-node hashed colours instead of raw u32 colours")
     ap.add_argument("--xcd", action="store_true",
@@ -244,7 +248,11 @@ def main():
         dg.relayout(args.parts)
         log(f"[bench] relayout into {args.parts} destination partitions in {time.perf_counter() - t0:.1f}s")
 
-    flags = (rbsc_amd.FLAG_HASHED64 if args.hashed64 else 0) | (rbsc_amd.FLAG_XCD if args.xcd else 0)
+    if args.hot:
+        t0 = time.perf_counter()
+        dg.split_hot(args.hot)
+        log(f"[bench] hot/cold split at {args.hot} heads in {time.perf_counter() - t0:.1f}s")
+    flags = (rbsc_amd.FLAG_FULLSORT if args.fullsort else 0) | (rbsc_amd.FLAG_HASHED64 if args.hashed64 else 0) | (rbsc_amd.FLAG_XCD if args.xcd else 0)
     for i in range(args.warmup):
         r = dg.refine(partition=False, flags=flags)
         log(f"[bench] warmup {i}: rounds={r.rounds} blocks={r.nodes} device_ms={r.device_ms:.2f}")
@@ -277,7 +285,7 @@ def main():
     round_ms = sum(per_round[k] for k in ("signature", "gather", "sort", "rank"))
     dom = max(("signature", "gather", "sort", "rank"), key=lambda k: per_round[k])
     sig_ms = per_round["signature"]
-    traffic = measured_traffic(args.config, "sig_kernel") if args.parts == 1 else None
+    traffic = measured_traffic(args.config, "sig_kernel") if args.parts == 1 and not args.hot else None
     out = {
         "metric": "edge-visits/s per WL round (% HBM roofline); batched graphs/s, p99 latency",
         "value": value,
@@ -292,7 +300,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "rows": U,
-                   "parts": args.parts, "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
+                   "parts": args.parts, "hot": args.hot, "fullsort": bool(args.fullsort), "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
                    "rounds_per_step": rounds // args.steps,
                    "blocks": r.nodes, "ms_per_round": ms_round},
         "roofline": {"bound": "hbm", "kernel": "sig_kernel",

@@ -78,15 +78,16 @@ typedef struct rbsc_opts {
  * partitions in one launch with workgroup b serving partition b mod 8, so
  * each of the 8 XCDs' L2 caches its own eighth of the colours. */
 #define RBSC_FLAG_XCD 2u
-/* opts.flags: sort the top 32 signature bits (4 passes instead of 8), then
- * order each run of equal top halves by the low halves; a run longer than
- * 4096 entries makes the round redo a full 64-bit sort.  Measured slower
- * than the full sort on R-MAT (large classes of equal signatures make long
- * runs), so it is off by default. */
+/* opts.flags: the default node-side sort orders the top 32 signature bits
+ * (4 passes instead of 8), then re-sorts only the runs whose members collide
+ * in the top half but differ in the low half (gathered into a compact array);
+ * RBSC_FLAG_SPLIT names that default.  RBSC_FLAG_FULLSORT sorts all 64 bits
+ * instead (results identical). */
 #define RBSC_FLAG_SPLIT 4u
-/* Verification hooks for RBSC_FLAG_SPLIT (results unchanged, slower): split
- * at 16 bits instead, so runs are common; cap runs at 2 entries, so the
- * full-sort fallback runs. */
+#define RBSC_FLAG_FULLSORT 32u
+/* Verification hooks for the split sort (results unchanged, slower): split
+ * at 16 bits instead, so colliding runs are common; cap the run list at 2
+ * entries, so the full-sort fallback runs. */
 #define RBSC_FLAG_SPLIT16 8u
 #define RBSC_FLAG_TIESMALL 16u
 
@@ -152,6 +153,11 @@ int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g);
  * unchanged.  No counterpart in the reference (its set_values gathers over
  * the whole colour array, rbsc.cu:80-87). */
 int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts);
+/* Two-pass line layout by head range: lines into the hot_heads most-gathered
+ * nodes (internal ids are in-degree order) in one pass, whose colours stay
+ * cache resident, and the rest in a second pass.  Results are unchanged.  No
+ * counterpart in the reference. */
+int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
 
 /* Per-phase device timing with hipEvents on the context's stream (off by
  * default).  Phases: 0 signature kernel + row fix-up, 1 key gather,

@@ -23,13 +23,13 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (FLAG_HASHED64, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
+from ._lib import (FLAG_FULLSORT, FLAG_HASHED64, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
                    RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
            "Server", "RequestResult",
            "RefineResult", "RbscError", "load",
-           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "gen"]
+           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "gen"]
 
 _MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
           "aggressive": MODE_QUOTIENT_AGGRESSIVE}
@@ -194,6 +194,11 @@ class DeviceGraph:
         """Group lines by destination (head mod n_parts) for cache-resident
         gathers; results are unchanged."""
         check(load().rbsc_graph_relayout(self.ctx.handle, self.handle, n_parts))
+
+    def split_hot(self, hot_heads: int):
+        """Two-pass layout: lines into the hot_heads most-gathered nodes
+        first (their colours stay cache resident); results are unchanged."""
+        check(load().rbsc_graph_split_hot(self.ctx.handle, self.handle, int(hot_heads)))
 
     def lines_to_host(self):
         """Keep the line arrays in pinned host memory; refinements then stream

@@ -29,6 +29,7 @@ FLAG_XCD = 2
 FLAG_SPLIT = 4
 FLAG_SPLIT16 = 8
 FLAG_TIESMALL = 16
+FLAG_FULLSORT = 32
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -180,6 +181,7 @@ SIGNATURES = {
                                        C.c_double, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "rbsc_graph_dims": (C.c_int, [C.c_void_p, u32p, u64p]),
     "rbsc_graph_rows": (C.c_uint32, [C.c_void_p]),
+    "rbsc_graph_split_hot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "rbsc_graph_download": (C.c_int, [C.c_void_p, C.c_void_p, u64p, u32p]),
     "rbsc_host_graph_free": (None, [C.POINTER(HostGraph)]),
     "rbsc_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,

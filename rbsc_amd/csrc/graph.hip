@@ -232,14 +232,21 @@ done:
 
 // ------------------------------------------------------------ partitioning
 namespace {
+// Partition of a line by its head: interleaved (head mod P) or, with a hot
+// range, 0 for heads below `hot` or without lines (never gathered) and 1 for
+// the other heads with lines.
+__device__ __forceinline__ u32 part_of(u32 d, u32 P, u32 hot, u32 U) {
+    if (hot) return (d >= hot && d < U) ? 1u : 0u;
+    return d & (P - 1);
+}
 // counts[p * U + j] = lines of row j whose head is in partition p.
 __global__ void part_count_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                  u64 U, u32 P, u64* __restrict__ counts) {
+                                  u64 U, u32 P, u32 hot, u64* __restrict__ counts) {
     const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= U) return;
     u64 c[16];
     for (u32 p = 0; p < P; ++p) c[p] = 0;
-    for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) c[dst[e] & (P - 1)]++;
+    for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) c[part_of(dst[e], P, hot, (u32)U)]++;
     for (u32 p = 0; p < P; ++p) counts[(u64)p * U + j] = c[p];
 }
 __global__ void flag_nonzero_kernel(const u64* __restrict__ a, u64 n, u64* __restrict__ f) {
@@ -250,7 +257,7 @@ __global__ void flag_nonzero_kernel(const u64* __restrict__ a, u64 n, u64* __res
 // segment rows and row-end bits.
 template <bool HAS_WT>
 __global__ void part_fill_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                 const u64* __restrict__ wt, u64 U, u32 P,
+                                 const u64* __restrict__ wt, u64 U, u32 P, u32 hot,
                                  const u64* __restrict__ line_off, const u64* __restrict__ seg_idx,
                                  const u64* __restrict__ counts, const u64* __restrict__ adj_line,
                                  const u64* __restrict__ adj_seg, u32* __restrict__ ndst,
@@ -272,7 +279,7 @@ __global__ void part_fill_kernel(const u64* __restrict__ row_ptr, const u32* __r
     }
     for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) {
         const u32 d = dst[e];
-        const u32 p = d & (P - 1);
+        const u32 p = part_of(d, P, hot, (u32)U);
         ndst[pos[p]] = d;
         if (HAS_WT) nwt[pos[p]] = wt[e];
         pos[p]++;
@@ -290,10 +297,11 @@ __global__ void pad_seg_kernel(u64* __restrict__ seg_ptr, u32* __restrict__ seg_
 }
 }  // namespace
 
-int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
+int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
     if (P == 1 && g->n_parts == 1) return RBSC_OK;
-    if (g->n_parts != 1 || P == 0 || P > 16 || (P & (P - 1))) {
-        rbsc_set_error("relayout: partitions must be a power of two <= 16, from the plain layout");
+    if (g->n_parts != 1 || P == 0 || P > 16 || (P & (P - 1)) || (hot && P != 2)) {
+        rbsc_set_error("relayout: partitions must be a power of two <= 16 (2 with a hot range), "
+                       "from the plain layout");
         return RBSC_ERR_INVALID;
     }
     if (!g->dst) {
@@ -326,7 +334,7 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
     R_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(PU + 1)));
     R_TRY(hipMalloc(&tot, sizeof(u64) * 2 * P));
     R_TRY(hipMalloc(&adj, sizeof(u64) * 2 * P));
-    part_count_kernel<<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, U, P, counts);
+    part_count_kernel<<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, U, P, hot, counts);
     flag_nonzero_kernel<<<grid_for(PU), BLK, 0, s>>>(counts, PU, flags);
     // per-partition totals of lines and segments
     for (u32 p = 0; p < P; ++p) {
@@ -367,12 +375,12 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P) {
     zero_u32<<<grid_for(end_words), BLK, 0, s>>>(nbits, end_words);
     if (nwt) R_TRY(hipMemsetAsync(nwt, 0, sizeof(u64) * E_pad, s));
     if (g->wt)
-        part_fill_kernel<true><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, g->wt, U, P, line_off,
+        part_fill_kernel<true><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, g->wt, U, P, hot, line_off,
                                                            flags, counts, adj, adj + P, ndst, nwt,
                                                            seg_ptr, seg_row, nbits);
     else
         part_fill_kernel<false><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, nullptr, U, P,
-                                                            line_off, flags, counts, adj, adj + P,
+                                                            hot, line_off, flags, counts, adj, adj + P,
                                                             ndst, nullptr, seg_ptr, seg_row, nbits);
     for (u32 p = 0; p < P; ++p) {
         const u64 a = adj_line[p] + Ep[p];

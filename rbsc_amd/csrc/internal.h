@@ -73,6 +73,12 @@ struct Workspace {
     u64 carry_cap = 0;
     u64* tie_runs = nullptr;           // (start, end) of runs to fix after a 32-bit sort
     u64 tie_cap = 0;
+    u64* tie_pre = nullptr;            // tie_cap + 1: run lengths -> offsets, total
+    u64* tie_tmp = nullptr;            // scan scratch
+    u32* tie_mark = nullptr;           // cap_n: epoch stamps of listed run starts
+    u32 tie_epoch = 0;
+    u64* tie_buf = nullptr;            // 8 x tie_buf_cap: compact run members
+    u64 tie_buf_cap = 0;
     int pc = 0;                        // P[pc] holds the last refinement's order
     u64 n_rank = 0;                    // nodes ranked by the last refinement
 };
@@ -132,7 +138,10 @@ void ws_release(rbsc_ctx* ctx);
 void raw_release(RawGraph* r);
 // Builds g from raw (raw arrays are left for the caller to release).
 int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g);
-int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 n_parts);
+// Regroups lines by head: n_parts interleaved partitions (head mod n_parts),
+// or with hot > 0 two partitions, heads below `hot` (the most-gathered
+// nodes) first.
+int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 n_parts, u32 hot);
 int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& p, RefineOut* out);
 // Raw colours (dense ranks of the last round) of the last refine_core on g,
 // by original node id, to host.

@@ -23,6 +23,7 @@
 // The reference's collision loop (rbsc.cu:404-419) always exits after one
 // pass because randomize is a bijection; it is not repeated here.
 #include "internal.h"
+#include "scan.h"
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
@@ -246,59 +247,140 @@ __global__ void gather_keys_kernel(const u64* __restrict__ P, const u64* __restr
     if (i < n) keys[i] = sig[(u32)P[i]];
 }
 
-// After a stable sort by the top 32 signature bits (colour order kept inside
-// equal keys), every run of equal top halves must be ordered by the low 32
-// bits, stably.  Such runs come from distinct signatures colliding in 32
-// bits (rare, short) or from equal signatures (already ordered: no "break"
-// between neighbours).  tie_find_kernel (read-only) lists each run that has a
-// break once, from its first break; tie_sort_kernel insertion-sorts the listed
-// runs (disjoint).  A run longer than TIE_MAX, or more runs than the list
-// holds, raise *overflow and the caller redoes a full 64-bit sort.
-// The split is at bit `lo` (32 normally; smaller splits are verification
-// hooks that make ties frequent).
-constexpr u64 TIE_MAX = 4096;
-__global__ void tie_find_kernel(const u64* __restrict__ keys, u64 n, u32 lo, u64 tie_max,
-                                u64* __restrict__ runs, u64 cap, u32* __restrict__ count,
-                                u32* __restrict__ overflow) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 || i >= n) return;
-    const u64 k = keys[i], pk = keys[i - 1];
-    if ((k >> lo) != (pk >> lo) || k == pk) return;  // not a break
-    u64 s = i - 1;
-    while (s > 0 && (keys[s - 1] >> lo) == (k >> lo)) {
-        if (keys[s - 1] != keys[s]) return;  // an earlier break lists the run
-        if (i - s > tie_max) { atomicOr(overflow, 1u); return; }
-        --s;
-    }
-    u64 e = i + 1;
-    while (e < n && (keys[e] >> lo) == (k >> lo)) {
-        if (e - s > tie_max) { atomicOr(overflow, 1u); return; }
-        ++e;
-    }
-    const u32 slot = atomicAdd(count, 1u);
-    if (slot >= cap) { atomicOr(overflow, 1u); return; }
-    runs[2 * (u64)slot] = s;
-    runs[2 * (u64)slot + 1] = e;
-}
-__global__ void tie_sort_kernel(u64* __restrict__ keys, u64* __restrict__ P,
-                                const u64* __restrict__ runs, const u32* __restrict__ count,
-                                u64 cap, u32 lo) {
-    const u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    const u64 nr = *count;
-    if (r >= nr || r >= cap) return;
-    const u64 s = runs[2 * r], e = runs[2 * r + 1];
-    const u64 mask = (1ULL << lo) - 1;
-    for (u64 a = s + 1; a < e; ++a) {  // stable insertion sort by the low bits
-        const u64 ka = keys[a], pa = P[a];
-        u64 b = a;
-        while (b > s && (keys[b - 1] & mask) > (ka & mask)) {
-            keys[b] = keys[b - 1];
-            P[b] = P[b - 1];
-            --b;
+// After a stable sort by the signature bits at and above `lo` (colour order
+// kept inside equal keys), every run of equal top bits must be ordered by
+// the low bits, stably.  Runs of equal signatures are already in order; only
+// runs holding a "break" (neighbours with equal top bits and different low
+// bits, from distinct signatures colliding in the top bits) need work.
+// tie_runs_kernel finds breaks in one coalesced pass and lists each such run
+// once (bounds by binary search on the sorted top bits; the first break to
+// stamp the run's start with this call's epoch records it).  The members of
+// the listed runs are then gathered into a compact array, sorted stably by
+// low bits and then by run (the same onesweep sort), and scattered back into
+// their runs' places.  More runs than the list holds raise *overflow and the
+// caller redoes a full 64-bit sort.  `lo` is 32 normally; 48 is a
+// verification hook that makes ties frequent.
+constexpr int TIE_BLK = 256;
+constexpr u32 TIE_LIST = 512;  // runs a workgroup collects in LDS before spilling
+constexpr unsigned TIE_GRID = 2048;
+__global__ __launch_bounds__(TIE_BLK) void tie_runs_kernel(
+    const u64* __restrict__ keys, u64 n, u32 lo, u32* __restrict__ mark, u32 epoch,
+    u64* __restrict__ runs, u64 cap, u32* __restrict__ count, u32* __restrict__ overflow) {
+    // Runs are gathered per workgroup in LDS and reserved in the global list
+    // with one atomic per workgroup (one global counter per break would
+    // serialise on a single address).
+    __shared__ u64 ls[2 * TIE_LIST];
+    __shared__ u32 lcount, lbase;
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
+    const u64 stride = (u64)gridDim.x * TIE_BLK;
+    for (u64 i = (u64)blockIdx.x * TIE_BLK + threadIdx.x; i < n; i += stride) {
+        if (i == 0) continue;
+        const u64 k = keys[i], pk = keys[i - 1];
+        const u64 top = k >> lo;
+        if (top != (pk >> lo) || k == pk) continue;  // not a break
+        // Run bounds by galloping outward from the break, then bisection, so
+        // a run of length L costs O(log L) probes.
+        u64 b = i - 1, step = 1;  // keys[b] is in the run
+        u64 a = b;
+        while (a > 0) {
+            const u64 m = a > step ? a - step : 0;
+            if ((keys[m] >> lo) != top) { a = m; break; }
+            b = m;
+            a = m;
+            step <<= 1;
         }
-        keys[b] = ka;
-        P[b] = pa;
+        if ((keys[a] >> lo) != top) {  // first index of the run lies in (a, b]
+            ++a;
+            while (a < b) {
+                const u64 m = a + (b - a) / 2;
+                if ((keys[m] >> lo) < top) a = m + 1; else b = m;
+            }
+        }
+        u64 c = i, d;  // keys[c] is in the run; find the first index past it
+        step = 1;
+        for (;;) {
+            const u64 m = c + step;
+            if (m >= n) { d = n; break; }
+            if ((keys[m] >> lo) != top) { d = m; break; }
+            c = m;
+            step <<= 1;
+        }
+        ++c;
+        while (c < d) {
+            const u64 m = c + (d - c) / 2;
+            if ((keys[m] >> lo) <= top) c = m + 1; else d = m;
+        }
+        if (atomicExch(&mark[a], epoch) == epoch) continue;  // run already listed
+        const u32 ls_slot = atomicAdd(&lcount, 1u);
+        if (ls_slot < TIE_LIST) {
+            ls[2 * ls_slot] = a;
+            ls[2 * ls_slot + 1] = c;
+        } else {  // LDS list full: take a global slot directly
+            const u32 slot = atomicAdd(count, 1u);
+            if (slot >= cap) {
+                atomicOr(overflow, 1u);
+            } else {
+                runs[2 * (u64)slot] = a;
+                runs[2 * (u64)slot + 1] = c;
+            }
+        }
     }
+    __syncthreads();
+    const u32 m = lcount < TIE_LIST ? lcount : TIE_LIST;
+    if (threadIdx.x == 0) lbase = m ? atomicAdd(count, m) : 0u;
+    __syncthreads();
+    for (u32 k = threadIdx.x; k < m; k += TIE_BLK) {
+        const u64 slot = (u64)lbase + k;
+        if (slot >= cap) {
+            atomicOr(overflow, 1u);
+        } else {
+            runs[2 * slot] = ls[2 * k];
+            runs[2 * slot + 1] = ls[2 * k + 1];
+        }
+    }
+}
+__global__ void tie_len_kernel(const u64* __restrict__ runs, const u32* __restrict__ count, u64 cap,
+                               u64* __restrict__ len) {
+    const u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= cap) return;
+    len[r] = r < *count ? runs[2 * r + 1] - runs[2 * r] : 0;
+}
+// Compact element j belongs to run r (largest r with pre[r] <= j).
+__global__ void tie_gather_kernel(const u64* __restrict__ keys, const u64* __restrict__ P,
+                                  const u64* __restrict__ runs, const u64* __restrict__ pre, u32 nr,
+                                  u64 total, u32 lo, u64* __restrict__ cl, u64* __restrict__ cv,
+                                  u64* __restrict__ crun, u64* __restrict__ cfull,
+                                  u64* __restrict__ cP, u64* __restrict__ corig) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= total) return;
+    u32 a = 0, b = nr - 1;
+    while (a < b) {
+        const u32 m = a + (b - a + 1) / 2;
+        if (pre[m] <= j) a = m; else b = m - 1;
+    }
+    const u64 i = runs[2 * (u64)a] + (j - pre[a]);
+    const u64 k = keys[i];
+    cl[j] = k & ((1ULL << lo) - 1);
+    cv[j] = j;
+    crun[j] = a;
+    cfull[j] = k;
+    cP[j] = P[i];
+    corig[j] = i;
+}
+__global__ void tie_key2_kernel(const u64* __restrict__ cv, const u64* __restrict__ crun, u64 total,
+                                u64* __restrict__ ck) {
+    const u64 m = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < total) ck[m] = crun[cv[m]];
+}
+__global__ void tie_scatter_kernel(const u64* __restrict__ cv, const u64* __restrict__ cfull,
+                                   const u64* __restrict__ cP, const u64* __restrict__ corig,
+                                   u64 total, u64* __restrict__ keys, u64* __restrict__ P) {
+    const u64 m = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= total) return;
+    const u64 src = cv[m];
+    keys[corig[m]] = cfull[src];
+    P[corig[m]] = cP[src];
 }
 
 constexpr int RANK_ITEMS = 16;
@@ -498,8 +580,9 @@ void ws_release(rbsc_ctx* ctx) {
     if (w.sig) (void)hipFree(w.sig);
     if (w.hc) (void)hipFree(w.hc);
     if (w.rank_status) (void)hipFree(w.rank_status);
-    if (w.tie_runs) (void)hipFree(w.tie_runs);
-    if (w.carry) (void)hipFree(w.carry);
+    for (void* p : {(void*)w.tie_runs, (void*)w.tie_pre, (void*)w.tie_tmp, (void*)w.tie_mark,
+                    (void*)w.tie_buf, (void*)w.carry})
+        if (p) (void)hipFree(p);
     w = Workspace();
     ctx->sort.extra_status = nullptr;
     ctx->sort.extra_words = 0;
@@ -518,6 +601,11 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     HIP_TRY(hipMalloc(&w.hc, sizeof(u64) * cap));
     w.tie_cap = cap / 16 + 1024;
     HIP_TRY(hipMalloc(&w.tie_runs, sizeof(u64) * 2 * w.tie_cap));
+    HIP_TRY(hipMalloc(&w.tie_pre, sizeof(u64) * (w.tie_cap + 1)));
+    HIP_TRY(hipMalloc(&w.tie_tmp, sizeof(u64) * scan_tmp_words(w.tie_cap)));
+    HIP_TRY(hipMalloc(&w.tie_mark, sizeof(u32) * cap));
+    HIP_TRY(hipMemsetAsync(w.tie_mark, 0, sizeof(u32) * cap, ctx->stream));
+    w.tie_epoch = 0;
     w.rank_tiles = (cap + RANK_TILE - 1) / RANK_TILE + 1;
     HIP_TRY(hipMalloc(&w.rank_status, sizeof(u64) * w.rank_tiles));
     HIP_TRY(hipMemsetAsync(w.rank_status, 0, sizeof(u64) * w.rank_tiles, ctx->stream));
@@ -792,6 +880,61 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         // Node side: keys in colour order, stable sort by signature, dense rank.
         // sorted_pc is the P buffer holding the sorted order (input of rank).
         int sorted_pc = pc;
+        // Orders every run of equal top bits by the low bits (see
+        // tie_runs_kernel); sets d_scalar[1] when the run list overflows.
+        auto fix_ties = [&](u64* keys, u64* P, u32 lo) -> int {
+            const u64 cap = (prm.flags & RBSC_FLAG_TIESMALL) ? 2 : w.tie_cap;
+            if (++w.tie_epoch == 0) {  // wrapped: clear the stamps
+                fill_u32_kernel<<<grid_for(w.cap_n), BLK, 0, s>>>(w.tie_mark, w.cap_n, 0);
+                w.tie_epoch = 1;
+            }
+            fill_u32<<<1, 1, 0, s>>>(ctx->d_scalar + 2, 1, 0);
+            const unsigned tg = (unsigned)std::min<u64>((n_rank + TIE_BLK - 1) / TIE_BLK, TIE_GRID);
+            tie_runs_kernel<<<tg, TIE_BLK, 0, s>>>(keys, n_rank, lo, w.tie_mark, w.tie_epoch,
+                                                   w.tie_runs, cap, ctx->d_scalar + 2,
+                                                   ctx->d_scalar + 1);
+            tie_len_kernel<<<grid_for(cap), BLK, 0, s>>>(w.tie_runs, ctx->d_scalar + 2, cap,
+                                                         w.tie_pre);
+            int r3 = scan_excl_u64(w.tie_pre, w.tie_pre, cap, w.tie_pre + cap, w.tie_tmp, s);
+            if (r3) return r3;
+            u64 total = 0;
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 1, ctx->d_scalar + 1, 2 * sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(&total, w.tie_pre + cap, sizeof(u64), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            const u32 nr = ctx->h_scalar[2];
+            if (ctx->h_scalar[1] || nr == 0) return RBSC_OK;  // overflow: the caller re-sorts
+            if (total > w.tie_buf_cap) {
+                if (w.tie_buf) HIP_TRY(hipFree(w.tie_buf));
+                w.tie_buf = nullptr;
+                w.tie_buf_cap = 0;
+                const u64 want = std::max<u64>(total, 4096);
+                HIP_TRY(hipMalloc(&w.tie_buf, sizeof(u64) * 8 * want));
+                w.tie_buf_cap = want;
+            }
+            const u64 C = w.tie_buf_cap;
+            u64* cl[2] = {w.tie_buf, w.tie_buf + C};
+            u64* cv[2] = {w.tie_buf + 2 * C, w.tie_buf + 3 * C};
+            u64 *crun = w.tie_buf + 4 * C, *cfull = w.tie_buf + 5 * C, *cP = w.tie_buf + 6 * C,
+                *corig = w.tie_buf + 7 * C;
+            tie_gather_kernel<<<grid_for(total), BLK, 0, s>>>(keys, P, w.tie_runs, w.tie_pre, nr,
+                                                             total, lo, cl[0], cv[0], crun, cfull,
+                                                             cP, corig);
+            int wa = 0;
+            r3 = radix_sort_u64(&ctx->sort, cl, cv, total, 0, (int)lo, &wa, s);  // by low bits
+            if (r3) return r3;
+            u64* ck[2] = {cl[wa ^ 1], cl[wa]};
+            u64* cw[2] = {cv[wa], cv[wa ^ 1]};
+            tie_key2_kernel<<<grid_for(total), BLK, 0, s>>>(cv[wa], crun, total, ck[0]);
+            int rb = 8;
+            while (rb < 32 && ((u64)(nr - 1) >> rb)) rb += 8;
+            int wb = 0;
+            r3 = radix_sort_u64(&ctx->sort, ck, cw, total, 0, rb, &wb, s);  // then by run
+            if (r3) return r3;
+            tie_scatter_kernel<<<grid_for(total), BLK, 0, s>>>(cw[wb], cfull, cP, corig, total,
+                                                               keys, P);
+            return RBSC_OK;
+        };
         auto sort_and_rank = [&](bool full, bool gather) -> int {
             if (gather) {
                 prof_begin(ctx, 1);
@@ -806,14 +949,8 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, full ? 0 : lo, 64, &which, s);
             if (r2) return r2;
             if (!full) {
-                const u64 cap = w.tie_cap;
-                fill_u32<<<1, 1, 0, s>>>(ctx->d_scalar + 2, 1, 0);
-                const u64 tie_max = (prm.flags & RBSC_FLAG_TIESMALL) ? 2 : TIE_MAX;
-                tie_find_kernel<<<grid_for(n_rank), BLK, 0, s>>>(kk[which], n_rank, lo, tie_max,
-                                                                 w.tie_runs, cap, ctx->d_scalar + 2,
-                                                                 ctx->d_scalar + 1);
-                tie_sort_kernel<<<grid_for(cap), BLK, 0, s>>>(kk[which], pp[which], w.tie_runs,
-                                                              ctx->d_scalar + 2, cap, lo);
+                r2 = fix_ties(kk[which], pp[which], lo);
+                if (r2) return r2;
             }
             prof_end(ctx);
             sorted_pc = (pp[which] == w.P[0]) ? 0 : 1;
@@ -828,7 +965,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             ctx->sort.ticket_total += tiles;
             return RBSC_OK;
         };
-        const bool full_sort = (prm.flags & RBSC_FLAG_SPLIT) == 0;
+        const bool full_sort = (prm.flags & RBSC_FLAG_FULLSORT) != 0;
         if (n_rank) {
             rc = sort_and_rank(full_sort, true);
             if (rc) return rc;
@@ -1094,7 +1231,16 @@ extern "C" int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_par
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    return dgraph_relayout(ctx, g, n_parts);
+    return dgraph_relayout(ctx, g, n_parts, 0);
+}
+
+extern "C" int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads) {
+    if (!ctx || !g || hot_heads == 0) {
+        rbsc_set_error("rbsc_graph_split_hot: null argument or empty hot range");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    return dgraph_relayout(ctx, g, 2, std::min<u32>(hot_heads, g->U ? g->U : 1));
 }
 
 extern "C" int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g) {

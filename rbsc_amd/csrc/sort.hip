@@ -81,7 +81,7 @@ __global__ void fill_u64_kernel(u64* p, u64 n, u64 v) {
 }
 
 template <bool HAS_VALS>
-__global__ __launch_bounds__(THREADS) void onesweep_kernel(
+__global__ __launch_bounds__(THREADS, 3) void onesweep_kernel(
     const u64* __restrict__ kin, const u64* __restrict__ vin, u64* __restrict__ kout,
     u64* __restrict__ vout, u64 n, u32 shift, const u64* __restrict__ digit_base,
     u64* status, u64* ticket, u64 ticket_base, u32 epoch, u32* err) {
@@ -107,6 +107,15 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(
     for (int j = 0; j < ITEMS; ++j) {
         const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
         key[j] = idx < n ? kin[idx] : ~0ULL;
+    }
+    // Values are loaded now so their latency hides behind ranking and look-back.
+    u64 val[HAS_VALS ? ITEMS : 1];
+    if (HAS_VALS) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
+            val[j] = idx < n ? vin[idx] : 0;
+        }
     }
     const u64 below_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
 #pragma unroll
@@ -194,7 +203,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
-            if (idx < n) s_stage[lpos[j]] = vin[idx];
+            if (idx < n) s_stage[lpos[j]] = val[j];
         }
         __syncthreads();
         for (u32 i = tid; i < tile_n; i += THREADS) vout[s_gbase[s_digit[i]] + i] = s_stage[i];

@@ -254,6 +254,25 @@ def test_destination_partitioned_layout(ctx, parts):
         dg.free()
 
 
+@pytest.mark.parametrize("hot", [1, 64, 1000, 1 << 30])
+def test_hot_cold_split_layout(ctx, hot):
+    """Lines into the `hot` most-gathered heads in a first pass and the rest
+    in a second give the same per-round colours (also hashed64)."""
+    for g in (gen.rmat(14, 1 << 18, clone_frac=0.2, seed=15),
+              _weighted(gen.rmat(12, 1 << 16, clone_frac=0.3, seed=3), 5),
+              gen.ba(20000, 4, 0.1, 3, seed=9)):
+        c, K, R, tr = _oracle_full(g, trace_rounds=64)
+        dg = ctx.upload(g)
+        dg.split_hot(hot)
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True)
+            assert np.array_equal(res.colors, tr[r - 1]), f"hot={hot} round {r}"
+        full = dg.refine(flags=rbsc_amd.FLAG_HASHED64)
+        assert (full.nodes, full.rounds) == (K, R)
+        assert np.array_equal(full.partition, oracle.canonical(c))
+        dg.free()
+
+
 @pytest.mark.parametrize("parts", [1, 4])
 def test_hashed64_per_node_hashing(ctx, parts):
     """Per-node hashed colours (instead of per-line hashing of raw colours)
@@ -291,10 +310,11 @@ def test_xcd_partitioned_single_launch(ctx, flags):
         dg.free()
 
 
-@pytest.mark.parametrize("flags", [0, 4, 4 | 8, 4 | 8 | 16, 4 | 16])
+@pytest.mark.parametrize("flags", [0, 32, 8, 8 | 16, 16])
 def test_sort_split_and_tie_fixup(ctx, flags):
-    """Full sort (default), top-half sort + run fix-up (4), frequent ties via a
-    16-bit split (8), and the full-sort fallback forced by tiny runs (16)."""
+    """Top-half sort + colliding-run fix-up (default, 0), full 64-bit sort
+    (32), frequent collisions via a 16-bit split (8), and the full-sort
+    fallback forced by a 2-entry run list (16)."""
     for g in (gen.rmat(15, 1 << 19, clone_frac=0.2, seed=15),
               gen.ba(30000, 3, 0.1, 3, seed=9)):
         c, K, R, tr = _oracle_full(g, trace_rounds=64)
