@@ -38,18 +38,55 @@ def log(*a):
 
 
 def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
-    """Single-core rsc restatement (oracle, C) on a bounded R-MAT sample."""
+    """Single-core CPU refinement of a bounded R-MAT sample of the same
+    family: the reference's own rsc and Valmari tools compiled from source
+    (oracle/_ref) when present, timed by their own timers (the figure the
+    reference's harness records), beside the oracle's rsc restatement."""
+    import subprocess
+    import tempfile
     import oracle
     from rbsc_amd import gen
     g = gen.rmat(scale, 1 << log_draws, clone_frac=clone, seed=seed)
+    E = g.n_edges
     src = g.sources()
     t = time.perf_counter()
     _, K, R = oracle.rsc_refine(g.n_nodes, src, g.dst)
     dt = time.perf_counter() - t
-    return {"value": R * g.n_edges / dt, "unit": "edge-visits/s", "cores": 1, "kind": "port",
-            "sample": f"rsc restatement (oracle/rbsc_oracle.c) on R-MAT scale {scale}, "
-                      f"2^{log_draws} draws, {int(clone * 100)}% clones: {g.n_edges} lines, "
-                      f"{R} rounds, {K} blocks, {dt:.2f} s"}
+    sample = (f"R-MAT scale {scale}, 2^{log_draws} draws, {int(clone * 100)}% clones: {E} lines, "
+              f"{R} rsc rounds, {K} blocks")
+    host = {"host_threads": os.cpu_count(), "cpu_model": _cpu_model()}
+    port = {"value": R * E / dt, "seconds": dt, "blocks": K}
+    out = {"value": port["value"], "unit": "edge-visits/s", "cores": 1, "kind": "port",
+           "sample": f"rsc restatement (oracle/rbsc_oracle.c) on {sample}", **host}
+    if not oracle.have_ref():
+        return out
+    with tempfile.TemporaryDirectory() as d:
+        lines, mdp = os.path.join(d, "g.txt"), os.path.join(d, "g.mdp")
+        g.write_text(lines, "lines")
+        g.write_text(mdp, "valmari")
+        runs = {}
+        for name, exe, path in (("rsc", oracle.REF_RSC, lines), ("valmari", oracle.REF_VALMARI, mdp)):
+            with open(path, "rb") as f:
+                p = subprocess.run([exe], stdin=f, stdout=subprocess.PIPE, timeout=600, check=True)
+            v = p.stdout.decode().split("\n")
+            runs[name] = {"seconds": float(v[0]), "blocks": int(v[2])}
+    out.update({"value": R * E / runs["rsc"]["seconds"], "kind": "reference",
+                "sample": f"reference rsc (src/rsc/rsc.cpp, compiled from source, its own timer) on "
+                          f"{sample}",
+                "port": port, "valmari": runs["valmari"],
+                "blocks_agree": runs["rsc"]["blocks"] == runs["valmari"]["blocks"] == K})
+    return out
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
@@ -187,6 +224,29 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
+def streaming_leg(dg, E: int, frac: float, steps: int) -> dict:
+    """The same graph with its lines in pinned host memory, refined in edge
+    batches of frac * E lines streamed over PCIe each round (double-buffered
+    copies on a side stream, overlapped with the signature kernel)."""
+    t0 = time.perf_counter()
+    dg.lines_to_host()
+    log(f"[bench] lines moved to pinned host memory in {time.perf_counter() - t0:.1f}s")
+    B = int(E * frac)
+    dg.refine(partition=False, max_batch_edges=B)
+    rounds, dev_ms = 0, 0.0
+    t = time.perf_counter()
+    for _ in range(steps):
+        r = dg.refine(partition=False, max_batch_edges=B)
+        rounds += r.rounds
+        dev_ms += r.device_ms
+    wall = time.perf_counter() - t
+    pcie = 4 * E  # every round streams the whole line array (unweighted heads)
+    return {"workload": "same graph, lines in pinned host memory", "batch_fraction": frac,
+            "max_batch_edges": B, "value": rounds * E / wall, "unit": "edge-visits/s",
+            "ms_per_round": dev_ms / rounds, "pcie_bytes_per_round": pcie,
+            "pcie_GBps": pcie * rounds / (dev_ms / 1e3) / 1e9, "blocks": r.nodes}
+
+
 def copy_probe(nbytes: int) -> None:
     """One plain device-to-device copy of nbytes through the HIP runtime, so a
     counter run can read FETCH_SIZE / WRITE_SIZE of a kernel of known size."""
@@ -209,7 +269,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="rmat_s27_d31_c20", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-scale", type=int, default=22)
+    ap.add_argument("--cpu-scale", type=int, default=21)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--parts", type=int, default=1,
                     help="destination partitions of the line layout (power of two)")
@@ -226,6 +286,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests per arrival rate in the serving leg (0 skips it)")
     ap.add_argument("--serve-rates", default="50,200,1000,2000")
+    ap.add_argument("--stream-frac", type=float, default=0.25,
+                    help="also time the pinned-host streaming path at this edge-batch fraction (0 skips)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -321,6 +383,10 @@ def main():
     if args.serve_requests:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         out["serving"] = serving_leg(ctx, rates, args.serve_requests, 64, 1005)
+    if args.stream_frac and args.parts == 1 and not args.hot:
+        out["streamed"] = streaming_leg(dg, E, args.stream_frac, max(1, min(args.steps, 2)))
+        log(f"[bench] streamed at {args.stream_frac}: {out['streamed']['value']:.3g} edge-visits/s, "
+            f"{out['streamed']['pcie_GBps']:.1f} GB/s over PCIe")
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_scale + 4, 0.20, seed)

@@ -50,6 +50,15 @@ int rbsc_gen_er_undirected(uint32_t n, double avg_deg, uint64_t seed, rbsc_host_
 int rbsc_gen_ba_undirected(uint32_t n, uint32_t m, uint64_t seed, rbsc_host_graph* out);
 int rbsc_gen_regular_undirected(uint32_t n, uint32_t k, uint64_t seed, rbsc_host_graph* out);
 
+/* Writes g as text: RBSC_TEXT_LINES is the reference's stdin format
+ * (rbsc.cu:168-180: "N\nE\n" then "src weight dst" lines); RBSC_TEXT_VALMARI
+ * is the 1-based MDP format the reference's valmari-adapter feeds part_ref
+ * (valmari-adapter:9-16, MDP_read.cc:51-64), with the initial colouring as
+ * initial blocks (MDP_read.cc:140-150). */
+#define RBSC_TEXT_LINES 0
+#define RBSC_TEXT_VALMARI 1
+int rbsc_host_graph_write_text(const rbsc_host_graph* g, const char* path, int format);
+
 /* A host batch allocated by the library (see rbsc_batch in rbsc.h). */
 typedef struct rbsc_host_batch {
     uint32_t n_graphs;

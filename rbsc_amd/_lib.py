@@ -174,6 +174,7 @@ SIGNATURES = {
     "rbsc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rbsc_refine_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Opts), u32p, u32p, u32p,
                                     C.POINTER(Result)]),
+    "rbsc_host_graph_write_text": (C.c_int, [C.POINTER(HostGraph), C.c_char_p, C.c_int]),
     "rbsc_host_batch_free": (None, [C.POINTER(HostBatch)]),
     "rbsc_gen_batch_mixed": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                        C.POINTER(HostBatch)]),

@@ -11,7 +11,11 @@
 // lines (bench.py:44-48) sees either a result or nothing parseable.
 //
 // Extra options (after the positionals):
-//   --mode stream|quotient|aggressive  edge-batch semantics when B < E
+//   --mode quotient|aggressive|stream  edge-batch semantics when B < E:
+//       quotient (default) is the reference's multi-batch contraction
+//       (rbsc.cu:274-487: `done` may be 0), aggressive its rbsc_aggressive.cu
+//       stop rule, stream the exact refinement with lines streamed from
+//       pinned host memory in batches of B (done = 1)
 //   --partition-out FILE   write the canonical partition, one label per line
 //   --init-colors FILE     N initial colours, whitespace separated
 //   --verify               exact stability check of the final partition
@@ -69,7 +73,7 @@ int fail(const char* what) {
 
 int main(int argc, char** argv) {
     std::vector<std::string> pos;
-    std::string mode = "stream", part_out, init_file;
+    std::string mode = "quotient", part_out, init_file;
     int verify = 0, device = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];

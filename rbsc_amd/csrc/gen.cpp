@@ -3,6 +3,7 @@
 #include "../../include/rbsc_gen.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -387,5 +388,99 @@ extern "C" int rbsc_gen_batch_mixed(uint32_t n_graphs, uint32_t n_min, uint32_t 
             }
         });
     for (auto& x : th) x.join();
+    return RBSC_OK;
+}
+
+// ------------------------------------------------------------ text output
+namespace {
+inline char* put_u64(char* p, uint64_t v) {
+    char tmp[24];
+    int k = 0;
+    do {
+        tmp[k++] = (char)('0' + v % 10);
+        v /= 10;
+    } while (v);
+    while (k) *p++ = tmp[--k];
+    return p;
+}
+}  // namespace
+
+extern "C" int rbsc_host_graph_write_text(const rbsc_host_graph* g, const char* path, int format) {
+    if (!g || !path || (format != RBSC_TEXT_LINES && format != RBSC_TEXT_VALMARI) ||
+        (g->n_edges && (!g->row_offsets || !g->dst))) {
+        rbsc_set_error("rbsc_host_graph_write_text: bad arguments");
+        return RBSC_ERR_INVALID;
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        rbsc_set_error("rbsc_host_graph_write_text: cannot open %s", path);
+        return RBSC_ERR_INPUT;
+    }
+    const int one = format == RBSC_TEXT_VALMARI ? 1 : 0;  // Valmari ids are 1-based
+    std::vector<char> buf(1 << 22);
+    char* p = buf.data();
+    char* end = buf.data() + buf.size() - 96;
+    bool ok = true;
+    auto flush = [&]() {
+        if (fwrite(buf.data(), 1, (size_t)(p - buf.data()), f) != (size_t)(p - buf.data())) ok = false;
+        p = buf.data();
+    };
+    if (format == RBSC_TEXT_LINES) {
+        // rbsc.cu:168-180: N, E, then "src weight dst"
+        p = put_u64(p, g->n_nodes);
+        *p++ = '\n';
+        p = put_u64(p, g->n_edges);
+        *p++ = '\n';
+    } else {
+        // MDP_read.cc:51-64 as valmari-adapter:9-16 writes it: "N 0 0 E blocks"
+        uint32_t blocks = 1;
+        if (g->init_color) {
+            std::vector<uint32_t> c(g->init_color, g->init_color + g->n_nodes);
+            std::sort(c.begin(), c.end());
+            blocks = (uint32_t)(std::unique(c.begin(), c.end()) - c.begin());
+            if (!blocks) blocks = 1;
+        }
+        p = put_u64(p, g->n_nodes);
+        memcpy(p, " 0 0 ", 5);
+        p += 5;
+        p = put_u64(p, g->n_edges);
+        *p++ = ' ';
+        p = put_u64(p, blocks);
+        *p++ = '\n';
+    }
+    for (uint32_t u = 0; u < g->n_nodes && ok; ++u)
+        for (uint64_t e = g->row_offsets[u]; e < g->row_offsets[u + 1]; ++e) {
+            p = put_u64(p, (uint64_t)u + one);
+            *p++ = ' ';
+            p = put_u64(p, g->weight ? g->weight[e] : 1);
+            *p++ = ' ';
+            p = put_u64(p, (uint64_t)g->dst[e] + one);
+            *p++ = '\n';
+            if (p >= end) flush();
+        }
+    if (format == RBSC_TEXT_VALMARI && g->init_color) {
+        // MDP_read.cc:140-150: every initial block but the first, members then 0
+        std::vector<std::pair<uint32_t, uint32_t>> cu(g->n_nodes);
+        for (uint32_t u = 0; u < g->n_nodes; ++u) cu[u] = {g->init_color[u], u};
+        std::sort(cu.begin(), cu.end());
+        for (size_t i = 0; i < cu.size() && ok; ++i) {
+            if (i == 0 || cu[i].first == cu[0].first) continue;
+            p = put_u64(p, (uint64_t)cu[i].second + 1);
+            const bool last = i + 1 == cu.size() || cu[i + 1].first != cu[i].first;
+            if (last) {
+                memcpy(p, " 0\n", 3);
+                p += 3;
+            } else {
+                *p++ = ' ';
+            }
+            if (p >= end) flush();
+        }
+    }
+    flush();
+    if (fclose(f) != 0) ok = false;
+    if (!ok) {
+        rbsc_set_error("rbsc_host_graph_write_text: write to %s failed", path);
+        return RBSC_ERR_INPUT;
+    }
     return RBSC_OK;
 }

@@ -222,17 +222,25 @@ def test_cli_batch_modes(tmp_path):
     src = g.sources()
     c, K, R, _ = _oracle_full(g)
     B = g.n_edges // 4
-    p = subprocess.run([CLI_PATH, str(B), "1"], input=g.to_text(), stdout=subprocess.PIPE,
-                       stderr=subprocess.PIPE, timeout=120)
+    p = subprocess.run([CLI_PATH, str(B), "1", "--mode", "stream"], input=g.to_text(),
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=120)
     assert p.returncode == 0, p.stderr.decode()
     lines = p.stdout.decode().split("\n")
     assert int(lines[1]) == 1 and int(lines[2]) == K
     want = oracle.rbsc_multibatch(g.n_nodes, src, g.dst, None, B)
-    p = subprocess.run([CLI_PATH, str(B), "1", "--mode", "quotient"], input=g.to_text(),
+    # the reference's semantics are the default for B < E
+    for extra in ([], ["--mode", "quotient"]):
+        p = subprocess.run([CLI_PATH, str(B), "1"] + extra, input=g.to_text(),
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=120)
+        assert p.returncode == 0, p.stderr.decode()
+        lines = p.stdout.decode().split("\n")
+        assert (int(lines[2]), int(lines[1])) == want[:2]
+    want_a = oracle.rbsc_multibatch(g.n_nodes, src, g.dst, None, B, aggressive=True)
+    p = subprocess.run([CLI_PATH, str(B), "1", "--mode", "aggressive"], input=g.to_text(),
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=120)
     assert p.returncode == 0, p.stderr.decode()
     lines = p.stdout.decode().split("\n")
-    assert (int(lines[2]), int(lines[1])) == want[:2]
+    assert (int(lines[2]), int(lines[1])) == want_a[:2]
 
 
 @pytest.mark.parametrize("parts", [2, 4, 8, 16])

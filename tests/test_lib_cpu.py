@@ -147,3 +147,17 @@ def test_host_batch_from_graphs_roundtrip():
     gs[1].init_color = np.zeros(gs[1].n_nodes, np.uint32)
     with pytest.raises(ValueError):
         HostBatch.from_graphs(gs)
+
+
+def test_write_text_matches_python_formats(tmp_path):
+    from rbsc_amd import gen
+    g = gen.ba(300, 3, 0.2, 3, seed=4)
+    g.weight = np.random.default_rng(2).integers(1, 9, g.n_edges).astype(np.uint64)
+    for fmt, want in (("lines", g.to_text()), ("valmari", g.to_valmari_text())):
+        p = tmp_path / f"g.{fmt}"
+        g.write_text(str(p), fmt)
+        assert p.read_bytes() == want
+    h = gen.rmat(8, 2000, clone_frac=0.3, seed=3)  # unweighted, no colours
+    p = tmp_path / "h.lines"
+    h.write_text(str(p))
+    assert p.read_bytes() == h.to_text()
