@@ -71,6 +71,14 @@ SPECS = [
 ]
 
 
+# The two named benchmark workloads beside the R-MAT scale-27 one: checked
+# on the GPU against these counts (Valmari alone takes about a minute on C2).
+LARGE_SPECS = [
+    ("rmat_2p20_clone30", gen.rmat_2p20_clone30),
+    ("ba_8m_m6_init3", gen.ba_8m_m6_init3),
+]
+
+
 def graph_hash(g: HostGraph) -> str:
     h = hashlib.sha256()
     h.update(np.uint64(g.n_nodes).tobytes())
@@ -83,18 +91,30 @@ def graph_hash(g: HostGraph) -> str:
     return h.hexdigest()
 
 
+def run_tool(exe, g: HostGraph, fmt: str) -> int:
+    """Block count printed by a reference tool on g (written by the library's
+    text writer, byte-identical to HostGraph.to_text / to_valmari_text)."""
+    import subprocess
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix="." + fmt) as f:
+        g.write_text(f.name, fmt)
+        with open(f.name, "rb") as inp:
+            p = subprocess.run([exe], stdin=inp, stdout=subprocess.PIPE, timeout=3600, check=True)
+    return int(p.stdout.decode().split("\n")[2])
+
+
 def main():
     if not oracle.have_ref():
         raise SystemExit("build oracle/_ref first: make -C oracle ref")
     out = []
-    for name, fn in SPECS:
+    for name, fn in SPECS + LARGE_SPECS:
         g = fn()
         rec = {"name": name, "n_nodes": g.n_nodes, "n_edges": g.n_edges, "sha256": graph_hash(g)}
         if g.init_color is None:
-            rec["rsc_nodes"] = oracle.run_ref_rsc(g.to_text())[2]
-        rec["valmari_nodes"] = oracle.run_ref_valmari(g.to_valmari_text())[2]
+            rec["rsc_nodes"] = run_tool(oracle.REF_RSC, g, "lines")
+        rec["valmari_nodes"] = run_tool(oracle.REF_VALMARI, g, "valmari")
         out.append(rec)
-        print(rec)
+        print(rec, flush=True)
     with open(OUT, "w") as f:
         json.dump({"tools": {"rsc": "src/rsc/rsc.cpp (g++ -std=c++17 -O2)",
                              "valmari": "src/part_ref/MDPmin.cc (g++ -std=c++11 -O2)"},

@@ -56,15 +56,40 @@ def test_per_round_colors_bit_exact(ctx, name):
     dg.free()
 
 
-def _golden_specs():
+def _make_golden():
     import importlib.util
     spec = importlib.util.spec_from_file_location("make_golden",
                                                   os.path.join(HERE, "golden", "make_golden.py"))
     mg = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mg)
+    return mg
+
+
+def _golden_specs(large=False):
+    mg = _make_golden()
     with open(os.path.join(HERE, "golden", "fixtures.json")) as f:
         recs = {r["name"]: r for r in json.load(f)["fixtures"]}
-    return [(n, fn, recs[n]) for n, fn in mg.SPECS]
+    return [(n, fn, recs[n]) for n, fn in (mg.LARGE_SPECS if large else mg.SPECS)]
+
+
+@pytest.mark.parametrize("spec", _golden_specs(large=True), ids=lambda s: s[0])
+def test_named_workloads_match_reference_tools(ctx, spec):
+    """The 2^20-node R-MAT (30% clones) and the 8M-node BA multigraph with 3
+    initial colours: block counts equal to those the reference's rsc and
+    Valmari printed for the same graphs, the exact stability check passes,
+    and the partition equals the oracle's."""
+    name, fn, rec = spec
+    g = fn()
+    assert _make_golden().graph_hash(g) == rec["sha256"], "generator output changed"
+    dg = ctx.upload(g)
+    r = dg.refine(verify=True)
+    dg.free()
+    assert r.nodes == rec["valmari_nodes"] and r.verified == 1
+    if "rsc_nodes" in rec:
+        assert r.nodes == rec["rsc_nodes"]
+    c, K, R, _ = _oracle_full(g)
+    assert (r.nodes, r.rounds) == (K, R)
+    assert np.array_equal(r.partition, oracle.canonical(c))
 
 
 @pytest.mark.parametrize("spec", _golden_specs(), ids=lambda s: s[0])

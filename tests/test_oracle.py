@@ -56,9 +56,12 @@ def test_randomize_is_a_bijection():
         assert _inv_randomize(oracle.randomize(v)) == v
 
 
-def _golden():
+LARGE = ("rmat_2p20_clone30", "ba_8m_m6_init3")  # checked on the GPU and below
+
+
+def _golden(large=False):
     with open(GOLDEN) as f:
-        return json.load(f)["fixtures"]
+        return [r for r in json.load(f)["fixtures"] if (r["name"] in LARGE) == large]
 
 
 def _build(name):
@@ -89,6 +92,24 @@ def test_oracle_matches_reference_tools(rec):
     ec, eK, _ = oracle.exact_refine(g.n_nodes, src, g.dst, g.weight, g.init_color)
     assert eK == K
     assert np.array_equal(oracle.canonical(ec), oracle.canonical(colors))
+
+
+def test_oracle_on_rmat_2p20_clone30_matches_reference_tools():
+    """The 2^20-node R-MAT workload (16.8M lines): the oracle's rbsc and rsc
+    restatements give the block count the reference's rsc and Valmari
+    printed for the same graph (recorded by tests/golden/make_golden.py)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_golden", os.path.join(os.path.dirname(__file__), "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    rec = {r["name"]: r for r in _golden(large=True)}["rmat_2p20_clone30"]
+    g = gen.rmat_2p20_clone30()
+    assert mg.graph_hash(g) == rec["sha256"]
+    src = g.sources()
+    _, K, _, _ = oracle.rbsc_refine(g.n_nodes, src, g.dst)
+    _, rK, _ = oracle.rsc_refine(g.n_nodes, src, g.dst)
+    assert K == rK == rec["rsc_nodes"] == rec["valmari_nodes"]
 
 
 @pytest.mark.skipif(not oracle.have_ref(), reason="reference tools not built (no reference tree)")
