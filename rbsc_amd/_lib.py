@@ -11,6 +11,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librbsc.so")
+WEBGRAPH_TO_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "bin", "webgraph-to")
 CLI_PATH = os.path.join(_HERE, "bin", "rbsc")
 
 RBSC_OK = 0
@@ -174,6 +175,8 @@ SIGNATURES = {
     "rbsc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rbsc_refine_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Opts), u32p, u32p, u32p,
                                     C.POINTER(Result)]),
+    "rbsc_bvgraph_load": (C.c_int, [C.c_char_p, C.POINTER(HostGraph)]),
+    "rbsc_bvgraph_check_offsets": (C.c_int, [C.c_char_p, u64p, u64p]),
     "rbsc_host_graph_write_text": (C.c_int, [C.POINTER(HostGraph), C.c_char_p, C.c_int]),
     "rbsc_host_batch_free": (None, [C.POINTER(HostBatch)]),
     "rbsc_gen_batch_mixed": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,

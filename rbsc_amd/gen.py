@@ -84,3 +84,19 @@ def batch_mixed(n_graphs: int, n_min=10, n_max=5000, seed=1004) -> HostBatch:
     finally:
         load().rbsc_host_batch_free(C.byref(hb))
     return HostBatch(no, lo, src, dst)
+
+
+def bvgraph(basename: str) -> HostGraph:
+    """A WebGraph BVGraph file (BASENAME.graph + .properties), decoded by the
+    library, lines in (src, dst) order."""
+    hg = _lib.HostGraph()
+    check(load().rbsc_bvgraph_load(basename.encode(), C.byref(hg)))
+    return _take(hg)
+
+
+def bvgraph_check_offsets(basename: str):
+    """(offsets read from BASENAME.offsets, nodes whose decoded position differs)."""
+    n = C.c_uint64()
+    m = C.c_uint64()
+    check(load().rbsc_bvgraph_check_offsets(basename.encode(), C.byref(n), C.byref(m)))
+    return int(n.value), int(m.value)

@@ -20,10 +20,12 @@ parse (time-out, crash) records ``time = TIMEOUT, done = 0, nodes = 0``
 columns run at edge batches of 100 %, 50 % and 25 % (bench.py:83-89).
 
 Inputs: ``tool = 'file'`` reads ``datasets/{name}.txt`` as the reference
-does (bench.py:9, 36-37); ``tool = 'gen'`` names a seeded workload of
-:data:`WORKLOADS`, written as text by the library; ``tool = 'law'`` needs a
-BVGraph decoder, which this build does not have: such rows fail (status -1)
-with a message.  The sigref columns stay empty results, as in the
+does (bench.py:9, 36-37); ``tool = 'law'`` reads the BVGraph
+``{name}/{name}.graph`` (where the reference's download.sh puts it) through
+this build's decoder, ``webgraph-to arcs-cuda|arcs-valmari`` in place of
+``./webgraph to ...`` (bench.py:8); nothing is downloaded, so a missing
+graph fails its row; ``tool = 'gen'`` names a seeded workload of
+:data:`WORKLOADS`, written as text by the library.  The sigref columns stay empty results, as in the
 reference, which removed that experiment (bench.py:92-103).
 
 Usage::
@@ -71,6 +73,8 @@ def _workloads() -> Dict[str, Callable]:
     }
 
 
+WEBGRAPH_TO = os.path.join(HERE, "bin", "webgraph-to")
+
 WORKLOADS = ("rmat_s20_d24_c30", "ba_8m_m6_init3", "rmat_s16_d20_c20_seed1010",
              "rmat_s12_d16_c30_seed1011", "er_n100000_d6_seed1012", "reg3_n20000_seed1013")
 
@@ -108,10 +112,21 @@ class Inputs:
     """Text inputs of one row: the reference's line format for rbsc and rsc,
     the MDP format for part_ref (what valmari-adapter writes)."""
 
-    def __init__(self, name: str, tool: str, workdir: str, datasets: str):
+    def __init__(self, name: str, tool: str, workdir: str, datasets: str, law_dir: str = "."):
         self.name, self.tool = name, tool
         self.dir = workdir
-        if tool == "file":
+        self.init = None
+        self.cmd_lines = self.cmd_mdp = None
+        if tool == "law":
+            base = os.path.join(law_dir, name, name)
+            if not os.path.exists(base + ".graph"):
+                raise FileNotFoundError(f"{base}.graph (LAW graphs are not downloaded here)")
+            q = shlex.quote
+            out = subprocess.run([WEBGRAPH_TO, "gdata", base], stdout=subprocess.PIPE, check=True)
+            self.nodes, self.edges = (int(v) for v in out.stdout.split())
+            self.cmd_lines = f"{q(WEBGRAPH_TO)} arcs-cuda {q(base)}"
+            self.cmd_mdp = f"{q(WEBGRAPH_TO)} arcs-valmari {q(base)}"
+        elif tool == "file":
             self.lines = os.path.join(datasets, f"{name}.txt")
             if not os.path.exists(self.lines):
                 raise FileNotFoundError(self.lines)
@@ -130,17 +145,15 @@ class Inputs:
             self.mdp = os.path.join(workdir, f"{name}.mdp")
             g.write_text(self.lines, "lines")
             g.write_text(self.mdp, "valmari")
-            self.init = None
             if g.init_color is not None:
                 self.init = os.path.join(workdir, f"{name}.init")
                 with open(self.init, "w") as f:
                     f.write("\n".join(map(str, g.init_color.tolist())) + "\n")
         else:
-            raise RuntimeError(f"row {name!r}: tool {tool!r} needs a BVGraph decoder, "
-                               "which this build does not have")
-
-    def cat(self, path: str) -> str:
-        return f"cat {shlex.quote(path)}"
+            raise RuntimeError(f"row {name!r}: unknown input tool {tool!r}")
+        if self.cmd_lines is None:
+            self.cmd_lines = f"cat {shlex.quote(self.lines)}"
+            self.cmd_mdp = f"cat {shlex.quote(self.mdp)}"
 
 
 def _lines_to_mdp(src: str, dst: str) -> None:
@@ -178,7 +191,7 @@ def init_db(path: str, large: bool, names=WORKLOADS) -> None:
 
 def run_db(path: str, gpus: int, large: bool = False, batch: Optional[int] = None,
            datasets: str = "datasets", rbsc: str = RBSC, rsc: str = RSC, part_ref: str = PART_REF,
-           timeout: Optional[int] = None) -> int:
+           timeout: Optional[int] = None, law_dir: str = ".") -> int:
     """Runs every row with status 0 (in order of edges); returns rows done."""
     T = timeout if timeout is not None else (TIMEOUT_LARGE if large else TIMEOUT)
     con = sqlite3.connect(path)
@@ -190,22 +203,22 @@ def run_db(path: str, gpus: int, large: bool = False, batch: Optional[int] = Non
         name = rc["name"]
         try:
             with tempfile.TemporaryDirectory() as wd:
-                inp = Inputs(name, rc["tool"], wd, datasets)
-                extra = f" --init-colors {shlex.quote(inp.init)}" if getattr(inp, "init", None) else ""
+                inp = Inputs(name, rc["tool"], wd, datasets, law_dir)
+                extra = f" --init-colors {shlex.quote(inp.init)}" if inp.init else ""
                 q = shlex.quote
-                valmari = experiment(f"{inp.cat(inp.mdp)} | timeout {T} {q(part_ref)}", T)
+                valmari = experiment(f"{inp.cmd_mdp} | timeout {T} {q(part_ref)}", T)
                 # rsc has no initial-partition input: it runs on the uncoloured
                 # graph, as the reference would
-                cpu = experiment(f"{inp.cat(inp.lines)} | timeout {T} {q(rsc)}", T)
+                cpu = experiment(f"{inp.cmd_lines} | timeout {T} {q(rsc)}", T)
                 if large:
                     edges = batch if batch else inp.edges
-                    cuda = {1: experiment(f"{inp.cat(inp.lines)} | timeout {T} {q(rbsc)} {edges} "
+                    cuda = {1: experiment(f"{inp.cmd_lines} | timeout {T} {q(rbsc)} {edges} "
                                           f"{gpus}{extra}", T)}
                     con.execute("UPDATE results SET nodes=?,edges=?,valmari=?,cpu=?,cuda=?,status=? "
                                 "WHERE name=?", (inp.nodes, inp.edges, valmari.cell(), cpu.cell(),
                                                  cuda[1].cell(), 1, name))
                 else:
-                    cuda = {p: experiment(f"{inp.cat(inp.lines)} | timeout {T} {q(rbsc)} "
+                    cuda = {p: experiment(f"{inp.cmd_lines} | timeout {T} {q(rbsc)} "
                                           f"{int(inp.edges * p)} {gpus}{extra}", T)
                             for p in (1, 0.50, 0.25)}
                     mc = Result()
@@ -238,11 +251,13 @@ def main(argv=None) -> int:
     b.add_argument("--batch", type=int, default=None, help="edge batch of the large table")
     b.add_argument("--datasets", default="datasets")
     b.add_argument("--timeout", type=int, default=None)
+    b.add_argument("--law-dir", default=".", help="directory holding NAME/NAME.graph")
     args = ap.parse_args(argv)
     if args.cmd == "init":
         init_db(args.db, args.large, tuple(args.only) if args.only else WORKLOADS)
         return 0
-    run_db(args.db, args.gpus, args.large, args.batch, args.datasets, timeout=args.timeout)
+    run_db(args.db, args.gpus, args.large, args.batch, args.datasets, timeout=args.timeout,
+           law_dir=args.law_dir)
     return 0
 
 

@@ -7,7 +7,8 @@ reference sources by oracle/Makefile into oracle/_ref/.  Graphs come from the
 seeded generators of rbsc_amd.gen (host side, no GPU), identified by their
 arguments and a SHA-256 of their arrays so a generator change is detected.
 
-Run from the repository root:  python tests/golden/make_golden.py
+Run from the repository root:  python tests/golden/make_golden.py [NAMES...]
+(with names, only those are recomputed and the other records are kept).
 """
 from __future__ import annotations
 
@@ -77,6 +78,11 @@ LARGE_SPECS = [
     ("rmat_2p20_clone30", gen.rmat_2p20_clone30),
     ("ba_8m_m6_init3", gen.ba_8m_m6_init3),
 ]
+# The LAW graph the reference's tests hold (BVGraph, decoded by the library).
+BVGRAPH_SPECS = [
+    ("cnr-2000", lambda: gen.bvgraph(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                  "bvgraph", "cnr-2000"))),
+]
 
 
 def graph_hash(g: HostGraph) -> str:
@@ -107,7 +113,15 @@ def main():
     if not oracle.have_ref():
         raise SystemExit("build oracle/_ref first: make -C oracle ref")
     out = []
-    for name, fn in SPECS + LARGE_SPECS:
+    old = {}
+    if os.path.exists(OUT):
+        with open(OUT) as f:
+            old = {r["name"]: r for r in json.load(f)["fixtures"]}
+    only = set(sys.argv[1:])  # names to (re)compute; others are kept as recorded
+    for name, fn in SPECS + LARGE_SPECS + BVGRAPH_SPECS:
+        if only and name not in only and name in old:
+            out.append(old[name])
+            continue
         g = fn()
         rec = {"name": name, "n_nodes": g.n_nodes, "n_edges": g.n_edges, "sha256": graph_hash(g)}
         if g.init_color is None:

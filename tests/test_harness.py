@@ -68,7 +68,7 @@ def test_run_life_cycle_with_stand_in_tools(tmp_path):
     for col in (6, 7, 8):
         assert json.loads(r[col]) == {"done": 1, "nodes": 7, "time": 0.25}
     assert json.loads(r[9]) == json.loads(r[10]) == {"done": 0, "nodes": 0, "time": 0.0}
-    assert rows["missing"][11] == -1 and rows["web"][11] == -1
+    assert rows["missing"][11] == -1 and rows["web"][11] == -1  # no such file / graph here
     assert rows["old"][4] == "x" and rows["old"][11] == 1  # finished rows are not rerun
     calls = log.read_text().split("\n")
     assert "60000 1" in calls and "30000 1" in calls and "15000 1" in calls  # 100/50/25 %
@@ -121,3 +121,23 @@ def test_harness_with_the_rbsc_command(tmp_path):
                                                 int(g.n_edges * p))
         cell = json.loads(r[col])
         assert (cell["nodes"], cell["done"]) == (nodes, done)
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="reference tools not built")
+def test_law_row_through_the_bvgraph_decoder(tmp_path):
+    """A 'law' row as the reference lays it out (NAME/NAME.graph), read by
+    webgraph-to: Valmari and rsc give the recorded cnr-2000 block count."""
+    src = os.path.join(os.path.dirname(__file__), "golden", "bvgraph")
+    (tmp_path / "cnr-2000").mkdir()
+    for ext in ("graph", "properties", "offsets"):
+        os.symlink(os.path.join(src, f"cnr-2000.{ext}"), tmp_path / "cnr-2000" / f"cnr-2000.{ext}")
+    db = str(tmp_path / "b.db")
+    con = sqlite3.connect(db)
+    harness.ensure_table(con, True)
+    con.execute("INSERT INTO results VALUES ('cnr-2000', 'law', 1, 1, '', '', '', 0)")
+    con.commit()
+    rbsc = _tool(tmp_path, "rbsc", "cat > /dev/null; printf '0\\n1\\n0\\n'")
+    assert harness.run_db(db, 1, large=True, rbsc=rbsc, timeout=600, law_dir=str(tmp_path)) == 1
+    r = con.execute("SELECT nodes, edges, valmari, cpu, status FROM results").fetchone()
+    assert r[:2] == (325557, 3216152) and r[4] == 1
+    assert json.loads(r[2])["nodes"] == 85418 and json.loads(r[3])["nodes"] == 85418

@@ -56,7 +56,7 @@ def test_randomize_is_a_bijection():
         assert _inv_randomize(oracle.randomize(v)) == v
 
 
-LARGE = ("rmat_2p20_clone30", "ba_8m_m6_init3")  # checked on the GPU and below
+LARGE = ("rmat_2p20_clone30", "ba_8m_m6_init3", "cnr-2000")  # checked elsewhere
 
 
 def _golden(large=False):
