@@ -240,7 +240,7 @@ def streaming_leg(dg, E: int, frac: float, steps: int) -> dict:
         rounds += r.rounds
         dev_ms += r.device_ms
     wall = time.perf_counter() - t
-    pcie = 4 * E  # every round streams the whole line array (unweighted heads)
+    pcie = 4 * dg.stored_lines  # every round streams the stored line array (unweighted heads)
     return {"workload": "same graph, lines in pinned host memory", "batch_fraction": frac,
             "max_batch_edges": B, "value": rounds * E / wall, "unit": "edge-visits/s",
             "ms_per_round": dev_ms / rounds, "pcie_bytes_per_round": pcie,
@@ -336,16 +336,18 @@ def main():
     visits = rounds * E
     value = visits / wall
     ms_round = dev_ms / rounds
-    U = dg.rows
-    # Whole round: 8 B per line (head id + neighbour colour) and 32 B per
-    # node (row offset, signature write and read, old and new colour).
+    U = dg.rows            # rows the signature pass visits (identical rows once)
+    E_st = dg.stored_lines  # their lines
+    # Whole round: 8 B per input line (head id + neighbour colour) and 32 B
+    # per node (row offset, signature write and read, old and new colour).
     b_round = 8 * E + 32 * N
-    # Dominant kernel, the signature pass: 8 B per line plus 16 B per row
-    # (row offset, signature write); one launch per round.
-    b_sig = 8 * E + 16 * U
+    # Dominant kernel, the signature pass: 8 B per line it reads plus 16 B
+    # per row it writes (row offset, signature); one launch per round.
+    b_sig = 8 * E_st + 16 * U
     per_round = {k: phases[k][0] / rounds for k in phases if k != "setup"}
-    round_ms = sum(per_round[k] for k in ("signature", "gather", "sort", "rank"))
-    dom = max(("signature", "gather", "sort", "rank"), key=lambda k: per_round[k])
+    ROUND = ("signature", "finish", "gather", "sort", "rank")
+    round_ms = sum(per_round[k] for k in ROUND)
+    dom = max(ROUND, key=lambda k: per_round[k])
     sig_ms = per_round["signature"]
     traffic = measured_traffic(args.config, "sig_kernel") if args.parts == 1 and not args.hot else None
     out = {
@@ -362,6 +364,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "rows": U,
+                   "stored_lines": E_st,
                    "parts": args.parts, "hot": args.hot, "fullsort": bool(args.fullsort), "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
                    "rounds_per_step": rounds // args.steps,
                    "blocks": r.nodes, "ms_per_round": ms_round},

@@ -160,10 +160,10 @@ int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts);
 int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
 
 /* Per-phase device timing with hipEvents on the context's stream (off by
- * default).  Phases: 0 signature kernel + row fix-up, 1 key gather,
- * 2 radix sort, 3 rank/scan, 4 round-0 set-up.  rbsc_ctx_profile_read fills
- * ms[5] (summed milliseconds) and launches[5] (phase executions) and resets
- * the sums when reset != 0.  No counterpart in the reference, which times
+ * default).  Phases: 0 signature kernel, 1 key gather, 2 radix sort,
+ * 3 rank/scan, 4 round-0 set-up, 5 signature finish (tile carries, repeated
+ * rows).  rbsc_ctx_profile_read fills ms[6] (summed milliseconds) and
+ * launches[6] (phase executions) and resets the sums when reset != 0.  No counterpart in the reference, which times
  * only the whole run (rbsc.cu:272, 489-490). */
 int rbsc_ctx_profile(rbsc_ctx* ctx, int enable);
 int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int reset);
@@ -285,8 +285,14 @@ int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_draws, double
 /* Copy a resident graph back to host CSR (for checks): row_offsets has
  * n_nodes + 1 entries, dst n_edges entries.  Sizes via rbsc_graph_dims. */
 int rbsc_graph_dims(const rbsc_dgraph* g, uint32_t* n_nodes, uint64_t* n_edges);
-/* Nodes with at least one line (the rows the signature pass visits). */
+/* Rows and lines the signature pass visits: nodes with lines, one per group
+ * of identical rows (same heads and weights in the same order), whose
+ * signatures the repeats copy.  No counterpart in the reference. */
 uint32_t rbsc_graph_rows(const rbsc_dgraph* g);
+uint64_t rbsc_graph_stored_lines(const rbsc_dgraph* g);
+/* Graphs built on ctx after this call store identical rows once (on, the
+ * default) or every row (off); results are identical either way. */
+int rbsc_ctx_set_dedup(rbsc_ctx* ctx, int on);
 int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t* row_offsets,
                         uint32_t* dst);
 

@@ -180,8 +180,13 @@ class DeviceGraph:
 
     @property
     def rows(self) -> int:
-        """Nodes with at least one line."""
+        """Rows the signature pass visits (identical rows stored once)."""
         return int(load().rbsc_graph_rows(self.handle))
+
+    @property
+    def stored_lines(self) -> int:
+        """Lines the signature pass visits."""
+        return int(load().rbsc_graph_stored_lines(self.handle))
 
     @property
     def nbytes(self) -> int:
@@ -396,14 +401,14 @@ class Context:
         except Exception:
             pass
 
-    PHASES = ("signature", "gather", "sort", "rank", "setup")
+    PHASES = ("signature", "gather", "sort", "rank", "setup", "finish")
 
     def profile(self, enable: bool = True):
         check(load().rbsc_ctx_profile(self.handle, 1 if enable else 0))
 
     def profile_read(self, reset: bool = True) -> dict:
-        ms = (C.c_double * 5)()
-        n = (C.c_uint64 * 5)()
+        ms = (C.c_double * 6)()
+        n = (C.c_uint64 * 6)()
         check(load().rbsc_ctx_profile_read(self.handle, ms, n, 1 if reset else 0))
         return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(self.PHASES)}
 
@@ -422,6 +427,10 @@ class Context:
         h = C.c_void_p()
         check(load().rbsc_graph_upload(self.handle, C.byref(st), C.byref(h)))
         return DeviceGraph(self, h)
+
+    def set_dedup(self, on: bool) -> None:
+        """Store identical rows once in graphs built from now on (default on)."""
+        check(load().rbsc_ctx_set_dedup(self.handle, 1 if on else 0))
 
     def upload_batch(self, b: HostBatch) -> DeviceBatch:
         st = b.as_struct()

@@ -185,6 +185,8 @@ SIGNATURES = {
                                        C.c_double, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "rbsc_graph_dims": (C.c_int, [C.c_void_p, u32p, u64p]),
     "rbsc_graph_rows": (C.c_uint32, [C.c_void_p]),
+    "rbsc_graph_stored_lines": (C.c_uint64, [C.c_void_p]),
+    "rbsc_ctx_set_dedup": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_graph_split_hot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "rbsc_graph_download": (C.c_int, [C.c_void_p, C.c_void_p, u64p, u32p]),
     "rbsc_host_graph_free": (None, [C.POINTER(HostGraph)]),

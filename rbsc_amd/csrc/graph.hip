@@ -18,6 +18,11 @@
 //     kernel finds row boundaries from one byte per 8 lines instead of
 //     searching row offsets.
 //   * pi[orig] = internal id, to map colours back.
+//   * identical rows (same heads and weights in the same order) are stored
+//     once: a signature depends only on a row's lines, so every round gives
+//     identical rows identical signatures.  Rows that repeat an earlier
+//     stored row get internal ids after the stored rows ([R, U)) and no lines;
+//     dup_rep[j - R] names the stored row whose signature they copy.
 #include "internal.h"
 #include "scan.h"
 #include <algorithm>
@@ -43,19 +48,28 @@ __global__ void indeg_kernel(const u32* __restrict__ dst, u64 E, u32* __restrict
 
 // keys: (~in-degree, original id) so an ascending sort gives in-degree
 // descending; vals: the raw row index.  Also marks nodes that have rows.
+// With rep_raw (identical rows found), rows that repeat another go after
+// the stored rows: key bit 63 = duplicate, in-degree clamped to 31 bits.
 __global__ void order_keys_kernel(const u32* __restrict__ row_node, u64 U,
-                                  const u32* __restrict__ indeg, u64* __restrict__ keys,
-                                  u64* __restrict__ vals, u64* __restrict__ dangling) {
+                                  const u32* __restrict__ indeg, const u32* __restrict__ rep_raw,
+                                  u64* __restrict__ keys, u64* __restrict__ vals,
+                                  u64* __restrict__ dangling) {
     const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= U) return;
     const u32 v = row_node[k];
-    keys[k] = ((u64)(0xFFFFFFFFu - indeg[v]) << 32) | v;
+    if (rep_raw) {
+        const u64 dup = rep_raw[k] != (u32)k ? 1ULL : 0ULL;
+        const u32 d = indeg[v] < 0x7FFFFFFFu ? indeg[v] : 0x7FFFFFFFu;
+        keys[k] = (dup << 63) | ((u64)(0x7FFFFFFFu - d) << 32) | v;
+    } else {
+        keys[k] = ((u64)(0xFFFFFFFFu - indeg[v]) << 32) | v;
+    }
     vals[k] = k;
     dangling[v] = 0;
 }
 
 __global__ void place_rows_kernel(const u64* __restrict__ keys, const u64* __restrict__ vals,
-                                  u64 U, const u64* __restrict__ row_ptr_raw,
+                                  u64 U, u64 R, const u64* __restrict__ row_ptr_raw,
                                   u32* __restrict__ pi, u64* __restrict__ deg_new,
                                   u32* __restrict__ src_row) {
     const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -63,8 +77,21 @@ __global__ void place_rows_kernel(const u64* __restrict__ keys, const u64* __res
     const u32 v = (u32)keys[j];
     const u64 k = vals[j];
     pi[v] = (u32)j;
-    deg_new[j] = row_ptr_raw[k + 1] - row_ptr_raw[k];
+    if (j < R) deg_new[j] = row_ptr_raw[k + 1] - row_ptr_raw[k];
     src_row[j] = (u32)k;
+}
+// *out += number of rows that repeat another (out zeroed by the caller).
+__global__ void count_dups_kernel(const u32* __restrict__ rep_raw, u64 U,
+                                  unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    unsigned long long m = 0;
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < U; k += (u64)gridDim.x * blockDim.x)
+        m += rep_raw[k] != (u32)k;
+    atomicAdd(&c, m);
+    __syncthreads();
+    if (threadIdx.x == 0 && c) atomicAdd(out, c);
 }
 
 __global__ void place_dangling_kernel(const u64* __restrict__ dangling,
@@ -72,6 +99,78 @@ __global__ void place_dangling_kernel(const u64* __restrict__ dangling,
                                       u32* __restrict__ pi) {
     const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (v < N && dangling[v]) pi[v] = (u32)(U + dscan[v]);
+}
+
+// Row fingerprints (sums of mixed (position, head, weight) triples, so rows
+// with the same lines in another order differ) and lengths; identical rows
+// share all three and so sort next to each other.
+__device__ __forceinline__ u64 mix_a(u64 x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    return x ^ (x >> 33);
+}
+template <bool HAS_WT>
+__global__ void row_hash_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
+                                const u64* __restrict__ wt, u64 U, u64* __restrict__ h1,
+                                u64* __restrict__ h2, u64* __restrict__ idx) {
+    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= U) return;
+    u64 a = 0, b = 0;
+    const u64 e0 = row_ptr[k];
+    for (u64 e = e0; e < row_ptr[k + 1]; ++e) {
+        const u64 w = HAS_WT ? wt[e] : 1;
+        const u64 j = (e - e0) * 0xbf58476d1ce4e5b9ULL;
+        a += mix_a(((u64)dst[e] << 1) ^ (w * 0x9e3779b97f4a7c15ULL) ^ j);
+        b += mix_a(((u64)dst[e] + 0x632be59bd9b4e019ULL) ^ (j >> 7)) * (2 * w + 1);
+    }
+    h1[k] = a;
+    h2[k] = b ^ ((row_ptr[k + 1] - row_ptr[k]) * 0xd6e8feb86659fd93ULL);
+    idx[k] = k;
+}
+// After sorting rows by h1: eq[i] = 1 if the row at sorted position i has
+// exactly the lines (heads and weights, in order) of the row at i - 1.
+template <bool HAS_WT>
+__global__ void row_eq_kernel(const u64* __restrict__ h1s, const u64* __restrict__ rows,
+                              const u64* __restrict__ h2, const u64* __restrict__ row_ptr,
+                              const u32* __restrict__ dst, const u64* __restrict__ wt, u64 U,
+                              u64* __restrict__ head) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= U) return;
+    bool eq = false;
+    if (i > 0 && h1s[i] == h1s[i - 1]) {
+        const u64 x = rows[i], y = rows[i - 1];
+        const u64 ax = row_ptr[x], ay = row_ptr[y], n = row_ptr[x + 1] - ax;
+        if (h2[x] == h2[y] && n == row_ptr[y + 1] - ay) {
+            eq = true;
+            for (u64 j = 0; j < n && eq; ++j) {
+                if (dst[ax + j] != dst[ay + j]) eq = false;
+                if (HAS_WT && wt[ax + j] != wt[ay + j]) eq = false;
+            }
+        }
+    }
+    head[i] = eq ? 0 : 1;
+}
+// Heads of runs of identical rows record their raw row per run.
+__global__ void run_leader_kernel(const u64* __restrict__ head, const u64* __restrict__ hscan,
+                                  const u64* __restrict__ rows, u64 U, u64* __restrict__ leader) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < U && head[i]) leader[hscan[i]] = rows[i];
+}
+// rep_raw[row] = raw row it repeats (itself for stored rows).
+__global__ void rep_raw_kernel(const u64* __restrict__ head, const u64* __restrict__ hscan,
+                               const u64* __restrict__ rows, const u64* __restrict__ leader,
+                               u64 U, u32* __restrict__ rep_raw) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < U) rep_raw[rows[i]] = (u32)leader[hscan[i] + head[i] - 1];
+}
+// Internal ids of the duplicate rows point at their stored row.
+__global__ void dup_rep_kernel(const u32* __restrict__ src_row, const u32* __restrict__ rep_raw,
+                               const u32* __restrict__ row_node, const u32* __restrict__ pi,
+                               u64 R, u64 U, u32* __restrict__ dup_rep) {
+    const u64 j = R + (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < U) dup_rep[j - R] = pi[row_node[rep_raw[src_row[j]]]];
 }
 
 // One thread per new row: copy its lines with heads renumbered.
@@ -139,15 +238,15 @@ void raw_release(RawGraph* r) {
 
 int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     hipStream_t s = ctx->stream;
-    const u64 N = raw->N, E = raw->E, U = raw->U;
+    const u64 N = raw->N, E_in = raw->E, U = raw->U;
     *g = rbsc_dgraph();
     g->N = (u32)N;
-    g->E = E;
+    g->E_in = E_in;
     g->U = (u32)U;
-    g->n_tiles = (E + SIG_TW - 1) / SIG_TW;
     u64 *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *dang = nullptr,
-        *deg = nullptr, *tmp = nullptr;
-    u32 *indeg = nullptr, *src_row = nullptr;
+        *deg = nullptr, *tmp = nullptr, *h2 = nullptr, *d_tot = nullptr;
+    u32 *indeg = nullptr, *src_row = nullptr, *rep_raw = nullptr;
+    u64 R = U, E = E_in, h_tot[2] = {0, 0};
     int rc = RBSC_OK;
 #define B_TRY(expr)                                                                  \
     do {                                                                             \
@@ -158,60 +257,111 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
             goto done;                                                               \
         }                                                                            \
     } while (0)
-    B_TRY(hipMalloc(&g->row_ptr, sizeof(u64) * (U + 1)));
     B_TRY(hipMalloc(&g->pi, sizeof(u32) * std::max<u64>(N, 1)));
-    B_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
-    if (raw->wt) B_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
     if (raw->init) B_TRY(hipMalloc(&g->init, sizeof(u32) * std::max<u64>(N, 1)));
-    g->end_words = (E + 31) / 32 + 64;  // padding covers whole-tile byte reads
-    B_TRY(hipMalloc(&g->end_bits, sizeof(u32) * g->end_words));
-    B_TRY(hipMalloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1)));
-    B_TRY(hipMalloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1)));
-    g->bytes = sizeof(u64) * (U + 1) + 4 * N + 4 * E + (raw->wt ? 8 * E : 0) +
-               (raw->init ? 4 * N : 0) + 4 * g->end_words + 16 * (g->n_tiles + 1);
-
     B_TRY(hipMalloc(&indeg, sizeof(u32) * std::max<u64>(N, 1)));
     B_TRY(hipMalloc(&dang, sizeof(u64) * (2 * N + 1)));
     B_TRY(hipMalloc(&deg, sizeof(u64) * (U + 1)));
     B_TRY(hipMalloc(&src_row, sizeof(u32) * std::max<u64>(U, 1)));
     B_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(std::max(N, U) + 1)));
+    B_TRY(hipMalloc(&d_tot, sizeof(u64) * 2));
     for (int i = 0; i < 2; ++i) {
         B_TRY(hipMalloc(&keys[i], sizeof(u64) * std::max<u64>(U, 1)));
         B_TRY(hipMalloc(&vals[i], sizeof(u64) * std::max<u64>(U, 1)));
     }
-    zero_u32<<<grid_for(N), BLK, 0, s>>>(indeg, N);
-    if (E) indeg_kernel<<<grid_for(E), BLK, 0, s>>>(raw->dst, E, indeg);
-    fill_ones_kernel<<<grid_for(N), BLK, 0, s>>>(dang, N);
     if (U) {
-        order_keys_kernel<<<grid_for(U), BLK, 0, s>>>(raw->row_node, U, indeg, keys[0], vals[0], dang);
         rc = sort_ws_init(&ctx->sort, U, ctx->d_err, s);
         if (rc) goto done;
+    }
+    zero_u32<<<grid_for(N), BLK, 0, s>>>(indeg, N);
+    if (E_in) indeg_kernel<<<grid_for(E_in), BLK, 0, s>>>(raw->dst, E_in, indeg);
+    fill_ones_kernel<<<grid_for(N), BLK, 0, s>>>(dang, N);
+
+    // Identical rows: sort rows by fingerprint, compare neighbours exactly,
+    // one leader (stored row) per run of identical neighbours.
+    if (U > 1 && ctx->dedup) {
+        B_TRY(hipMalloc(&h2, sizeof(u64) * U));
+        B_TRY(hipMalloc(&rep_raw, sizeof(u32) * U));
+        if (raw->wt)
+            row_hash_kernel<true><<<grid_for(U), BLK, 0, s>>>(raw->row_ptr, raw->dst, raw->wt, U,
+                                                              keys[0], h2, vals[0]);
+        else
+            row_hash_kernel<false><<<grid_for(U), BLK, 0, s>>>(raw->row_ptr, raw->dst, nullptr, U,
+                                                               keys[0], h2, vals[0]);
+        int w = 0;
+        rc = radix_sort_u64(&ctx->sort, keys, vals, U, 0, 64, &w, s);
+        if (rc) goto done;
+        u64* head = keys[w ^ 1];
+        u64* hscan = vals[w ^ 1];
+        if (raw->wt)
+            row_eq_kernel<true><<<grid_for(U), BLK, 0, s>>>(keys[w], vals[w], h2, raw->row_ptr,
+                                                            raw->dst, raw->wt, U, head);
+        else
+            row_eq_kernel<false><<<grid_for(U), BLK, 0, s>>>(keys[w], vals[w], h2, raw->row_ptr,
+                                                             raw->dst, nullptr, U, head);
+        rc = scan_excl_u64(head, hscan, U, nullptr, tmp, s);
+        if (rc) goto done;
+        run_leader_kernel<<<grid_for(U), BLK, 0, s>>>(head, hscan, vals[w], U, h2);  // h2 free now
+        rep_raw_kernel<<<grid_for(U), BLK, 0, s>>>(head, hscan, vals[w], h2, U, rep_raw);
+        B_TRY(hipMemsetAsync(d_tot, 0, sizeof(u64), s));
+        count_dups_kernel<<<(unsigned)std::min<u64>(grid_for(U), 1024), BLK, 0, s>>>(
+            rep_raw, U, reinterpret_cast<unsigned long long*>(d_tot));
+        B_TRY(hipMemcpyAsync(h_tot, d_tot, sizeof(u64), hipMemcpyDeviceToHost, s));
+        B_TRY(hipStreamSynchronize(s));
+        R = U - h_tot[0];
+    }
+    if (U) {
+        order_keys_kernel<<<grid_for(U), BLK, 0, s>>>(raw->row_node, U, indeg, rep_raw, keys[0],
+                                                      vals[0], dang);
         int which = 0;
         rc = radix_sort_u64(&ctx->sort, keys, vals, U, 0, 64, &which, s);
         if (rc) goto done;
-        place_rows_kernel<<<grid_for(U), BLK, 0, s>>>(keys[which], vals[which], U, raw->row_ptr,
+        place_rows_kernel<<<grid_for(U), BLK, 0, s>>>(keys[which], vals[which], U, R, raw->row_ptr,
                                                        g->pi, deg, src_row);
     }
     rc = scan_excl_u64(dang, dang + N, N, nullptr, tmp, s);  // scan into the upper half
     if (rc) goto done;
     place_dangling_kernel<<<grid_for(N), BLK, 0, s>>>(dang, dang + N, N, U, g->pi);
-    rc = scan_excl_u64(deg, g->row_ptr, U, nullptr, tmp, s);
+    // Row offsets of the stored rows; their total is the stored line count.
+    B_TRY(hipMalloc(&g->row_ptr, sizeof(u64) * (R + 1)));
+    rc = scan_excl_u64(deg, g->row_ptr, R, d_tot + 1, tmp, s);
     if (rc) goto done;
-    set_last_kernel<<<1, 1, 0, s>>>(g->row_ptr, U, E);
+    if (R < U) {
+        B_TRY(hipMemcpyAsync(h_tot + 1, d_tot + 1, sizeof(u64), hipMemcpyDeviceToHost, s));
+        B_TRY(hipStreamSynchronize(s));
+        E = h_tot[1];
+    }
+    g->R = (u32)R;
+    g->E = E;
+    g->n_tiles = (E + SIG_TW - 1) / SIG_TW;
+    B_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
+    if (raw->wt) B_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
+    g->end_words = (E + 31) / 32 + 64;  // padding covers whole-tile byte reads
+    B_TRY(hipMalloc(&g->end_bits, sizeof(u32) * g->end_words));
+    B_TRY(hipMalloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1)));
+    B_TRY(hipMalloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1)));
+    if (R < U) {
+        B_TRY(hipMalloc(&g->dup_rep, sizeof(u32) * (U - R)));
+        dup_rep_kernel<<<grid_for(U - R), BLK, 0, s>>>(src_row, rep_raw, raw->row_node, g->pi, R, U,
+                                                       g->dup_rep);
+    }
+    g->bytes = sizeof(u64) * (R + 1) + 4 * N + 4 * E + (raw->wt ? 8 * E : 0) +
+               (raw->init ? 4 * N : 0) + 4 * g->end_words + 16 * (g->n_tiles + 1) + 4 * (U - R);
+    set_last_kernel<<<1, 1, 0, s>>>(g->row_ptr, R, E);
     zero_u32<<<grid_for(g->end_words), BLK, 0, s>>>(g->end_bits, g->end_words);
-    if (U) {
+    if (R) {
         if (raw->wt)
-            copy_rows_kernel<true><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
-                                                               raw->dst, raw->wt, g->pi, U, g->dst,
+            copy_rows_kernel<true><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
+                                                               raw->dst, raw->wt, g->pi, R, g->dst,
                                                                g->wt, g->end_bits);
         else
-            copy_rows_kernel<false><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
-                                                                raw->dst, nullptr, g->pi, U, g->dst,
+            copy_rows_kernel<false><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
+                                                                raw->dst, nullptr, g->pi, R, g->dst,
                                                                 nullptr, g->end_bits);
     }
     if (raw->init) permute_init_kernel<<<grid_for(N), BLK, 0, s>>>(raw->init, g->pi, N, g->init);
     if (g->n_tiles)
-        tile_setup_kernel<<<grid_for(g->n_tiles + 1), BLK, 0, s>>>(g->row_ptr, (u32)U, E, g->n_tiles,
+        tile_setup_kernel<<<grid_for(g->n_tiles + 1), BLK, 0, s>>>(g->row_ptr, (u32)R, E, g->n_tiles,
                                                                    g->tile_row, g->carry_row);
     B_TRY(hipGetLastError());
     B_TRY(hipStreamSynchronize(s));
@@ -221,11 +371,9 @@ done:
         if (keys[i]) (void)hipFree(keys[i]);
         if (vals[i]) (void)hipFree(vals[i]);
     }
-    if (dang) (void)hipFree(dang);
-    if (deg) (void)hipFree(deg);
-    if (tmp) (void)hipFree(tmp);
-    if (indeg) (void)hipFree(indeg);
-    if (src_row) (void)hipFree(src_row);
+    for (void* p : {(void*)dang, (void*)deg, (void*)tmp, (void*)indeg, (void*)src_row, (void*)h2,
+                    (void*)rep_raw, (void*)d_tot})
+        if (p) (void)hipFree(p);
     return rc;
 #undef B_TRY
 }
@@ -241,12 +389,12 @@ __device__ __forceinline__ u32 part_of(u32 d, u32 P, u32 hot, u32 U) {
 }
 // counts[p * U + j] = lines of row j whose head is in partition p.
 __global__ void part_count_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                  u64 U, u32 P, u32 hot, u64* __restrict__ counts) {
+                                  u64 U, u32 P, u32 hot, u32 lined, u64* __restrict__ counts) {
     const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= U) return;
     u64 c[16];
     for (u32 p = 0; p < P; ++p) c[p] = 0;
-    for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) c[part_of(dst[e], P, hot, (u32)U)]++;
+    for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) c[part_of(dst[e], P, hot, lined)]++;
     for (u32 p = 0; p < P; ++p) counts[(u64)p * U + j] = c[p];
 }
 __global__ void flag_nonzero_kernel(const u64* __restrict__ a, u64 n, u64* __restrict__ f) {
@@ -257,7 +405,7 @@ __global__ void flag_nonzero_kernel(const u64* __restrict__ a, u64 n, u64* __res
 // segment rows and row-end bits.
 template <bool HAS_WT>
 __global__ void part_fill_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                 const u64* __restrict__ wt, u64 U, u32 P, u32 hot,
+                                 const u64* __restrict__ wt, u64 U, u32 P, u32 hot, u32 lined,
                                  const u64* __restrict__ line_off, const u64* __restrict__ seg_idx,
                                  const u64* __restrict__ counts, const u64* __restrict__ adj_line,
                                  const u64* __restrict__ adj_seg, u32* __restrict__ ndst,
@@ -279,7 +427,7 @@ __global__ void part_fill_kernel(const u64* __restrict__ row_ptr, const u32* __r
     }
     for (u64 e = row_ptr[j]; e < row_ptr[j + 1]; ++e) {
         const u32 d = dst[e];
-        const u32 p = part_of(d, P, hot, (u32)U);
+        const u32 p = part_of(d, P, hot, lined);
         ndst[pos[p]] = d;
         if (HAS_WT) nwt[pos[p]] = wt[e];
         pos[p]++;
@@ -309,7 +457,9 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
         return RBSC_ERR_INVALID;
     }
     hipStream_t s = ctx->stream;
-    const u64 U = g->U, E = g->E, N = g->N;
+    // rows: the stored rows; heads below g->U are nodes with lines
+    const u64 U = g->R, E = g->E, N = g->N;
+    const u32 lined = g->U;
     if (U == 0 || E == 0) return RBSC_OK;
     const u64 PU = (u64)P * U;
     u64 *counts = nullptr, *line_off = nullptr, *flags = nullptr, *tmp = nullptr, *tot = nullptr,
@@ -334,7 +484,7 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
     R_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(PU + 1)));
     R_TRY(hipMalloc(&tot, sizeof(u64) * 2 * P));
     R_TRY(hipMalloc(&adj, sizeof(u64) * 2 * P));
-    part_count_kernel<<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, U, P, hot, counts);
+    part_count_kernel<<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, U, P, hot, lined, counts);
     flag_nonzero_kernel<<<grid_for(PU), BLK, 0, s>>>(counts, PU, flags);
     // per-partition totals of lines and segments
     for (u32 p = 0; p < P; ++p) {
@@ -375,12 +525,12 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
     zero_u32<<<grid_for(end_words), BLK, 0, s>>>(nbits, end_words);
     if (nwt) R_TRY(hipMemsetAsync(nwt, 0, sizeof(u64) * E_pad, s));
     if (g->wt)
-        part_fill_kernel<true><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, g->wt, U, P, hot, line_off,
+        part_fill_kernel<true><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, g->wt, U, P, hot, lined, line_off,
                                                            flags, counts, adj, adj + P, ndst, nwt,
                                                            seg_ptr, seg_row, nbits);
     else
         part_fill_kernel<false><<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, nullptr, U, P,
-                                                            hot, line_off, flags, counts, adj, adj + P,
+                                                            hot, lined, line_off, flags, counts, adj, adj + P,
                                                             ndst, nullptr, seg_ptr, seg_row, nbits);
     for (u32 p = 0; p < P; ++p) {
         const u64 a = adj_line[p] + Ep[p];

@@ -31,9 +31,12 @@ struct RawGraph {
 // signature 0 (rbsc.cu:110-111).
 struct rbsc_dgraph {
     u32 N = 0;
-    u64 E = 0;
-    u32 U = 0;                  // nodes with >= 1 line = rows
-    u64* row_ptr = nullptr;     // U + 1
+    u64 E = 0;                  // stored lines (the rows' lines below)
+    u64 E_in = 0;               // input lines (E plus the lines of duplicate rows)
+    u32 U = 0;                  // nodes with >= 1 line
+    u32 R = 0;                  // stored rows: nodes [0, R); [R, U) repeat a stored row
+    u32* dup_rep = nullptr;     // U - R: stored row whose signature node R + i copies
+    u64* row_ptr = nullptr;     // R + 1
     u32* pi = nullptr;          // N: original id -> internal id
     u32* dst = nullptr;         // E, internal ids (null when streamed)
     u64* wt = nullptr;          // E or null (all 1)
@@ -104,6 +107,7 @@ struct rbsc_ctx {
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;
     u64 budget = 0;
+    bool dedup = true;         // store identical rows once (graph build)
     SortWorkspace sort;
     Workspace ws;
     u32* d_err = nullptr;
@@ -119,11 +123,11 @@ struct rbsc_ctx {
     hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
     // per-phase profiling (rbsc_ctx_profile)
     bool prof = false;
-    static constexpr int NPHASE = 5;
+    static constexpr int NPHASE = 6;
     std::vector<hipEvent_t> prof_ev;   // pairs (start, stop) recorded this refine
     std::vector<int> prof_phase;       // phase of each pair
-    double prof_ms[NPHASE] = {0, 0, 0, 0, 0};
-    u64 prof_n[NPHASE] = {0, 0, 0, 0, 0};
+    double prof_ms[NPHASE] = {0, 0, 0, 0, 0, 0};
+    u64 prof_n[NPHASE] = {0, 0, 0, 0, 0, 0};
     size_t prof_used = 0;
 };
 

@@ -210,6 +210,12 @@ __global__ void sig_fixup_kernel(const u32* __restrict__ carry_row,
                   (unsigned long long)carry_val[t]);
 }
 
+// Rows that repeat a stored row take its signature.
+__global__ void dup_sig_kernel(const u32* __restrict__ dup_rep, u64 R, u64 n_dup, u64* sig) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_dup) sig[R + i] = sig[dup_rep[i]];
+}
+
 // Round-0 state when every node with a line has colour N+1 (rbsc.cu:51-63)
 // and nodes without lines are left out of the node side: positions are
 // already sorted by colour.
@@ -631,6 +637,7 @@ void dgraph_release(rbsc_dgraph* g) {
     if (g->wt_host) (void)hipHostFree(g->wt_host);
     if (g->tile_row) (void)hipFree(g->tile_row);
     if (g->carry_row) (void)hipFree(g->carry_row);
+    if (g->dup_rep) (void)hipFree(g->dup_rep);
     *g = rbsc_dgraph();
 }
 
@@ -872,10 +879,15 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                 }
             }
         }
+        prof_end(ctx);
+        prof_begin(ctx, 5);
         if (n_tiles)
             sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, w.carry,
                                                                g->n_parts > 1 ? g->seg_row : nullptr,
                                                                w.sig, n_tiles);
+        if (g->U > g->R)
+            dup_sig_kernel<<<grid_for(g->U - g->R), BLK, 0, s>>>(g->dup_rep, g->R, g->U - g->R,
+                                                                 w.sig);
         prof_end(ctx);
         // Node side: keys in colour order, stable sort by signature, dense rank.
         // sorted_pc is the P buffer holding the sorted order (input of rank).
@@ -1048,7 +1060,7 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
     RefineParams prm;
     prm.max_rounds = opts ? opts->max_rounds : 0;
     prm.flags = opts ? opts->flags : 0;
-    if (opts && opts->max_batch_edges && opts->max_batch_edges < g->E)
+    if (opts && opts->max_batch_edges && opts->max_batch_edges < g->E_in)
         prm.slice_lines = opts->max_batch_edges;
     RefineOut o;
     int rc = refine_core(ctx, g, prm, &o);
@@ -1076,7 +1088,7 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
     auto t1 = std::chrono::steady_clock::now();
     rbsc_result r{};
     r.passes = 1;
-    r.edges_per_round = g->E;
+    r.edges_per_round = g->E_in;
     r.nodes = N ? o.K : 0;
     r.done = 1;
     r.rounds = o.rounds;
@@ -1286,11 +1298,21 @@ extern "C" int rbsc_graph_dims(const rbsc_dgraph* g, uint32_t* n_nodes, uint64_t
         return RBSC_ERR_INVALID;
     }
     if (n_nodes) *n_nodes = g->N;
-    if (n_edges) *n_edges = g->E;
+    if (n_edges) *n_edges = g->E_in;
     return RBSC_OK;
 }
 
-extern "C" uint32_t rbsc_graph_rows(const rbsc_dgraph* g) { return g ? g->U : 0; }
+extern "C" uint32_t rbsc_graph_rows(const rbsc_dgraph* g) { return g ? g->R : 0; }
+extern "C" uint64_t rbsc_graph_stored_lines(const rbsc_dgraph* g) { return g ? g->E : 0; }
+
+extern "C" int rbsc_ctx_set_dedup(rbsc_ctx* ctx, int on) {
+    if (!ctx) {
+        rbsc_set_error("rbsc_ctx_set_dedup: null context");
+        return RBSC_ERR_INVALID;
+    }
+    ctx->dedup = on != 0;
+    return RBSC_OK;
+}
 
 extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t* row_offsets,
                                    uint32_t* dst) {
@@ -1303,10 +1325,12 @@ extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    const u64 N = g->N, U = g->U, E = g->E;
-    std::vector<u64> rp(U + 1);
-    std::vector<u32> pi(N), inv(N), d(E);
-    HIP_TRY(hipMemcpy(rp.data(), g->row_ptr, sizeof(u64) * (U + 1), hipMemcpyDeviceToHost));
+    const u64 N = g->N, U = g->U, R = g->R, E = g->E;
+    std::vector<u64> rp(R + 1);
+    std::vector<u32> pi(N), inv(N), d(E), dup(U - R);
+    HIP_TRY(hipMemcpy(rp.data(), g->row_ptr, sizeof(u64) * (R + 1), hipMemcpyDeviceToHost));
+    if (U > R)
+        HIP_TRY(hipMemcpy(dup.data(), g->dup_rep, sizeof(u32) * (U - R), hipMemcpyDeviceToHost));
     if (N) HIP_TRY(hipMemcpy(pi.data(), g->pi, sizeof(u32) * N, hipMemcpyDeviceToHost));
     if (E) {
         if (g->dst) HIP_TRY(hipMemcpy(d.data(), g->dst, sizeof(u32) * E, hipMemcpyDeviceToHost));
@@ -1318,8 +1342,9 @@ extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t
     u64 o = 0;
     if (row_offsets) row_offsets[0] = 0;
     for (u64 v = 0; v < N; ++v) {
-        const u64 j = pi[v];
+        u64 j = pi[v];
         if (j < U) {
+            if (j >= R) j = dup[j - R];  // a repeated row: the stored row's lines
             for (u64 p = rp[j]; p < rp[j + 1]; ++p, ++o)
                 if (dst) dst[o] = inv[d[p]];
         }

@@ -445,7 +445,7 @@ extern "C" int rbsc_server_submit_resident(rbsc_server* s, rbsc_dgraph* g, doubl
     r->kind = REQ_RESIDENT;
     r->g = g;
     r->n = g->N;
-    r->e = g->E;
+    r->e = g->E_in;
     r->small = false;  // already laid out for the single-graph path
     r->need = need_resident(g->N);
     std::lock_guard<std::mutex> lk(s->mu);

@@ -147,6 +147,50 @@ __global__ void vruns_kernel(const u64* __restrict__ sk, const u64* __restrict__
         atomicAdd(bad, 1ULL);
 }
 
+// Nodes [R, U) repeat a stored row (dup_rep) and have no lines of their
+// own: their row must match the row of their block's representative, unless
+// it is that very row.  first == nullptr: unweighted (compare sorted colour
+// sequences); else compare (colour, summed weight) runs.
+__global__ void vdups_kernel(const u64* __restrict__ sk, const u64* __restrict__ row_ptr,
+                             const u32* __restrict__ dup_rep, u64 R, u64 U,
+                             const u32* __restrict__ col, const u32* __restrict__ rep, u32 cbits,
+                             const u64* __restrict__ first, const u64* __restrict__ run_start,
+                             const u64* __restrict__ run_w, unsigned long long* bad) {
+    const u64 i = (u64)blockIdx.x * VB + threadIdx.x;
+    if (i >= U - R) return;
+    const u64 d = R + i;
+    const u64 r = rep[col[d]];
+    if (r >= U) return;  // counted by vnodes_kernel
+    const u64 a = dup_rep[i], b = r < R ? r : dup_rep[r - R];
+    if (a == b) return;
+    const u64 cmask = (1ULL << cbits) - 1;
+    if (!first) {
+        const u64 n = row_ptr[a + 1] - row_ptr[a];
+        if (n != row_ptr[b + 1] - row_ptr[b]) {
+            atomicAdd(bad, 1ULL);
+            return;
+        }
+        for (u64 j = 0; j < n; ++j)
+            if ((sk[row_ptr[a] + j] & cmask) != (sk[row_ptr[b] + j] & cmask)) {
+                atomicAdd(bad, 1ULL);
+                return;
+            }
+    } else {
+        const u64 n = first[a + 1] - first[a];
+        if (n != first[b + 1] - first[b]) {
+            atomicAdd(bad, 1ULL);
+            return;
+        }
+        for (u64 j = 0; j < n; ++j) {
+            const u64 ka = first[a] + j, kb = first[b] + j;
+            if ((sk[run_start[ka]] & cmask) != (sk[run_start[kb]] & cmask) || run_w[ka] != run_w[kb]) {
+                atomicAdd(bad, 1ULL);
+                return;
+            }
+        }
+    }
+}
+
 __global__ void vpermute_kernel(const u32* __restrict__ host_col, const u32* __restrict__ pi,
                                 u64 N, u32* __restrict__ col) {
     const u64 v = (u64)blockIdx.x * VB + threadIdx.x;
@@ -165,7 +209,7 @@ u32 bits_for(u64 x) {  // bits to hold values 0..x (at least 1)
 // ids, values < K) on g; *bad_out == 0 iff stable.
 int verify_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* d_col, u32 K, u64* bad_out) {
     *bad_out = 0;
-    const u64 N = g->N, E = g->E, U = g->U;
+    const u64 N = g->N, E = g->E, U = g->U, R = g->R;  // R stored rows, U nodes with lines
     if (N == 0) return RBSC_OK;
     if (g->n_parts > 1) {
         rbsc_set_error("exact check needs the default line layout (graph was relaid out)");
@@ -174,7 +218,7 @@ int verify_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* d_col, u32 K, u64
     hipStream_t s = ctx->stream;
     const bool weighted = g->wt != nullptr || g->wt_host != nullptr;
     const u32 cbits = bits_for(K ? K - 1 : 0);
-    const u32 rbits = bits_for(U ? U - 1 : 0);
+    const u32 rbits = bits_for(R ? R - 1 : 0);
     if (rbits + cbits > 64) {
         rbsc_set_error("exact check: %u row bits + %u colour bits exceed 64", rbits, cbits);
         return RBSC_ERR_INVALID;
@@ -244,22 +288,28 @@ int verify_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* d_col, u32 K, u64
         const u64* sk = kk[which];
         if (!weighted) {
             vlines_kernel<<<vgrid(E), VB, 0, s>>>(sk, E, g->row_ptr, d_col, rep, U, cbits, d_bad);
+            if (U > R)
+                vdups_kernel<<<vgrid(U - R), VB, 0, s>>>(sk, g->row_ptr, g->dup_rep, R, U, d_col, rep,
+                                                         cbits, nullptr, nullptr, nullptr, d_bad);
         } else {
             u64* hidx = kk[which ^ 1];  // free after the sort
             u64* W = vv[which];         // sorted weights -> exclusive prefix, in place
             u64* run_start = vv[which ^ 1];
             u64 *run_w = nullptr, *first = nullptr, *tmp = nullptr;
             if ((rc = alloc((void**)&run_w, sizeof(u64) * E))) return done(rc);
-            if ((rc = alloc((void**)&first, sizeof(u64) * (U + 1)))) return done(rc);
+            if ((rc = alloc((void**)&first, sizeof(u64) * (R + 1)))) return done(rc);
             if ((rc = alloc((void**)&tmp, sizeof(u64) * scan_tmp_words(E)))) return done(rc);
             vheads_kernel<<<vgrid(E), VB, 0, s>>>(sk, E, hidx);
             if ((rc = scan_excl_u64(hidx, hidx, E, d_tot, tmp, s))) return done(rc);
             if ((rc = scan_excl_u64(W, W, E, d_tot + 1, tmp, s))) return done(rc);
             vrun_start_kernel<<<vgrid(E), VB, 0, s>>>(sk, hidx, E, run_start);
             vrun_w_kernel<<<vgrid(E), VB, 0, s>>>(run_start, W, d_tot + 1, d_tot, E, run_w);
-            vfirst_kernel<<<vgrid(U + 1), VB, 0, s>>>(g->row_ptr, hidx, U, d_tot, first);
+            vfirst_kernel<<<vgrid(R + 1), VB, 0, s>>>(g->row_ptr, hidx, R, d_tot, first);
             vruns_kernel<<<vgrid(E), VB, 0, s>>>(sk, run_start, run_w, first, d_tot, d_col, rep, U,
                                                  cbits, d_bad);
+            if (U > R)
+                vdups_kernel<<<vgrid(U - R), VB, 0, s>>>(sk, g->row_ptr, g->dup_rep, R, U, d_col, rep,
+                                                         cbits, first, run_start, run_w, d_bad);
         }
     }
     if (hipGetLastError() != hipSuccess) return done(RBSC_ERR_HIP);
