@@ -281,6 +281,11 @@ def main():
                     help="gather per-node hashed colours instead of raw u32 colours")
     ap.add_argument("--xcd", action="store_true",
                     help="with --parts 8: one launch, partition b mod 8 per workgroup b")
+    ap.add_argument("--bins", type=int, default=0,
+                    help="1: binned signature layout (rbsc_graph_bin) before timing")
+    ap.add_argument("--bin-hot", type=int, default=0, help="hot heads of the binned layout (0 = default)")
+    ap.add_argument("--bin-rows", type=int, default=0, help="rows per bin (0 = default)")
+    ap.add_argument("--bin-chunk", type=int, default=0, help="slots per chunk (0 = default)")
     ap.add_argument("--batch-graphs", type=int, default=50000,
                     help="graphs in the many-small-graphs leg (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=200,
@@ -314,6 +319,12 @@ def main():
         t0 = time.perf_counter()
         dg.split_hot(args.hot)
         log(f"[bench] hot/cold split at {args.hot} heads in {time.perf_counter() - t0:.1f}s")
+    bin_info = None
+    if args.bins and args.parts == 1 and not args.hot:
+        t0 = time.perf_counter()
+        dg.bin(args.bin_hot, args.bin_rows, args.bin_chunk)
+        bin_info = dict(dg.bin_info, build_s=time.perf_counter() - t0)
+        log(f"[bench] binned layout in {bin_info['build_s']:.1f}s: {bin_info}")
     flags = (rbsc_amd.FLAG_FULLSORT if args.fullsort else 0) | (rbsc_amd.FLAG_HASHED64 if args.hashed64 else 0) | (rbsc_amd.FLAG_XCD if args.xcd else 0)
     for i in range(args.warmup):
         r = dg.refine(partition=False, flags=flags)
@@ -345,7 +356,7 @@ def main():
     # per row it writes (row offset, signature); one launch per round.
     b_sig = 8 * E_st + 16 * U
     per_round = {k: phases[k][0] / rounds for k in phases if k != "setup"}
-    ROUND = ("signature", "finish", "gather", "sort", "rank")
+    ROUND = ("scatter", "signature", "finish", "gather", "sort", "rank")
     round_ms = sum(per_round[k] for k in ROUND)
     dom = max(ROUND, key=lambda k: per_round[k])
     sig_ms = per_round["signature"]
@@ -365,7 +376,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "rows": U,
                    "stored_lines": E_st,
-                   "parts": args.parts, "hot": args.hot, "fullsort": bool(args.fullsort), "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
+                   "parts": args.parts, "hot": args.hot, "bins": bin_info, "fullsort": bool(args.fullsort), "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
                    "rounds_per_step": rounds // args.steps,
                    "blocks": r.nodes, "ms_per_round": ms_round},
         "roofline": {"bound": "hbm", "kernel": "sig_kernel",

@@ -90,6 +90,9 @@ typedef struct rbsc_opts {
  * entries, so the full-sort fallback runs. */
 #define RBSC_FLAG_SPLIT16 8u
 #define RBSC_FLAG_TIESMALL 16u
+/* opts.flags: on a binned graph (rbsc_graph_bin) compute signatures with the
+ * plain line kernel instead (results identical). */
+#define RBSC_FLAG_NOBINS 64u
 
 /* Result.  nodes / done / time_s are the reference's three output lines
  * (rbsc.cu:492-494). */
@@ -159,11 +162,32 @@ int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts);
  * counterpart in the reference. */
 int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
 
+/* Binned signature layout of a resident graph, an addition to its plain
+ * layout (about 6 bytes per gathered line plus 8 per line into a cold head;
+ * 14 with weights).  Rows are cut into bins of rows_per_bin (a power of two
+ * from 256 to 16384; 0 = 4096) whose signatures are summed in LDS; lines
+ * into the hot_heads most-gathered nodes (0 = 2^20) gather their colours
+ * in head order, the colours of the other gathered heads are scattered to
+ * their lines' slots once per round in head order; bins with more than
+ * chunk_slots lines (0 = 2^19) are split over several workgroups.  Later
+ * refinements of g use it unless RBSC_FLAG_NOBINS, RBSC_FLAG_HASHED64 or
+ * streamed lines are asked for; relayout, split_hot and lines_to_host drop
+ * it.  Signatures and results are identical.  No counterpart in the
+ * reference (its set_values + ReduceByKey, rbsc.cu:357-377, read the lines
+ * in input order). */
+int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads, uint32_t rows_per_bin,
+                   uint64_t chunk_slots);
+int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g);
+/* info[8]: binned (0/1), bins, hot lines, cold lines, workgroup chunks, bins
+ * split over several chunks, bytes, hot_heads | log2(rows_per_bin) << 32. */
+int rbsc_graph_bin_info(const rbsc_dgraph* g, uint64_t* info);
+
 /* Per-phase device timing with hipEvents on the context's stream (off by
  * default).  Phases: 0 signature kernel, 1 key gather, 2 radix sort,
  * 3 rank/scan, 4 round-0 set-up, 5 signature finish (tile carries, repeated
- * rows).  rbsc_ctx_profile_read fills ms[6] (summed milliseconds) and
- * launches[6] (phase executions) and resets the sums when reset != 0.  No counterpart in the reference, which times
+ * rows), 6 cold-colour scatter of a binned graph.  rbsc_ctx_profile_read
+ * fills ms[7] (summed milliseconds) and launches[7] (phase executions) and
+ * resets the sums when reset != 0.  No counterpart in the reference, which times
  * only the whole run (rbsc.cu:272, 489-490). */
 int rbsc_ctx_profile(rbsc_ctx* ctx, int enable);
 int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int reset);

@@ -23,13 +23,13 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (FLAG_FULLSORT, FLAG_HASHED64, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
+from ._lib import (FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
                    RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
            "Server", "RequestResult",
            "RefineResult", "RbscError", "load",
-           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "gen"]
+           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "gen"]
 
 _MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
           "aggressive": MODE_QUOTIENT_AGGRESSIVE}
@@ -225,6 +225,28 @@ class DeviceGraph:
         first (their colours stay cache resident); results are unchanged."""
         check(load().rbsc_graph_split_hot(self.ctx.handle, self.handle, int(hot_heads)))
 
+    def bin(self, hot_heads: int = 0, rows_per_bin: int = 0, chunk_slots: int = 0):
+        """Build the binned signature layout (row sums in LDS per bin of
+        rows, hot heads gathered in head order, the other colours scattered
+        to their lines in head order); later refinements use it.  0 picks
+        the defaults (2^20 hot heads, 4096 rows per bin, 2^19 slots per
+        chunk).  Results are unchanged."""
+        check(load().rbsc_graph_bin(self.ctx.handle, self.handle, int(hot_heads),
+                                    int(rows_per_bin), int(chunk_slots)))
+
+    def unbin(self):
+        check(load().rbsc_graph_unbin(self.ctx.handle, self.handle))
+
+    @property
+    def bin_info(self) -> dict:
+        """Sizes of the binned layout (all zero when the graph has none)."""
+        v = (C.c_uint64 * 8)()
+        check(load().rbsc_graph_bin_info(self.handle, v))
+        return {"binned": bool(v[0]), "bins": int(v[1]), "hot_lines": int(v[2]),
+                "cold_lines": int(v[3]), "chunks": int(v[4]), "split_bins": int(v[5]),
+                "bytes": int(v[6]), "hot_heads": int(v[7] & 0xFFFFFFFF),
+                "rows_per_bin": 1 << int(v[7] >> 32) if v[0] else 0}
+
     def lines_to_host(self):
         """Keep the line arrays in pinned host memory; refinements then stream
         them in slices of ``max_batch_edges`` lines."""
@@ -401,14 +423,14 @@ class Context:
         except Exception:
             pass
 
-    PHASES = ("signature", "gather", "sort", "rank", "setup", "finish")
+    PHASES = ("signature", "gather", "sort", "rank", "setup", "finish", "scatter")
 
     def profile(self, enable: bool = True):
         check(load().rbsc_ctx_profile(self.handle, 1 if enable else 0))
 
     def profile_read(self, reset: bool = True) -> dict:
-        ms = (C.c_double * 6)()
-        n = (C.c_uint64 * 6)()
+        ms = (C.c_double * len(self.PHASES))()
+        n = (C.c_uint64 * len(self.PHASES))()
         check(load().rbsc_ctx_profile_read(self.handle, ms, n, 1 if reset else 0))
         return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(self.PHASES)}
 

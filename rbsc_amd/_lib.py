@@ -31,6 +31,7 @@ FLAG_SPLIT = 4
 FLAG_SPLIT16 = 8
 FLAG_TIESMALL = 16
 FLAG_FULLSORT = 32
+FLAG_NOBINS = 64
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -188,6 +189,9 @@ SIGNATURES = {
     "rbsc_graph_stored_lines": (C.c_uint64, [C.c_void_p]),
     "rbsc_ctx_set_dedup": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_graph_split_hot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "rbsc_graph_bin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64]),
+    "rbsc_graph_unbin": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "rbsc_graph_bin_info": (C.c_int, [C.c_void_p, u64p]),
     "rbsc_graph_download": (C.c_int, [C.c_void_p, C.c_void_p, u64p, u32p]),
     "rbsc_host_graph_free": (None, [C.POINTER(HostGraph)]),
     "rbsc_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,

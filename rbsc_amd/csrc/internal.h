@@ -23,6 +23,45 @@ struct RawGraph {
     u32* init = nullptr;        // N or null
 };
 
+// Binned signature layout (bins.hip): stored rows cut into bins of S rows
+// whose signatures are summed in LDS; each bin lists its lines as slots, hot
+// slots (head < H) first, sorted by head, then cold slots (H <= head < GL),
+// sorted by head, whose colours a scatter pass writes every round.  Lines
+// into heads >= GL (never gathered) are summed per row into dsum.
+struct BinChunk {
+    u64 lo, hi;    // slot range
+    u32 bin;
+    u32 flags;     // CHUNK_FIRST: adds the bin's dsum terms; CHUNK_ONLY: the
+};                 // bin's only chunk (stores; otherwise adds atomically)
+constexpr u32 CHUNK_FIRST = 1, CHUNK_ONLY = 2;
+constexpr int BIN_GROUPS = 8;  // scatter groups, one per XCD
+struct rbsc_bins {
+    u32 log_s = 0, H = 0, GL = 0;
+    u64 n_bins = 0, n_slots = 0, n_hot = 0, n_cold = 0, n_chunks = 0, n_zero = 0;
+    u64* bin_slot = nullptr;        // n_bins + 1: first slot of each bin
+    u64* bin_hot = nullptr;         // n_bins: hot slots of each bin
+    u32* slot_key = nullptr;        // n_slots: hot = head; cold = colour of the round
+    unsigned short* slot_row = nullptr;  // n_slots: row - first row of the bin
+    u64* slot_wt = nullptr;         // n_slots or null
+    u64* dsum = nullptr;            // R: weight of the row's lines into heads >= GL
+    u32* csc_head = nullptr;        // n_cold, by (group, head)
+    u32* csc_slot = nullptr;        // n_cold: slot receiving that line's colour
+    u64 grp_line[BIN_GROUPS + 1] = {};  // first cold line of each group
+    u64* d_grp_line = nullptr;
+    BinChunk* chunks = nullptr;     // n_chunks
+    u32* zero_bins = nullptr;       // n_zero: bins with several chunks
+    u64 bytes = 0;
+};
+void bins_release(rbsc_bins* b);
+// Builds g->bins (hot heads below H, 2^log_s rows per bin, chunks of at most
+// chunk_slots slots).
+int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots);
+// One round on the binned layout: cold colours into their slots, then the
+// signatures of the stored rows into sig (col: colours by internal id).
+void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col);
+void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col, u64 salt, u64 dang_h,
+              u64* sig);
+
 // A graph resident in HBM in the round's layout (graph.hip).  Refinement
 // only reads it (all per-refinement state lives in the context), so several
 // contexts may refine one resident graph at the same time.  Internal ids
@@ -61,6 +100,7 @@ struct rbsc_dgraph {
     u64 n_segs = 0;
     std::vector<u64> part_tile;  // n_parts + 1: first tile of each partition
     u64* d_part_tile = nullptr;  // the same on the device
+    rbsc_bins* bins = nullptr;   // binned signature layout (rbsc_graph_bin) or null
 };
 
 // Per-context device workspace for one refinement of up to cap_n nodes.
@@ -123,11 +163,11 @@ struct rbsc_ctx {
     hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
     // per-phase profiling (rbsc_ctx_profile)
     bool prof = false;
-    static constexpr int NPHASE = 6;
+    static constexpr int NPHASE = 7;
     std::vector<hipEvent_t> prof_ev;   // pairs (start, stop) recorded this refine
     std::vector<int> prof_phase;       // phase of each pair
-    double prof_ms[NPHASE] = {0, 0, 0, 0, 0, 0};
-    u64 prof_n[NPHASE] = {0, 0, 0, 0, 0, 0};
+    double prof_ms[NPHASE] = {0, 0, 0, 0, 0, 0, 0};
+    u64 prof_n[NPHASE] = {0, 0, 0, 0, 0, 0, 0};
     size_t prof_used = 0;
 };
 

@@ -638,6 +638,7 @@ void dgraph_release(rbsc_dgraph* g) {
     if (g->tile_row) (void)hipFree(g->tile_row);
     if (g->carry_row) (void)hipFree(g->carry_row);
     if (g->dup_rep) (void)hipFree(g->dup_rep);
+    if (g->bins) bins_release(g->bins);
     *g = rbsc_dgraph();
 }
 
@@ -838,11 +839,22 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     u32 K = prm.k0, Knew = prm.k0;
     u32 it = 0;
     const u64 n_tiles = g->n_tiles;
+    // The binned layout serves resident rounds over raw u32 colours.
+    const bool binned = g->bins && !streaming && c32 && g->n_parts == 1 &&
+                        g->bins->GL == gather_limit && !(prm.flags & RBSC_FLAG_NOBINS);
     do {
         K = Knew;
-        prof_begin(ctx, 0);
         const u64 dang_h = rbsc_randomize(0 + (u64)it);
-        if (n_tiles && !streaming && g->n_parts == 8 && (prm.flags & RBSC_FLAG_XCD)) {
+        if (binned) {
+            const u32* col = reinterpret_cast<const u32*>(w.hc);
+            prof_begin(ctx, 6);
+            bins_scatter(ctx, g, col);
+            prof_end(ctx);
+        }
+        prof_begin(ctx, 0);
+        if (binned) {
+            bins_sig(ctx, g, reinterpret_cast<const u32*>(w.hc), (u64)it, dang_h, w.sig);
+        } else if (n_tiles && !streaming && g->n_parts == 8 && (prm.flags & RBSC_FLAG_XCD)) {
             fill_u64<<<grid_for(g->U), BLK, 0, s>>>(w.sig, g->U, 0);
             launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0, gather_limit, dang_h, it, c32, true);
         } else if (n_tiles && !streaming && g->n_parts > 1) {
@@ -881,7 +893,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         }
         prof_end(ctx);
         prof_begin(ctx, 5);
-        if (n_tiles)
+        if (n_tiles && !binned)
             sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, w.carry,
                                                                g->n_parts > 1 ? g->seg_row : nullptr,
                                                                w.sig, n_tiles);
@@ -1237,12 +1249,61 @@ extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgrap
     return graph_upload_host(ctx, gr, false, out);
 }
 
+static void drop_bins(rbsc_dgraph* g) {
+    if (!g->bins) return;
+    g->bytes -= g->bins->bytes;
+    bins_release(g->bins);
+    g->bins = nullptr;
+}
+
+extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads,
+                              uint32_t rows_per_bin, uint64_t chunk_slots) {
+    if (!ctx || !g) {
+        rbsc_set_error("rbsc_graph_bin: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    if (rows_per_bin == 0) rows_per_bin = 1u << 12;
+    if (rows_per_bin & (rows_per_bin - 1)) {
+        rbsc_set_error("rbsc_graph_bin: rows_per_bin must be a power of two");
+        return RBSC_ERR_INVALID;
+    }
+    u32 log_s = 0;
+    while ((1u << log_s) < rows_per_bin) ++log_s;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (g->R == 0) {
+        drop_bins(g);
+        return RBSC_OK;
+    }
+    return dgraph_bin(ctx, g, hot_heads ? hot_heads : (1u << 20), log_s, chunk_slots);
+}
+
+extern "C" int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g) {
+    if (!g) return RBSC_OK;
+    if (ctx) HIP_TRY(hipSetDevice(ctx->device));
+    drop_bins(g);
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_graph_bin_info(const rbsc_dgraph* g, uint64_t* info) {
+    if (!g || !info) {
+        rbsc_set_error("rbsc_graph_bin_info: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    const rbsc_bins* b = g->bins;
+    const uint64_t v[8] = {b ? 1u : 0u, b ? b->n_bins : 0, b ? b->n_hot : 0, b ? b->n_cold : 0,
+                           b ? b->n_chunks : 0, b ? b->n_zero : 0, b ? b->bytes : 0,
+                           b ? ((u64)b->H | ((u64)b->log_s << 32)) : 0};
+    for (int i = 0; i < 8; ++i) info[i] = v[i];
+    return RBSC_OK;
+}
+
 extern "C" int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_parts) {
     if (!ctx || !g) {
         rbsc_set_error("rbsc_graph_relayout: null argument");
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(ctx->device));
+    drop_bins(g);
     return dgraph_relayout(ctx, g, n_parts, 0);
 }
 
@@ -1252,6 +1313,7 @@ extern "C" int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(ctx->device));
+    drop_bins(g);
     return dgraph_relayout(ctx, g, 2, std::min<u32>(hot_heads, g->U ? g->U : 1));
 }
 
@@ -1266,6 +1328,7 @@ extern "C" int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g) {
     }
     if (!g->dst) return RBSC_OK;  // already on the host (or no lines)
     HIP_TRY(hipSetDevice(ctx->device));
+    drop_bins(g);  // streamed rounds read the plain line array
     const u64 E = g->E;
     HIP_TRY(hipHostMalloc(&g->dst_host, sizeof(u32) * std::max<u64>(E, 1), hipHostMallocDefault));
     HIP_TRY(hipMemcpy(g->dst_host, g->dst, sizeof(u32) * E, hipMemcpyDeviceToHost));

@@ -1,0 +1,128 @@
+"""Binned signature layout (rbsc_graph_bin): every round's colours equal the
+oracle's and the plain kernel's, for every bin size, hot range and chunk
+split, with weights, initial colours, repeated rows and rows whose lines all
+go to nodes without lines."""
+import numpy as np
+import pytest
+
+import oracle
+import rbsc_amd
+from rbsc_amd import HostGraph, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def _weighted(g, k, seed=3):
+    g.weight = np.random.default_rng(seed).integers(1, k + 1, g.n_edges).astype(np.uint64)
+    return g
+
+
+def _wide_weighted(g, seed=4):
+    # full 64-bit weights: products and sums wrap mod 2^64
+    g.weight = np.random.default_rng(seed).integers(0, 2**63, g.n_edges, dtype=np.uint64) * 2 + 1
+    return g
+
+
+CASES = {
+    "rmat_s14_c30": lambda: gen.rmat(14, 1 << 18, clone_frac=0.3, seed=31),
+    "rmat_s12_w3": lambda: _weighted(gen.rmat(12, 1 << 16, clone_frac=0.2, seed=32), 3),
+    "rmat_s11_w64": lambda: _wide_weighted(gen.rmat(11, 1 << 14, seed=33)),
+    "ba_init3": lambda: gen.ba(20000, 3, 0.1, 3, seed=34),
+}
+
+# (hot heads, rows per bin, slots per chunk): defaults, everything cold,
+# everything hot, small bins split over many chunks, the largest bins
+SHAPES = [(0, 0, 0), (1, 256, 0), (1 << 30, 512, 0), (64, 256, 300), (4096, 16384, 1000)]
+
+
+def _trace(g, rounds=64):
+    return oracle.rbsc_refine(g.n_nodes, g.sources(), g.dst, g.weight, g.init_color,
+                              trace_rounds=rounds)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_binned_rounds_match_oracle(ctx, name):
+    g = CASES[name]()
+    c, K, R, tr = _trace(g)
+    dg = ctx.upload(g)
+    plain = dg.refine(colors=True)
+    assert (plain.nodes, plain.rounds) == (K, R)
+    for hot, rows, chunk in SHAPES:
+        dg.bin(hot, rows, chunk)
+        info = dg.bin_info
+        assert info["binned"]
+        assert info["hot_lines"] + info["cold_lines"] <= dg.stored_lines
+        if chunk and chunk < 1000:
+            assert info["split_bins"] > 0
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True)
+            assert np.array_equal(res.colors, tr[r - 1]), f"{(hot, rows, chunk)} round {r}"
+        full = dg.refine(verify=True, colors=True)
+        assert (full.nodes, full.rounds, full.verified) == (K, R, 1)
+        assert np.array_equal(full.colors, plain.colors)
+        assert np.array_equal(full.partition, oracle.canonical(c))
+    # the plain kernel on the binned graph
+    res = dg.refine(colors=True, flags=rbsc_amd.FLAG_NOBINS)
+    assert np.array_equal(res.colors, plain.colors)
+    dg.free()
+
+
+def _coo(n, lines, weight=None):
+    order = sorted(range(len(lines)), key=lambda i: lines[i][0])
+    src = np.array([lines[i][0] for i in order], np.uint32)
+    dst = np.array([lines[i][1] for i in order], np.uint32)
+    w = None if weight is None else np.array([weight[i] for i in order], np.uint64)
+    return HostGraph(n, dst, src=src, weight=w)
+
+
+def test_rows_into_nodes_without_lines_only(ctx):
+    # nodes 0..3 have lines; 4..8 have none.  Row 0 reaches only line-less
+    # nodes (its whole signature is the per-row term), row 1 mixes, row 2 is
+    # a multi-line row into line-less nodes with weights, row 3 is all lined.
+    lines = [(0, 4), (0, 5), (1, 4), (1, 2), (2, 6), (2, 7), (2, 8), (3, 0), (3, 1), (3, 2)]
+    w = [1, 2, 1, 1, 3, 1, 5, 1, 1, 2]
+    for weight in (None, w):
+        g = _coo(9, lines, weight)
+        c, K, R, tr = _trace(g)
+        dg = ctx.upload(g)
+        for hot in (1, 2, 1 << 20):
+            dg.bin(hot, 256, 0)
+            for r in range(1, R + 1):
+                res = dg.refine(max_rounds=r, partition=False, colors=True)
+                assert np.array_equal(res.colors, tr[r - 1])
+            assert dg.refine(verify=True).verified == 1
+        dg.free()
+
+
+def test_bins_follow_the_line_layout(ctx):
+    g = gen.rmat(12, 1 << 15, clone_frac=0.2, seed=35)
+    c, K, R, _ = _trace(g)
+    dg = ctx.upload(g)
+    dg.bin()
+    n0 = dg.nbytes
+    info = dg.bin_info
+    assert info["bytes"] > 0 and n0 >= info["bytes"]
+    dg.unbin()
+    assert not dg.bin_info["binned"] and dg.nbytes == n0 - info["bytes"]
+    dg.bin()
+    dg.lines_to_host()  # streamed rounds read the plain lines: bins dropped
+    assert not dg.bin_info["binned"]
+    st = dg.refine(max_batch_edges=dg.stored_lines // 3 + 1, colors=True)
+    assert (st.nodes, st.rounds) == (K, R)
+    dg.free()
+    dg = ctx.upload(g)
+    dg.bin()
+    dg.relayout(4)  # a partitioned layout replaces the bins
+    assert not dg.bin_info["binned"]
+    assert dg.refine().nodes == K
+    dg.free()
+
+
+def test_hashed64_on_a_binned_graph(ctx):
+    g = gen.rmat(12, 1 << 15, clone_frac=0.2, seed=36)
+    c, K, R, tr = _trace(g)
+    dg = ctx.upload(g)
+    dg.bin(256, 256, 0)
+    res = dg.refine(colors=True, flags=rbsc_amd.FLAG_HASHED64)
+    assert (res.nodes, res.rounds) == (K, R) and np.array_equal(res.colors, tr[R - 1])
+    dg.free()
