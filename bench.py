@@ -210,9 +210,10 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int) -> dict:
 
 
 def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    counter summary (FETCH_SIZE and WRITE_SIZE runs, calibrated against a
-    copy of known size), or None if there is none."""
+    """HBM bytes per launch of `kernel` (or per round of the signature pass)
+    on `workload` from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE runs, calibrated against a copy of known size), or None if
+    there is none."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_hbm_traffic.md")
     try:
         with open(path) as f:
@@ -281,11 +282,16 @@ def main():
                     help="gather per-node hashed colours instead of raw u32 colours")
     ap.add_argument("--xcd", action="store_true",
                     help="with --parts 8: one launch, partition b mod 8 per workgroup b")
-    ap.add_argument("--bins", type=int, default=0,
-                    help="1: binned signature layout (rbsc_graph_bin) before timing")
+    ap.add_argument("--bins", type=int, default=1,
+                    help="1: binned signature layout (rbsc_graph_bin) built before timing; "
+                         "0: plain line layout")
     ap.add_argument("--bin-hot", type=int, default=0, help="hot heads of the binned layout (0 = default)")
     ap.add_argument("--bin-rows", type=int, default=0, help="rows per bin (0 = default)")
     ap.add_argument("--bin-chunk", type=int, default=0, help="slots per chunk (0 = default)")
+    ap.add_argument("--bin-scatter", type=int, default=0,
+                    help="cold lines per scatter workgroup (0 = default)")
+    ap.add_argument("--bin-slot-order", action="store_true",
+                    help="scatter records in slot order inside each workgroup's share")
     ap.add_argument("--batch-graphs", type=int, default=50000,
                     help="graphs in the many-small-graphs leg (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=200,
@@ -322,7 +328,7 @@ def main():
     bin_info = None
     if args.bins and args.parts == 1 and not args.hot:
         t0 = time.perf_counter()
-        dg.bin(args.bin_hot, args.bin_rows, args.bin_chunk)
+        dg.bin(args.bin_hot, args.bin_rows, args.bin_chunk, args.bin_scatter, args.bin_slot_order)
         bin_info = dict(dg.bin_info, build_s=time.perf_counter() - t0)
         log(f"[bench] binned layout in {bin_info['build_s']:.1f}s: {bin_info}")
     flags = (rbsc_amd.FLAG_FULLSORT if args.fullsort else 0) | (rbsc_amd.FLAG_HASHED64 if args.hashed64 else 0) | (rbsc_amd.FLAG_XCD if args.xcd else 0)
@@ -352,15 +358,23 @@ def main():
     # Whole round: 8 B per input line (head id + neighbour colour) and 32 B
     # per node (row offset, signature write and read, old and new colour).
     b_round = 8 * E + 32 * N
-    # Dominant kernel, the signature pass: 8 B per line it reads plus 16 B
-    # per row it writes (row offset, signature); one launch per round.
+    # The signature pass (the reference's set_values + ReduceByKey): 8 B per
+    # stored line (head id + neighbour colour) plus 16 B per stored row (row
+    # offset or per-row term, signature write), once per round.  On the
+    # binned layout the pass is two launches, cold_scatter_kernel (phase
+    # "scatter") and bin_sig_kernel (phase "signature"); on the plain layout
+    # it is sig_kernel alone.
     b_sig = 8 * E_st + 16 * U
     per_round = {k: phases[k][0] / rounds for k in phases if k != "setup"}
     ROUND = ("scatter", "signature", "finish", "gather", "sort", "rank")
     round_ms = sum(per_round[k] for k in ROUND)
     dom = max(ROUND, key=lambda k: per_round[k])
-    sig_ms = per_round["signature"]
-    traffic = measured_traffic(args.config, "sig_kernel") if args.parts == 1 and not args.hot else None
+    binned = bin_info is not None and not (flags & rbsc_amd.FLAG_HASHED64)
+    sig_ms = per_round["signature"] + (per_round["scatter"] if binned else 0.0)
+    pass_name = "signature_pass" if binned else "sig_kernel"
+    kernels = ({"cold_scatter_kernel": per_round["scatter"], "bin_sig_kernel": per_round["signature"]}
+               if binned else {"sig_kernel": per_round["signature"]})
+    traffic = measured_traffic(args.config, pass_name) if args.parts == 1 and not args.hot else None
     out = {
         "metric": "edge-visits/s per WL round (% HBM roofline); batched graphs/s, p99 latency",
         "value": value,
@@ -379,11 +393,12 @@ def main():
                    "parts": args.parts, "hot": args.hot, "bins": bin_info, "fullsort": bool(args.fullsort), "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
                    "rounds_per_step": rounds // args.steps,
                    "blocks": r.nodes, "ms_per_round": ms_round},
-        "roofline": {"bound": "hbm", "kernel": "sig_kernel",
+        "roofline": {"bound": "hbm", "kernel": " + ".join(kernels),
                      "achieved": b_sig / (sig_ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": b_sig / (sig_ms / 1e3) / HBM_PEAK,
                      "traffic": traffic,
                      "alg_bytes_per_launch": b_sig, "launch_ms": sig_ms,
+                     "kernel_ms": kernels,
                      "dominant_phase": dom,
                      "round": {"alg_bytes": b_round, "ms": round_ms,
                                "achieved": b_round / (round_ms / 1e3) / 1e9,

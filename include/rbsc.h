@@ -164,19 +164,30 @@ int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
 
 /* Binned signature layout of a resident graph, an addition to its plain
  * layout (about 6 bytes per gathered line plus 8 per line into a cold head;
- * 14 with weights).  Rows are cut into bins of rows_per_bin (a power of two
- * from 256 to 16384; 0 = 4096) whose signatures are summed in LDS; lines
- * into the hot_heads most-gathered nodes (0 = 2^20) gather their colours
- * in head order, the colours of the other gathered heads are scattered to
- * their lines' slots once per round in head order; bins with more than
- * chunk_slots lines (0 = 2^19) are split over several workgroups.  Later
- * refinements of g use it unless RBSC_FLAG_NOBINS, RBSC_FLAG_HASHED64 or
- * streamed lines are asked for; relayout, split_hot and lines_to_host drop
- * it.  Signatures and results are identical.  No counterpart in the
- * reference (its set_values + ReduceByKey, rbsc.cu:357-377, read the lines
- * in input order). */
-int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads, uint32_t rows_per_bin,
-                   uint64_t chunk_slots);
+ * 8 more per line with weights).  Stored rows are cut into bins of
+ * rows_per_bin (a power of two from 256 to 16384; 0 = 4096) whose
+ * signatures are summed in LDS; lines into the hot_heads most-gathered nodes
+ * (0 = 2^20) gather their colours in head order inside the bin; the colours
+ * of the other gathered heads are scattered to their lines' slots once per
+ * round, one workgroup per scatter_lines of them (0 = 2048) in head order,
+ * kept in head order (colours read in order) or, with RBSC_BIN_SLOT_ORDER,
+ * reordered by slot inside each workgroup's share (stores coalesced); bins with
+ * more than chunk_slots lines (0 = 2^19) are split over several
+ * workgroups.  opts may be null (all defaults).  Later refinements of g use
+ * the layout unless RBSC_FLAG_NOBINS, RBSC_FLAG_HASHED64 or streamed lines
+ * are asked for; relayout, split_hot and lines_to_host drop it.  Signatures
+ * and results are identical.  No counterpart in the reference (its
+ * set_values + ReduceByKey, rbsc.cu:357-377, read the lines in input
+ * order). */
+typedef struct {
+    uint32_t hot_heads;
+    uint32_t rows_per_bin;
+    uint64_t chunk_slots;
+    uint32_t scatter_lines;
+    uint32_t flags;           /* RBSC_BIN_* */
+} rbsc_bin_opts;
+#define RBSC_BIN_SLOT_ORDER 1u
+int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts* opts);
 int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g);
 /* info[8]: binned (0/1), bins, hot lines, cold lines, workgroup chunks, bins
  * split over several chunks, bytes, hot_heads | log2(rows_per_bin) << 32. */

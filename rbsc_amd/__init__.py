@@ -225,14 +225,17 @@ class DeviceGraph:
         first (their colours stay cache resident); results are unchanged."""
         check(load().rbsc_graph_split_hot(self.ctx.handle, self.handle, int(hot_heads)))
 
-    def bin(self, hot_heads: int = 0, rows_per_bin: int = 0, chunk_slots: int = 0):
+    def bin(self, hot_heads: int = 0, rows_per_bin: int = 0, chunk_slots: int = 0,
+            scatter_lines: int = 0, slot_order: bool = False):
         """Build the binned signature layout (row sums in LDS per bin of
         rows, hot heads gathered in head order, the other colours scattered
-        to their lines in head order); later refinements use it.  0 picks
-        the defaults (2^20 hot heads, 4096 rows per bin, 2^19 slots per
-        chunk).  Results are unchanged."""
-        check(load().rbsc_graph_bin(self.ctx.handle, self.handle, int(hot_heads),
-                                    int(rows_per_bin), int(chunk_slots)))
+        to their lines' slots in chunks of lines); later refinements use it.
+        0 picks the defaults (2^20 hot heads, 4096 rows per bin, 2^19 slots
+        per chunk, 2048 lines per scatter workgroup, kept in head order).
+        Results are unchanged."""
+        o = _lib.BinOpts(int(hot_heads), int(rows_per_bin), int(chunk_slots), int(scatter_lines),
+                         _lib.BIN_SLOT_ORDER if slot_order else 0)
+        check(load().rbsc_graph_bin(self.ctx.handle, self.handle, C.byref(o)))
 
     def unbin(self):
         check(load().rbsc_graph_unbin(self.ctx.handle, self.handle))

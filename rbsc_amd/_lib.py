@@ -32,6 +32,13 @@ FLAG_SPLIT16 = 8
 FLAG_TIESMALL = 16
 FLAG_FULLSORT = 32
 FLAG_NOBINS = 64
+BIN_SLOT_ORDER = 1
+
+
+class BinOpts(C.Structure):
+    _fields_ = [("hot_heads", C.c_uint32), ("rows_per_bin", C.c_uint32),
+                ("chunk_slots", C.c_uint64), ("scatter_lines", C.c_uint32),
+                ("flags", C.c_uint32)]
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -189,7 +196,7 @@ SIGNATURES = {
     "rbsc_graph_stored_lines": (C.c_uint64, [C.c_void_p]),
     "rbsc_ctx_set_dedup": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_graph_split_hot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
-    "rbsc_graph_bin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64]),
+    "rbsc_graph_bin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rbsc_graph_unbin": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_graph_bin_info": (C.c_int, [C.c_void_p, u64p]),
     "rbsc_graph_download": (C.c_int, [C.c_void_p, C.c_void_p, u64p, u32p]),

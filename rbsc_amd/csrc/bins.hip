@@ -7,13 +7,20 @@
 //   sig(u) = sum over u's lines (u, v, w) of randomize(c(v) + r) * w  (mod 2^64)
 // (rbsc.cu:80-87 set_values + rbsc.cu:366-377 ReduceByKey) in two passes:
 //
-//   cold_scatter_kernel  walks the cold lines (H <= head < gather limit) in
-//                        head order: colours are read in order, and each
-//                        line's colour is written into its slot.  The slots
-//                        of one bin are in head order too, so every bin's
-//                        slot run fills front to back; bins are split into 8
-//                        groups, one per XCD, so a slot's cache line is
-//                        written from one L2 only.
+//   cold_scatter_kernel  writes each cold line's colour (H <= head < gather
+//                        limit) into its slot.  Bins are split into 8 groups,
+//                        one per XCD; a group's cold lines, in head order, are
+//                        cut into chunks of a fixed number of lines, one
+//                        workgroup per chunk (chunk c of group g is workgroup
+//                        8c + g: workgroups are dealt to the XCDs round
+//                        robin).  A chunk spans a narrow head range, so its
+//                        colour reads stay in cache; inside the chunk the
+//                        lines are kept in slot order (default) -- a bin's
+//                        cold slots are in head order, so a chunk's lines for
+//                        one bin fill one run of consecutive slots and
+//                        neighbouring lines write neighbouring words -- or in
+//                        head order (RBSC_BIN_HEAD_ORDER: colour reads in
+//                        order, stores scattered).
 //   bin_sig_kernel       one workgroup per chunk of a bin of 2^log_s rows:
 //                        row sums in LDS, hot slots (head < H) gathered from
 //                        the L2-resident hot colours in head order (a wave's
@@ -36,10 +43,7 @@ inline int bitlen(u64 v) {
     while (b < 64 && (v >> b)) ++b;
     return b;
 }
-constexpr int SCATTER_V = 8;                          // lines per thread
-constexpr u64 SCATTER_WG = (u64)BLK * SCATTER_V;      // lines per workgroup
 constexpr int SIG_THREADS = 512;
-constexpr int SIG_VB = 8;                             // slots per thread per step
 
 // ---------------------------------------------------------------- build
 // Per stored row: lines into hot heads, into cold heads, and the total
@@ -129,7 +133,7 @@ __global__ void place_hot_kernel(const u64* __restrict__ keys, const u64* __rest
 }
 
 // Sorted cold records into their slots (after the bin's hot slots), and the
-// (group, head) -> slot records of the scatter pass.
+// scatter records: key (group, head), value slot << 32 | head.
 template <bool HAS_WT>
 __global__ void place_cold_kernel(const u64* __restrict__ keys, const u64* __restrict__ vals,
                                   u64 n, int gb, const u64* __restrict__ bin_slot,
@@ -150,42 +154,70 @@ __global__ void place_cold_kernel(const u64* __restrict__ keys, const u64* __res
     slot_row[s] = (unsigned short)(v & 0xFFFF);
     if (HAS_WT) slot_wt[s] = wt[v >> 16];
     k2[i] = ((u64)grp_of_bin[b] << gb) | head;
-    v2[i] = s;
+    v2[i] = (s << 32) | head;
 }
 
-__global__ void csc_fill_kernel(const u64* __restrict__ k2, const u64* __restrict__ v2, u64 n,
-                                int gb, u32* __restrict__ csc_head, u32* __restrict__ csc_slot) {
+// Records in (group, head) order -> key (tile, slot) or (tile, position), where
+// tile = 8 * (position in the group / chunk) + group.
+__global__ void chunk_key_kernel(const u64* __restrict__ k2, const u64* __restrict__ v2, u64 n,
+                                 int gb, const u64* __restrict__ grp_first, u64 chunk,
+                                 int by_slot, u64* __restrict__ k3) {
     const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    csc_head[i] = (u32)(k2[i] & ((1ull << gb) - 1));
-    csc_slot[i] = (u32)v2[i];
+    const u64 g = k2[i] >> gb;
+    const u64 t = ((i - grp_first[g]) / chunk) * BIN_GROUPS + g;
+    k3[i] = (t << 32) | (by_slot ? (v2[i] >> 32) : (i & 0xFFFFFFFFull));
+}
+
+// Records sorted by tile -> compact arrays and the first record of every
+// tile (tile_line[n_tiles] = n).
+__global__ void csc_fill_kernel(const u64* __restrict__ k3, const u64* __restrict__ v3, u64 n,
+                                u64 n_tiles, u32* __restrict__ csc_head,
+                                u32* __restrict__ csc_slot, u64* __restrict__ tile_line) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64 t = k3[i] >> 32;
+    csc_head[i] = (u32)(v3[i] & 0xFFFFFFFFull);
+    csc_slot[i] = (u32)(v3[i] >> 32);
+    const u64 first = i == 0 ? 0 : (k3[i - 1] >> 32) + 1;
+    for (u64 u = first; u <= t; ++u) tile_line[u] = i;
+    if (i == n - 1)
+        for (u64 u = t + 1; u <= n_tiles; ++u) tile_line[u] = n;
 }
 
 // ---------------------------------------------------------------- rounds
-// Writes the current colour of every cold line's head into the line's slot.
-// Workgroup b serves group b mod 8 (workgroups are dealt to the XCDs round
-// robin), lines [grp_line[g] + (b / 8) * SCATTER_WG, ...) in head order.
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+
+// Writes the current colour of every cold line's head into the line's slot,
+// one workgroup per tile (records [tile_line[t], tile_line[t + 1])).  Records
+// are read 8 at a time with 16-byte loads: thread i takes the aligned groups
+// [8q, 8q + 8) for q = lo / 8 + i, + BLK, ...; records outside the tile (a
+// neighbour's, or the zeroed padding past the end) are loaded, and their
+// heads (valid ids or 0) read, but never stored.
 __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict__ csc_head,
                                                            const u32* __restrict__ csc_slot,
-                                                           const u64* __restrict__ grp_line,
+                                                           const u64* __restrict__ tile_line,
                                                            const u32* __restrict__ col,
                                                            u32* __restrict__ slot_key) {
-    const u32 g = blockIdx.x & (BIN_GROUPS - 1);
-    const u64 lo = grp_line[g] + (u64)(blockIdx.x / BIN_GROUPS) * SCATTER_WG;
-    const u64 end = grp_line[g + 1];
-    if (lo >= end) return;
-    const u64 hi = lo + SCATTER_WG < end ? lo + SCATTER_WG : end;
-    u32 hd[SCATTER_V], sl[SCATTER_V];
+    const u64 t = blockIdx.x;
+    const u64 lo = tile_line[t], hi = tile_line[t + 1];
+    const u64 q1 = (hi + 7) / 8;
+    for (u64 q = lo / 8 + threadIdx.x; q < q1; q += BLK) {
+        const u32x4 ha = *reinterpret_cast<const u32x4*>(csc_head + 8 * q);
+        const u32x4 hb = *reinterpret_cast<const u32x4*>(csc_head + 8 * q + 4);
+        const u32x4 sa = *reinterpret_cast<const u32x4*>(csc_slot + 8 * q);
+        const u32x4 sb = *reinterpret_cast<const u32x4*>(csc_slot + 8 * q + 4);
+        const u32 hd[8] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y, hb.z, hb.w};
+        const u32 sl[8] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
+        u32 c[8];
 #pragma unroll
-    for (int k = 0; k < SCATTER_V; ++k) {
-        const u64 j = lo + (u64)k * BLK + threadIdx.x;
-        hd[k] = j < hi ? csc_head[j] : 0u;
-        sl[k] = j < hi ? csc_slot[j] : 0u;
-    }
+        for (int k = 0; k < 8; ++k) c[k] = col[hd[k]];
 #pragma unroll
-    for (int k = 0; k < SCATTER_V; ++k) {
-        const u64 j = lo + (u64)k * BLK + threadIdx.x;
-        if (j < hi) slot_key[sl[k]] = col[hd[k]];
+        for (int k = 0; k < 8; ++k) {
+            const u64 j = 8 * q + k;
+            if (j >= lo && j < hi) slot_key[sl[k]] = c[k];
+        }
     }
 }
 
@@ -197,6 +229,43 @@ __global__ void bin_zero_kernel(const u32* __restrict__ zero_bins, u64 n_zero, u
     if (k >= n_zero) return;
     const u64 row = ((u64)zero_bins[k] << log_s) + (i & ((1ull << log_s) - 1));
     if (row < R) sig[row] = 0;
+}
+
+// Adds the hashed colours of slots [a, e) into the LDS row sums: slots below
+// cold0 (hot) hold heads whose colours are gathered, the others hold
+// colours.  Slots are read 8 at a time as in scatter_lines (aligned groups,
+// slots outside [a, e) loaded but not added).
+template <bool HAS_WT>
+__device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0,
+                                               const u32* __restrict__ slot_key,
+                                               const unsigned short* __restrict__ slot_row,
+                                               const u64* __restrict__ slot_wt,
+                                               const u32* __restrict__ col, u64 salt, u64* sums) {
+    const u64 q1 = (e + 7) / 8;
+    for (u64 q = a / 8 + threadIdx.x; q < q1; q += SIG_THREADS) {
+        const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + 8 * q);
+        const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + 8 * q + 4);
+        const u16x8 r8 = *reinterpret_cast<const u16x8*>(slot_row + 8 * q);
+        u32 key[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+        const u32 row[8] = {r8.s0, r8.s1, r8.s2, r8.s3, r8.s4, r8.s5, r8.s6, r8.s7};
+        if (8 * q < cold0) {  // the group holds hot slots
+            u32 g[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g[k] = col[8 * q + k < cold0 ? key[k] : 0u];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (8 * q + k < cold0) key[k] = g[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const u64 s = 8 * q + k;
+            if (s >= a && s < e) {
+                u64 h = rbsc_randomize((u64)key[k] + salt);
+                if (HAS_WT) h *= slot_wt[s];
+                atomicAdd((unsigned long long*)&sums[row[k]], (unsigned long long)h);
+            }
+        }
+    }
 }
 
 template <int LOG_S, bool HAS_WT>
@@ -216,32 +285,7 @@ __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
         sums[i] = (first && i < nrows) ? dang_h * dsum[row0 + i] : 0ull;
     __syncthreads();
     const u64 cold0 = bin_slot[ch.bin] + bin_hot[ch.bin];
-    for (u64 base = ch.lo; base < ch.hi; base += (u64)SIG_THREADS * SIG_VB) {
-        u32 key[SIG_VB], row[SIG_VB];
-        u64 wv[SIG_VB];
-#pragma unroll
-        for (int k = 0; k < SIG_VB; ++k) {
-            const u64 s = base + (u64)k * SIG_THREADS + threadIdx.x;
-            const bool ok = s < ch.hi;
-            key[k] = ok ? slot_key[s] : 0u;
-            row[k] = ok ? (u32)slot_row[s] : 0u;
-            if (HAS_WT) wv[k] = ok ? slot_wt[s] : 0ull;
-        }
-#pragma unroll
-        for (int k = 0; k < SIG_VB; ++k) {
-            const u64 s = base + (u64)k * SIG_THREADS + threadIdx.x;
-            if (s < cold0 && s < ch.hi) key[k] = col[key[k]];
-        }
-#pragma unroll
-        for (int k = 0; k < SIG_VB; ++k) {
-            const u64 s = base + (u64)k * SIG_THREADS + threadIdx.x;
-            if (s < ch.hi) {
-                u64 h = rbsc_randomize((u64)key[k] + salt);
-                if (HAS_WT) h *= wv[k];
-                atomicAdd((unsigned long long*)&sums[row[k]], (unsigned long long)h);
-            }
-        }
-    }
+    bin_accumulate<HAS_WT>(ch.lo, ch.hi, cold0, slot_key, slot_row, slot_wt, col, salt, sums);
     __syncthreads();
     const bool only = (ch.flags & CHUNK_ONLY) != 0;
     for (u32 i = threadIdx.x; i < nrows; i += SIG_THREADS) {
@@ -267,11 +311,8 @@ void launch_bin_sig(hipStream_t s, const rbsc_bins* b, const u32* col, u64 salt,
 void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col) {
     const rbsc_bins* b = g->bins;
     if (!b->n_cold) return;
-    u64 most = 0;
-    for (int k = 0; k < BIN_GROUPS; ++k) most = std::max(most, b->grp_line[k + 1] - b->grp_line[k]);
-    const u64 per = (most + SCATTER_WG - 1) / SCATTER_WG;
-    cold_scatter_kernel<<<(unsigned)(per * BIN_GROUPS), BLK, 0, ctx->stream>>>(
-        b->csc_head, b->csc_slot, b->d_grp_line, col, b->slot_key);
+    cold_scatter_kernel<<<(unsigned)b->n_tiles, BLK, 0, ctx->stream>>>(
+        b->csc_head, b->csc_slot, b->tile_line, col, b->slot_key);
 }
 
 void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col, u64 salt, u64 dang_h,
@@ -296,18 +337,20 @@ void bins_release(rbsc_bins* b) {
     if (!b) return;
     for (void* p : {(void*)b->bin_slot, (void*)b->bin_hot, (void*)b->slot_key, (void*)b->slot_row,
                     (void*)b->slot_wt, (void*)b->dsum, (void*)b->csc_head, (void*)b->csc_slot,
-                    (void*)b->d_grp_line, (void*)b->chunks, (void*)b->zero_bins})
+                    (void*)b->tile_line, (void*)b->chunks, (void*)b->zero_bins})
         if (p) (void)hipFree(p);
     delete b;
 }
 
-int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots) {
+int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
+               u64 scatter_chunk, bool head_order) {
     if (!g->dst || g->n_parts != 1) {
         rbsc_set_error("bin: needs the plain layout with lines resident on the device");
         return RBSC_ERR_INVALID;
     }
-    if (log_s < 8 || log_s > 14) {
-        rbsc_set_error("bin: rows per bin must be a power of two from 256 to 16384");
+    if (log_s < 8 || log_s > 14 || scatter_chunk == 0) {
+        rbsc_set_error("bin: rows per bin must be a power of two from 256 to 16384, scatter "
+                       "chunks nonempty");
         return RBSC_ERR_INVALID;
     }
     hipStream_t s = ctx->stream;
@@ -323,10 +366,13 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots)
     b->H = H;
     b->GL = GL;
     b->n_bins = n_bins;
+    b->scatter_chunk = scatter_chunk;
+    b->head_order = head_order;
     u64 *n_hot = nullptr, *n_cold = nullptr, *tmp = nullptr, *tot = nullptr, *hot_start = nullptr,
         *cold_start = nullptr, *bin_n = nullptr, *keys[2] = {nullptr, nullptr},
         *vals[2] = {nullptr, nullptr};
     u32* grp_of_bin = nullptr;
+    u64* grp_first = nullptr;
     std::vector<u64> h_bin_n, h_bin_hot, h_bin_slot;
     std::vector<u32> h_grp(n_bins + 1, 0), h_zero;
     std::vector<BinChunk> h_chunks;
@@ -396,9 +442,6 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots)
             h_grp[k] = grp;
             seen += cold;
         }
-        std::fill(b->grp_line, b->grp_line + BIN_GROUPS + 1, 0);
-        for (u64 k = 0; k < n_bins; ++k) b->grp_line[h_grp[k] + 1] += h_bin_n[k] - h_bin_hot[k];
-        for (int k = 0; k < BIN_GROUPS; ++k) b->grp_line[k + 1] += b->grp_line[k];
     }
     // Chunks: a bin in one chunk when its slots fit, else split (those bins'
     // rows are zeroed each round and their chunks add atomically).
@@ -429,12 +472,12 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots)
         B_TRY(hipMemcpyAsync(b->zero_bins, h_zero.data(), sizeof(u32) * b->n_zero,
                              hipMemcpyHostToDevice, s));
     }
-    B_TRY(hipMalloc(&b->d_grp_line, sizeof(u64) * (BIN_GROUPS + 1)));
-    B_TRY(hipMemcpyAsync(b->d_grp_line, b->grp_line, sizeof(u64) * (BIN_GROUPS + 1),
-                         hipMemcpyHostToDevice, s));
     // slot arrays
-    B_TRY(hipMalloc(&b->slot_key, sizeof(u32) * std::max<u64>(b->n_slots, 1)));
-    B_TRY(hipMalloc(&b->slot_row, sizeof(unsigned short) * std::max<u64>(b->n_slots, 1)));
+    // + 16 zeroed entries: the 8-wide loads of the last group stay in bounds
+    B_TRY(hipMalloc(&b->slot_key, sizeof(u32) * (b->n_slots + 16)));
+    B_TRY(hipMalloc(&b->slot_row, sizeof(unsigned short) * (b->n_slots + 16)));
+    B_TRY(hipMemsetAsync(b->slot_key, 0, sizeof(u32) * (b->n_slots + 16), s));
+    B_TRY(hipMemsetAsync(b->slot_row, 0, sizeof(unsigned short) * (b->n_slots + 16), s));
     if (g->wt) B_TRY(hipMalloc(&b->slot_wt, sizeof(u64) * std::max<u64>(b->n_slots, 1)));
     if ((rc = sort_ws_init(&ctx->sort, std::max(b->n_hot, b->n_cold), ctx->d_err, s))) goto done;
     // hot slots: records sorted by (bin, head)
@@ -490,15 +533,39 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots)
         int w2 = 0;
         bits = (gb + 3 + 7) / 8 * 8;
         if ((rc = radix_sort_u64(&ctx->sort, k2, v2, b->n_cold, 0, bits, &w2, s))) goto done;
-        B_TRY(hipMalloc(&b->csc_head, sizeof(u32) * b->n_cold));
-        B_TRY(hipMalloc(&b->csc_slot, sizeof(u32) * b->n_cold));
-        csc_fill_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(k2[w2], v2[w2], b->n_cold, gb,
-                                                           b->csc_head, b->csc_slot);
+        // tiles: chunks of each group's lines in head order, interleaved by group
+        u64 h_first[BIN_GROUPS + 1] = {0};
+        u64 most = 0;
+        for (u64 k = 0; k < n_bins; ++k) h_first[h_grp[k] + 1] += h_bin_n[k] - h_bin_hot[k];
+        for (u64 k = 0; k < BIN_GROUPS; ++k) {
+            most = std::max(most, h_first[k + 1]);
+            h_first[k + 1] += h_first[k];
+        }
+        b->n_tiles = (most + scatter_chunk - 1) / scatter_chunk * BIN_GROUPS;
+        B_TRY(hipMalloc(&grp_first, sizeof(u64) * (BIN_GROUPS + 1)));
+        B_TRY(hipMemcpyAsync(grp_first, h_first, sizeof(u64) * (BIN_GROUPS + 1),
+                             hipMemcpyHostToDevice, s));
+        u64* k3[2] = {k2[w2 ^ 1], k2[w2]};
+        u64* v3[2] = {v2[w2], v2[w2 ^ 1]};
+        chunk_key_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(k2[w2], v2[w2], b->n_cold, gb,
+                                                            grp_first, scatter_chunk,
+                                                            head_order ? 0 : 1, k3[0]);
+        int w3 = 0;
+        bits = (32 + bitlen(b->n_tiles) + 7) / 8 * 8;
+        if ((rc = radix_sort_u64(&ctx->sort, k3, v3, b->n_cold, 0, bits, &w3, s))) goto done;
+        B_TRY(hipMalloc(&b->csc_head, sizeof(u32) * (b->n_cold + 16)));
+        B_TRY(hipMalloc(&b->csc_slot, sizeof(u32) * (b->n_cold + 16)));
+        B_TRY(hipMemsetAsync(b->csc_head, 0, sizeof(u32) * (b->n_cold + 16), s));
+        B_TRY(hipMemsetAsync(b->csc_slot, 0, sizeof(u32) * (b->n_cold + 16), s));
+        B_TRY(hipMalloc(&b->tile_line, sizeof(u64) * (b->n_tiles + 1)));
+        csc_fill_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(k3[w3], v3[w3], b->n_cold, b->n_tiles,
+                                                           b->csc_head, b->csc_slot, b->tile_line);
     }
     B_TRY(hipGetLastError());
     B_TRY(hipStreamSynchronize(s));
     b->bytes = 16 * (n_bins + 1) + 6 * b->n_slots + (g->wt ? 8 * b->n_slots : 0) + 8 * R +
-               8 * b->n_cold + sizeof(BinChunk) * b->n_chunks + 4 * b->n_zero;
+               8 * b->n_cold + 8 * (b->n_tiles + 1) + sizeof(BinChunk) * b->n_chunks +
+               4 * b->n_zero;
     if (g->bins) {
         g->bytes -= g->bins->bytes;
         bins_release(g->bins);
@@ -508,7 +575,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots)
     b = nullptr;
 done:
     for (u64* p : {n_hot, n_cold, tmp, tot, hot_start, cold_start, bin_n, keys[0], keys[1],
-                   vals[0], vals[1]})
+                   vals[0], vals[1], grp_first})
         if (p) (void)hipFree(p);
     if (grp_of_bin) (void)hipFree(grp_of_bin);
     if (b) bins_release(b);

@@ -34,7 +34,7 @@ struct BinChunk {
     u32 flags;     // CHUNK_FIRST: adds the bin's dsum terms; CHUNK_ONLY: the
 };                 // bin's only chunk (stores; otherwise adds atomically)
 constexpr u32 CHUNK_FIRST = 1, CHUNK_ONLY = 2;
-constexpr int BIN_GROUPS = 8;  // scatter groups, one per XCD
+constexpr u64 BIN_GROUPS = 8;  // scatter groups, one per XCD
 struct rbsc_bins {
     u32 log_s = 0, H = 0, GL = 0;
     u64 n_bins = 0, n_slots = 0, n_hot = 0, n_cold = 0, n_chunks = 0, n_zero = 0;
@@ -44,18 +44,24 @@ struct rbsc_bins {
     unsigned short* slot_row = nullptr;  // n_slots: row - first row of the bin
     u64* slot_wt = nullptr;         // n_slots or null
     u64* dsum = nullptr;            // R: weight of the row's lines into heads >= GL
-    u32* csc_head = nullptr;        // n_cold, by (group, head)
-    u32* csc_slot = nullptr;        // n_cold: slot receiving that line's colour
-    u64 grp_line[BIN_GROUPS + 1] = {};  // first cold line of each group
-    u64* d_grp_line = nullptr;
+    // Scatter tiles: tile t = 8c + g is chunk c (scatter_chunk lines in head
+    // order) of the cold lines of the bins in group g; records of a tile in
+    // slot order, or in head order with head_order.
+    u64 scatter_chunk = 0;
+    bool head_order = false;
+    u64 n_tiles = 0;
+    u32* csc_head = nullptr;        // n_cold + 16 (zero padding)
+    u32* csc_slot = nullptr;        // n_cold + 16: slot receiving that line's colour
+    u64* tile_line = nullptr;       // n_tiles + 1: first record of each tile
     BinChunk* chunks = nullptr;     // n_chunks
     u32* zero_bins = nullptr;       // n_zero: bins with several chunks
     u64 bytes = 0;
 };
 void bins_release(rbsc_bins* b);
 // Builds g->bins (hot heads below H, 2^log_s rows per bin, chunks of at most
-// chunk_slots slots).
-int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots);
+// chunk_slots slots, scatter tiles of scatter_chunk lines).
+int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
+               u64 scatter_chunk, bool head_order);
 // One round on the binned layout: cold colours into their slots, then the
 // signatures of the stored rows into sig (col: colours by internal id).
 void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col);

@@ -1256,25 +1256,29 @@ static void drop_bins(rbsc_dgraph* g) {
     g->bins = nullptr;
 }
 
-extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads,
-                              uint32_t rows_per_bin, uint64_t chunk_slots) {
+extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts* opts) {
     if (!ctx || !g) {
         rbsc_set_error("rbsc_graph_bin: null argument");
         return RBSC_ERR_INVALID;
     }
-    if (rows_per_bin == 0) rows_per_bin = 1u << 12;
-    if (rows_per_bin & (rows_per_bin - 1)) {
+    rbsc_bin_opts o = opts ? *opts : rbsc_bin_opts{0, 0, 0, 0, 0};
+    if (o.hot_heads == 0) o.hot_heads = 1u << 20;
+    if (o.rows_per_bin == 0) o.rows_per_bin = 1u << 12;
+    if (o.chunk_slots == 0) o.chunk_slots = 1ull << 19;
+    if (o.scatter_lines == 0) o.scatter_lines = 1u << 11;
+    if (o.rows_per_bin & (o.rows_per_bin - 1)) {
         rbsc_set_error("rbsc_graph_bin: rows_per_bin must be a power of two");
         return RBSC_ERR_INVALID;
     }
     u32 log_s = 0;
-    while ((1u << log_s) < rows_per_bin) ++log_s;
+    while ((1u << log_s) < o.rows_per_bin) ++log_s;
     HIP_TRY(hipSetDevice(ctx->device));
     if (g->R == 0) {
         drop_bins(g);
         return RBSC_OK;
     }
-    return dgraph_bin(ctx, g, hot_heads ? hot_heads : (1u << 20), log_s, chunk_slots);
+    return dgraph_bin(ctx, g, o.hot_heads, log_s, o.chunk_slots, o.scatter_lines,
+                      (o.flags & RBSC_BIN_SLOT_ORDER) == 0);
 }
 
 extern "C" int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g) {

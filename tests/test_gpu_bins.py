@@ -30,9 +30,13 @@ CASES = {
     "ba_init3": lambda: gen.ba(20000, 3, 0.1, 3, seed=34),
 }
 
-# (hot heads, rows per bin, slots per chunk): defaults, everything cold,
-# everything hot, small bins split over many chunks, the largest bins
-SHAPES = [(0, 0, 0), (1, 256, 0), (1 << 30, 512, 0), (64, 256, 300), (4096, 16384, 1000)]
+# (hot heads, rows per bin, slots per chunk, cold lines per scatter
+# workgroup, scatter records in slot order): defaults, everything cold,
+# everything hot, small bins split over many chunks, the largest bins,
+# scatter chunks of a few lines (not multiples of 8)
+SHAPES = [(0, 0, 0, 0, False), (1, 256, 0, 0, True), (1 << 30, 512, 0, 0, False),
+          (64, 256, 300, 13, False), (4096, 16384, 1000, 1 << 20, True), (16, 1024, 0, 1, False),
+          (3, 256, 0, 77, True)]
 
 
 def _trace(g, rounds=64):
@@ -47,8 +51,8 @@ def test_binned_rounds_match_oracle(ctx, name):
     dg = ctx.upload(g)
     plain = dg.refine(colors=True)
     assert (plain.nodes, plain.rounds) == (K, R)
-    for hot, rows, chunk in SHAPES:
-        dg.bin(hot, rows, chunk)
+    for hot, rows, chunk, scatter, slot_order in SHAPES:
+        dg.bin(hot, rows, chunk, scatter, slot_order)
         info = dg.bin_info
         assert info["binned"]
         assert info["hot_lines"] + info["cold_lines"] <= dg.stored_lines
@@ -56,7 +60,7 @@ def test_binned_rounds_match_oracle(ctx, name):
             assert info["split_bins"] > 0
         for r in range(1, R + 1):
             res = dg.refine(max_rounds=r, partition=False, colors=True)
-            assert np.array_equal(res.colors, tr[r - 1]), f"{(hot, rows, chunk)} round {r}"
+            assert np.array_equal(res.colors, tr[r - 1]), f"{(hot, rows, chunk, scatter, slot_order)} round {r}"
         full = dg.refine(verify=True, colors=True)
         assert (full.nodes, full.rounds, full.verified) == (K, R, 1)
         assert np.array_equal(full.colors, plain.colors)
