@@ -49,11 +49,18 @@ constexpr int SIG_THREADS = 512;
 // Per stored row: lines into hot heads, into cold heads, and the total
 // weight of lines into heads that are never gathered.  Entry R is the zero
 // that makes the exclusive scans end with the totals.
+// Head of a line as gathered: with `rep` (no initial colours), a repeated
+// row's node [R, U) always has its stored row's colour, so lines into it
+// gather the stored row instead.
+__device__ __forceinline__ u32 gathered_head(u32 v, const u32* __restrict__ rep, u32 R, u32 U) {
+    return (rep && v >= R && v < U) ? rep[v - R] : v;
+}
+
 template <bool HAS_WT>
 __global__ void bin_count_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
                                  const u64* __restrict__ wt, u64 R, u32 H, u32 GL,
-                                 u64* __restrict__ n_hot, u64* __restrict__ n_cold,
-                                 u64* __restrict__ dsum) {
+                                 const u32* __restrict__ rep, u32 U, u64* __restrict__ n_hot,
+                                 u64* __restrict__ n_cold, u64* __restrict__ dsum) {
     const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (u > R) return;
     if (u == R) {
@@ -63,7 +70,7 @@ __global__ void bin_count_kernel(const u64* __restrict__ row_ptr, const u32* __r
     }
     u64 h = 0, c = 0, d = 0;
     for (u64 e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
-        const u32 v = dst[e];
+        const u32 v = gathered_head(dst[e], rep, (u32)R, U);
         if (v < H) ++h;
         else if (v < GL) ++c;
         else d += HAS_WT ? wt[e] : 1;
@@ -77,16 +84,16 @@ __global__ void bin_count_kernel(const u64* __restrict__ row_ptr, const u32* __r
 // (line << 16 | row in bin).
 template <bool HOT>
 __global__ void bin_fill_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                u64 R, u32 H, u32 GL, u32 log_s, int hb,
-                                const u64* __restrict__ off, u64* __restrict__ keys,
-                                u64* __restrict__ vals) {
+                                u64 R, u32 H, u32 GL, const u32* __restrict__ rep, u32 U,
+                                u32 log_s, int hb, const u64* __restrict__ off,
+                                u64* __restrict__ keys, u64* __restrict__ vals) {
     const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= R) return;
     u64 p = off[u];
     const u64 bin = u >> log_s;
     const u64 rib = u & ((1ull << log_s) - 1);
     for (u64 e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
-        const u32 v = dst[e];
+        const u32 v = gathered_head(dst[e], rep, (u32)R, U);
         const bool take = HOT ? v < H : (v >= H && v < GL);
         if (take) {
             keys[p] = (bin << hb) | v;
@@ -359,6 +366,9 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     // initial colours are given (their colour is 0 every round).
     const u32 GL = g->init ? g->N : g->U;
     H = std::min(H, GL);
+    // Without initial colours, lines into repeated rows gather their stored
+    // rows (refinement then keeps no colours for repeated rows).
+    const u32* rep = (!g->init && g->U > g->R) ? g->dup_rep : nullptr;
     const u64 n_bins = (R + (1ull << log_s) - 1) >> log_s;
     if (chunk_slots == 0) chunk_slots = 1ull << 19;
     rbsc_bins* b = new rbsc_bins();
@@ -366,6 +376,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     b->H = H;
     b->GL = GL;
     b->n_bins = n_bins;
+    b->remap = rep != nullptr;
     b->scatter_chunk = scatter_chunk;
     b->head_order = head_order;
     u64 *n_hot = nullptr, *n_cold = nullptr, *tmp = nullptr, *tot = nullptr, *hot_start = nullptr,
@@ -400,10 +411,10 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     B_TRY(hipMalloc(&tot, sizeof(u64) * 2));
     if (g->wt)
         bin_count_kernel<true><<<grid_for(R + 1), BLK, 0, s>>>(g->row_ptr, g->dst, g->wt, R, H, GL,
-                                                             n_hot, n_cold, b->dsum);
+                                                             rep, g->U, n_hot, n_cold, b->dsum);
     else
         bin_count_kernel<false><<<grid_for(R + 1), BLK, 0, s>>>(g->row_ptr, g->dst, nullptr, R, H,
-                                                              GL, n_hot, n_cold, b->dsum);
+                                                              GL, rep, g->U, n_hot, n_cold, b->dsum);
     if ((rc = scan_excl_u64(n_hot, n_hot, R + 1, tot, tmp, s))) goto done;
     if ((rc = scan_excl_u64(n_cold, n_cold, R + 1, tot + 1, tmp, s))) goto done;
     B_TRY(hipMemcpyAsync(h_tot, tot, sizeof(u64) * 2, hipMemcpyDeviceToHost, s));
@@ -486,8 +497,8 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
             B_TRY(hipMalloc(&keys[i], sizeof(u64) * b->n_hot));
             B_TRY(hipMalloc(&vals[i], sizeof(u64) * b->n_hot));
         }
-        bin_fill_kernel<true><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, g->dst, R, H, GL, log_s, hb,
-                                                          n_hot, keys[0], vals[0]);
+        bin_fill_kernel<true><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, g->dst, R, H, GL, rep, g->U,
+                                                          log_s, hb, n_hot, keys[0], vals[0]);
         int w = 0;
         const int bits = (hb + bb + 7) / 8 * 8;
         if ((rc = radix_sort_u64(&ctx->sort, keys, vals, b->n_hot, 0, bits, &w, s))) goto done;
@@ -515,8 +526,8 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
         B_TRY(hipMalloc(&grp_of_bin, sizeof(u32) * (n_bins + 1)));
         B_TRY(hipMemcpyAsync(grp_of_bin, h_grp.data(), sizeof(u32) * (n_bins + 1),
                              hipMemcpyHostToDevice, s));
-        bin_fill_kernel<false><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, g->dst, R, H, GL, log_s, gb,
-                                                           n_cold, keys[0], vals[0]);
+        bin_fill_kernel<false><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, g->dst, R, H, GL, rep, g->U,
+                                                           log_s, gb, n_cold, keys[0], vals[0]);
         int w = 0;
         int bits = (gb + bb + 7) / 8 * 8;
         if ((rc = radix_sort_u64(&ctx->sort, keys, vals, b->n_cold, 0, bits, &w, s))) goto done;

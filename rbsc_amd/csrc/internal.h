@@ -37,6 +37,7 @@ constexpr u32 CHUNK_FIRST = 1, CHUNK_ONLY = 2;
 constexpr u64 BIN_GROUPS = 8;  // scatter groups, one per XCD
 struct rbsc_bins {
     u32 log_s = 0, H = 0, GL = 0;
+    bool remap = false;             // lines into repeated rows point at their stored rows
     u64 n_bins = 0, n_slots = 0, n_hot = 0, n_cold = 0, n_chunks = 0, n_zero = 0;
     u64* bin_slot = nullptr;        // n_bins + 1: first slot of each bin
     u64* bin_hot = nullptr;         // n_bins: hot slots of each bin
@@ -130,6 +131,7 @@ struct Workspace {
     u64 tie_buf_cap = 0;
     int pc = 0;                        // P[pc] holds the last refinement's order
     u64 n_rank = 0;                    // nodes ranked by the last refinement
+    bool follow = false;               // its repeated rows took their stored rows' colours
 };
 
 // Inputs of one refinement beyond the graph itself.

@@ -216,6 +216,21 @@ __global__ void dup_sig_kernel(const u32* __restrict__ dup_rep, u64 R, u64 n_dup
     if (i < n_dup) sig[R + i] = sig[dup_rep[i]];
 }
 
+// Colours of repeated rows from their stored rows (u32 colours, or u64 hashed
+// colours when wide).
+__global__ void dup_colors_kernel(const u32* __restrict__ dup_rep, u64 R, u64 n_dup, void* col,
+                                  int wide) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_dup) return;
+    if (wide) {
+        u64* c = static_cast<u64*>(col);
+        c[R + i] = c[dup_rep[i]];
+    } else {
+        u32* c = static_cast<u32*>(col);
+        c[R + i] = c[dup_rep[i]];
+    }
+}
+
 // Round-0 state when every node with a line has colour N+1 (rbsc.cu:51-63)
 // and nodes without lines are left out of the node side: positions are
 // already sorted by colour.
@@ -777,10 +792,17 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         rbsc_set_error("signature defaults need initial colours");
         return RBSC_ERR_INVALID;
     }
-    const u64 n_rank = excluded ? g->U : N;
+    // Without initial colours, a row that repeats a stored row has the same
+    // signature as it every round and starts with the same colour, so it is
+    // always in its stored row's block: repeated rows [R, U) are left out of
+    // the node side too (their pairs equal their stored rows' pairs, so the
+    // dense ranks are unchanged) and take their stored rows' colours.  With
+    // initial colours every node takes part.
+    const u64 n_rank = excluded ? g->R : N;
     const u64 rank_offset = excluded && N > g->U ? 1 : 0;
     const u32 gather_limit = excluded ? g->U : (u32)N;
     const int c32 = (prm.flags & RBSC_FLAG_HASHED64) ? 0 : 1;
+    const bool follow = excluded && g->U > g->R;
 
     // Round-0 state: P sorted by c0, hc = randomize(c0 + 0).
     prof_begin(ctx, 4);
@@ -842,6 +864,15 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     // The binned layout serves resident rounds over raw u32 colours.
     const bool binned = g->bins && !streaming && c32 && g->n_parts == 1 &&
                         g->bins->GL == gather_limit && !(prm.flags & RBSC_FLAG_NOBINS);
+    // Repeated rows' colours are kept only where lines still gather them (the
+    // binned layout points its lines at the stored rows instead).
+    const bool copy_rep = follow && !(binned && g->bins->remap);
+    auto rep_colors = [&]() {
+        if (copy_rep)
+            dup_colors_kernel<<<grid_for(g->U - g->R), BLK, 0, s>>>(g->dup_rep, g->R,
+                                                                    g->U - g->R, w.hc, c32 ? 0 : 1);
+    };
+    rep_colors();
     do {
         K = Knew;
         const u64 dang_h = rbsc_randomize(0 + (u64)it);
@@ -897,7 +928,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, w.carry,
                                                                g->n_parts > 1 ? g->seg_row : nullptr,
                                                                w.sig, n_tiles);
-        if (g->U > g->R)
+        if (g->U > g->R && !follow)  // ranked repeats need their signatures
             dup_sig_kernel<<<grid_for(g->U - g->R), BLK, 0, s>>>(g->dup_rep, g->R, g->U - g->R,
                                                                  w.sig);
         prof_end(ctx);
@@ -985,6 +1016,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                 kk[which], pp[which], pp[which ^ 1], w.hc, n_rank, rank_offset, (u64)it + 1, c32,
                 w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
                 ctx->d_err);
+            rep_colors();
             prof_end(ctx);
             ctx->sort.ticket_total += tiles;
             return RBSC_OK;
@@ -1037,6 +1069,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     w.pc = pc;
     w.n_rank = n_rank;
+    w.follow = follow;
     out->K = Knew;
     out->rounds = it;
     out->device_ms = ms;
@@ -1049,6 +1082,9 @@ void device_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* d_int) {
     hipStream_t s = ctx->stream;
     if (w.n_rank < N) fill_u32_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, N, 0);
     if (w.n_rank) colors_kernel<<<grid_for(w.n_rank), BLK, 0, s>>>(w.P[w.pc], d_int, w.n_rank);
+    if (w.follow)
+        dup_colors_kernel<<<grid_for(g->U - g->R), BLK, 0, s>>>(g->dup_rep, g->R, g->U - g->R,
+                                                                d_int, 0);
 }
 
 int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host) {
