@@ -6,7 +6,9 @@
 // round kernels read:
 //
 //   * internal node ids: nodes with lines first, by in-degree descending
-//     (ties by original id), then the nodes without lines in original order.
+//     (ties by original id; without initial colours a stored row's node also
+//     counts the in-degree of the nodes whose rows repeat it, since its colour
+//     is gathered for them), then the nodes without lines in original order.
 //     Colours depend only on (signature, colour) pairs, never on node ids, so
 //     every round's colour of every node is unchanged by the renumbering; it
 //     makes the hashed colours of the most-gathered heads a compact prefix
@@ -80,6 +82,18 @@ __global__ void place_rows_kernel(const u64* __restrict__ keys, const u64* __res
     if (j < R) deg_new[j] = row_ptr_raw[k + 1] - row_ptr_raw[k];
     src_row[j] = (u32)k;
 }
+// Without initial colours a repeated row's node always has its stored row's
+// colour, and the signature layouts gather the stored row for it: the stored
+// row's node absorbs the repeat's in-degree, so internal ids order nodes by
+// how often their colour is gathered.
+__global__ void absorb_indeg_kernel(const u32* __restrict__ row_node,
+                                    const u32* __restrict__ rep_raw, u64 U, u32* indeg) {
+    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= U || rep_raw[k] == (u32)k) return;
+    const u32 d = indeg[row_node[k]];  // repeats are never stored rows: read only
+    if (d) atomicAdd(&indeg[row_node[rep_raw[k]]], d);
+}
+
 // *out += number of rows that repeat another (out zeroed by the caller).
 __global__ void count_dups_kernel(const u32* __restrict__ rep_raw, u64 U,
                                   unsigned long long* __restrict__ out) {
@@ -309,6 +323,8 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
         B_TRY(hipMemcpyAsync(h_tot, d_tot, sizeof(u64), hipMemcpyDeviceToHost, s));
         B_TRY(hipStreamSynchronize(s));
         R = U - h_tot[0];
+        if (R < U && !raw->init)
+            absorb_indeg_kernel<<<grid_for(U), BLK, 0, s>>>(raw->row_node, rep_raw, U, indeg);
     }
     if (U) {
         order_keys_kernel<<<grid_for(U), BLK, 0, s>>>(raw->row_node, U, indeg, rep_raw, keys[0],
