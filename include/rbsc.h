@@ -165,9 +165,9 @@ int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
 /* Binned signature layout of a resident graph, an addition to its plain
  * layout (about 6 bytes per gathered line plus 8 per line into a cold head;
  * 8 more per line with weights).  Stored rows are cut into bins of
- * rows_per_bin (a power of two from 256 to 16384; 0 = 4096) whose
+ * rows_per_bin (a power of two from 256 to 16384; 0 = 8192) whose
  * signatures are summed in LDS; lines into the hot_heads most-gathered nodes
- * (0 = 2^20) gather their colours in head order inside the bin; the colours
+ * (0 = 2^22) gather their colours in head order inside the bin; the colours
  * of the other gathered heads are scattered to their lines' slots once per
  * round, one workgroup per scatter_lines of them (0 = 2048) in head order,
  * kept in head order (colours read in order) or, with RBSC_BIN_SLOT_ORDER,

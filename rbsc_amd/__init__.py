@@ -230,7 +230,7 @@ class DeviceGraph:
         """Build the binned signature layout (row sums in LDS per bin of
         rows, hot heads gathered in head order, the other colours scattered
         to their lines' slots in chunks of lines); later refinements use it.
-        0 picks the defaults (2^20 hot heads, 4096 rows per bin, 2^19 slots
+        0 picks the defaults (2^22 hot heads, 8192 rows per bin, 2^19 slots
         per chunk, 2048 lines per scatter workgroup, kept in head order).
         Results are unchanged."""
         o = _lib.BinOpts(int(hot_heads), int(rows_per_bin), int(chunk_slots), int(scatter_lines),

@@ -1298,8 +1298,8 @@ extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts
         return RBSC_ERR_INVALID;
     }
     rbsc_bin_opts o = opts ? *opts : rbsc_bin_opts{0, 0, 0, 0, 0};
-    if (o.hot_heads == 0) o.hot_heads = 1u << 20;
-    if (o.rows_per_bin == 0) o.rows_per_bin = 1u << 12;
+    if (o.hot_heads == 0) o.hot_heads = 1u << 22;
+    if (o.rows_per_bin == 0) o.rows_per_bin = 1u << 13;
     if (o.chunk_slots == 0) o.chunk_slots = 1ull << 19;
     if (o.scatter_lines == 0) o.scatter_lines = 1u << 11;
     if (o.rows_per_bin & (o.rows_per_bin - 1)) {
