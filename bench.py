@@ -290,6 +290,8 @@ def main():
     ap.add_argument("--bin-chunk", type=int, default=0, help="slots per chunk (0 = default)")
     ap.add_argument("--bin-scatter", type=int, default=0,
                     help="cold lines per scatter workgroup (0 = default)")
+    ap.add_argument("--bin-hashed", action="store_true",
+                    help="hashed binned layout: per-node u64 hashed colours, no hashing per line")
     ap.add_argument("--bin-slot-order", action="store_true",
                     help="scatter records in slot order inside each workgroup's share")
     ap.add_argument("--batch-graphs", type=int, default=50000,
@@ -328,7 +330,8 @@ def main():
     bin_info = None
     if args.bins and args.parts == 1 and not args.hot:
         t0 = time.perf_counter()
-        dg.bin(args.bin_hot, args.bin_rows, args.bin_chunk, args.bin_scatter, args.bin_slot_order)
+        dg.bin(args.bin_hot, args.bin_rows, args.bin_chunk, args.bin_scatter, args.bin_slot_order,
+               args.bin_hashed)
         bin_info = dict(dg.bin_info, build_s=time.perf_counter() - t0)
         log(f"[bench] binned layout in {bin_info['build_s']:.1f}s: {bin_info}")
     flags = (rbsc_amd.FLAG_FULLSORT if args.fullsort else 0) | (rbsc_amd.FLAG_HASHED64 if args.hashed64 else 0) | (rbsc_amd.FLAG_XCD if args.xcd else 0)
@@ -369,7 +372,7 @@ def main():
     ROUND = ("scatter", "signature", "finish", "gather", "sort", "rank")
     round_ms = sum(per_round[k] for k in ROUND)
     dom = max(ROUND, key=lambda k: per_round[k])
-    binned = bin_info is not None and not (flags & rbsc_amd.FLAG_HASHED64)
+    binned = bin_info is not None and (bin_info["hashed"] or not (flags & rbsc_amd.FLAG_HASHED64))
     sig_ms = per_round["signature"] + (per_round["scatter"] if binned else 0.0)
     pass_name = "signature_pass" if binned else "sig_kernel"
     kernels = ({"cold_scatter_kernel": per_round["scatter"], "bin_sig_kernel": per_round["signature"]}

@@ -93,6 +93,13 @@ typedef struct rbsc_opts {
 /* opts.flags: on a binned graph (rbsc_graph_bin) compute signatures with the
  * plain line kernel instead (results identical). */
 #define RBSC_FLAG_NOBINS 64u
+/* opts.flags: without initial colours the node side sorts the nodes by
+ * signature straight from node-id order (no gather of signatures into
+ * colour order; equal signatures have equal colours from a uniform start,
+ * and a round with a pair out of colour order is redone exactly).  This flag
+ * keeps positions in colour order instead, as with initial colours (results
+ * identical). */
+#define RBSC_FLAG_COLOUR_ORDER 128u
 
 /* Result.  nodes / done / time_s are the reference's three output lines
  * (rbsc.cu:492-494). */
@@ -174,9 +181,9 @@ int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
  * reordered by slot inside each workgroup's share (stores coalesced); bins with
  * more than chunk_slots lines (0 = 2^19) are split over several
  * workgroups.  opts may be null (all defaults).  Later refinements of g use
- * the layout unless RBSC_FLAG_NOBINS, RBSC_FLAG_HASHED64 or streamed lines
- * are asked for; relayout, split_hot and lines_to_host drop it.  Signatures
- * and results are identical.  No counterpart in the reference (its
+ * the layout unless RBSC_FLAG_NOBINS or streamed lines are asked for (or
+ * RBSC_FLAG_HASHED64 on the raw form); relayout, split_hot and
+ * lines_to_host drop it.  Signatures and results are identical.  No counterpart in the reference (its
  * set_values + ReduceByKey, rbsc.cu:357-377, read the lines in input
  * order). */
 typedef struct {
@@ -187,10 +194,19 @@ typedef struct {
     uint32_t flags;           /* RBSC_BIN_* */
 } rbsc_bin_opts;
 #define RBSC_BIN_SLOT_ORDER 1u
+/* Hashed form of the layout: refinements keep u64 hashed colours per node,
+ * randomize(c + r) computed once per node per round by the rank pass; hot
+ * lines gather them and cold slots receive them, so no line hashes.  The
+ * default (raw) form gathers u32 raw colours and hashes each line (four
+ * 64-bit multiplies per line): it moves half the gathered bytes and is the
+ * faster of the two on large graphs, where the gathers bound the pass.
+ * Results are identical. */
+#define RBSC_BIN_HASHED 2u
 int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts* opts);
 int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g);
 /* info[8]: binned (0/1), bins, hot lines, cold lines, workgroup chunks, bins
- * split over several chunks, bytes, hot_heads | log2(rows_per_bin) << 32. */
+ * split over several chunks, bytes, hot_heads | log2(rows_per_bin) << 32 |
+ * hashed form << 40. */
 int rbsc_graph_bin_info(const rbsc_dgraph* g, uint64_t* info);
 
 /* Per-phase device timing with hipEvents on the context's stream (off by

@@ -23,13 +23,13 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
+from ._lib import (FLAG_COLOUR_ORDER, FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
                    RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
            "Server", "RequestResult",
            "RefineResult", "RbscError", "load",
-           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "gen"]
+           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "FLAG_COLOUR_ORDER", "gen"]
 
 _MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
           "aggressive": MODE_QUOTIENT_AGGRESSIVE}
@@ -226,15 +226,16 @@ class DeviceGraph:
         check(load().rbsc_graph_split_hot(self.ctx.handle, self.handle, int(hot_heads)))
 
     def bin(self, hot_heads: int = 0, rows_per_bin: int = 0, chunk_slots: int = 0,
-            scatter_lines: int = 0, slot_order: bool = False):
+            scatter_lines: int = 0, slot_order: bool = False, hashed: bool = False):
         """Build the binned signature layout (row sums in LDS per bin of
         rows, hot heads gathered in head order, the other colours scattered
         to their lines' slots in chunks of lines); later refinements use it.
         0 picks the defaults (2^22 hot heads, 8192 rows per bin, 2^19 slots
         per chunk, 2048 lines per scatter workgroup, kept in head order).
-        Results are unchanged."""
+        hashed: keep per-node hashed u64 colours (no hashing per line)
+        instead of gathering raw u32 colours.  Results are unchanged."""
         o = _lib.BinOpts(int(hot_heads), int(rows_per_bin), int(chunk_slots), int(scatter_lines),
-                         _lib.BIN_SLOT_ORDER if slot_order else 0)
+                         (_lib.BIN_SLOT_ORDER if slot_order else 0) | (_lib.BIN_HASHED if hashed else 0))
         check(load().rbsc_graph_bin(self.ctx.handle, self.handle, C.byref(o)))
 
     def unbin(self):
@@ -248,7 +249,8 @@ class DeviceGraph:
         return {"binned": bool(v[0]), "bins": int(v[1]), "hot_lines": int(v[2]),
                 "cold_lines": int(v[3]), "chunks": int(v[4]), "split_bins": int(v[5]),
                 "bytes": int(v[6]), "hot_heads": int(v[7] & 0xFFFFFFFF),
-                "rows_per_bin": 1 << int(v[7] >> 32) if v[0] else 0}
+                "rows_per_bin": 1 << int((v[7] >> 32) & 0xFF) if v[0] else 0,
+                "hashed": bool((v[7] >> 40) & 1)}
 
     def lines_to_host(self):
         """Keep the line arrays in pinned host memory; refinements then stream

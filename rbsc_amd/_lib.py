@@ -32,7 +32,9 @@ FLAG_SPLIT16 = 8
 FLAG_TIESMALL = 16
 FLAG_FULLSORT = 32
 FLAG_NOBINS = 64
+FLAG_COLOUR_ORDER = 128
 BIN_SLOT_ORDER = 1
+BIN_HASHED = 2
 
 
 class BinOpts(C.Structure):

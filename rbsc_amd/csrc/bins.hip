@@ -44,6 +44,9 @@ inline int bitlen(u64 v) {
     return b;
 }
 constexpr int SIG_THREADS = 512;
+// Zeroed entries past the end of the slot arrays and scatter records (the
+// last aligned group's loads stay in bounds).
+constexpr u64 SLOT_RUN = 512;
 
 // ---------------------------------------------------------------- build
 // Per stored row: lines into hot heads, into cold heads, and the total
@@ -141,12 +144,15 @@ __global__ void place_hot_kernel(const u64* __restrict__ keys, const u64* __rest
 
 // Sorted cold records into their slots (after the bin's hot slots), and the
 // scatter records: key (group, head), value slot << 32 | head.
+// With cold_pos (hashed layout) a record's target is its cold_val index
+// instead of its slot.
 template <bool HAS_WT>
 __global__ void place_cold_kernel(const u64* __restrict__ keys, const u64* __restrict__ vals,
                                   u64 n, int gb, const u64* __restrict__ bin_slot,
                                   const u64* __restrict__ bin_hot,
                                   const u64* __restrict__ cold_start,
-                                  const u32* __restrict__ grp_of_bin, const u64* __restrict__ wt,
+                                  const u32* __restrict__ grp_of_bin,
+                                  const u64* __restrict__ cold_pos, const u64* __restrict__ wt,
                                   u32* __restrict__ slot_key,
                                   unsigned short* __restrict__ slot_row,
                                   u64* __restrict__ slot_wt, u64* __restrict__ k2,
@@ -156,12 +162,14 @@ __global__ void place_cold_kernel(const u64* __restrict__ keys, const u64* __res
     const u64 k = keys[i], v = vals[i];
     const u64 b = k >> gb;
     const u64 head = k & ((1ull << gb) - 1);
-    const u64 s = bin_slot[b] + bin_hot[b] + (i - cold_start[b]);
+    const u64 c0 = bin_slot[b] + bin_hot[b];
+    const u64 s = c0 + (i - cold_start[b]);
     slot_key[s] = 0;
     slot_row[s] = (unsigned short)(v & 0xFFFF);
     if (HAS_WT) slot_wt[s] = wt[v >> 16];
+    const u64 target = cold_pos ? cold_pos[b] + (s - (c0 & ~7ull)) : s;
     k2[i] = ((u64)grp_of_bin[b] << gb) | head;
-    v2[i] = (s << 32) | head;
+    v2[i] = (target << 32) | head;
 }
 
 // Records in (group, head) order -> key (tile, slot) or (tile, position), where
@@ -195,18 +203,24 @@ __global__ void csc_fill_kernel(const u64* __restrict__ k3, const u64* __restric
 // ---------------------------------------------------------------- rounds
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+typedef u64 u64x2 __attribute__((ext_vector_type(2)));
 
 // Writes the current colour of every cold line's head into the line's slot,
 // one workgroup per tile (records [tile_line[t], tile_line[t + 1])).  Records
 // are read 8 at a time with 16-byte loads: thread i takes the aligned groups
 // [8q, 8q + 8) for q = lo / 8 + i, + BLK, ...; records outside the tile (a
 // neighbour's, or the zeroed padding past the end) are loaded, and their
-// heads (valid ids or 0) read, but never stored.
+// heads (valid ids or 0) read, but never stored.  On the raw layout the
+// colour is the u32 raw colour and the target a slot of slot_key; on the
+// hashed layout it is the u64 hashed colour and the target an index of
+// cold_val.  (Lanes taking 64 consecutive records per load instruction
+// instead measured slower: the stores scatter more.)
+template <typename C>
 __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict__ csc_head,
                                                            const u32* __restrict__ csc_slot,
                                                            const u64* __restrict__ tile_line,
-                                                           const u32* __restrict__ col,
-                                                           u32* __restrict__ slot_key) {
+                                                           const C* __restrict__ col,
+                                                           C* __restrict__ out) {
     const u64 t = blockIdx.x;
     const u64 lo = tile_line[t], hi = tile_line[t + 1];
     const u64 q1 = (hi + 7) / 8;
@@ -217,13 +231,13 @@ __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict
         const u32x4 sb = *reinterpret_cast<const u32x4*>(csc_slot + 8 * q + 4);
         const u32 hd[8] = {ha.x, ha.y, ha.z, ha.w, hb.x, hb.y, hb.z, hb.w};
         const u32 sl[8] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
-        u32 c[8];
+        C c[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) c[k] = col[hd[k]];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const u64 j = 8 * q + k;
-            if (j >= lo && j < hi) slot_key[sl[k]] = c[k];
+            if (j >= lo && j < hi) out[sl[k]] = c[k];
         }
     }
 }
@@ -240,47 +254,84 @@ __global__ void bin_zero_kernel(const u32* __restrict__ zero_bins, u64 n_zero, u
 
 // Adds the hashed colours of slots [a, e) into the LDS row sums: slots below
 // cold0 (hot) hold heads whose colours are gathered, the others hold
-// colours.  Slots are read 8 at a time as in scatter_lines (aligned groups,
-// slots outside [a, e) loaded but not added).
-template <bool HAS_WT>
-__device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0,
+// colours.  Slots are read 8 at a time with 16-byte loads (aligned groups,
+// slots outside [a, e) loaded but not added).  (Lanes taking 64 consecutive
+// slots per load instruction, so one gather instruction reads 64
+// neighbouring heads, measured no faster.)
+//
+// Raw layout (HASHED false): col is u32 raw colours, cold slots hold raw
+// colours in slot_key, and every line hashes randomize(c + salt).
+// Hashed layout: col is the round's u64 hashed colours, cold slots hold
+// hashed colours in cold_val (index s + cbase, aligned groups of 8 stay
+// aligned), and no line hashes.
+template <bool HAS_WT, bool HASHED>
+__device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0, u64 cbase,
                                                const u32* __restrict__ slot_key,
                                                const unsigned short* __restrict__ slot_row,
                                                const u64* __restrict__ slot_wt,
-                                               const u32* __restrict__ col, u64 salt, u64* sums) {
+                                               const void* __restrict__ colv,
+                                               const u64* __restrict__ cold_val, u64 salt,
+                                               u64* sums) {
     const u64 q1 = (e + 7) / 8;
     for (u64 q = a / 8 + threadIdx.x; q < q1; q += SIG_THREADS) {
-        const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + 8 * q);
-        const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + 8 * q + 4);
         const u16x8 r8 = *reinterpret_cast<const u16x8*>(slot_row + 8 * q);
-        u32 key[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
         const u32 row[8] = {r8.s0, r8.s1, r8.s2, r8.s3, r8.s4, r8.s5, r8.s6, r8.s7};
-        if (8 * q < cold0) {  // the group holds hot slots
-            u32 g[8];
+        u64 h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const u64 g0 = 8 * q;
+        if (HASHED) {
+            const u64* col = static_cast<const u64*>(colv);
+            if (g0 < cold0) {  // the group holds hot slots
+                const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + g0);
+                const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + g0 + 4);
+                const u32 key[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
 #pragma unroll
-            for (int k = 0; k < 8; ++k) g[k] = col[8 * q + k < cold0 ? key[k] : 0u];
+                for (int k = 0; k < 8; ++k) h[k] = col[g0 + k < cold0 ? key[k] : 0u];
+            }
+            // Cold slots of this chunk in the group: [max(g0, cold0), min(g0 + 8, e)).
+            const u64 c_lo = g0 > cold0 ? g0 : cold0;
+            if (c_lo < e && c_lo < g0 + 8) {
+                const u64x2* cv = reinterpret_cast<const u64x2*>(cold_val + (g0 + cbase));
+                const u64x2 v0 = cv[0], v1 = cv[1], v2 = cv[2], v3 = cv[3];
+                const u64 c8[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (8 * q + k < cold0) key[k] = g[k];
+                for (int k = 0; k < 8; ++k)
+                    if (g0 + k >= cold0) h[k] = c8[k];
+            }
+        } else {
+            const u32* col = static_cast<const u32*>(colv);
+            const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + g0);
+            const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + g0 + 4);
+            u32 key[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+            if (g0 < cold0) {  // the group holds hot slots
+                u32 g[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) g[k] = col[g0 + k < cold0 ? key[k] : 0u];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (g0 + k < cold0) key[k] = g[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) h[k] = (u64)key[k];
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const u64 s = 8 * q + k;
+            const u64 s = g0 + k;
             if (s >= a && s < e) {
-                u64 h = rbsc_randomize((u64)key[k] + salt);
-                if (HAS_WT) h *= slot_wt[s];
-                atomicAdd((unsigned long long*)&sums[row[k]], (unsigned long long)h);
+                u64 v = HASHED ? h[k] : rbsc_randomize(h[k] + salt);
+                if (HAS_WT) v *= slot_wt[s];
+                atomicAdd((unsigned long long*)&sums[row[k]], (unsigned long long)v);
             }
         }
     }
 }
 
-template <int LOG_S, bool HAS_WT>
+template <int LOG_S, bool HAS_WT, bool HASHED>
 __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
     const BinChunk* __restrict__ chunks, const u64* __restrict__ bin_slot,
-    const u64* __restrict__ bin_hot, const u32* __restrict__ slot_key,
-    const unsigned short* __restrict__ slot_row, const u64* __restrict__ slot_wt,
-    const u64* __restrict__ dsum, const u32* __restrict__ col, u64 salt, u64 dang_h, u64 R,
+    const u64* __restrict__ bin_hot, const u64* __restrict__ cold_pos,
+    const u32* __restrict__ slot_key, const unsigned short* __restrict__ slot_row,
+    const u64* __restrict__ slot_wt, const u64* __restrict__ cold_val,
+    const u64* __restrict__ dsum, const void* __restrict__ col, u64 salt, u64 dang_h, u64 R,
     u64* __restrict__ sig) {
     constexpr u32 S = 1u << LOG_S;
     __shared__ u64 sums[S];
@@ -292,7 +343,11 @@ __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
         sums[i] = (first && i < nrows) ? dang_h * dsum[row0 + i] : 0ull;
     __syncthreads();
     const u64 cold0 = bin_slot[ch.bin] + bin_hot[ch.bin];
-    bin_accumulate<HAS_WT>(ch.lo, ch.hi, cold0, slot_key, slot_row, slot_wt, col, salt, sums);
+    // cold_val index of slot s: cold_pos + s - (cold0 rounded down to 8), in
+    // Z/2^64 (only ever applied to the bin's cold groups).
+    const u64 cbase = HASHED ? cold_pos[ch.bin] - (cold0 & ~7ull) : 0;
+    bin_accumulate<HAS_WT, HASHED>(ch.lo, ch.hi, cold0, cbase, slot_key, slot_row, slot_wt, col,
+                                   cold_val, salt, sums);
     __syncthreads();
     const bool only = (ch.flags & CHUNK_ONLY) != 0;
     for (u32 i = threadIdx.x; i < nrows; i += SIG_THREADS) {
@@ -301,28 +356,38 @@ __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
     }
 }
 
-template <int LOG_S>
-void launch_bin_sig(hipStream_t s, const rbsc_bins* b, const u32* col, u64 salt, u64 dang_h,
-                    u64 R, u64* sig) {
+template <int LOG_S, bool HASHED>
+void launch_bin_sig2(hipStream_t s, const rbsc_bins* b, const void* col, u64 salt, u64 dang_h,
+                     u64 R, u64* sig) {
     if (b->slot_wt)
-        bin_sig_kernel<LOG_S, true><<<(unsigned)b->n_chunks, SIG_THREADS, 0, s>>>(
-            b->chunks, b->bin_slot, b->bin_hot, b->slot_key, b->slot_row, b->slot_wt, b->dsum, col,
-            salt, dang_h, R, sig);
+        bin_sig_kernel<LOG_S, true, HASHED><<<(unsigned)b->n_chunks, SIG_THREADS, 0, s>>>(
+            b->chunks, b->bin_slot, b->bin_hot, b->cold_pos, b->slot_key, b->slot_row, b->slot_wt,
+            b->cold_val, b->dsum, col, salt, dang_h, R, sig);
     else
-        bin_sig_kernel<LOG_S, false><<<(unsigned)b->n_chunks, SIG_THREADS, 0, s>>>(
-            b->chunks, b->bin_slot, b->bin_hot, b->slot_key, b->slot_row, nullptr, b->dsum, col,
-            salt, dang_h, R, sig);
+        bin_sig_kernel<LOG_S, false, HASHED><<<(unsigned)b->n_chunks, SIG_THREADS, 0, s>>>(
+            b->chunks, b->bin_slot, b->bin_hot, b->cold_pos, b->slot_key, b->slot_row, nullptr,
+            b->cold_val, b->dsum, col, salt, dang_h, R, sig);
+}
+template <int LOG_S>
+void launch_bin_sig(hipStream_t s, const rbsc_bins* b, const void* col, u64 salt, u64 dang_h,
+                    u64 R, u64* sig) {
+    if (b->hashed) launch_bin_sig2<LOG_S, true>(s, b, col, salt, dang_h, R, sig);
+    else launch_bin_sig2<LOG_S, false>(s, b, col, salt, dang_h, R, sig);
 }
 }  // namespace
 
-void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col) {
+void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col) {
     const rbsc_bins* b = g->bins;
     if (!b->n_cold) return;
-    cold_scatter_kernel<<<(unsigned)b->n_tiles, BLK, 0, ctx->stream>>>(
-        b->csc_head, b->csc_slot, b->tile_line, col, b->slot_key);
+    if (b->hashed)
+        cold_scatter_kernel<u64><<<(unsigned)b->n_tiles, BLK, 0, ctx->stream>>>(
+            b->csc_head, b->csc_slot, b->tile_line, static_cast<const u64*>(col), b->cold_val);
+    else
+        cold_scatter_kernel<u32><<<(unsigned)b->n_tiles, BLK, 0, ctx->stream>>>(
+            b->csc_head, b->csc_slot, b->tile_line, static_cast<const u32*>(col), b->slot_key);
 }
 
-void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col, u64 salt, u64 dang_h,
+void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, u64 salt, u64 dang_h,
               u64* sig) {
     const rbsc_bins* b = g->bins;
     hipStream_t s = ctx->stream;
@@ -344,13 +409,14 @@ void bins_release(rbsc_bins* b) {
     if (!b) return;
     for (void* p : {(void*)b->bin_slot, (void*)b->bin_hot, (void*)b->slot_key, (void*)b->slot_row,
                     (void*)b->slot_wt, (void*)b->dsum, (void*)b->csc_head, (void*)b->csc_slot,
-                    (void*)b->tile_line, (void*)b->chunks, (void*)b->zero_bins})
+                    (void*)b->tile_line, (void*)b->chunks, (void*)b->zero_bins,
+                    (void*)b->cold_pos, (void*)b->cold_val})
         if (p) (void)hipFree(p);
     delete b;
 }
 
 int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
-               u64 scatter_chunk, bool head_order) {
+               u64 scatter_chunk, bool head_order, bool hashed) {
     if (!g->dst || g->n_parts != 1) {
         rbsc_set_error("bin: needs the plain layout with lines resident on the device");
         return RBSC_ERR_INVALID;
@@ -379,6 +445,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     b->remap = rep != nullptr;
     b->scatter_chunk = scatter_chunk;
     b->head_order = head_order;
+    b->hashed = hashed;
     u64 *n_hot = nullptr, *n_cold = nullptr, *tmp = nullptr, *tot = nullptr, *hot_start = nullptr,
         *cold_start = nullptr, *bin_n = nullptr, *keys[2] = {nullptr, nullptr},
         *vals[2] = {nullptr, nullptr};
@@ -443,6 +510,27 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     B_TRY(hipStreamSynchronize(s));
     h_bin_slot.assign(n_bins + 1, 0);
     for (u64 k = 0; k < n_bins; ++k) h_bin_slot[k + 1] = h_bin_slot[k] + h_bin_n[k];
+    if (hashed) {
+        // Each bin's cold slots [c0, c1) as whole aligned groups of 8 in cold_val.
+        std::vector<u64> h_pos(n_bins, 0);
+        u64 pos = 0;
+        for (u64 k = 0; k < n_bins; ++k) {
+            h_pos[k] = pos;
+            const u64 c0 = h_bin_slot[k] + h_bin_hot[k], c1 = h_bin_slot[k + 1];
+            if (c1 > c0) pos += ((c1 + 7) & ~7ull) - (c0 & ~7ull);
+        }
+        b->n_cold_pad = pos;
+        if (pos + SLOT_RUN >= (1ull << 32)) {
+            rbsc_set_error("bin: %llu cold slots exceed 2^32", (unsigned long long)pos);
+            rc = RBSC_ERR_INVALID;
+            goto done;
+        }
+        B_TRY(hipMalloc(&b->cold_pos, sizeof(u64) * std::max<u64>(n_bins, 1)));
+        B_TRY(hipMemcpyAsync(b->cold_pos, h_pos.data(), sizeof(u64) * n_bins,
+                             hipMemcpyHostToDevice, s));
+        B_TRY(hipMalloc(&b->cold_val, sizeof(u64) * (pos + SLOT_RUN)));
+        B_TRY(hipMemsetAsync(b->cold_val, 0, sizeof(u64) * (pos + SLOT_RUN), s));
+    }
     // Scatter groups: contiguous bin ranges with about equal cold lines.
     {
         u64 seen = 0;
@@ -484,12 +572,15 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
                              hipMemcpyHostToDevice, s));
     }
     // slot arrays
-    // + 16 zeroed entries: the 8-wide loads of the last group stay in bounds
-    B_TRY(hipMalloc(&b->slot_key, sizeof(u32) * (b->n_slots + 16)));
-    B_TRY(hipMalloc(&b->slot_row, sizeof(unsigned short) * (b->n_slots + 16)));
-    B_TRY(hipMemsetAsync(b->slot_key, 0, sizeof(u32) * (b->n_slots + 16), s));
-    B_TRY(hipMemsetAsync(b->slot_row, 0, sizeof(unsigned short) * (b->n_slots + 16), s));
-    if (g->wt) B_TRY(hipMalloc(&b->slot_wt, sizeof(u64) * std::max<u64>(b->n_slots, 1)));
+    // + SLOT_RUN zeroed entries: the loads of the last run stay in bounds
+    B_TRY(hipMalloc(&b->slot_key, sizeof(u32) * (b->n_slots + SLOT_RUN)));
+    B_TRY(hipMalloc(&b->slot_row, sizeof(unsigned short) * (b->n_slots + SLOT_RUN)));
+    B_TRY(hipMemsetAsync(b->slot_key, 0, sizeof(u32) * (b->n_slots + SLOT_RUN), s));
+    B_TRY(hipMemsetAsync(b->slot_row, 0, sizeof(unsigned short) * (b->n_slots + SLOT_RUN), s));
+    if (g->wt) {
+        B_TRY(hipMalloc(&b->slot_wt, sizeof(u64) * (b->n_slots + SLOT_RUN)));
+        B_TRY(hipMemsetAsync(b->slot_wt + b->n_slots, 0, sizeof(u64) * SLOT_RUN, s));
+    }
     if ((rc = sort_ws_init(&ctx->sort, std::max(b->n_hot, b->n_cold), ctx->d_err, s))) goto done;
     // hot slots: records sorted by (bin, head)
     if (b->n_hot) {
@@ -534,11 +625,13 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
         if (g->wt)
             place_cold_kernel<true><<<grid_for(b->n_cold), BLK, 0, s>>>(
                 keys[w], vals[w], b->n_cold, gb, b->bin_slot, b->bin_hot, cold_start, grp_of_bin,
-                g->wt, b->slot_key, b->slot_row, b->slot_wt, keys[w ^ 1], vals[w ^ 1]);
+                b->cold_pos, g->wt, b->slot_key, b->slot_row, b->slot_wt, keys[w ^ 1],
+                vals[w ^ 1]);
         else
             place_cold_kernel<false><<<grid_for(b->n_cold), BLK, 0, s>>>(
                 keys[w], vals[w], b->n_cold, gb, b->bin_slot, b->bin_hot, cold_start, grp_of_bin,
-                nullptr, b->slot_key, b->slot_row, nullptr, keys[w ^ 1], vals[w ^ 1]);
+                b->cold_pos, nullptr, b->slot_key, b->slot_row, nullptr, keys[w ^ 1],
+                vals[w ^ 1]);
         u64* k2[2] = {keys[w ^ 1], keys[w]};
         u64* v2[2] = {vals[w ^ 1], vals[w]};
         int w2 = 0;
@@ -564,10 +657,10 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
         int w3 = 0;
         bits = (32 + bitlen(b->n_tiles) + 7) / 8 * 8;
         if ((rc = radix_sort_u64(&ctx->sort, k3, v3, b->n_cold, 0, bits, &w3, s))) goto done;
-        B_TRY(hipMalloc(&b->csc_head, sizeof(u32) * (b->n_cold + 16)));
-        B_TRY(hipMalloc(&b->csc_slot, sizeof(u32) * (b->n_cold + 16)));
-        B_TRY(hipMemsetAsync(b->csc_head, 0, sizeof(u32) * (b->n_cold + 16), s));
-        B_TRY(hipMemsetAsync(b->csc_slot, 0, sizeof(u32) * (b->n_cold + 16), s));
+        B_TRY(hipMalloc(&b->csc_head, sizeof(u32) * (b->n_cold + SLOT_RUN)));
+        B_TRY(hipMalloc(&b->csc_slot, sizeof(u32) * (b->n_cold + SLOT_RUN)));
+        B_TRY(hipMemsetAsync(b->csc_head, 0, sizeof(u32) * (b->n_cold + SLOT_RUN), s));
+        B_TRY(hipMemsetAsync(b->csc_slot, 0, sizeof(u32) * (b->n_cold + SLOT_RUN), s));
         B_TRY(hipMalloc(&b->tile_line, sizeof(u64) * (b->n_tiles + 1)));
         csc_fill_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(k3[w3], v3[w3], b->n_cold, b->n_tiles,
                                                            b->csc_head, b->csc_slot, b->tile_line);
@@ -576,7 +669,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     B_TRY(hipStreamSynchronize(s));
     b->bytes = 16 * (n_bins + 1) + 6 * b->n_slots + (g->wt ? 8 * b->n_slots : 0) + 8 * R +
                8 * b->n_cold + 8 * (b->n_tiles + 1) + sizeof(BinChunk) * b->n_chunks +
-               4 * b->n_zero;
+               4 * b->n_zero + (hashed ? 8 * (n_bins + b->n_cold_pad + SLOT_RUN) : 0);
     if (g->bins) {
         g->bytes -= g->bins->bytes;
         bins_release(g->bins);

@@ -38,10 +38,24 @@ constexpr u64 BIN_GROUPS = 8;  // scatter groups, one per XCD
 struct rbsc_bins {
     u32 log_s = 0, H = 0, GL = 0;
     bool remap = false;             // lines into repeated rows point at their stored rows
+    // Hashed layout: refinements on it keep per-node hashed colours
+    // randomize(c + r) (u64, written once per node by the rank pass), hot
+    // slots gather those and cold slots receive them in cold_val, so no line
+    // is hashed.  Otherwise (the default) raw u32 colours are gathered and
+    // every line hashes its colour.
+    bool hashed = false;
     u64 n_bins = 0, n_slots = 0, n_hot = 0, n_cold = 0, n_chunks = 0, n_zero = 0;
     u64* bin_slot = nullptr;        // n_bins + 1: first slot of each bin
     u64* bin_hot = nullptr;         // n_bins: hot slots of each bin
-    u32* slot_key = nullptr;        // n_slots: hot = head; cold = colour of the round
+    // Hashed layout: bin b's cold slots [c0, c1) (c0 = bin_slot + bin_hot)
+    // live in cold_val at cold_pos[b] + (s - c0 rounded down to 8), each bin's
+    // region padded to whole aligned groups of 8 (64-byte aligned runs).
+    u64* cold_pos = nullptr;        // n_bins
+    u64* cold_val = nullptr;        // n_cold_pad + 512: hashed colour of each cold slot
+    u64 n_cold_pad = 0;
+    // Slot arrays and scatter records carry 512 zeroed entries past the end.
+    u32* slot_key = nullptr;        // n_slots: hot = head; cold = colour of the round (raw
+                                    // layout) or 0 (hashed)
     unsigned short* slot_row = nullptr;  // n_slots: row - first row of the bin
     u64* slot_wt = nullptr;         // n_slots or null
     u64* dsum = nullptr;            // R: weight of the row's lines into heads >= GL
@@ -51,8 +65,9 @@ struct rbsc_bins {
     u64 scatter_chunk = 0;
     bool head_order = false;
     u64 n_tiles = 0;
-    u32* csc_head = nullptr;        // n_cold + 16 (zero padding)
-    u32* csc_slot = nullptr;        // n_cold + 16: slot receiving that line's colour
+    u32* csc_head = nullptr;        // n_cold
+    u32* csc_slot = nullptr;        // n_cold: slot receiving that line's colour (raw
+                                    // layout) or its cold_val index (hashed)
     u64* tile_line = nullptr;       // n_tiles + 1: first record of each tile
     BinChunk* chunks = nullptr;     // n_chunks
     u32* zero_bins = nullptr;       // n_zero: bins with several chunks
@@ -62,11 +77,13 @@ void bins_release(rbsc_bins* b);
 // Builds g->bins (hot heads below H, 2^log_s rows per bin, chunks of at most
 // chunk_slots slots, scatter tiles of scatter_chunk lines).
 int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
-               u64 scatter_chunk, bool head_order);
+               u64 scatter_chunk, bool head_order, bool hashed);
 // One round on the binned layout: cold colours into their slots, then the
-// signatures of the stored rows into sig (col: colours by internal id).
-void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col);
-void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const u32* col, u64 salt, u64 dang_h,
+// signatures of the stored rows into sig.  col: per-node colours by internal
+// id, u32 raw colours on the raw layout, u64 hashed colours of the round
+// (randomize(c + salt)) on the hashed layout.
+void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col);
+void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, u64 salt, u64 dang_h,
               u64* sig);
 
 // A graph resident in HBM in the round's layout (graph.hip).  Refinement
@@ -116,7 +133,9 @@ struct Workspace {
     u64* keys[2] = {nullptr, nullptr};
     u64* P[2] = {nullptr, nullptr};   // (colour << 32) | node, sorted by colour
     u64* sig = nullptr;
-    u64* hc = nullptr;                 // randomize(colour + salt) per node
+    u64* hc = nullptr;                 // colours of the round per node: raw u32, or
+                                       // randomize(colour + salt) (u64)
+    u64* hc2 = nullptr;                // the next round's, written by the rank pass
     u64* rank_status = nullptr;
     u64 rank_tiles = 0;
     u64* carry = nullptr;              // per tile: partial signature of its carry row
@@ -132,6 +151,7 @@ struct Workspace {
     int pc = 0;                        // P[pc] holds the last refinement's order
     u64 n_rank = 0;                    // nodes ranked by the last refinement
     bool follow = false;               // its repeated rows took their stored rows' colours
+    bool node_order = false;           // its final colours are in hc (raw, by node), not P
 };
 
 // Inputs of one refinement beyond the graph itself.
@@ -160,7 +180,7 @@ struct rbsc_ctx {
     Workspace ws;
     u32* d_err = nullptr;
     u32* d_scalar = nullptr;   // [0] = K of the last rank pass, [1] = tie fix-up overflow,
-                               // [2] = tie runs listed
+                               // [2] = tie runs listed, [3] = rank input out of colour order
     u64 full_sort_redos = 0;   // rounds whose tie fix-up overflowed
     u32* h_scalar = nullptr;   // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -262,10 +262,38 @@ __global__ void permute_fill_kernel(const u64* __restrict__ fill, const u32* __r
     if (v < N) sig[pi[v]] = fill[v];
 }
 
-__global__ void gather_keys_kernel(const u64* __restrict__ P, const u64* __restrict__ sig,
-                                   u64* __restrict__ keys, u64 n) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = sig[(u32)P[i]];
+// Sort keys in colour order, keys[i] = sig[node of P[i]], accumulating the
+// histograms of the npass 8-bit digits from bit_begin into the sort's
+// counters (the sort then skips its own pass over the keys).
+constexpr unsigned GK_BLOCKS = 2048;
+__global__ __launch_bounds__(256) void gather_keys_hist_kernel(
+    const u64* __restrict__ P, const u64* __restrict__ sig, u64* __restrict__ keys, u64 n,
+    int bit_begin, int npass, u32* __restrict__ ghist) {
+    __shared__ u32 lh[8][256];
+    for (int i = threadIdx.x; i < 8 * 256; i += 256) (&lh[0][0])[i] = 0;
+    __syncthreads();
+    constexpr int IT = 8;  // keys per thread per step, all gathered before any is counted
+    const u64 stride = (u64)gridDim.x * 256 * IT;
+    for (u64 b = (u64)blockIdx.x * 256 * IT; b < n; b += stride) {  // uniform per workgroup
+        u64 k[IT];
+#pragma unroll
+        for (int j = 0; j < IT; ++j) {
+            const u64 i = b + (u64)j * 256 + threadIdx.x;
+            k[j] = i < n ? sig[(u32)P[i]] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < IT; ++j) {
+            const u64 i = b + (u64)j * 256 + threadIdx.x;
+            if (i < n) keys[i] = k[j];
+            for (int p = 0; p < npass; ++p)
+                sort_hist_add(lh[p], (u32)(k[j] >> (bit_begin + 8 * p)) & 255u, i < n);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * 256; i += 256) {
+        const u32 v = (&lh[0][0])[i];
+        if (v) atomicAdd(&ghist[i], v);
+    }
 }
 
 // After a stable sort by the signature bits at and above `lo` (colour order
@@ -409,58 +437,65 @@ constexpr u64 RANK_TILE = 256 * RANK_ITEMS;
 
 // Dense rank of (sig, colour) over the sorted sequence (rbsc.cu:392-396):
 // flag = position starts a new distinct pair, rank = inclusive scan of flags.
-// Writes Pout[i] = (rank << 32) | node and hc[node] = randomize(rank + salt).
+// Writes Pout[i] = (rank << 32) | node (unless Pout is null) and
+// hc[node] = randomize(rank + salt) (or the raw rank with c32).  A position
+// whose key equals its predecessor's but whose colour is smaller (the
+// sequence is not in (sig, colour) order there) sets *disorder.
+//
+// A workgroup takes a tile of RANK_TILE positions by ticket; wave w reads
+// positions [base + 1024 w, + 1024) in 16 coalesced steps of 64 (lane l of
+// step j holds position 64 j + l), takes each position's predecessor from the
+// neighbouring lane (lane 63 of the step before for lane 0), counts flags
+// with ballots, and ranks with the wave's running count plus the flags of
+// lower lanes in the step.  The tile's count is published by decoupled
+// look-back over earlier tiles.
 __global__ __launch_bounds__(256) void rank_kernel(
     const u64* __restrict__ keys, const u64* __restrict__ Pin, u64* __restrict__ Pout,
     u64* __restrict__ hc, u64 n, u64 rank_offset, u64 salt_next, int c32, u64* status,
-    u64* ticket, u64 ticket_base, u32 epoch, u32* K_out, u32* err) {
+    u64* ticket, u64 ticket_base, u32 epoch, u32* K_out, u32* disorder, u32* err) {
     __shared__ u64 s_wave[4];
     __shared__ u64 s_tile, s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) s_tile = atomicAdd((unsigned long long*)ticket, 1ULL) - ticket_base;
     __syncthreads();
     const u64 tile = s_tile;
-    const u64 base = tile * RANK_TILE + (u64)tid * RANK_ITEMS;
+    const u64 wbase = tile * RANK_TILE + (u64)wave * (64 * RANK_ITEMS);
     u64 k[RANK_ITEMS], p[RANK_ITEMS];
 #pragma unroll
     for (int j = 0; j < RANK_ITEMS; ++j) {
-        const u64 i = base + j;
+        const u64 i = wbase + 64 * j + lane;
         k[j] = i < n ? keys[i] : 0;
         p[j] = i < n ? Pin[i] : 0;
     }
-    u64 pk = 0, pp = 0;
-    if (base > 0 && base < n) {
-        pk = keys[base - 1];
-        pp = Pin[base - 1];
+    // predecessor of the wave's first position
+    u64 lk = 0, lc = 0;
+    if (wbase > 0 && wbase < n) {
+        lk = keys[wbase - 1];
+        lc = Pin[wbase - 1] >> 32;
     }
-    u32 flags = 0;  // bit j set: item j starts a new pair
-    u32 cnt = 0;
+    u64 bal[RANK_ITEMS];
+    u64 wcount = 0;
 #pragma unroll
     for (int j = 0; j < RANK_ITEMS; ++j) {
-        const u64 i = base + j;
-        const u64 prevk = j ? k[j - 1] : pk;
-        const u64 prevc = (j ? p[j - 1] : pp) >> 32;
-        const bool f = i < n && i > 0 && (k[j] != prevk || (p[j] >> 32) != prevc);
-        flags |= (u32)f << j;
-        cnt += f;
+        const u64 i = wbase + 64 * j + lane;
+        const u64 c = p[j] >> 32;
+        u64 pk = __shfl_up(k[j], 1);
+        u64 pc = __shfl_up(c, 1);
+        if (lane == 0) {
+            pk = lk;
+            pc = lc;
+        }
+        const bool f = i < n && i > 0 && (k[j] != pk || c != pc);
+        if (i < n && i > 0 && k[j] == pk && c < pc) atomicOr(disorder, 1u);
+        bal[j] = __ballot(f);
+        wcount += (u64)__popcll(bal[j]);
+        lk = __shfl(k[j], 63);  // predecessor of the next step's lane 0
+        lc = __shfl(c, 63);
     }
-    // Block exclusive scan of per-thread counts.
-    u64 x = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const u64 y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    if (lane == 63) s_wave[wave] = x;
+    if (lane == 0) s_wave[wave] = wcount;
     __syncthreads();
-    u64 woff = 0, agg = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        if (w < wave) woff += s_wave[w];
-        agg += s_wave[w];
-    }
-    const u64 texcl = woff + x - cnt;
     if (tid == 0) {
+        const u64 agg = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
         u64* st = status + tile;
         u64 excl = 0;
         if (tile == 0) {
@@ -470,9 +505,9 @@ __global__ __launch_bounds__(256) void rank_kernel(
             long long q = (long long)tile - 1;
             u32 spins = 0;
             while (q >= 0) {
-                const u64 s = lb_load(status + q);
-                const u64 state = s >> LB_STATE_SHIFT;
-                const u32 ep = (u32)((s >> LB_EPOCH_SHIFT) & LB_EPOCH_MASK);
+                const u64 sw = lb_load(status + q);
+                const u64 state = sw >> LB_STATE_SHIFT;
+                const u32 ep = (u32)((sw >> LB_EPOCH_SHIFT) & LB_EPOCH_MASK);
                 if (state == 0 || ep != (epoch & LB_EPOCH_MASK)) {
                     if (++spins > LB_SPIN_LIMIT) {
                         atomicOr(err, KERR_SPIN);
@@ -481,7 +516,7 @@ __global__ __launch_bounds__(256) void rank_kernel(
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                excl += s & LB_COUNT_MASK;
+                excl += sw & LB_COUNT_MASK;
                 if (state == LB_PREFIX) break;
                 --q;
             }
@@ -490,18 +525,32 @@ __global__ __launch_bounds__(256) void rank_kernel(
         s_prefix = excl;
     }
     __syncthreads();
-    u64 r = s_prefix + texcl + rank_offset;
+    u64 r = s_prefix + rank_offset;
+    for (int w = 0; w < wave; ++w) r += s_wave[w];
+    const u64 below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    const u64 upto = below | (1ULL << lane);
 #pragma unroll
     for (int j = 0; j < RANK_ITEMS; ++j) {
-        const u64 i = base + j;
+        const u64 i = wbase + 64 * j + lane;
+        const u64 rj = r + (u64)__popcll(bal[j] & upto);
+        r += (u64)__popcll(bal[j]);
         if (i < n) {
-            r += (flags >> j) & 1u;
             const u32 node = (u32)p[j];
-            Pout[i] = (r << 32) | node;
-            if (c32) reinterpret_cast<u32*>(hc)[node] = (u32)r;
-            else hc[node] = rbsc_randomize(r + salt_next);
-            if (i == n - 1) *K_out = (u32)(r + 1);
+            if (Pout) Pout[i] = (rj << 32) | node;
+            if (c32) reinterpret_cast<u32*>(hc)[node] = (u32)rj;
+            else hc[node] = rbsc_randomize(rj + salt_next);
+            if (i == n - 1) *K_out = (u32)(rj + 1);
         }
+    }
+}
+
+// keys[i] = colour of node i, P[i] = (colour << 32) | i (raw u32 colours).
+__global__ void colour_keys_kernel(const u32* __restrict__ col, u64 n, u64* __restrict__ keys,
+                                   u64* __restrict__ P) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        keys[i] = col[i];
+        P[i] = ((u64)col[i] << 32) | i;
     }
 }
 
@@ -600,6 +649,7 @@ void ws_release(rbsc_ctx* ctx) {
     }
     if (w.sig) (void)hipFree(w.sig);
     if (w.hc) (void)hipFree(w.hc);
+    if (w.hc2) (void)hipFree(w.hc2);
     if (w.rank_status) (void)hipFree(w.rank_status);
     for (void* p : {(void*)w.tie_runs, (void*)w.tie_pre, (void*)w.tie_tmp, (void*)w.tie_mark,
                     (void*)w.tie_buf, (void*)w.carry})
@@ -620,6 +670,7 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     }
     HIP_TRY(hipMalloc(&w.sig, sizeof(u64) * (cap + 1)));  // + spare slot for padding
     HIP_TRY(hipMalloc(&w.hc, sizeof(u64) * cap));
+    HIP_TRY(hipMalloc(&w.hc2, sizeof(u64) * cap));
     w.tie_cap = cap / 16 + 1024;
     HIP_TRY(hipMalloc(&w.tie_runs, sizeof(u64) * 2 * w.tie_cap));
     HIP_TRY(hipMalloc(&w.tie_pre, sizeof(u64) * (w.tie_cap + 1)));
@@ -801,11 +852,20 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     const u64 n_rank = excluded ? g->R : N;
     const u64 rank_offset = excluded && N > g->U ? 1 : 0;
     const u32 gather_limit = excluded ? g->U : (u32)N;
-    const int c32 = (prm.flags & RBSC_FLAG_HASHED64) ? 0 : 1;
+    const bool streaming = g->dst == nullptr && g->E > 0;
+    // The binned layout serves resident rounds.  Its hashed form keeps u64
+    // hashed colours per node; its raw form gathers u32 raw colours (and is
+    // left out when hashed colours are asked for).
+    const bool bins_fit = g->bins && !streaming && g->n_parts == 1 &&
+                          g->bins->GL == gather_limit && !(prm.flags & RBSC_FLAG_NOBINS);
+    const bool binned = bins_fit && (g->bins->hashed || !(prm.flags & RBSC_FLAG_HASHED64));
+    const int c32 = binned ? (g->bins->hashed ? 0 : 1) : ((prm.flags & RBSC_FLAG_HASHED64) ? 0 : 1);
     const bool follow = excluded && g->U > g->R;
+    const bool node_order = excluded && c32 && !(prm.flags & RBSC_FLAG_COLOUR_ORDER);
 
     // Round-0 state: P sorted by c0, hc = randomize(c0 + 0).
     prof_begin(ctx, 4);
+    fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 3, 1, 0);
     int which = 0;
     int pc = 0;
     if (excluded) {
@@ -826,7 +886,6 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     prof_end(ctx);
 
     // Streaming set-up: slices of whole wave tiles, double-buffered.
-    const bool streaming = g->dst == nullptr && g->E > 0;
     u64 Bs = 0, n_slices = 0;
     const bool weighted = streaming ? g->wt_host != nullptr : g->wt != nullptr;
     if (streaming) {
@@ -861,30 +920,26 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     u32 K = prm.k0, Knew = prm.k0;
     u32 it = 0;
     const u64 n_tiles = g->n_tiles;
-    // The binned layout serves resident rounds over raw u32 colours.
-    const bool binned = g->bins && !streaming && c32 && g->n_parts == 1 &&
-                        g->bins->GL == gather_limit && !(prm.flags & RBSC_FLAG_NOBINS);
     // Repeated rows' colours are kept only where lines still gather them (the
     // binned layout points its lines at the stored rows instead).
     const bool copy_rep = follow && !(binned && g->bins->remap);
-    auto rep_colors = [&]() {
+    auto rep_colors = [&](u64* hcbuf) {
         if (copy_rep)
             dup_colors_kernel<<<grid_for(g->U - g->R), BLK, 0, s>>>(g->dup_rep, g->R,
-                                                                    g->U - g->R, w.hc, c32 ? 0 : 1);
+                                                                    g->U - g->R, hcbuf, c32 ? 0 : 1);
     };
-    rep_colors();
+    rep_colors(w.hc);
     do {
         K = Knew;
         const u64 dang_h = rbsc_randomize(0 + (u64)it);
         if (binned) {
-            const u32* col = reinterpret_cast<const u32*>(w.hc);
             prof_begin(ctx, 6);
-            bins_scatter(ctx, g, col);
+            bins_scatter(ctx, g, w.hc);
             prof_end(ctx);
         }
         prof_begin(ctx, 0);
         if (binned) {
-            bins_sig(ctx, g, reinterpret_cast<const u32*>(w.hc), (u64)it, dang_h, w.sig);
+            bins_sig(ctx, g, w.hc, (u64)it, dang_h, w.sig);
         } else if (n_tiles && !streaming && g->n_parts == 8 && (prm.flags & RBSC_FLAG_XCD)) {
             fill_u64<<<grid_for(g->U), BLK, 0, s>>>(w.sig, g->U, 0);
             launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0, gather_limit, dang_h, it, c32, true);
@@ -991,17 +1046,19 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             return RBSC_OK;
         };
         auto sort_and_rank = [&](bool full, bool gather) -> int {
+            const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
+            const int bit0 = full ? 0 : (int)lo;
             if (gather) {
                 prof_begin(ctx, 1);
-                gather_keys_kernel<<<grid_for(n_rank), BLK, 0, s>>>(w.P[pc], w.sig, w.keys[0],
-                                                                    n_rank);
+                const unsigned gb = (unsigned)std::min<u64>((n_rank + 4095) / 4096, GK_BLOCKS);
+                gather_keys_hist_kernel<<<gb, 256, 0, s>>>(w.P[pc], w.sig, w.keys[0], n_rank, bit0,
+                                                           (64 - bit0) / 8, ctx->sort.hist);
                 prof_end(ctx);
             }
             u64* kk[2] = {w.keys[0], w.keys[1]};
             u64* pp[2] = {w.P[pc], w.P[pc ^ 1]};
             prof_begin(ctx, 2);
-            const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
-            int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, full ? 0 : lo, 64, &which, s);
+            int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, bit0, 64, &which, s, gather);
             if (r2) return r2;
             if (!full) {
                 r2 = fix_ties(kk[which], pp[which], lo);
@@ -1013,27 +1070,85 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             const u32 ep = lb_next_epoch(&ctx->sort, s);
             prof_begin(ctx, 3);
             rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
-                kk[which], pp[which], pp[which ^ 1], w.hc, n_rank, rank_offset, (u64)it + 1, c32,
+                kk[which], pp[which], pp[which ^ 1], w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
                 w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
-                ctx->d_err);
-            rep_colors();
+                ctx->d_scalar + 3, ctx->d_err);
+            rep_colors(w.hc2);
+            prof_end(ctx);
+            ctx->sort.ticket_total += tiles;
+            return RBSC_OK;
+        };
+        // Node order (no initial colours, raw colours): the sort reads the
+        // signatures and colours by node id, so no gather; inside equal
+        // signatures the order is by node, which is (signature, colour) order
+        // as long as equal signatures have equal colours.  From a uniform
+        // start they do: equal signatures mean equal neighbour-colour
+        // multisets in this and, the colours refining, every earlier round,
+        // hence equal colours -- unless two different multisets hash to the
+        // same 64-bit sum.  The rank pass flags any pair out of colour order
+        // and the round's node side is then redone from positions in colour
+        // order.
+        auto sort_and_rank_nodes = [&](bool full) -> int {
+            u64* kk[2] = {w.keys[0], w.keys[1]};
+            u64* pp[2] = {w.P[0], w.P[1]};
+            const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
+            prof_begin(ctx, 2);
+            int r2 = radix_sort_nodes(&ctx->sort, w.sig, reinterpret_cast<const u32*>(w.hc), kk,
+                                      pp, n_rank, full ? 0 : (int)lo, 64, s);
+            if (r2) return r2;
+            if (!full) {
+                r2 = fix_ties(kk[0], pp[0], lo);
+                if (r2) return r2;
+            }
+            prof_end(ctx);
+            const u64 tiles = (n_rank + RANK_TILE - 1) / RANK_TILE;
+            const u32 ep = lb_next_epoch(&ctx->sort, s);
+            prof_begin(ctx, 3);
+            rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
+                kk[0], pp[0], nullptr, w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
+                w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
+                ctx->d_scalar + 3, ctx->d_err);
+            rep_colors(w.hc2);
             prof_end(ctx);
             ctx->sort.ticket_total += tiles;
             return RBSC_OK;
         };
         const bool full_sort = (prm.flags & RBSC_FLAG_FULLSORT) != 0;
         if (n_rank) {
-            rc = sort_and_rank(full_sort, true);
+            rc = node_order ? sort_and_rank_nodes(full_sort) : sort_and_rank(full_sort, true);
             if (rc) return rc;
         }
         HIP_TRY(hipGetLastError());
         if (n_rank)
-            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 2 * sizeof(u32),
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 4 * sizeof(u32),
                                    hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 4, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (ctx->h_scalar[4]) return check_kernel_errors(ctx);  // K of a failed round is void
-        if (n_rank && ctx->h_scalar[1]) {
+        if (n_rank && node_order && (ctx->h_scalar[1] || ctx->h_scalar[3])) {
+            // Run list overflow or pairs out of colour order: redo the node
+            // side from positions sorted by the (unchanged) old colours, with
+            // the stable full 64-bit sort by signature.
+            ++ctx->full_sort_redos;
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 1, 3, 0);
+            colour_keys_kernel<<<grid_for(n_rank), BLK, 0, s>>>(
+                reinterpret_cast<const u32*>(w.hc), n_rank, w.keys[0], w.P[0]);
+            u64* kk[2] = {w.keys[0], w.keys[1]};
+            u64* pp[2] = {w.P[0], w.P[1]};
+            int wc = 0;
+            rc = radix_sort_u64(&ctx->sort, kk, pp, n_rank, 0, 32, &wc, s);
+            if (rc) return rc;
+            pc = wc;
+            rc = sort_and_rank(true, true);
+            if (rc) return rc;
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 4 * sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (ctx->h_scalar[1] || ctx->h_scalar[3]) {
+                rbsc_set_error("node side redo left pairs out of order");
+                return RBSC_ERR_KERNEL;
+            }
+        } else if (n_rank && ctx->h_scalar[1]) {
             // A run of signatures equal in their top 32 bits was too long for
             // the fix-up: sort this round's (signature, P) again on all 64 bits.
             // Inside equal signatures the arrays still hold colour order, so
@@ -1048,6 +1163,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             HIP_TRY(hipStreamSynchronize(s));
         }
         if (n_rank) pc = sorted_pc ^ 1;
+        std::swap(w.hc, w.hc2);  // next round reads the colours ranked here
         Knew = n_rank ? ctx->h_scalar[0] : (u32)rank_offset;
         ++it;
         // K grows every round that is not the last and cannot pass N.
@@ -1070,6 +1186,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     w.pc = pc;
     w.n_rank = n_rank;
     w.follow = follow;
+    w.node_order = node_order;
     out->K = Knew;
     out->rounds = it;
     out->device_ms = ms;
@@ -1081,7 +1198,10 @@ void device_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* d_int) {
     Workspace& w = ctx->ws;
     hipStream_t s = ctx->stream;
     if (w.n_rank < N) fill_u32_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, N, 0);
-    if (w.n_rank) colors_kernel<<<grid_for(w.n_rank), BLK, 0, s>>>(w.P[w.pc], d_int, w.n_rank);
+    if (w.n_rank && w.node_order)
+        (void)hipMemcpyAsync(d_int, w.hc, sizeof(u32) * w.n_rank, hipMemcpyDeviceToDevice, s);
+    else if (w.n_rank)
+        colors_kernel<<<grid_for(w.n_rank), BLK, 0, s>>>(w.P[w.pc], d_int, w.n_rank);
     if (w.follow)
         dup_colors_kernel<<<grid_for(g->U - g->R), BLK, 0, s>>>(g->dup_rep, g->R, g->U - g->R,
                                                                 d_int, 0);
@@ -1314,7 +1434,7 @@ extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts
         return RBSC_OK;
     }
     return dgraph_bin(ctx, g, o.hot_heads, log_s, o.chunk_slots, o.scatter_lines,
-                      (o.flags & RBSC_BIN_SLOT_ORDER) == 0);
+                      (o.flags & RBSC_BIN_SLOT_ORDER) == 0, (o.flags & RBSC_BIN_HASHED) != 0);
 }
 
 extern "C" int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g) {
@@ -1332,7 +1452,7 @@ extern "C" int rbsc_graph_bin_info(const rbsc_dgraph* g, uint64_t* info) {
     const rbsc_bins* b = g->bins;
     const uint64_t v[8] = {b ? 1u : 0u, b ? b->n_bins : 0, b ? b->n_hot : 0, b ? b->n_cold : 0,
                            b ? b->n_chunks : 0, b ? b->n_zero : 0, b ? b->bytes : 0,
-                           b ? ((u64)b->H | ((u64)b->log_s << 32)) : 0};
+                           b ? ((u64)b->H | ((u64)b->log_s << 32) | ((u64)b->hashed << 40)) : 0};
     for (int i = 0; i < 8; ++i) info[i] = v[i];
     return RBSC_OK;
 }

@@ -29,8 +29,32 @@ void sort_ws_free(SortWorkspace* ws);
 // Sort (keys[0], vals[0]) by key bits [bit_begin, bit_end) (multiple of 8 wide),
 // stably, ping-ponging with (keys[1], vals[1]).  vals may be null (keys only).
 // Returns in *which the buffer index (0 or 1) that holds the result.
+// hist_ready: ws->hist already holds the digit counts of keys[0] for this
+// bit range (accumulated with sort_hist_add by the kernel that wrote them).
 int radix_sort_u64(SortWorkspace* ws, u64* keys[2], u64* vals[2], u64 n, int bit_begin,
-                   int bit_end, int* which, hipStream_t s);
+                   int bit_end, int* which, hipStream_t s, bool hist_ready = false);
+
+// Stable sort of n < 2^32 nodes in id order by key0 bits [bit_begin,
+// bit_end): the first pass reads key0[i] and makes the value
+// (col[i] << 32) | i; the result is in (keys[0], vals[0]).
+int radix_sort_nodes(SortWorkspace* ws, const u64* key0, const u32* col, u64* keys[2],
+                     u64* vals[2], u64 n, int bit_begin, int bit_end, hipStream_t s);
+
+// Adds one key's digit d (0..255) to an LDS histogram h[256], one LDS atomic
+// per distinct digit in the wave: lanes with equal digits are found with
+// eight ballots and their leader adds their number (many equal keys, as in
+// the early rounds of a refinement, would otherwise serialise on one
+// address).  Every lane of the wave must call it; valid = the lane has a key.
+__device__ __forceinline__ void sort_hist_add(u32* h, u32 d, bool valid) {
+    u64 peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const u64 bal = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    if (valid && (int)(__ffsll((long long)peers) - 1) == (int)(threadIdx.x & 63))
+        atomicAdd(&h[d], (u32)__popcll(peers));
+}
 
 // Epoch/ticket bookkeeping shared with the other look-back kernels.
 u32 lb_next_epoch(SortWorkspace* ws, hipStream_t s);

@@ -20,6 +20,8 @@ constexpr int RADIX = 256;
 static_assert(TILE == (int)SORT_TILE, "tile size mismatch");
 static_assert(THREADS == RADIX, "one thread per digit in the look-back");
 
+constexpr int HIST_ITEMS = 16;
+
 // Histograms of every 8-bit digit of the sort.
 __global__ __launch_bounds__(THREADS) void hist_kernel(const u64* __restrict__ keys, u64 n,
                                                        int bit_begin, int npass,
@@ -27,11 +29,22 @@ __global__ __launch_bounds__(THREADS) void hist_kernel(const u64* __restrict__ k
     __shared__ u32 lh[8][RADIX];
     for (int i = threadIdx.x; i < 8 * RADIX; i += THREADS) (&lh[0][0])[i] = 0;
     __syncthreads();
-    const u64 stride = (u64)gridDim.x * THREADS;
-    for (u64 i = (u64)blockIdx.x * THREADS + threadIdx.x; i < n; i += stride) {
-        const u64 k = keys[i];
-        for (int p = 0; p < npass; ++p)
-            atomicAdd(&lh[p][(k >> (bit_begin + 8 * p)) & 255], 1u);
+    // HIST_ITEMS keys per thread per step, all loaded before any is counted
+    // (one load in flight per thread would leave the pass latency-bound).
+    const u64 stride = (u64)gridDim.x * THREADS * HIST_ITEMS;
+    for (u64 b = (u64)blockIdx.x * THREADS * HIST_ITEMS; b < n; b += stride) {  // uniform
+        u64 k[HIST_ITEMS];
+#pragma unroll
+        for (int j = 0; j < HIST_ITEMS; ++j) {
+            const u64 i = b + (u64)j * THREADS + threadIdx.x;
+            k[j] = i < n ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < HIST_ITEMS; ++j) {
+            const bool valid = b + (u64)j * THREADS + threadIdx.x < n;
+            for (int p = 0; p < npass; ++p)
+                sort_hist_add(lh[p], (u32)(k[j] >> (bit_begin + 8 * p)) & 255u, valid);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < npass * RADIX; i += THREADS) {
@@ -80,11 +93,15 @@ __global__ void fill_u64_kernel(u64* p, u64 n, u64 v) {
     if (i < n) p[i] = v;
 }
 
-template <bool HAS_VALS>
+// VM: 0 keys only; 1 values from vin; 2 values made from a u32 array and the
+// position, vin_col[i] << 32 | i (the first pass of a sort of nodes in id
+// order, values (colour << 32) | node).
+template <int VM>
 __global__ __launch_bounds__(THREADS, 3) void onesweep_kernel(
     const u64* __restrict__ kin, const u64* __restrict__ vin, u64* __restrict__ kout,
     u64* __restrict__ vout, u64 n, u32 shift, const u64* __restrict__ digit_base,
     u64* status, u64* ticket, u64 ticket_base, u32 epoch, u32* err) {
+    constexpr bool HAS_VALS = VM != 0;
     __shared__ u64 s_stage[TILE];
     __shared__ unsigned char s_digit[TILE];
     __shared__ u32 s_whist[WAVES][RADIX];
@@ -110,11 +127,18 @@ __global__ __launch_bounds__(THREADS, 3) void onesweep_kernel(
     }
     // Values are loaded now so their latency hides behind ranking and look-back.
     u64 val[HAS_VALS ? ITEMS : 1];
-    if (HAS_VALS) {
+    if (VM == 1) {
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
             val[j] = idx < n ? vin[idx] : 0;
+        }
+    } else if (VM == 2) {
+        const u32* col = reinterpret_cast<const u32*>(vin);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
+            val[j] = idx < n ? ((u64)col[idx] << 32) | idx : 0;
         }
     }
     const u64 below_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
@@ -254,7 +278,7 @@ void sort_ws_free(SortWorkspace* ws) {
 }
 
 int radix_sort_u64(SortWorkspace* ws, u64* keys[2], u64* vals[2], u64 n, int bit_begin,
-                   int bit_end, int* which, hipStream_t s) {
+                   int bit_end, int* which, hipStream_t s, bool hist_ready) {
     *which = 0;
     if (n == 0 || bit_end <= bit_begin) return RBSC_OK;
     if ((bit_end - bit_begin) % 8 != 0 || bit_end > 64 || bit_begin < 0) {
@@ -268,19 +292,21 @@ int radix_sort_u64(SortWorkspace* ws, u64* keys[2], u64* vals[2], u64 n, int bit
         return RBSC_ERR_INVALID;
     }
     const int npass = (bit_end - bit_begin) / 8;
-    const unsigned hist_blocks = (unsigned)std::min<u64>((n + THREADS * 16 - 1) / (THREADS * 16), 2048);
-    hist_kernel<<<hist_blocks, THREADS, 0, s>>>(keys[0], n, bit_begin, npass, ws->hist);
+    const unsigned hist_blocks =
+        (unsigned)std::min<u64>((n + THREADS * HIST_ITEMS - 1) / (THREADS * HIST_ITEMS), 2048);
+    if (!hist_ready)
+        hist_kernel<<<hist_blocks, THREADS, 0, s>>>(keys[0], n, bit_begin, npass, ws->hist);
     scan_hist_kernel<<<1, THREADS, 0, s>>>(ws->hist, npass, ws->digit_base);
     int cur = 0;
     for (int p = 0; p < npass; ++p) {
         const u32 ep = lb_next_epoch(ws, s);
         const u32 shift = (u32)(bit_begin + 8 * p);
         if (vals) {
-            onesweep_kernel<true><<<(unsigned)tiles, THREADS, 0, s>>>(
+            onesweep_kernel<1><<<(unsigned)tiles, THREADS, 0, s>>>(
                 keys[cur], vals[cur], keys[cur ^ 1], vals[cur ^ 1], n, shift,
                 ws->digit_base + p * RADIX, ws->status, ws->ticket, ws->ticket_total, ep, ws->err);
         } else {
-            onesweep_kernel<false><<<(unsigned)tiles, THREADS, 0, s>>>(
+            onesweep_kernel<0><<<(unsigned)tiles, THREADS, 0, s>>>(
                 keys[cur], nullptr, keys[cur ^ 1], nullptr, n, shift,
                 ws->digit_base + p * RADIX, ws->status, ws->ticket, ws->ticket_total, ep, ws->err);
         }
@@ -289,5 +315,42 @@ int radix_sort_u64(SortWorkspace* ws, u64* keys[2], u64* vals[2], u64 n, int bit
     }
     HIP_TRY(hipGetLastError());
     *which = cur;
+    return RBSC_OK;
+}
+
+int radix_sort_nodes(SortWorkspace* ws, const u64* key0, const u32* col, u64* keys[2],
+                     u64* vals[2], u64 n, int bit_begin, int bit_end, hipStream_t s) {
+    if (n == 0 || bit_end <= bit_begin) return RBSC_OK;
+    if ((bit_end - bit_begin) % 8 != 0 || bit_end > 64 || bit_begin < 0 || n >= (1ull << 32)) {
+        rbsc_set_error("radix_sort_nodes: bit range [%d,%d) or %llu keys not supported",
+                       bit_begin, bit_end, (unsigned long long)n);
+        return RBSC_ERR_INVALID;
+    }
+    const u64 tiles = (n + TILE - 1) / TILE;
+    if (tiles + 1 > ws->status_tiles) {
+        rbsc_set_error("radix_sort_nodes: workspace sized for fewer keys");
+        return RBSC_ERR_INVALID;
+    }
+    const int npass = (bit_end - bit_begin) / 8;
+    const unsigned hist_blocks =
+        (unsigned)std::min<u64>((n + THREADS * HIST_ITEMS - 1) / (THREADS * HIST_ITEMS), 2048);
+    hist_kernel<<<hist_blocks, THREADS, 0, s>>>(key0, n, bit_begin, npass, ws->hist);
+    scan_hist_kernel<<<1, THREADS, 0, s>>>(ws->hist, npass, ws->digit_base);
+    // pass p writes buffer (npass - 1 - p) & 1, so the last pass writes buffer 0
+    for (int p = 0; p < npass; ++p) {
+        const u32 ep = lb_next_epoch(ws, s);
+        const u32 shift = (u32)(bit_begin + 8 * p);
+        const int out = (npass - 1 - p) & 1;
+        if (p == 0)
+            onesweep_kernel<2><<<(unsigned)tiles, THREADS, 0, s>>>(
+                key0, reinterpret_cast<const u64*>(col), keys[out], vals[out], n, shift,
+                ws->digit_base, ws->status, ws->ticket, ws->ticket_total, ep, ws->err);
+        else
+            onesweep_kernel<1><<<(unsigned)tiles, THREADS, 0, s>>>(
+                keys[out ^ 1], vals[out ^ 1], keys[out], vals[out], n, shift,
+                ws->digit_base + p * RADIX, ws->status, ws->ticket, ws->ticket_total, ep, ws->err);
+        ws->ticket_total += tiles;
+    }
+    HIP_TRY(hipGetLastError());
     return RBSC_OK;
 }

@@ -31,12 +31,16 @@ CASES = {
 }
 
 # (hot heads, rows per bin, slots per chunk, cold lines per scatter
-# workgroup, scatter records in slot order): defaults, everything cold,
-# everything hot, small bins split over many chunks, the largest bins,
-# scatter chunks of a few lines (not multiples of 8)
-SHAPES = [(0, 0, 0, 0, False), (1, 256, 0, 0, True), (1 << 30, 512, 0, 0, False),
-          (64, 256, 300, 13, False), (4096, 16384, 1000, 1 << 20, True), (16, 1024, 0, 1, False),
-          (3, 256, 0, 77, True)]
+# workgroup, scatter records in slot order, hashed form): defaults,
+# everything cold, everything hot, small bins split over many chunks, the
+# largest bins, scatter chunks of a few lines (not multiples of 8); the raw
+# form on all of them, the hashed form on four
+SHAPES = [(0, 0, 0, 0, False, False), (1, 256, 0, 0, True, False),
+          (1 << 30, 512, 0, 0, False, False), (64, 256, 300, 13, False, False),
+          (4096, 16384, 1000, 1 << 20, True, False), (16, 1024, 0, 1, False, False),
+          (3, 256, 0, 77, True, False),
+          (0, 0, 0, 0, False, True), (1, 256, 0, 0, True, True),
+          (64, 256, 300, 13, False, True), (3, 256, 0, 77, True, True)]
 
 
 def _trace(g, rounds=64):
@@ -51,16 +55,17 @@ def test_binned_rounds_match_oracle(ctx, name):
     dg = ctx.upload(g)
     plain = dg.refine(colors=True)
     assert (plain.nodes, plain.rounds) == (K, R)
-    for hot, rows, chunk, scatter, slot_order in SHAPES:
-        dg.bin(hot, rows, chunk, scatter, slot_order)
+    for hot, rows, chunk, scatter, slot_order, hashed in SHAPES:
+        dg.bin(hot, rows, chunk, scatter, slot_order, hashed)
         info = dg.bin_info
-        assert info["binned"]
+        assert info["binned"] and info["hashed"] == hashed
         assert info["hot_lines"] + info["cold_lines"] <= dg.stored_lines
         if chunk and chunk < 1000:
             assert info["split_bins"] > 0
         for r in range(1, R + 1):
             res = dg.refine(max_rounds=r, partition=False, colors=True)
-            assert np.array_equal(res.colors, tr[r - 1]), f"{(hot, rows, chunk, scatter, slot_order)} round {r}"
+            assert np.array_equal(res.colors, tr[r - 1]), \
+                f"{(hot, rows, chunk, scatter, slot_order, hashed)} round {r}"
         full = dg.refine(verify=True, colors=True)
         assert (full.nodes, full.rounds, full.verified) == (K, R, 1)
         assert np.array_equal(full.colors, plain.colors)
@@ -89,8 +94,8 @@ def test_rows_into_nodes_without_lines_only(ctx):
         g = _coo(9, lines, weight)
         c, K, R, tr = _trace(g)
         dg = ctx.upload(g)
-        for hot in (1, 2, 1 << 20):
-            dg.bin(hot, 256, 0)
+        for hot, hashed in ((1, False), (2, False), (1 << 20, False), (1, True), (1 << 20, True)):
+            dg.bin(hot, 256, 0, hashed=hashed)
             for r in range(1, R + 1):
                 res = dg.refine(max_rounds=r, partition=False, colors=True)
                 assert np.array_equal(res.colors, tr[r - 1])
@@ -123,10 +128,44 @@ def test_bins_follow_the_line_layout(ctx):
 
 
 def test_hashed64_on_a_binned_graph(ctx):
+    # the hashed form serves hashed colours; the raw form leaves them to the
+    # plain kernel
     g = gen.rmat(12, 1 << 15, clone_frac=0.2, seed=36)
     c, K, R, tr = _trace(g)
     dg = ctx.upload(g)
-    dg.bin(256, 256, 0)
-    res = dg.refine(colors=True, flags=rbsc_amd.FLAG_HASHED64)
-    assert (res.nodes, res.rounds) == (K, R) and np.array_equal(res.colors, tr[R - 1])
+    for hashed in (False, True):
+        dg.bin(256, 256, 0, hashed=hashed)
+        res = dg.refine(colors=True, flags=rbsc_amd.FLAG_HASHED64)
+        assert (res.nodes, res.rounds) == (K, R) and np.array_equal(res.colors, tr[R - 1])
+    dg.free()
+
+
+def test_hashed_form_bin_boundaries(ctx):
+    # rows into a few heavily gathered heads only, rows into rarely gathered
+    # heads only and mixed rows of 1-11 lines: bins whose cold regions start
+    # and end inside aligned groups of 8, bins with few or no cold slots,
+    # chunks of a few slots, a short last bin; every round against the oracle
+    rng = np.random.default_rng(37)
+    n = 3000
+    lines = []
+    for u in range(2000):
+        k = int(rng.integers(1, 12))
+        for _ in range(k):
+            if u % 7 == 0:
+                v = int(rng.integers(0, 40))        # hot only
+            elif u % 11 == 0:
+                v = int(rng.integers(40, 2000))     # cold only
+            else:
+                v = int(rng.integers(0, n))
+            lines.append((u, v))
+    g = _coo(n, lines)
+    c, K, R, tr = _trace(g)
+    dg = ctx.upload(g)
+    for hot, rows, chunk, hashed in ((40, 256, 0, True), (40, 256, 37, True), (5, 256, 9, True),
+                                     (1 << 20, 256, 0, True), (40, 256, 37, False),
+                                     (5, 256, 9, False)):
+        dg.bin(hot, rows, chunk, 3, hashed=hashed)
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True)
+            assert np.array_equal(res.colors, tr[r - 1]), f"{(hot, rows, chunk, hashed)} round {r}"
     dg.free()

@@ -343,11 +343,14 @@ def test_xcd_partitioned_single_launch(ctx, flags):
         dg.free()
 
 
-@pytest.mark.parametrize("flags", [0, 32, 8, 8 | 16, 16])
+@pytest.mark.parametrize("flags", [0, 32, 8, 8 | 16, 16, 128, 128 | 32, 128 | 8, 128 | 8 | 16])
 def test_sort_split_and_tie_fixup(ctx, flags):
     """Top-half sort + colliding-run fix-up (default, 0), full 64-bit sort
     (32), frequent collisions via a 16-bit split (8), and the full-sort
-    fallback forced by a 2-entry run list (16)."""
+    fallback forced by a 2-entry run list (16); on the R-MAT (no initial
+    colours) the default sorts nodes from id order, where the forced
+    fallback takes the exact redo from colour order, and 128 keeps
+    positions in colour order instead; the BA graph has initial colours."""
     for g in (gen.rmat(15, 1 << 19, clone_frac=0.2, seed=15),
               gen.ba(30000, 3, 0.1, 3, seed=9)):
         c, K, R, tr = _oracle_full(g, trace_rounds=64)
