@@ -9,6 +9,29 @@ typedef uint64_t u64;
 
 #define RBSC_WAVE 64
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// v * 5 and v * 9 (mod 2^64) as one full-rate v_lshl_add_u64 each
+// ((v << 2) + v, (v << 3) + v); the compiler otherwise multiplies by the
+// small constants with two quarter-rate 32-bit multiply-adds.
+__device__ __forceinline__ u64 rbsc_mul5(u64 v) {
+    u64 r;
+    asm("v_lshl_add_u64 %0, %1, 2, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ u64 rbsc_mul9(u64 v) {
+    u64 r;
+    asm("v_lshl_add_u64 %0, %1, 3, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+// rotate left by 7 as two funnel shifts of the 32-bit halves
+__device__ __forceinline__ u64 rbsc_rotl7(u64 v) {
+    const u32 lo = (u32)v, hi = (u32)(v >> 32);
+    const u32 nhi = __builtin_amdgcn_alignbit(hi, lo, 25);
+    const u32 nlo = __builtin_amdgcn_alignbit(lo, hi, 25);
+    return ((u64)nhi << 32) | nlo;
+}
+#endif
+
 // The signature hash of rbsc.cu:65-71 (also rsc.cpp:63-69): the SplitMix64
 // finalizer followed by the xoshiro** scrambler.  Every step is a bijection on
 // Z/2^64, so distinct colours never collide under it.
@@ -16,8 +39,12 @@ __host__ __device__ __forceinline__ u64 rbsc_randomize(u64 v) {
     v = v + 0x9e3779b97f4a7c15ULL;
     v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
     v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rbsc_mul9(rbsc_rotl7(rbsc_mul5(v ^ (v >> 31))));
+#else
     v = (v ^ (v >> 31)) * 5ULL;
     return ((v << 7) | (v >> 57)) * 9ULL;
+#endif
 }
 
 // Plain SplitMix64 step, used by the seeded generators.

@@ -262,19 +262,25 @@ __global__ void permute_fill_kernel(const u64* __restrict__ fill, const u32* __r
     if (v < N) sig[pi[v]] = fill[v];
 }
 
-// Sort keys in colour order, keys[i] = sig[node of P[i]], accumulating the
-// histograms of the npass 8-bit digits from bit_begin into the sort's
-// counters (the sort then skips its own pass over the keys).
+// Sort keys in colour order: keys[i] = sig[node of P[i]].
+__global__ void gather_keys_kernel(const u64* __restrict__ P, const u64* __restrict__ sig,
+                                   u64* __restrict__ keys, u64 n) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = sig[(u32)P[i]];
+}
+// The same, accumulating the
+// histograms of the npass (<= SORT_HD) 8-bit digits from bit_begin into the
+// sort's counters (the sort then skips its own pass over the keys).
 constexpr unsigned GK_BLOCKS = 2048;
 __global__ __launch_bounds__(256) void gather_keys_hist_kernel(
     const u64* __restrict__ P, const u64* __restrict__ sig, u64* __restrict__ keys, u64 n,
     int bit_begin, int npass, u32* __restrict__ ghist) {
-    __shared__ u32 lh[8][256];
-    for (int i = threadIdx.x; i < 8 * 256; i += 256) (&lh[0][0])[i] = 0;
+    __shared__ u32 lh[SORT_HWORDS];
+    for (int i = threadIdx.x; i < SORT_HWORDS; i += 256) lh[i] = 0;
     __syncthreads();
     constexpr int IT = 8;  // keys per thread per step, all gathered before any is counted
     const u64 stride = (u64)gridDim.x * 256 * IT;
-    for (u64 b = (u64)blockIdx.x * 256 * IT; b < n; b += stride) {  // uniform per workgroup
+    for (u64 b = (u64)blockIdx.x * 256 * IT; b < n; b += stride) {
         u64 k[IT];
 #pragma unroll
         for (int j = 0; j < IT; ++j) {
@@ -284,16 +290,14 @@ __global__ __launch_bounds__(256) void gather_keys_hist_kernel(
 #pragma unroll
         for (int j = 0; j < IT; ++j) {
             const u64 i = b + (u64)j * 256 + threadIdx.x;
-            if (i < n) keys[i] = k[j];
+            if (i >= n) continue;
+            keys[i] = k[j];
             for (int p = 0; p < npass; ++p)
-                sort_hist_add(lh[p], (u32)(k[j] >> (bit_begin + 8 * p)) & 255u, i < n);
+                sort_hist_add(lh, p, (u32)(k[j] >> (bit_begin + 8 * p)) & 255u);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < npass * 256; i += 256) {
-        const u32 v = (&lh[0][0])[i];
-        if (v) atomicAdd(&ghist[i], v);
-    }
+    sort_hist_flush(lh, npass, ghist);
 }
 
 // After a stable sort by the signature bits at and above `lo` (colour order
@@ -1048,7 +1052,14 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         auto sort_and_rank = [&](bool full, bool gather) -> int {
             const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
             const int bit0 = full ? 0 : (int)lo;
-            if (gather) {
+            // the fused histogram covers at most SORT_HD digits
+            const bool fused = gather && (64 - bit0) / 8 <= SORT_HD;
+            if (gather && !fused) {
+                prof_begin(ctx, 1);
+                gather_keys_kernel<<<grid_for(n_rank), BLK, 0, s>>>(w.P[pc], w.sig, w.keys[0],
+                                                                    n_rank);
+                prof_end(ctx);
+            } else if (fused) {
                 prof_begin(ctx, 1);
                 const unsigned gb = (unsigned)std::min<u64>((n_rank + 4095) / 4096, GK_BLOCKS);
                 gather_keys_hist_kernel<<<gb, 256, 0, s>>>(w.P[pc], w.sig, w.keys[0], n_rank, bit0,
@@ -1058,7 +1069,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             u64* kk[2] = {w.keys[0], w.keys[1]};
             u64* pp[2] = {w.P[pc], w.P[pc ^ 1]};
             prof_begin(ctx, 2);
-            int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, bit0, 64, &which, s, gather);
+            int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, bit0, 64, &which, s, fused);
             if (r2) return r2;
             if (!full) {
                 r2 = fix_ties(kk[which], pp[which], lo);

@@ -40,20 +40,27 @@ int radix_sort_u64(SortWorkspace* ws, u64* keys[2], u64* vals[2], u64 n, int bit
 int radix_sort_nodes(SortWorkspace* ws, const u64* key0, const u32* col, u64* keys[2],
                      u64* vals[2], u64 n, int bit_begin, int bit_end, hipStream_t s);
 
-// Adds one key's digit d (0..255) to an LDS histogram h[256], one LDS atomic
-// per distinct digit in the wave: lanes with equal digits are found with
-// eight ballots and their leader adds their number (many equal keys, as in
-// the early rounds of a refinement, would otherwise serialise on one
-// address).  Every lane of the wave must call it; valid = the lane has a key.
-__device__ __forceinline__ void sort_hist_add(u32* h, u32 d, bool valid) {
-    u64 peers = __ballot(valid);
+// Digit histograms in LDS: SORT_HC copies per digit position, copy = lane
+// mod SORT_HC, each padded to 257 words so equal digits of neighbouring lanes
+// fall into different banks; many equal keys (the early rounds of a
+// refinement) then contend on SORT_HC addresses instead of one.  Layout
+// lh[p][copy][257] for up to SORT_HD digit positions per launch.
+constexpr int SORT_HC = 8;
+constexpr int SORT_HD = 4;
+constexpr int SORT_HSTRIDE = 257;
+constexpr int SORT_HWORDS = SORT_HD * SORT_HC * SORT_HSTRIDE;
+__device__ __forceinline__ void sort_hist_add(u32* lh, int p, u32 d) {
+    atomicAdd(&lh[(p * SORT_HC + (int)(threadIdx.x % SORT_HC)) * SORT_HSTRIDE + d], 1u);
+}
+// Adds the copies of digit positions [0, npass) to ghist[p * 256 + d].
+__device__ __forceinline__ void sort_hist_flush(const u32* lh, int npass, u32* ghist) {
+    for (int i = threadIdx.x; i < npass * 256; i += blockDim.x) {
+        const int p = i >> 8, d = i & 255;
+        u32 v = 0;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const u64 bal = __ballot((d >> b) & 1u);
-        peers &= ((d >> b) & 1u) ? bal : ~bal;
+        for (int c = 0; c < SORT_HC; ++c) v += lh[(p * SORT_HC + c) * SORT_HSTRIDE + d];
+        if (v) atomicAdd(&ghist[i], v);
     }
-    if (valid && (int)(__ffsll((long long)peers) - 1) == (int)(threadIdx.x & 63))
-        atomicAdd(&h[d], (u32)__popcll(peers));
 }
 
 // Epoch/ticket bookkeeping shared with the other look-back kernels.

@@ -22,17 +22,18 @@ static_assert(THREADS == RADIX, "one thread per digit in the look-back");
 
 constexpr int HIST_ITEMS = 16;
 
-// Histograms of every 8-bit digit of the sort.
+// Histograms of up to SORT_HD 8-bit digits of the sort (positions
+// [0, npass) from bit_begin) into ghist.
 __global__ __launch_bounds__(THREADS) void hist_kernel(const u64* __restrict__ keys, u64 n,
                                                        int bit_begin, int npass,
                                                        u32* __restrict__ ghist) {
-    __shared__ u32 lh[8][RADIX];
-    for (int i = threadIdx.x; i < 8 * RADIX; i += THREADS) (&lh[0][0])[i] = 0;
+    __shared__ u32 lh[SORT_HWORDS];
+    for (int i = threadIdx.x; i < SORT_HWORDS; i += THREADS) lh[i] = 0;
     __syncthreads();
     // HIST_ITEMS keys per thread per step, all loaded before any is counted
     // (one load in flight per thread would leave the pass latency-bound).
     const u64 stride = (u64)gridDim.x * THREADS * HIST_ITEMS;
-    for (u64 b = (u64)blockIdx.x * THREADS * HIST_ITEMS; b < n; b += stride) {  // uniform
+    for (u64 b = (u64)blockIdx.x * THREADS * HIST_ITEMS; b < n; b += stride) {
         u64 k[HIST_ITEMS];
 #pragma unroll
         for (int j = 0; j < HIST_ITEMS; ++j) {
@@ -41,16 +42,24 @@ __global__ __launch_bounds__(THREADS) void hist_kernel(const u64* __restrict__ k
         }
 #pragma unroll
         for (int j = 0; j < HIST_ITEMS; ++j) {
-            const bool valid = b + (u64)j * THREADS + threadIdx.x < n;
+            if (b + (u64)j * THREADS + threadIdx.x >= n) continue;
             for (int p = 0; p < npass; ++p)
-                sort_hist_add(lh[p], (u32)(k[j] >> (bit_begin + 8 * p)) & 255u, valid);
+                sort_hist_add(lh, p, (u32)(k[j] >> (bit_begin + 8 * p)) & 255u);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < npass * RADIX; i += THREADS) {
-        const u32 v = (&lh[0][0])[i];
-        if (v) atomicAdd(&ghist[i], v);
-    }
+    sort_hist_flush(lh, npass, ghist);
+}
+
+// Histograms of the digits of [bit_begin, bit_begin + 8 * npass) into
+// ghist[p * 256 + d], SORT_HD positions per launch.
+static void launch_hist(const u64* keys, u64 n, int bit_begin, int npass, u32* ghist,
+                        hipStream_t s) {
+    const unsigned blocks =
+        (unsigned)std::min<u64>((n + THREADS * HIST_ITEMS - 1) / (THREADS * HIST_ITEMS), 2048);
+    for (int p0 = 0; p0 < npass; p0 += SORT_HD)
+        hist_kernel<<<blocks, THREADS, 0, s>>>(keys, n, bit_begin + 8 * p0,
+                                               std::min(SORT_HD, npass - p0), ghist + 256 * p0);
 }
 
 // Exclusive scan over a 256-thread block, one value per thread.
@@ -292,10 +301,7 @@ int radix_sort_u64(SortWorkspace* ws, u64* keys[2], u64* vals[2], u64 n, int bit
         return RBSC_ERR_INVALID;
     }
     const int npass = (bit_end - bit_begin) / 8;
-    const unsigned hist_blocks =
-        (unsigned)std::min<u64>((n + THREADS * HIST_ITEMS - 1) / (THREADS * HIST_ITEMS), 2048);
-    if (!hist_ready)
-        hist_kernel<<<hist_blocks, THREADS, 0, s>>>(keys[0], n, bit_begin, npass, ws->hist);
+    if (!hist_ready) launch_hist(keys[0], n, bit_begin, npass, ws->hist, s);
     scan_hist_kernel<<<1, THREADS, 0, s>>>(ws->hist, npass, ws->digit_base);
     int cur = 0;
     for (int p = 0; p < npass; ++p) {
@@ -332,9 +338,7 @@ int radix_sort_nodes(SortWorkspace* ws, const u64* key0, const u32* col, u64* ke
         return RBSC_ERR_INVALID;
     }
     const int npass = (bit_end - bit_begin) / 8;
-    const unsigned hist_blocks =
-        (unsigned)std::min<u64>((n + THREADS * HIST_ITEMS - 1) / (THREADS * HIST_ITEMS), 2048);
-    hist_kernel<<<hist_blocks, THREADS, 0, s>>>(key0, n, bit_begin, npass, ws->hist);
+    launch_hist(key0, n, bit_begin, npass, ws->hist, s);
     scan_hist_kernel<<<1, THREADS, 0, s>>>(ws->hist, npass, ws->digit_base);
     // pass p writes buffer (npass - 1 - p) & 1, so the last pass writes buffer 0
     for (int p = 0; p < npass; ++p) {
