@@ -351,6 +351,7 @@ def main():
     wall = time.perf_counter() - t
     phases = ctx.profile_read(reset=True)
     ctx.profile(False)
+    slow = ctx.stats(reset=True)
     log(f"[bench] timed {args.steps} steps: rounds={rounds} wall={wall:.3f}s device={dev_ms:.1f}ms")
 
     visits = rounds * E
@@ -394,7 +395,7 @@ def main():
         "config": {"workload": args.config, "n_nodes": N, "n_lines": E, "rows": U,
                    "stored_lines": E_st,
                    "parts": args.parts, "hot": args.hot, "bins": bin_info, "fullsort": bool(args.fullsort), "hashed64": bool(args.hashed64), "xcd": bool(args.xcd),
-                   "rounds_per_step": rounds // args.steps,
+                   "rounds_per_step": rounds // args.steps, "node_side_slow_paths": slow,
                    "blocks": r.nodes, "ms_per_round": ms_round},
         "roofline": {"bound": "hbm", "kernel": " + ".join(kernels),
                      "achieved": b_sig / (sig_ms / 1e3) / 1e9, "peak": HBM_PEAK / 1e9,

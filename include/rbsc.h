@@ -218,6 +218,12 @@ int rbsc_graph_bin_info(const rbsc_dgraph* g, uint64_t* info);
  * only the whole run (rbsc.cu:272, 489-490). */
 int rbsc_ctx_profile(rbsc_ctx* ctx, int enable);
 int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int reset);
+/* Counters of the node side's slow paths since the context was made (or
+ * last reset): out[0] rounds whose node side was redone with a full sort
+ * (a tie run list overflowed, or equal signatures out of colour order),
+ * out[1] rounds with tie runs too long for the in-kernel sorts.  No
+ * counterpart in the reference. */
+int rbsc_ctx_stats(rbsc_ctx* ctx, uint64_t* out, int reset);
 
 /* Exact, hash-free stability check of a colouring of g (colors: one u32
  * label per node id, any values).  *bad_out = number of violations: 0 iff

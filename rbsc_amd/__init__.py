@@ -439,6 +439,13 @@ class Context:
         check(load().rbsc_ctx_profile_read(self.handle, ms, n, 1 if reset else 0))
         return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(self.PHASES)}
 
+    def stats(self, reset: bool = False) -> dict:
+        """Node-side slow-path counters: rounds redone with a full sort, and
+        rounds with tie runs too long for the in-kernel sorts."""
+        v = (C.c_uint64 * 2)()
+        check(load().rbsc_ctx_stats(self.handle, v, 1 if reset else 0))
+        return {"full_sort_redos": int(v[0]), "long_run_rounds": int(v[1])}
+
     def refine(self, g: HostGraph, max_batch_edges=0, mode="stream", max_rounds=0,
                verify=False, partition=True, flags=0) -> RefineResult:
         st = g.as_struct()

@@ -164,6 +164,7 @@ SIGNATURES = {
     "rbsc_graph_lines_to_host": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "rbsc_ctx_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), u64p, C.c_int]),
+    "rbsc_ctx_stats": (C.c_int, [C.c_void_p, u64p, C.c_int]),
     "rbsc_graph_check_partition": (C.c_int, [C.c_void_p, C.c_void_p, u32p, C.POINTER(C.c_uint64)]),
     "rbsc_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "rbsc_server_destroy": (C.c_int, [C.c_void_p]),

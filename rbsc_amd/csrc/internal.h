@@ -140,9 +140,11 @@ struct Workspace {
     u64 rank_tiles = 0;
     u64* carry = nullptr;              // per tile: partial signature of its carry row
     u64 carry_cap = 0;
-    u64* tie_runs = nullptr;           // (start, end) of runs to fix after a 32-bit sort
+    u64* tie_runs = nullptr;           // (start, end) of small runs to fix after a 32-bit sort
     u64 tie_cap = 0;
-    u64* tie_pre = nullptr;            // tie_cap + 1: run lengths -> offsets, total
+    u64* tie_runs_m = nullptr;         // mid runs
+    u64* tie_runs_l = nullptr;         // long runs
+    u64* tie_pre = nullptr;            // long runs: lengths -> offsets, total
     u64* tie_tmp = nullptr;            // scan scratch
     u32* tie_mark = nullptr;           // cap_n: epoch stamps of listed run starts
     u32 tie_epoch = 0;
@@ -180,8 +182,11 @@ struct rbsc_ctx {
     Workspace ws;
     u32* d_err = nullptr;
     u32* d_scalar = nullptr;   // [0] = K of the last rank pass, [1] = tie fix-up overflow,
-                               // [2] = tie runs listed, [3] = rank input out of colour order
-    u64 full_sort_redos = 0;   // rounds whose tie fix-up overflowed
+                               // [3] = rank input out of colour order, [8..10] = small, mid
+                               // and long tie runs listed
+    u64 full_sort_redos = 0;   // rounds whose node side was redone (run list overflow, or
+                               // equal signatures out of colour order)
+    u64 long_run_rounds = 0;   // rounds with runs too long for the in-kernel tie sorts
     u32* h_scalar = nullptr;   // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // streaming double buffer

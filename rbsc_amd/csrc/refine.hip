@@ -305,92 +305,219 @@ __global__ __launch_bounds__(256) void gather_keys_hist_kernel(
 // the low bits, stably.  Runs of equal signatures are already in order; only
 // runs holding a "break" (neighbours with equal top bits and different low
 // bits, from distinct signatures colliding in the top bits) need work.
-// tie_runs_kernel finds breaks in one coalesced pass and lists each such run
-// once (bounds by binary search on the sorted top bits; the first break to
-// stamp the run's start with this call's epoch records it).  The members of
-// the listed runs are then gathered into a compact array, sorted stably by
-// low bits and then by run (the same onesweep sort), and scattered back into
-// their runs' places.  More runs than the list holds raise *overflow and the
-// caller redoes a full 64-bit sort.  `lo` is 32 normally; 48 is a
-// verification hook that makes ties frequent.
+// tie_find_kernel finds breaks in one coalesced pass and lists each such run
+// once (bounds by galloping and bisection on the sorted top bits; the first
+// break to stamp the run's start with this call's epoch records it), in one
+// of three lists by length:
+//   small (<= 64 members): tie_small_kernel, one wave per run, a bitonic sort
+//     of (low bits, position in run) across the wave's lanes;
+//   mid (<= TIE_MID members): tie_mid_kernel, one workgroup per run, a
+//     bitonic sort of the same keys in LDS;
+//   long: gathered into a compact array and sorted there with the onesweep
+//     sort (by low bits, then by run), driven by the host.
+// Sorting by (low bits, position) is stable.  A list that overflows raises
+// *overflow and the caller redoes the round's sort on all 64 bits.  `lo` is
+// 32 normally; 48 is a verification hook that makes ties frequent.
 constexpr int TIE_BLK = 256;
-constexpr u32 TIE_LIST = 512;  // runs a workgroup collects in LDS before spilling
+constexpr u32 TIE_LIST = 512;  // small runs a workgroup collects in LDS before spilling
 constexpr unsigned TIE_GRID = 2048;
-__global__ __launch_bounds__(TIE_BLK) void tie_runs_kernel(
+constexpr int TF_ITEMS = 16;
+constexpr u64 TF_TILE = 256 * TF_ITEMS;
+constexpr u32 TIE_SMALL = 64;
+constexpr u32 TIE_MID = 4096;
+constexpr u64 TIE_MID_CAP = 65536;   // mid runs listed per round
+constexpr u64 TIE_LONG_CAP = 4096;   // long runs listed per round
+// counts[0..2]: small, mid, long runs listed.
+__global__ __launch_bounds__(TIE_BLK) void tie_find_kernel(
     const u64* __restrict__ keys, u64 n, u32 lo, u32* __restrict__ mark, u32 epoch,
-    u64* __restrict__ runs, u64 cap, u32* __restrict__ count, u32* __restrict__ overflow) {
-    // Runs are gathered per workgroup in LDS and reserved in the global list
-    // with one atomic per workgroup (one global counter per break would
-    // serialise on a single address).
+    u64* __restrict__ runs_s, u64 cap_s, u64* __restrict__ runs_m, u64* __restrict__ runs_l,
+    u32* __restrict__ counts, u32* __restrict__ overflow) {
+    // Small runs are gathered per workgroup in LDS and reserved in the global
+    // list with one atomic per workgroup.
     __shared__ u64 ls[2 * TIE_LIST];
     __shared__ u32 lcount, lbase;
     if (threadIdx.x == 0) lcount = 0;
     __syncthreads();
-    const u64 stride = (u64)gridDim.x * TIE_BLK;
-    for (u64 i = (u64)blockIdx.x * TIE_BLK + threadIdx.x; i < n; i += stride) {
-        if (i == 0) continue;
-        const u64 k = keys[i], pk = keys[i - 1];
-        const u64 top = k >> lo;
-        if (top != (pk >> lo) || k == pk) continue;  // not a break
-        // Run bounds by galloping outward from the break, then bisection, so
-        // a run of length L costs O(log L) probes.
-        u64 b = i - 1, step = 1;  // keys[b] is in the run
-        u64 a = b;
-        while (a > 0) {
-            const u64 m = a > step ? a - step : 0;
-            if ((keys[m] >> lo) != top) { a = m; break; }
-            b = m;
-            a = m;
-            step <<= 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (u64 t = blockIdx.x; t * TF_TILE < n; t += gridDim.x) {  // uniform per workgroup
+        const u64 wbase = t * TF_TILE + (u64)wave * (64 * TF_ITEMS);
+        u64 k[TF_ITEMS];
+#pragma unroll
+        for (int j = 0; j < TF_ITEMS; ++j) {
+            const u64 i = wbase + 64 * j + lane;
+            k[j] = i < n ? keys[i] : 0;
         }
-        if ((keys[a] >> lo) != top) {  // first index of the run lies in (a, b]
-            ++a;
-            while (a < b) {
-                const u64 m = a + (b - a) / 2;
-                if ((keys[m] >> lo) < top) a = m + 1; else b = m;
+        u64 lk = (wbase > 0 && wbase < n) ? keys[wbase - 1] : 0;
+#pragma unroll
+        for (int j = 0; j < TF_ITEMS; ++j) {
+            u64 pk = __shfl_up(k[j], 1);
+            if (lane == 0) pk = lk;
+            lk = __shfl(k[j], 63);
+            const u64 i = wbase + 64 * j + lane;
+            const u64 top = k[j] >> lo;
+            if (!(i < n && i > 0 && top == (pk >> lo) && k[j] != pk)) continue;  // not a break
+            // Run bounds by galloping outward from the break, then bisection,
+            // so a run of length L costs O(log L) probes.
+            u64 b = i - 1, step = 1;  // keys[b] is in the run
+            u64 a = b;
+            while (a > 0) {
+                const u64 m = a > step ? a - step : 0;
+                if ((keys[m] >> lo) != top) { a = m; break; }
+                b = m;
+                a = m;
+                step <<= 1;
             }
-        }
-        u64 c = i, d;  // keys[c] is in the run; find the first index past it
-        step = 1;
-        for (;;) {
-            const u64 m = c + step;
-            if (m >= n) { d = n; break; }
-            if ((keys[m] >> lo) != top) { d = m; break; }
-            c = m;
-            step <<= 1;
-        }
-        ++c;
-        while (c < d) {
-            const u64 m = c + (d - c) / 2;
-            if ((keys[m] >> lo) <= top) c = m + 1; else d = m;
-        }
-        if (atomicExch(&mark[a], epoch) == epoch) continue;  // run already listed
-        const u32 ls_slot = atomicAdd(&lcount, 1u);
-        if (ls_slot < TIE_LIST) {
-            ls[2 * ls_slot] = a;
-            ls[2 * ls_slot + 1] = c;
-        } else {  // LDS list full: take a global slot directly
-            const u32 slot = atomicAdd(count, 1u);
-            if (slot >= cap) {
-                atomicOr(overflow, 1u);
+            if ((keys[a] >> lo) != top) {  // first index of the run lies in (a, b]
+                ++a;
+                while (a < b) {
+                    const u64 m = a + (b - a) / 2;
+                    if ((keys[m] >> lo) < top) a = m + 1; else b = m;
+                }
+            }
+            u64 c = i, d;  // keys[c] is in the run; find the first index past it
+            step = 1;
+            for (;;) {
+                const u64 m = c + step;
+                if (m >= n) { d = n; break; }
+                if ((keys[m] >> lo) != top) { d = m; break; }
+                c = m;
+                step <<= 1;
+            }
+            ++c;
+            while (c < d) {
+                const u64 m = c + (d - c) / 2;
+                if ((keys[m] >> lo) <= top) c = m + 1; else d = m;
+            }
+            if (atomicExch(&mark[a], epoch) == epoch) continue;  // run already listed
+            const u64 L = c - a;
+            if (L <= TIE_SMALL) {
+                const u32 ls_slot = atomicAdd(&lcount, 1u);
+                if (ls_slot < TIE_LIST) {
+                    ls[2 * ls_slot] = a;
+                    ls[2 * ls_slot + 1] = c;
+                } else {  // LDS list full: take a global slot directly
+                    const u32 slot = atomicAdd(&counts[0], 1u);
+                    if (slot >= cap_s) {
+                        atomicOr(overflow, 1u);
+                    } else {
+                        runs_s[2 * (u64)slot] = a;
+                        runs_s[2 * (u64)slot + 1] = c;
+                    }
+                }
             } else {
-                runs[2 * (u64)slot] = a;
-                runs[2 * (u64)slot + 1] = c;
+                const bool mid = L <= TIE_MID;
+                const u32 slot = atomicAdd(&counts[mid ? 1 : 2], 1u);
+                u64* runs = mid ? runs_m : runs_l;
+                if (slot >= (mid ? TIE_MID_CAP : TIE_LONG_CAP)) {
+                    atomicOr(overflow, 1u);
+                } else {
+                    runs[2 * (u64)slot] = a;
+                    runs[2 * (u64)slot + 1] = c;
+                }
             }
         }
     }
     __syncthreads();
     const u32 m = lcount < TIE_LIST ? lcount : TIE_LIST;
-    if (threadIdx.x == 0) lbase = m ? atomicAdd(count, m) : 0u;
+    if (threadIdx.x == 0) lbase = m ? atomicAdd(&counts[0], m) : 0u;
     __syncthreads();
-    for (u32 k = threadIdx.x; k < m; k += TIE_BLK) {
-        const u64 slot = (u64)lbase + k;
-        if (slot >= cap) {
+    for (u32 q = threadIdx.x; q < m; q += TIE_BLK) {
+        const u64 slot = (u64)lbase + q;
+        if (slot >= cap_s) {
             atomicOr(overflow, 1u);
         } else {
-            runs[2 * slot] = ls[2 * k];
-            runs[2 * slot + 1] = ls[2 * k + 1];
+            runs_s[2 * slot] = ls[2 * q];
+            runs_s[2 * slot + 1] = ls[2 * q + 1];
         }
+    }
+}
+
+// Small runs: wave w sorts runs w, w + waves, ...; lane l holds member l with
+// the key (low bits << 6) | l (lanes past the run: all ones), a bitonic
+// network over the next power of two sorts the keys, and lane m writes the
+// member whose position the m-th key names.
+__global__ __launch_bounds__(256) void tie_small_kernel(u64* __restrict__ keys,
+                                                        u64* __restrict__ P,
+                                                        const u64* __restrict__ runs,
+                                                        const u32* __restrict__ count, u64 cap,
+                                                        u32 lo) {
+    const u32 lane = threadIdx.x & 63;
+    const u64 nr = *count < cap ? *count : cap;
+    const u64 lowmask = (1ULL << lo) - 1;
+    const u64 nw = (u64)gridDim.x * (256 / 64);
+    for (u64 r = ((u64)blockIdx.x * 256 + threadIdx.x) >> 6; r < nr; r += nw) {  // uniform
+        const u64 a = runs[2 * r], c = runs[2 * r + 1];
+        const u32 L = (u32)(c - a);
+        u64 kv = 0, pv = 0, comp = ~0ULL;
+        if (lane < L) {
+            kv = keys[a + lane];
+            pv = P[a + lane];
+            comp = ((kv & lowmask) << 6) | lane;
+        }
+        u32 K = 2;
+        while (K < L) K <<= 1;
+        for (u32 kk = 2; kk <= K; kk <<= 1) {
+            for (u32 j = kk >> 1; j > 0; j >>= 1) {
+                const u64 other = __shfl_xor(comp, (int)j);
+                const bool keep_min = ((lane & kk) == 0) == ((lane & j) == 0);
+                const u64 mn = other < comp ? other : comp;
+                const u64 mx = other < comp ? comp : other;
+                comp = keep_min ? mn : mx;
+            }
+        }
+        const int src = (int)(comp & 63);
+        const u64 ks = __shfl(kv, src), ps = __shfl(pv, src);
+        if (lane < L) {
+            keys[a + lane] = ks;
+            P[a + lane] = ps;
+        }
+    }
+}
+
+// Mid runs: workgroup g sorts runs g, g + grid, ... in LDS; keys
+// (low bits << 12) | position, a bitonic network over the next power of two.
+__global__ __launch_bounds__(256) void tie_mid_kernel(u64* __restrict__ keys, u64* __restrict__ P,
+                                                      const u64* __restrict__ runs,
+                                                      const u32* __restrict__ count, u32 lo) {
+    __shared__ u64 s_comp[TIE_MID], s_k[TIE_MID], s_p[TIE_MID];
+    const u64 nr = *count < TIE_MID_CAP ? *count : TIE_MID_CAP;
+    const u64 lowmask = (1ULL << lo) - 1;
+    for (u64 r = blockIdx.x; r < nr; r += gridDim.x) {  // uniform per workgroup
+        const u64 a = runs[2 * r], c = runs[2 * r + 1];
+        const u32 L = (u32)(c - a);
+        u32 K = 2;
+        while (K < L) K <<= 1;
+        for (u32 i = threadIdx.x; i < K; i += 256) {
+            if (i < L) {
+                const u64 kv = keys[a + i];
+                s_k[i] = kv;
+                s_p[i] = P[a + i];
+                s_comp[i] = ((kv & lowmask) << 12) | i;
+            } else {
+                s_comp[i] = ~0ULL;
+            }
+        }
+        __syncthreads();
+        for (u32 kk = 2; kk <= K; kk <<= 1) {
+            for (u32 j = kk >> 1; j > 0; j >>= 1) {
+                for (u32 i = threadIdx.x; i < K / 2; i += 256) {
+                    const u32 l = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                    const u32 m = l + j;
+                    const u64 x = s_comp[l], y = s_comp[m];
+                    if ((x > y) == ((l & kk) == 0)) {
+                        s_comp[l] = y;
+                        s_comp[m] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (u32 i = threadIdx.x; i < L; i += 256) {
+            const u32 src = (u32)(s_comp[i] & (TIE_MID - 1));
+            keys[a + i] = s_k[src];
+            P[a + i] = s_p[src];
+        }
+        __syncthreads();
     }
 }
 __global__ void tie_len_kernel(const u64* __restrict__ runs, const u32* __restrict__ count, u64 cap,
@@ -655,8 +782,9 @@ void ws_release(rbsc_ctx* ctx) {
     if (w.hc) (void)hipFree(w.hc);
     if (w.hc2) (void)hipFree(w.hc2);
     if (w.rank_status) (void)hipFree(w.rank_status);
-    for (void* p : {(void*)w.tie_runs, (void*)w.tie_pre, (void*)w.tie_tmp, (void*)w.tie_mark,
-                    (void*)w.tie_buf, (void*)w.carry})
+    for (void* p : {(void*)w.tie_runs, (void*)w.tie_runs_m, (void*)w.tie_runs_l,
+                    (void*)w.tie_pre, (void*)w.tie_tmp, (void*)w.tie_mark, (void*)w.tie_buf,
+                    (void*)w.carry})
         if (p) (void)hipFree(p);
     w = Workspace();
     ctx->sort.extra_status = nullptr;
@@ -677,8 +805,10 @@ int ws_reserve(rbsc_ctx* ctx, u64 n) {
     HIP_TRY(hipMalloc(&w.hc2, sizeof(u64) * cap));
     w.tie_cap = cap / 16 + 1024;
     HIP_TRY(hipMalloc(&w.tie_runs, sizeof(u64) * 2 * w.tie_cap));
-    HIP_TRY(hipMalloc(&w.tie_pre, sizeof(u64) * (w.tie_cap + 1)));
-    HIP_TRY(hipMalloc(&w.tie_tmp, sizeof(u64) * scan_tmp_words(w.tie_cap)));
+    HIP_TRY(hipMalloc(&w.tie_runs_m, sizeof(u64) * 2 * TIE_MID_CAP));
+    HIP_TRY(hipMalloc(&w.tie_runs_l, sizeof(u64) * 2 * TIE_LONG_CAP));
+    HIP_TRY(hipMalloc(&w.tie_pre, sizeof(u64) * (TIE_LONG_CAP + 1)));
+    HIP_TRY(hipMalloc(&w.tie_tmp, sizeof(u64) * scan_tmp_words(TIE_LONG_CAP)));
     HIP_TRY(hipMalloc(&w.tie_mark, sizeof(u32) * cap));
     HIP_TRY(hipMemsetAsync(w.tie_mark, 0, sizeof(u32) * cap, ctx->stream));
     w.tie_epoch = 0;
@@ -995,29 +1125,37 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         // sorted_pc is the P buffer holding the sorted order (input of rank).
         int sorted_pc = pc;
         // Orders every run of equal top bits by the low bits (see
-        // tie_runs_kernel); sets d_scalar[1] when the run list overflows.
-        auto fix_ties = [&](u64* keys, u64* P, u32 lo) -> int {
+        // tie_find_kernel): small and mid runs on the device; long runs are
+        // left for fix_long after the round's host sync (d_scalar[10] counts
+        // them).  Sets d_scalar[1] when a run list overflows.
+        const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
+        auto fix_ties = [&](u64* keys, u64* P) {
             const u64 cap = (prm.flags & RBSC_FLAG_TIESMALL) ? 2 : w.tie_cap;
             if (++w.tie_epoch == 0) {  // wrapped: clear the stamps
                 fill_u32_kernel<<<grid_for(w.cap_n), BLK, 0, s>>>(w.tie_mark, w.cap_n, 0);
                 w.tie_epoch = 1;
             }
-            fill_u32<<<1, 1, 0, s>>>(ctx->d_scalar + 2, 1, 0);
-            const unsigned tg = (unsigned)std::min<u64>((n_rank + TIE_BLK - 1) / TIE_BLK, TIE_GRID);
-            tie_runs_kernel<<<tg, TIE_BLK, 0, s>>>(keys, n_rank, lo, w.tie_mark, w.tie_epoch,
-                                                   w.tie_runs, cap, ctx->d_scalar + 2,
-                                                   ctx->d_scalar + 1);
-            tie_len_kernel<<<grid_for(cap), BLK, 0, s>>>(w.tie_runs, ctx->d_scalar + 2, cap,
-                                                         w.tie_pre);
-            int r3 = scan_excl_u64(w.tie_pre, w.tie_pre, cap, w.tie_pre + cap, w.tie_tmp, s);
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
+            const unsigned tg = (unsigned)std::min<u64>((n_rank + TF_TILE - 1) / TF_TILE, TIE_GRID);
+            tie_find_kernel<<<tg, TIE_BLK, 0, s>>>(keys, n_rank, lo, w.tie_mark, w.tie_epoch,
+                                                   w.tie_runs, cap, w.tie_runs_m, w.tie_runs_l,
+                                                   ctx->d_scalar + 8, ctx->d_scalar + 1);
+            tie_small_kernel<<<1024, 256, 0, s>>>(keys, P, w.tie_runs, ctx->d_scalar + 8, cap, lo);
+            tie_mid_kernel<<<256, 256, 0, s>>>(keys, P, w.tie_runs_m, ctx->d_scalar + 9, lo);
+        };
+        // The nl long runs listed this round (host-driven): members gathered
+        // into a compact array, sorted by low bits and then by run (both
+        // stable), scattered back.
+        auto fix_long = [&](u64* keys, u64* P, u32 nl) -> int {
+            tie_len_kernel<<<grid_for(TIE_LONG_CAP), BLK, 0, s>>>(w.tie_runs_l, ctx->d_scalar + 10,
+                                                                  TIE_LONG_CAP, w.tie_pre);
+            int r3 = scan_excl_u64(w.tie_pre, w.tie_pre, TIE_LONG_CAP, w.tie_pre + TIE_LONG_CAP,
+                                   w.tie_tmp, s);
             if (r3) return r3;
             u64 total = 0;
-            HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 1, ctx->d_scalar + 1, 2 * sizeof(u32),
+            HIP_TRY(hipMemcpyAsync(&total, w.tie_pre + TIE_LONG_CAP, sizeof(u64),
                                    hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(&total, w.tie_pre + cap, sizeof(u64), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
-            const u32 nr = ctx->h_scalar[2];
-            if (ctx->h_scalar[1] || nr == 0) return RBSC_OK;  // overflow: the caller re-sorts
             if (total > w.tie_buf_cap) {
                 if (w.tie_buf) HIP_TRY(hipFree(w.tie_buf));
                 w.tie_buf = nullptr;
@@ -1031,7 +1169,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             u64* cv[2] = {w.tie_buf + 2 * C, w.tie_buf + 3 * C};
             u64 *crun = w.tie_buf + 4 * C, *cfull = w.tie_buf + 5 * C, *cP = w.tie_buf + 6 * C,
                 *corig = w.tie_buf + 7 * C;
-            tie_gather_kernel<<<grid_for(total), BLK, 0, s>>>(keys, P, w.tie_runs, w.tie_pre, nr,
+            tie_gather_kernel<<<grid_for(total), BLK, 0, s>>>(keys, P, w.tie_runs_l, w.tie_pre, nl,
                                                              total, lo, cl[0], cv[0], crun, cfull,
                                                              cP, corig);
             int wa = 0;
@@ -1041,7 +1179,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             u64* cw[2] = {cv[wa], cv[wa ^ 1]};
             tie_key2_kernel<<<grid_for(total), BLK, 0, s>>>(cv[wa], crun, total, ck[0]);
             int rb = 8;
-            while (rb < 32 && ((u64)(nr - 1) >> rb)) rb += 8;
+            while (rb < 32 && ((u64)(nl - 1) >> rb)) rb += 8;
             int wb = 0;
             r3 = radix_sort_u64(&ctx->sort, ck, cw, total, 0, rb, &wb, s);  // then by run
             if (r3) return r3;
@@ -1049,8 +1187,23 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                                                                keys, P);
             return RBSC_OK;
         };
+        // Rank input of this round (sorted keys and positions) and the
+        // positions' output (null in node order); kept for a second rank pass
+        // after long runs are fixed.
+        u64 *rk_keys = nullptr, *rk_P = nullptr, *rk_Pout = nullptr;
+        auto launch_rank = [&]() {
+            const u64 tiles = (n_rank + RANK_TILE - 1) / RANK_TILE;
+            const u32 ep = lb_next_epoch(&ctx->sort, s);
+            prof_begin(ctx, 3);
+            rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
+                rk_keys, rk_P, rk_Pout, w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
+                w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
+                ctx->d_scalar + 3, ctx->d_err);
+            rep_colors(w.hc2);
+            prof_end(ctx);
+            ctx->sort.ticket_total += tiles;
+        };
         auto sort_and_rank = [&](bool full, bool gather) -> int {
-            const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
             const int bit0 = full ? 0 : (int)lo;
             // the fused histogram covers at most SORT_HD digits
             const bool fused = gather && (64 - bit0) / 8 <= SORT_HD;
@@ -1071,22 +1224,14 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             prof_begin(ctx, 2);
             int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, bit0, 64, &which, s, fused);
             if (r2) return r2;
-            if (!full) {
-                r2 = fix_ties(kk[which], pp[which], lo);
-                if (r2) return r2;
-            }
+            if (!full) fix_ties(kk[which], pp[which]);
+            else fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
             prof_end(ctx);
             sorted_pc = (pp[which] == w.P[0]) ? 0 : 1;
-            const u64 tiles = (n_rank + RANK_TILE - 1) / RANK_TILE;
-            const u32 ep = lb_next_epoch(&ctx->sort, s);
-            prof_begin(ctx, 3);
-            rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
-                kk[which], pp[which], pp[which ^ 1], w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
-                w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
-                ctx->d_scalar + 3, ctx->d_err);
-            rep_colors(w.hc2);
-            prof_end(ctx);
-            ctx->sort.ticket_total += tiles;
+            rk_keys = kk[which];
+            rk_P = pp[which];
+            rk_Pout = pp[which ^ 1];
+            launch_rank();
             return RBSC_OK;
         };
         // Node order (no initial colours, raw colours): the sort reads the
@@ -1102,26 +1247,17 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         auto sort_and_rank_nodes = [&](bool full) -> int {
             u64* kk[2] = {w.keys[0], w.keys[1]};
             u64* pp[2] = {w.P[0], w.P[1]};
-            const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
             prof_begin(ctx, 2);
             int r2 = radix_sort_nodes(&ctx->sort, w.sig, reinterpret_cast<const u32*>(w.hc), kk,
                                       pp, n_rank, full ? 0 : (int)lo, 64, s);
             if (r2) return r2;
-            if (!full) {
-                r2 = fix_ties(kk[0], pp[0], lo);
-                if (r2) return r2;
-            }
+            if (!full) fix_ties(kk[0], pp[0]);
+            else fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
             prof_end(ctx);
-            const u64 tiles = (n_rank + RANK_TILE - 1) / RANK_TILE;
-            const u32 ep = lb_next_epoch(&ctx->sort, s);
-            prof_begin(ctx, 3);
-            rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
-                kk[0], pp[0], nullptr, w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
-                w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
-                ctx->d_scalar + 3, ctx->d_err);
-            rep_colors(w.hc2);
-            prof_end(ctx);
-            ctx->sort.ticket_total += tiles;
+            rk_keys = kk[0];
+            rk_P = pp[0];
+            rk_Pout = nullptr;
+            launch_rank();
             return RBSC_OK;
         };
         const bool full_sort = (prm.flags & RBSC_FLAG_FULLSORT) != 0;
@@ -1130,12 +1266,28 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             if (rc) return rc;
         }
         HIP_TRY(hipGetLastError());
-        if (n_rank)
-            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 4 * sizeof(u32),
+        auto read_scalars = [&]() -> int {
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 12 * sizeof(u32),
                                    hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 4, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 4, ctx->d_err, sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            return RBSC_OK;
+        };
+        if ((rc = read_scalars())) return rc;
         if (ctx->h_scalar[4]) return check_kernel_errors(ctx);  // K of a failed round is void
+        if (n_rank && !ctx->h_scalar[1] && ctx->h_scalar[10]) {
+            // Runs too long for the in-kernel sorts: sort them in a compact
+            // array and rank again (the rank pass is a pure function of the
+            // sorted arrays).
+            ++ctx->long_run_rounds;
+            rc = fix_long(rk_keys, rk_P, ctx->h_scalar[10]);
+            if (rc) return rc;
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 3, 1, 0);
+            launch_rank();
+            if ((rc = read_scalars())) return rc;
+            if (ctx->h_scalar[4]) return check_kernel_errors(ctx);
+        }
         if (n_rank && node_order && (ctx->h_scalar[1] || ctx->h_scalar[3])) {
             // Run list overflow or pairs out of colour order: redo the node
             // side from positions sorted by the (unchanged) old colours, with
@@ -1152,26 +1304,24 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             pc = wc;
             rc = sort_and_rank(true, true);
             if (rc) return rc;
-            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 4 * sizeof(u32),
-                                   hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
+            if ((rc = read_scalars())) return rc;
+            if (ctx->h_scalar[4]) return check_kernel_errors(ctx);
             if (ctx->h_scalar[1] || ctx->h_scalar[3]) {
                 rbsc_set_error("node side redo left pairs out of order");
                 return RBSC_ERR_KERNEL;
             }
         } else if (n_rank && ctx->h_scalar[1]) {
-            // A run of signatures equal in their top 32 bits was too long for
-            // the fix-up: sort this round's (signature, P) again on all 64 bits.
-            // Inside equal signatures the arrays still hold colour order, so
-            // the full stable sort gives (signature, colour) order.
+            // A run list overflowed: sort this round's (signature, P) again
+            // on all 64 bits.  Inside equal signatures the arrays still hold
+            // colour order, so the full stable sort gives (signature, colour)
+            // order.
             ++ctx->full_sort_redos;
             fill_u32<<<1, 1, 0, s>>>(ctx->d_scalar + 1, 1, 0);
             pc = sorted_pc;
             rc = sort_and_rank(true, false);
             if (rc) return rc;
-            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(u32),
-                                   hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipStreamSynchronize(s));
+            if ((rc = read_scalars())) return rc;
+            if (ctx->h_scalar[4]) return check_kernel_errors(ctx);
         }
         if (n_rank) pc = sorted_pc ^ 1;
         std::swap(w.hc, w.hc2);  // next round reads the colours ranked here
@@ -1637,6 +1787,17 @@ extern "C" int rbsc_ctx_profile(rbsc_ctx* ctx, int enable) {
     ctx->prof = enable != 0;
     ctx->prof_used = 0;
     ctx->prof_phase.clear();
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_ctx_stats(rbsc_ctx* ctx, uint64_t* out, int reset) {
+    if (!ctx || !out) {
+        rbsc_set_error("rbsc_ctx_stats: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    out[0] = ctx->full_sort_redos;
+    out[1] = ctx->long_run_rounds;
+    if (reset) ctx->full_sort_redos = ctx->long_run_rounds = 0;
     return RBSC_OK;
 }
 

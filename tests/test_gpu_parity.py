@@ -359,3 +359,30 @@ def test_sort_split_and_tie_fixup(ctx, flags):
             res = dg.refine(max_rounds=r, partition=False, colors=True, flags=flags)
             assert np.array_equal(res.colors, tr[r - 1]), f"flags={flags} round {r}"
         dg.free()
+
+
+@pytest.mark.parametrize("flags", [8, 8 | 128])
+def test_tie_runs_of_every_length(ctx, flags):
+    """A class of 6000 rows with one signature (each row one line into its own
+    line-less node) beside an R-MAT with many distinct signatures: under the
+    16-bit split (8) the class's run collides with other signatures, so the
+    long-run path (sorted in a compact array, rank redone) runs next to the
+    in-kernel small and mid sorts; every round equals the oracle's, in node
+    order and in colour order (128)."""
+    base = gen.rmat(16, 1 << 20, clone_frac=0.1, seed=41)
+    n0 = base.n_nodes
+    src = [base.sources()]
+    dst = [base.dst]
+    k = 6000
+    src.append(np.arange(n0, n0 + k, dtype=np.uint32))
+    dst.append(np.arange(n0 + k, n0 + 2 * k, dtype=np.uint32))
+    g = HostGraph(n0 + 2 * k, np.concatenate(dst), src=np.concatenate(src))
+    c, K, R, tr = _oracle_full(g, trace_rounds=64)
+    dg = ctx.upload(g)
+    ctx.stats(reset=True)
+    for r in range(1, R + 1):
+        res = dg.refine(max_rounds=r, partition=False, colors=True, flags=flags)
+        assert np.array_equal(res.colors, tr[r - 1]), f"flags={flags} round {r}"
+    st = ctx.stats(reset=True)
+    assert st["long_run_rounds"] > 0, st
+    dg.free()
