@@ -20,8 +20,19 @@ struct SortWorkspace {
     u64 extra_words = 0;
 };
 
-// Number of keys per onesweep tile.
-constexpr u64 SORT_TILE = 4096;
+// Keys per thread and per onesweep tile (256 threads).  RBSC_SORT_ITEMS and
+// RBSC_SORT_OCC (workgroups per CU asked of the compiler) are build-time
+// tuning knobs.
+// 24 x 256 = 6144-key tiles, two workgroups per CU (62.5 KB of LDS, 248
+// VGPRs): longer digit runs per tile than 4096-key tiles (write runs of 24
+// keys on average) and no spills; 28 and more keys per thread spill.
+#ifndef RBSC_SORT_ITEMS
+#define RBSC_SORT_ITEMS 24
+#endif
+#ifndef RBSC_SORT_OCC
+#define RBSC_SORT_OCC 2
+#endif
+constexpr u64 SORT_TILE = 256 * RBSC_SORT_ITEMS;
 
 int sort_ws_init(SortWorkspace* ws, u64 max_n, u32* err_flag, hipStream_t s);
 void sort_ws_free(SortWorkspace* ws);

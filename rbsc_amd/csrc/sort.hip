@@ -1,7 +1,7 @@
 // sort.hip -- onesweep LSD radix sort for gfx950 (64-wide waves, LDS-staged
 // scatter, decoupled look-back with epoch-stamped status words).
 //
-// Per pass, one workgroup of 256 threads takes a 4096-key tile through an
+// Per pass, one workgroup of 256 threads takes a SORT_TILE-key tile through an
 // atomic ticket (so every tile it waits on is already resident), ranks its
 // keys stably with wave ballots, publishes its per-digit counts, looks back
 // over earlier tiles for its per-digit global offset, reorders the tile in
@@ -14,7 +14,7 @@ namespace {
 
 constexpr int THREADS = 256;
 constexpr int WAVES = THREADS / RBSC_WAVE;
-constexpr int ITEMS = 16;
+constexpr int ITEMS = RBSC_SORT_ITEMS;
 constexpr int TILE = THREADS * ITEMS;  // == SORT_TILE
 constexpr int RADIX = 256;
 static_assert(TILE == (int)SORT_TILE, "tile size mismatch");
@@ -106,7 +106,7 @@ __global__ void fill_u64_kernel(u64* p, u64 n, u64 v) {
 // position, vin_col[i] << 32 | i (the first pass of a sort of nodes in id
 // order, values (colour << 32) | node).
 template <int VM>
-__global__ __launch_bounds__(THREADS, 3) void onesweep_kernel(
+__global__ __launch_bounds__(THREADS, RBSC_SORT_OCC) void onesweep_kernel(
     const u64* __restrict__ kin, const u64* __restrict__ vin, u64* __restrict__ kout,
     u64* __restrict__ vout, u64 n, u32 shift, const u64* __restrict__ digit_base,
     u64* status, u64* ticket, u64 ticket_base, u32 epoch, u32* err) {
@@ -216,15 +216,15 @@ __global__ __launch_bounds__(THREADS, 3) void onesweep_kernel(
     __syncthreads();
 
     // Stage keys in LDS by digit, then write digit runs.
-    u32 lpos[ITEMS];
+    // rank[j] becomes the item's LDS position (one register array for both)
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
         if (idx < n) {
             const u32 dj = (u32)(key[j] >> shift) & 255u;
-            lpos[j] = s_tstart[dj] + s_whist[wave][dj] + rank[j];
-            s_stage[lpos[j]] = key[j];
-            s_digit[lpos[j]] = (unsigned char)dj;
+            rank[j] += s_tstart[dj] + s_whist[wave][dj];
+            s_stage[rank[j]] = key[j];
+            s_digit[rank[j]] = (unsigned char)dj;
         }
     }
     __syncthreads();
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(THREADS, 3) void onesweep_kernel(
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
-            if (idx < n) s_stage[lpos[j]] = val[j];
+            if (idx < n) s_stage[rank[j]] = val[j];
         }
         __syncthreads();
         for (u32 i = tid; i < tile_n; i += THREADS) vout[s_gbase[s_digit[i]] + i] = s_stage[i];
