@@ -102,12 +102,12 @@ def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
         f"built and uploaded in {time.perf_counter() - t0:.1f}s")
     threads, wgs, tune_ms = db.autotune()
     log(f"[bench] batch autotune: {threads} threads x {wgs} workgroups ({tune_ms:.1f} ms)")
-    db.refine(labels=True)
+    db.refine(labels=True, pinned=True)
     reps = 5
     t = time.perf_counter()
     dev = 0.0
     for _ in range(reps):
-        r = db.refine(labels=True)
+        r = db.refine(labels=True, pinned=True)
         dev += r.total.device_ms
     wall = (time.perf_counter() - t) / reps
     blocks = int(r.blocks.sum())

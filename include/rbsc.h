@@ -123,6 +123,14 @@ const char* rbsc_last_error(void);
 /* Library version string. */
 const char* rbsc_version(void);
 
+/* Page-locked host memory for callers' output buffers (labels, colours,
+ * partitions): device-to-host copies into it run at the link's rate, where
+ * copies into pageable memory are staged through the runtime's own pinned
+ * buffers.  No counterpart in the reference (its outputs are host vectors,
+ * rbsc.cu:427-445). */
+int rbsc_host_alloc(uint64_t bytes, void** out);
+int rbsc_host_free(void* p);
+
 /* Context on HIP device `device`; hbm_budget = 0 means no budget.  Replaces
  * the per-GPU setup of rbsc.cu:234-270 (streams, scratch, CUB temp). */
 int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out);

@@ -27,6 +27,7 @@ from ._lib import (FLAG_COLOUR_ORDER, FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS,
                    RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
+           "PinnedArray",
            "Server", "RequestResult",
            "RefineResult", "RbscError", "load",
            "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "FLAG_COLOUR_ORDER", "gen"]
@@ -355,6 +356,29 @@ class BatchResult:
     total: RefineResult
 
 
+class PinnedArray:
+    """A numpy uint32 array over page-locked host memory (``rbsc_host_alloc``):
+    device-to-host copies into it are not staged through pageable memory."""
+
+    def __init__(self, n: int):
+        self._p = C.c_void_p()
+        check(load().rbsc_host_alloc(4 * max(int(n), 1), C.byref(self._p)))
+        buf = (C.c_uint32 * max(int(n), 1)).from_address(self._p.value)
+        self.array = np.ctypeslib.as_array(buf)
+
+    def free(self):
+        if self._p:
+            self.array = None
+            load().rbsc_host_free(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class DeviceBatch:
     """A batch resident in HBM (``rbsc_dbatch``)."""
 
@@ -363,11 +387,20 @@ class DeviceBatch:
         self.handle = handle
         self.n_graphs = host.n_graphs
         self.n_nodes = host.n_nodes
+        self._pinned = None
 
-    def refine(self, max_rounds=0, labels=True, colors=False) -> BatchResult:
+    def refine(self, max_rounds=0, labels=True, colors=False, pinned=False) -> BatchResult:
+        """pinned: download the labels into page-locked memory owned by the
+        batch (faster); the returned labels array is then reused, and
+        overwritten, by the next pinned refine of this batch."""
         blocks = np.empty(max(self.n_graphs, 1), np.uint32)
         rounds = np.empty(max(self.n_graphs, 1), np.uint32)
-        lab = np.empty(max(self.n_nodes, 1), np.uint32) if labels else None
+        if labels and pinned:
+            if self._pinned is None:
+                self._pinned = PinnedArray(self.n_nodes)
+            lab = self._pinned.array
+        else:
+            lab = np.empty(max(self.n_nodes, 1), np.uint32) if labels else None
         col = np.empty(max(self.n_nodes, 1), np.uint32) if colors else None
         r = _lib.Result()
         o = _opts(max_rounds=max_rounds)
@@ -390,6 +423,9 @@ class DeviceBatch:
         check(load().rbsc_batch_set_launch(self.ctx.handle, self.handle, threads, wgs))
 
     def free(self):
+        if self._pinned is not None:
+            self._pinned.free()
+            self._pinned = None
         if self.handle:
             load().rbsc_batch_free(self.ctx.handle, self.handle)
             self.handle = None

@@ -153,6 +153,8 @@ PENDING = 1
 SIGNATURES = {
     "rbsc_last_error": (C.c_char_p, []),
     "rbsc_version": (C.c_char_p, []),
+    "rbsc_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "rbsc_host_free": (C.c_int, [C.c_void_p]),
     "rbsc_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     "rbsc_ctx_destroy": (C.c_int, [C.c_void_p]),
     "rbsc_refine": (C.c_int, [C.c_void_p, C.POINTER(Graph), C.POINTER(Opts), C.POINTER(Result), u32p]),

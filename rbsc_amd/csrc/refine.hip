@@ -42,6 +42,27 @@ extern "C" void rbsc_set_error(const char* fmt, ...) {
 extern "C" const char* rbsc_last_error(void) { return g_err; }
 extern "C" const char* rbsc_version(void) { return "rbsc-gfx950 0.1"; }
 
+extern "C" int rbsc_host_alloc(uint64_t bytes, void** out) {
+    if (!out) {
+        rbsc_set_error("rbsc_host_alloc: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    *out = nullptr;
+    const hipError_t e = hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        rbsc_set_error("hipHostMalloc of %llu bytes failed: %s", (unsigned long long)bytes,
+                       hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;
+    }
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_host_free(void* p) {
+    if (p) HIP_TRY(hipHostFree(p));
+    return RBSC_OK;
+}
+
 namespace {
 
 constexpr int BLK = 256;

@@ -71,6 +71,21 @@ def test_batch_mixed_matches_oracle(ctx):
     db.free()
 
 
+def test_batch_labels_into_pinned_memory(ctx):
+    # labels downloaded into page-locked memory owned by the batch equal the
+    # pageable download; the pinned array is reused by the next pinned call
+    b = gen.batch_mixed(300, 10, 5000, seed=22)
+    db = ctx.upload_batch(b)
+    ref = db.refine(labels=True)
+    pin = db.refine(labels=True, pinned=True)
+    assert np.array_equal(pin.labels, ref.labels)
+    assert np.array_equal(pin.blocks, ref.blocks)
+    first = pin.labels.copy()
+    again = db.refine(labels=True, pinned=True)
+    assert np.array_equal(again.labels, first)
+    db.free()
+
+
 def test_batch_directed_sizes_and_edges(ctx):
     gs = _mixed_graphs(3)
     gs += [HostGraph(0, np.zeros(0, np.uint32), src=np.zeros(0, np.uint32)),
