@@ -389,7 +389,8 @@ class DeviceBatch:
         self.n_nodes = host.n_nodes
         self._pinned = None
 
-    def refine(self, max_rounds=0, labels=True, colors=False, pinned=False) -> BatchResult:
+    def refine(self, max_rounds=0, labels=True, colors=False, pinned=False,
+               flags=0) -> BatchResult:
         """pinned: download the labels into page-locked memory owned by the
         batch (faster); the returned labels array is then reused, and
         overwritten, by the next pinned refine of this batch."""
@@ -403,7 +404,7 @@ class DeviceBatch:
             lab = np.empty(max(self.n_nodes, 1), np.uint32) if labels else None
         col = np.empty(max(self.n_nodes, 1), np.uint32) if colors else None
         r = _lib.Result()
-        o = _opts(max_rounds=max_rounds)
+        o = _opts(max_rounds=max_rounds, flags=flags)
         check(load().rbsc_batch_refine(self.ctx.handle, self.handle, C.byref(o),
                                        _ptr(blocks, C.c_uint32), _ptr(rounds, C.c_uint32),
                                        _ptr(lab, C.c_uint32), _ptr(col, C.c_uint32), C.byref(r)))

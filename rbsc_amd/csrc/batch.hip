@@ -2,7 +2,8 @@
 //
 // Each workgroup takes graphs from an atomic queue (largest first) and runs
 // the whole refinement of one graph inside LDS: colours, signatures, a
-// stable LDS radix sort of (signature) over positions kept in colour order,
+// stable LDS radix sort of the top 32 signature bits over positions kept in
+// colour order (the rare runs colliding in those bits ordered afterwards),
 // the dense rank and the fixpoint test, then the first-occurrence canonical
 // labels.  Every graph gets exactly the colours the single-graph path gives
 // it alone (same round function, same dense ranks, same stop rule), so a
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
     const u32* __restrict__ rp_all, const unsigned short* __restrict__ dst_all,
     const u64* __restrict__ wt_all, const u32* __restrict__ init_all, u32 max_rounds,
     u32* __restrict__ labels_out, u32* __restrict__ colors_out, u32* __restrict__ blocks_out,
-    u32* __restrict__ rounds_out, u32* __restrict__ queue) {
+    u32* __restrict__ rounds_out, u32* __restrict__ queue, int split_lo) {
     constexpr int IPT = Cfg<BT>::IPT;
     __shared__ Lds<BT> s;
     // Every job runs the whole body: no `continue` back to the thread-0 queue
@@ -233,8 +234,38 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
             }
             __syncthreads();
             for (int i = threadIdx.x; i < n; i += BT) s.key[0][i] = s.key[1][s.val[0][i] & 0xFFFFu];
+            if (threadIdx.x == 0) s.scal[1] = 0;
             __syncthreads();
-            lds_sort(&s, n, 0, 64);
+            // Stable sort by the top 32 signature bits (4 passes; split_lo
+            // is 32, or 48 as a verification hook that makes collisions
+            // common), then neighbours equal there and different below --
+            // distinct signatures colliding in 32 bits, about n^2 / 2^33
+            // pairs a round -- are put in order by thread 0 with a stable
+            // insertion pass over the full keys (inversions exist only
+            // inside such runs).
+            lds_sort(&s, n, split_lo, 64);
+            for (int i = threadIdx.x + 1; i < n; i += BT) {
+                const u64 a = s.key[0][i - 1], b = s.key[0][i];
+                if ((a >> split_lo) == (b >> split_lo) && a != b) s.scal[1] = 1;
+            }
+            __syncthreads();
+            if (s.scal[1]) {  // uniform
+                if (threadIdx.x == 0) {
+                    for (int i = 1; i < n; ++i) {
+                        const u64 k = s.key[0][i];
+                        const u32 v = s.val[0][i];
+                        int j = i;
+                        while (j > 0 && s.key[0][j - 1] > k) {
+                            s.key[0][j] = s.key[0][j - 1];
+                            s.val[0][j] = s.val[0][j - 1];
+                            --j;
+                        }
+                        s.key[0][j] = k;
+                        s.val[0][j] = v;
+                    }
+                }
+                __syncthreads();
+            }
             // dense rank of (signature, colour)
             u32 f[IPT];
             u32 cnt = 0;
@@ -309,11 +340,13 @@ __global__ void zero_queue(u32* q) { *q = 0; }
 
 template <int BT>
 static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 max_rounds,
-                      u32* d_labels, u32* d_colors, u32* d_blocks, u32* d_rounds, u32* d_queue) {
+                      u32* d_labels, u32* d_colors, u32* d_blocks, u32* d_rounds, u32* d_queue,
+                      int split_lo) {
 #define BL(W, I)                                                                        \
     batch_refine_kernel<BT, W, I><<<grid, BT, 0, s>>>(b.desc, b.order, b.n_small, b.rp, b.dst, \
                                                       b.wt, b.init, max_rounds, d_labels,       \
-                                                      d_colors, d_blocks, d_rounds, d_queue)
+                                                      d_colors, d_blocks, d_rounds, d_queue,    \
+                                                      split_lo)
     if (b.wt) {
         if (b.init) BL(true, true); else BL(true, false);
     } else {
@@ -325,15 +358,15 @@ static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 ma
 }  // namespace
 
 int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
-                 u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg) {
+                 u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg, int split_lo) {
     hipStream_t s = ctx->stream;
     zero_queue<<<1, 1, 0, s>>>(d_queue);
     if (b.n_small == 0) return RBSC_OK;
     const unsigned grid = (unsigned)std::max(1, std::min<int>(n_wg, (int)b.n_small));
     switch (threads) {
-        case 256: launch_bt<256>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue); break;
-        case 1024: launch_bt<1024>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue); break;
-        default: launch_bt<512>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue); break;
+        case 256: launch_bt<256>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
+        case 1024: launch_bt<1024>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
+        default: launch_bt<512>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
     }
     HIP_TRY(hipGetLastError());
     return RBSC_OK;
@@ -526,8 +559,9 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
     const u32 max_rounds = opts ? opts->max_rounds : 0;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ctx->ev0, s));
+    const int split_lo = (opts && (opts->flags & RBSC_FLAG_SPLIT16)) ? 48 : 32;
     int rc = batch_launch(ctx, b->dev, max_rounds, b->d_labels, b->d_colors, b->d_blocks,
-                          b->d_rounds, b->d_queue, threads, wgs);
+                          b->d_rounds, b->d_queue, threads, wgs, split_lo);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev1, s));
     const u32 G = b->n_graphs;

@@ -71,6 +71,19 @@ def test_batch_mixed_matches_oracle(ctx):
     db.free()
 
 
+def test_batch_sort_collisions_fixed(ctx):
+    # With the 16-bit split hook (flag 8) neighbours equal in the sorted
+    # bits and different below are common; the in-LDS fix-up must still give
+    # every graph exactly the oracle's colours.
+    b = gen.batch_mixed(200, 10, 5000, seed=23)
+    db = ctx.upload_batch(b)
+    res = db.refine(labels=True, colors=True, flags=8)
+    _check_against_oracle(b, res)
+    ref = db.refine(labels=True, colors=True)
+    assert np.array_equal(res.colors, ref.colors)
+    db.free()
+
+
 def test_batch_labels_into_pinned_memory(ctx):
     # labels downloaded into page-locked memory owned by the batch equal the
     # pageable download; the pinned array is reused by the next pinned call
