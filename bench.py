@@ -137,7 +137,7 @@ def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
 def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int) -> dict:
     """Poisson arrivals of refinement requests over a pool of R-MAT graphs
     (E log-uniform in [1e3, 5e8], N = E/16, colour 0): small graphs as host
-    graphs (batched), larger ones resident in HBM.  Reports p50/p99 latency
+    graphs (batched), larger ones resident in HBM (binned from 2^22 lines).  Reports p50/p99 latency
     (arrival -> result) and graphs/s per rate, against a plain loop that
     refines the same requests one after another, and checks that every
     served partition equals the loop's."""
@@ -154,6 +154,8 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int) -> dict:
             pool.append(("host", gen.rmat(scale, int(e), seed=seed + k)))
         else:
             dg = ctx.gen_rmat(scale, int(e), 0.57, 0.19, 0.19, 0.0, seed + k)
+            if e >= (1 << 22):  # served many times: binned once, like the main workload
+                dg.bin()
             pool_bytes += dg.nbytes
             pool.append(("resident", dg))
     log(f"[bench] serving pool: {pool_size} graphs, {int(E.sum())} lines, "

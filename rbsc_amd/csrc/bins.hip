@@ -8,25 +8,29 @@
 // (rbsc.cu:80-87 set_values + rbsc.cu:366-377 ReduceByKey) in two passes:
 //
 //   cold_scatter_kernel  writes each cold line's colour (H <= head < gather
-//                        limit) into its slot.  Bins are split into 8 groups,
-//                        one per XCD; a group's cold lines, in head order, are
-//                        cut into chunks of a fixed number of lines, one
-//                        workgroup per chunk (chunk c of group g is workgroup
-//                        8c + g: workgroups are dealt to the XCDs round
-//                        robin).  A chunk spans a narrow head range, so its
-//                        colour reads stay in cache; inside the chunk the
-//                        lines are kept in slot order (default) -- a bin's
-//                        cold slots are in head order, so a chunk's lines for
-//                        one bin fill one run of consecutive slots and
-//                        neighbouring lines write neighbouring words -- or in
-//                        head order (RBSC_BIN_HEAD_ORDER: colour reads in
-//                        order, stores scattered).
+//                        limit) into the refinement's cold-colour buffer, at
+//                        the line's slot position.  Bins are split into 8
+//                        groups, one per XCD; a group's cold lines, in head
+//                        order, are cut into chunks of a fixed number of
+//                        lines, one workgroup per chunk (chunk c of group g is
+//                        workgroup 8c + g: workgroups are dealt to the XCDs
+//                        round robin).  A chunk spans a narrow head range, so
+//                        its colour reads stay in cache; inside the chunk the
+//                        lines stay in head order (default: colour reads in
+//                        order, stores scattered) or are put in slot order
+//                        (RBSC_BIN_SLOT_ORDER: a bin's cold slots are in head
+//                        order, so a chunk's lines for one bin fill one run of
+//                        consecutive positions).
 //   bin_sig_kernel       one workgroup per chunk of a bin of 2^log_s rows:
 //                        row sums in LDS, hot slots (head < H) gathered from
 //                        the L2-resident hot colours in head order (a wave's
 //                        64 gathers fall into few cache lines), cold slots
-//                        read as written, lines into heads that are never
+//                        read from the buffer, lines into heads that are never
 //                        gathered folded in per row as dang_h * dsum[row].
+//
+// The layout is read-only during refinement: the per-round cold colours
+// live in the refinement's context, so several contexts may refine one
+// binned graph at the same time.
 //
 // Only the order of the additions differs from the plain kernel, and they are
 // exact in Z/2^64: signatures are identical.
@@ -144,8 +148,9 @@ __global__ void place_hot_kernel(const u64* __restrict__ keys, const u64* __rest
 
 // Sorted cold records into their slots (after the bin's hot slots), and the
 // scatter records: key (group, head), value slot << 32 | head.
-// With cold_pos (hashed layout) a record's target is its cold_val index
-// instead of its slot.
+// A record's target is its index in a refinement's cold-colour buffer
+// (cold_pos[bin] + slot - the bin's first cold slot rounded down to 8), or
+// its slot when cold_pos is null.
 template <bool HAS_WT>
 __global__ void place_cold_kernel(const u64* __restrict__ keys, const u64* __restrict__ vals,
                                   u64 n, int gb, const u64* __restrict__ bin_slot,
@@ -210,11 +215,11 @@ typedef u64 u64x2 __attribute__((ext_vector_type(2)));
 // are read 8 at a time with 16-byte loads: thread i takes the aligned groups
 // [8q, 8q + 8) for q = lo / 8 + i, + BLK, ...; records outside the tile (a
 // neighbour's, or the zeroed padding past the end) are loaded, and their
-// heads (valid ids or 0) read, but never stored.  On the raw layout the
-// colour is the u32 raw colour and the target a slot of slot_key; on the
-// hashed layout it is the u64 hashed colour and the target an index of
-// cold_val.  (Lanes taking 64 consecutive records per load instruction
-// instead measured slower: the stores scatter more.)
+// heads (valid ids or 0) read, but never stored.  The target is the line's
+// index in the refinement's cold-colour buffer; the colour is the u32 raw
+// colour on the raw layout, the u64 hashed colour on the hashed layout.
+// (Lanes taking 64 consecutive records per load instruction instead
+// measured slower: the stores scatter more.)
 template <typename C>
 __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict__ csc_head,
                                                            const u32* __restrict__ csc_slot,
@@ -259,18 +264,20 @@ __global__ void bin_zero_kernel(const u32* __restrict__ zero_bins, u64 n_zero, u
 // slots per load instruction, so one gather instruction reads 64
 // neighbouring heads, measured no faster.)
 //
-// Raw layout (HASHED false): col is u32 raw colours, cold slots hold raw
-// colours in slot_key, and every line hashes randomize(c + salt).
-// Hashed layout: col is the round's u64 hashed colours, cold slots hold
-// hashed colours in cold_val (index s + cbase, aligned groups of 8 stay
-// aligned), and no line hashes.
+// Cold slots take their colours from the refinement's cold-colour buffer
+// (index s + cbase; aligned groups of 8 stay aligned), which the scatter
+// fills every round; the graph's own arrays are only read, so several
+// refinements of one graph may run at once.
+// Raw layout (HASHED false): col and the buffer hold u32 raw colours, and
+// every line hashes randomize(c + salt).  Hashed layout: col and the buffer
+// hold the round's u64 hashed colours, and no line hashes.
 template <bool HAS_WT, bool HASHED>
 __device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0, u64 cbase,
                                                const u32* __restrict__ slot_key,
                                                const unsigned short* __restrict__ slot_row,
                                                const u64* __restrict__ slot_wt,
                                                const void* __restrict__ colv,
-                                               const u64* __restrict__ cold_val, u64 salt,
+                                               const void* __restrict__ cold_buf, u64 salt,
                                                u64* sums) {
     const u64 q1 = (e + 7) / 8;
     for (u64 q = a / 8 + threadIdx.x; q < q1; q += SIG_THREADS) {
@@ -290,7 +297,8 @@ __device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0, u64 cbas
             // Cold slots of this chunk in the group: [max(g0, cold0), min(g0 + 8, e)).
             const u64 c_lo = g0 > cold0 ? g0 : cold0;
             if (c_lo < e && c_lo < g0 + 8) {
-                const u64x2* cv = reinterpret_cast<const u64x2*>(cold_val + (g0 + cbase));
+                const u64x2* cv =
+                    reinterpret_cast<const u64x2*>(static_cast<const u64*>(cold_buf) + (g0 + cbase));
                 const u64x2 v0 = cv[0], v1 = cv[1], v2 = cv[2], v3 = cv[3];
                 const u64 c8[8] = {v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, v3.x, v3.y};
 #pragma unroll
@@ -302,6 +310,18 @@ __device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0, u64 cbas
             const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + g0);
             const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + g0 + 4);
             u32 key[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+            // Cold slots of this chunk in the group: [max(g0, cold0), min(g0 + 8, e)),
+            // colours from the refinement's buffer (index s + cbase).
+            const u64 c_lo = g0 > cold0 ? g0 : cold0;
+            if (c_lo < e && c_lo < g0 + 8) {
+                const u32* cc = static_cast<const u32*>(cold_buf) + (g0 + cbase);
+                const u32x4 ca = *reinterpret_cast<const u32x4*>(cc);
+                const u32x4 cb = *reinterpret_cast<const u32x4*>(cc + 4);
+                const u32 c8[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (g0 + k >= cold0) key[k] = c8[k];
+            }
             if (g0 < cold0) {  // the group holds hot slots
                 u32 g[8];
 #pragma unroll
@@ -330,7 +350,7 @@ __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
     const BinChunk* __restrict__ chunks, const u64* __restrict__ bin_slot,
     const u64* __restrict__ bin_hot, const u64* __restrict__ cold_pos,
     const u32* __restrict__ slot_key, const unsigned short* __restrict__ slot_row,
-    const u64* __restrict__ slot_wt, const u64* __restrict__ cold_val,
+    const u64* __restrict__ slot_wt, const void* __restrict__ cold_buf,
     const u64* __restrict__ dsum, const void* __restrict__ col, u64 salt, u64 dang_h, u64 R,
     u64* __restrict__ sig) {
     constexpr u32 S = 1u << LOG_S;
@@ -343,11 +363,11 @@ __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
         sums[i] = (first && i < nrows) ? dang_h * dsum[row0 + i] : 0ull;
     __syncthreads();
     const u64 cold0 = bin_slot[ch.bin] + bin_hot[ch.bin];
-    // cold_val index of slot s: cold_pos + s - (cold0 rounded down to 8), in
-    // Z/2^64 (only ever applied to the bin's cold groups).
-    const u64 cbase = HASHED ? cold_pos[ch.bin] - (cold0 & ~7ull) : 0;
+    // cold-buffer index of slot s: cold_pos + s - (cold0 rounded down to 8),
+    // in Z/2^64 (only ever applied to the bin's cold groups).
+    const u64 cbase = cold_pos[ch.bin] - (cold0 & ~7ull);
     bin_accumulate<HAS_WT, HASHED>(ch.lo, ch.hi, cold0, cbase, slot_key, slot_row, slot_wt, col,
-                                   cold_val, salt, sums);
+                                   cold_buf, salt, sums);
     __syncthreads();
     const bool only = (ch.flags & CHUNK_ONLY) != 0;
     for (u32 i = threadIdx.x; i < nrows; i += SIG_THREADS) {
@@ -357,51 +377,57 @@ __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
 }
 
 template <int LOG_S, bool HASHED>
-void launch_bin_sig2(hipStream_t s, const rbsc_bins* b, const void* col, u64 salt, u64 dang_h,
-                     u64 R, u64* sig) {
+void launch_bin_sig2(hipStream_t s, const rbsc_bins* b, const void* col, const void* cold_buf,
+                     u64 salt, u64 dang_h, u64 R, u64* sig) {
     if (b->slot_wt)
         bin_sig_kernel<LOG_S, true, HASHED><<<(unsigned)b->n_chunks, SIG_THREADS, 0, s>>>(
             b->chunks, b->bin_slot, b->bin_hot, b->cold_pos, b->slot_key, b->slot_row, b->slot_wt,
-            b->cold_val, b->dsum, col, salt, dang_h, R, sig);
+            cold_buf, b->dsum, col, salt, dang_h, R, sig);
     else
         bin_sig_kernel<LOG_S, false, HASHED><<<(unsigned)b->n_chunks, SIG_THREADS, 0, s>>>(
             b->chunks, b->bin_slot, b->bin_hot, b->cold_pos, b->slot_key, b->slot_row, nullptr,
-            b->cold_val, b->dsum, col, salt, dang_h, R, sig);
+            cold_buf, b->dsum, col, salt, dang_h, R, sig);
 }
 template <int LOG_S>
-void launch_bin_sig(hipStream_t s, const rbsc_bins* b, const void* col, u64 salt, u64 dang_h,
-                    u64 R, u64* sig) {
-    if (b->hashed) launch_bin_sig2<LOG_S, true>(s, b, col, salt, dang_h, R, sig);
-    else launch_bin_sig2<LOG_S, false>(s, b, col, salt, dang_h, R, sig);
+void launch_bin_sig(hipStream_t s, const rbsc_bins* b, const void* col, const void* cold_buf,
+                    u64 salt, u64 dang_h, u64 R, u64* sig) {
+    if (b->hashed) launch_bin_sig2<LOG_S, true>(s, b, col, cold_buf, salt, dang_h, R, sig);
+    else launch_bin_sig2<LOG_S, false>(s, b, col, cold_buf, salt, dang_h, R, sig);
 }
 }  // namespace
 
-void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col) {
+u64 bins_cold_bytes(const rbsc_bins* b) {
+    return (b->n_cold_pad + SLOT_RUN) * (b->hashed ? sizeof(u64) : sizeof(u32));
+}
+
+void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, void* cold_buf) {
     const rbsc_bins* b = g->bins;
     if (!b->n_cold) return;
     if (b->hashed)
         cold_scatter_kernel<u64><<<(unsigned)b->n_tiles, BLK, 0, ctx->stream>>>(
-            b->csc_head, b->csc_slot, b->tile_line, static_cast<const u64*>(col), b->cold_val);
+            b->csc_head, b->csc_slot, b->tile_line, static_cast<const u64*>(col),
+            static_cast<u64*>(cold_buf));
     else
         cold_scatter_kernel<u32><<<(unsigned)b->n_tiles, BLK, 0, ctx->stream>>>(
-            b->csc_head, b->csc_slot, b->tile_line, static_cast<const u32*>(col), b->slot_key);
+            b->csc_head, b->csc_slot, b->tile_line, static_cast<const u32*>(col),
+            static_cast<u32*>(cold_buf));
 }
 
-void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, u64 salt, u64 dang_h,
-              u64* sig) {
+void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, const void* cold_buf,
+              u64 salt, u64 dang_h, u64* sig) {
     const rbsc_bins* b = g->bins;
     hipStream_t s = ctx->stream;
     if (b->n_zero)
         bin_zero_kernel<<<grid_for(b->n_zero << b->log_s), BLK, 0, s>>>(b->zero_bins, b->n_zero,
                                                                      b->log_s, g->R, sig);
     switch (b->log_s) {
-        case 8: launch_bin_sig<8>(s, b, col, salt, dang_h, g->R, sig); break;
-        case 9: launch_bin_sig<9>(s, b, col, salt, dang_h, g->R, sig); break;
-        case 10: launch_bin_sig<10>(s, b, col, salt, dang_h, g->R, sig); break;
-        case 11: launch_bin_sig<11>(s, b, col, salt, dang_h, g->R, sig); break;
-        case 12: launch_bin_sig<12>(s, b, col, salt, dang_h, g->R, sig); break;
-        case 13: launch_bin_sig<13>(s, b, col, salt, dang_h, g->R, sig); break;
-        default: launch_bin_sig<14>(s, b, col, salt, dang_h, g->R, sig); break;
+        case 8: launch_bin_sig<8>(s, b, col, cold_buf, salt, dang_h, g->R, sig); break;
+        case 9: launch_bin_sig<9>(s, b, col, cold_buf, salt, dang_h, g->R, sig); break;
+        case 10: launch_bin_sig<10>(s, b, col, cold_buf, salt, dang_h, g->R, sig); break;
+        case 11: launch_bin_sig<11>(s, b, col, cold_buf, salt, dang_h, g->R, sig); break;
+        case 12: launch_bin_sig<12>(s, b, col, cold_buf, salt, dang_h, g->R, sig); break;
+        case 13: launch_bin_sig<13>(s, b, col, cold_buf, salt, dang_h, g->R, sig); break;
+        default: launch_bin_sig<14>(s, b, col, cold_buf, salt, dang_h, g->R, sig); break;
     }
 }
 
@@ -410,7 +436,7 @@ void bins_release(rbsc_bins* b) {
     for (void* p : {(void*)b->bin_slot, (void*)b->bin_hot, (void*)b->slot_key, (void*)b->slot_row,
                     (void*)b->slot_wt, (void*)b->dsum, (void*)b->csc_head, (void*)b->csc_slot,
                     (void*)b->tile_line, (void*)b->chunks, (void*)b->zero_bins,
-                    (void*)b->cold_pos, (void*)b->cold_val})
+                    (void*)b->cold_pos})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -510,8 +536,9 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     B_TRY(hipStreamSynchronize(s));
     h_bin_slot.assign(n_bins + 1, 0);
     for (u64 k = 0; k < n_bins; ++k) h_bin_slot[k + 1] = h_bin_slot[k] + h_bin_n[k];
-    if (hashed) {
-        // Each bin's cold slots [c0, c1) as whole aligned groups of 8 in cold_val.
+    {
+        // Each bin's cold slots [c0, c1) as whole aligned groups of 8 in a
+        // refinement's cold-colour buffer.
         std::vector<u64> h_pos(n_bins, 0);
         u64 pos = 0;
         for (u64 k = 0; k < n_bins; ++k) {
@@ -528,8 +555,6 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
         B_TRY(hipMalloc(&b->cold_pos, sizeof(u64) * std::max<u64>(n_bins, 1)));
         B_TRY(hipMemcpyAsync(b->cold_pos, h_pos.data(), sizeof(u64) * n_bins,
                              hipMemcpyHostToDevice, s));
-        B_TRY(hipMalloc(&b->cold_val, sizeof(u64) * (pos + SLOT_RUN)));
-        B_TRY(hipMemsetAsync(b->cold_val, 0, sizeof(u64) * (pos + SLOT_RUN), s));
     }
     // Scatter groups: contiguous bin ranges with about equal cold lines.
     {
@@ -669,7 +694,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     B_TRY(hipStreamSynchronize(s));
     b->bytes = 16 * (n_bins + 1) + 6 * b->n_slots + (g->wt ? 8 * b->n_slots : 0) + 8 * R +
                8 * b->n_cold + 8 * (b->n_tiles + 1) + sizeof(BinChunk) * b->n_chunks +
-               4 * b->n_zero + (hashed ? 8 * (n_bins + b->n_cold_pad + SLOT_RUN) : 0);
+               4 * b->n_zero + 8 * n_bins;
     if (g->bins) {
         g->bytes -= g->bins->bytes;
         bins_release(g->bins);

@@ -40,22 +40,23 @@ struct rbsc_bins {
     bool remap = false;             // lines into repeated rows point at their stored rows
     // Hashed layout: refinements on it keep per-node hashed colours
     // randomize(c + r) (u64, written once per node by the rank pass), hot
-    // slots gather those and cold slots receive them in cold_val, so no line
-    // is hashed.  Otherwise (the default) raw u32 colours are gathered and
-    // every line hashes its colour.
+    // slots gather those and cold slots receive them, so no line is hashed.
+    // Otherwise (the default) raw u32 colours are gathered and every line
+    // hashes its colour.
     bool hashed = false;
     u64 n_bins = 0, n_slots = 0, n_hot = 0, n_cold = 0, n_chunks = 0, n_zero = 0;
     u64* bin_slot = nullptr;        // n_bins + 1: first slot of each bin
     u64* bin_hot = nullptr;         // n_bins: hot slots of each bin
-    // Hashed layout: bin b's cold slots [c0, c1) (c0 = bin_slot + bin_hot)
-    // live in cold_val at cold_pos[b] + (s - c0 rounded down to 8), each bin's
-    // region padded to whole aligned groups of 8 (64-byte aligned runs).
+    // The colours of bin b's cold slots [c0, c1) (c0 = bin_slot + bin_hot)
+    // change every round; they live in the refinement's own cold-colour
+    // buffer (rbsc_ctx::cold, n_cold_pad + 512 entries of u32, or u64 on the
+    // hashed layout) at cold_pos[b] + (s - c0 rounded down to 8), each bin's
+    // region padded to whole aligned groups of 8.  The layout itself is only
+    // read by refinements.
     u64* cold_pos = nullptr;        // n_bins
-    u64* cold_val = nullptr;        // n_cold_pad + 512: hashed colour of each cold slot
     u64 n_cold_pad = 0;
     // Slot arrays and scatter records carry 512 zeroed entries past the end.
-    u32* slot_key = nullptr;        // n_slots: hot = head; cold = colour of the round (raw
-                                    // layout) or 0 (hashed)
+    u32* slot_key = nullptr;        // n_slots: hot = head; cold = 0
     unsigned short* slot_row = nullptr;  // n_slots: row - first row of the bin
     u64* slot_wt = nullptr;         // n_slots or null
     u64* dsum = nullptr;            // R: weight of the row's lines into heads >= GL
@@ -66,8 +67,7 @@ struct rbsc_bins {
     bool head_order = false;
     u64 n_tiles = 0;
     u32* csc_head = nullptr;        // n_cold
-    u32* csc_slot = nullptr;        // n_cold: slot receiving that line's colour (raw
-                                    // layout) or its cold_val index (hashed)
+    u32* csc_slot = nullptr;        // n_cold: cold-buffer index receiving that line's colour
     u64* tile_line = nullptr;       // n_tiles + 1: first record of each tile
     BinChunk* chunks = nullptr;     // n_chunks
     u32* zero_bins = nullptr;       // n_zero: bins with several chunks
@@ -82,9 +82,11 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
 // signatures of the stored rows into sig.  col: per-node colours by internal
 // id, u32 raw colours on the raw layout, u64 hashed colours of the round
 // (randomize(c + salt)) on the hashed layout.
-void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col);
-void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, u64 salt, u64 dang_h,
-              u64* sig);
+// cold_buf: the refinement's cold-colour buffer, bins_cold_bytes(b) bytes.
+u64 bins_cold_bytes(const rbsc_bins* b);
+void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, void* cold_buf);
+void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, const void* cold_buf,
+              u64 salt, u64 dang_h, u64* sig);
 
 // A graph resident in HBM in the round's layout (graph.hip).  Refinement
 // only reads it (all per-refinement state lives in the context), so several
@@ -137,6 +139,8 @@ struct Workspace {
                                        // randomize(colour + salt) (u64)
     u64* hc2 = nullptr;                // the next round's, written by the rank pass
     u64* rank_status = nullptr;
+    void* cold = nullptr;              // cold-colour buffer of a binned graph's rounds
+    u64 cold_bytes = 0;
     u64 rank_tiles = 0;
     u64* carry = nullptr;              // per tile: partial signature of its carry row
     u64 carry_cap = 0;

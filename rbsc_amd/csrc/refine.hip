@@ -803,6 +803,7 @@ void ws_release(rbsc_ctx* ctx) {
     if (w.hc) (void)hipFree(w.hc);
     if (w.hc2) (void)hipFree(w.hc2);
     if (w.rank_status) (void)hipFree(w.rank_status);
+    if (w.cold) (void)hipFree(w.cold);
     for (void* p : {(void*)w.tie_runs, (void*)w.tie_runs_m, (void*)w.tie_runs_l,
                     (void*)w.tie_pre, (void*)w.tie_tmp, (void*)w.tie_mark, (void*)w.tie_buf,
                     (void*)w.carry})
@@ -1015,6 +1016,16 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                           g->bins->GL == gather_limit && !(prm.flags & RBSC_FLAG_NOBINS);
     const bool binned = bins_fit && (g->bins->hashed || !(prm.flags & RBSC_FLAG_HASHED64));
     const int c32 = binned ? (g->bins->hashed ? 0 : 1) : ((prm.flags & RBSC_FLAG_HASHED64) ? 0 : 1);
+    if (binned && bins_cold_bytes(g->bins) > w.cold_bytes) {
+        // the rounds' cold-slot colours belong to this refinement, not to the
+        // graph, so concurrent refinements of one graph do not share them
+        if (w.cold) HIP_TRY(hipFree(w.cold));
+        w.cold = nullptr;
+        w.cold_bytes = 0;
+        HIP_TRY(hipMalloc(&w.cold, bins_cold_bytes(g->bins)));
+        HIP_TRY(hipMemsetAsync(w.cold, 0, bins_cold_bytes(g->bins), s));
+        w.cold_bytes = bins_cold_bytes(g->bins);
+    }
     const bool follow = excluded && g->U > g->R;
     const bool node_order = excluded && c32 && !(prm.flags & RBSC_FLAG_COLOUR_ORDER);
 
@@ -1089,12 +1100,12 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         const u64 dang_h = rbsc_randomize(0 + (u64)it);
         if (binned) {
             prof_begin(ctx, 6);
-            bins_scatter(ctx, g, w.hc);
+            bins_scatter(ctx, g, w.hc, w.cold);
             prof_end(ctx);
         }
         prof_begin(ctx, 0);
         if (binned) {
-            bins_sig(ctx, g, w.hc, (u64)it, dang_h, w.sig);
+            bins_sig(ctx, g, w.hc, w.cold, (u64)it, dang_h, w.sig);
         } else if (n_tiles && !streaming && g->n_parts == 8 && (prm.flags & RBSC_FLAG_XCD)) {
             fill_u64<<<grid_for(g->U), BLK, 0, s>>>(w.sig, g->U, 0);
             launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0, gather_limit, dang_h, it, c32, true);

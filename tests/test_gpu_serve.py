@@ -126,12 +126,17 @@ def test_serve_rmat_and_resident(ctx):
     assert np.array_equal(rs[1].partition, hl.partition)
 
 
-def test_serve_same_resident_graph_concurrently(ctx):
-    # rows longer than a 512-line tile carry partial sums between kernels;
-    # those live in each worker's context, so concurrent refinements of one
-    # resident graph cannot disturb each other
+@pytest.mark.parametrize("binned", [False, True])
+def test_serve_same_resident_graph_concurrently(ctx, binned):
+    # rows longer than a 512-line tile carry partial sums between kernels,
+    # and a binned graph's cold slots take new colours every round; both live
+    # in each worker's context, so concurrent refinements of one resident
+    # graph cannot disturb each other
     dg = ctx.gen_rmat(15, 1 << 21, 0.57, 0.19, 0.19, 0.1, 17)
-    ref = dg.refine()
+    if binned:
+        dg.bin(256, 1024, 0)  # few hot heads: most lines are cold
+        assert dg.bin_info["cold_lines"] > dg.bin_info["hot_lines"]
+    ref = dg.refine(colors=True)
     with rbsc_amd.Server(workers=4) as srv:
         ts = [srv.submit_resident(dg) for _ in range(16)]
         rs = [srv.wait(t) for t in ts]
