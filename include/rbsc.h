@@ -184,9 +184,11 @@ int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
  * signatures are summed in LDS; lines into the hot_heads most-gathered nodes
  * (0 = 2^22) gather their colours in head order inside the bin; the colours
  * of the other gathered heads are scattered to their lines' slots once per
- * round, one workgroup per scatter_lines of them (0 = 2048) in head order,
- * kept in head order (colours read in order) or, with RBSC_BIN_SLOT_ORDER,
- * reordered by slot inside each workgroup's share (stores coalesced); bins with
+ * round, one workgroup per scatter_lines of them in head order: by default
+ * (0 = 16384, at most 16384, 8192 on the hashed form) the workgroup reads
+ * the colours in head order into LDS and writes them in slot order (stores
+ * in runs); with RBSC_BIN_SLOT_ORDER (0 = 2048) the records are kept in slot
+ * order inside each workgroup's share (colours read out of order); bins with
  * more than chunk_slots lines (0 = 2^19) are split over several
  * workgroups.  opts may be null (all defaults).  Later refinements of g use
  * the layout unless RBSC_FLAG_NOBINS or streamed lines are asked for (or

@@ -232,7 +232,7 @@ class DeviceGraph:
         rows, hot heads gathered in head order, the other colours scattered
         to their lines' slots in chunks of lines); later refinements use it.
         0 picks the defaults (2^22 hot heads, 8192 rows per bin, 2^19 slots
-        per chunk, 2048 lines per scatter workgroup, kept in head order).
+        per chunk, 16384 lines per scatter workgroup staged through LDS).
         hashed: keep per-node hashed u64 colours (no hashing per line)
         instead of gathering raw u32 colours.  Results are unchanged."""
         o = _lib.BinOpts(int(hot_heads), int(rows_per_bin), int(chunk_slots), int(scatter_lines),

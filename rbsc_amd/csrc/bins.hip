@@ -15,12 +15,14 @@
 //                        lines, one workgroup per chunk (chunk c of group g is
 //                        workgroup 8c + g: workgroups are dealt to the XCDs
 //                        round robin).  A chunk spans a narrow head range, so
-//                        its colour reads stay in cache; inside the chunk the
-//                        lines stay in head order (default: colour reads in
-//                        order, stores scattered) or are put in slot order
-//                        (RBSC_BIN_SLOT_ORDER: a bin's cold slots are in head
-//                        order, so a chunk's lines for one bin fill one run of
-//                        consecutive positions).
+//                        its colour reads stay in cache.  Default (staged):
+//                        the workgroup reads the colours in head order into
+//                        LDS and writes them out in slot order; a bin's cold
+//                        slots are in head order, so a chunk's lines for one
+//                        bin fill one run of consecutive positions, and a
+//                        wave's stores cover a few runs.  RBSC_BIN_SLOT_ORDER
+//                        keeps the records in slot order instead (colour
+//                        reads out of order, no LDS).
 //   bin_sig_kernel       one workgroup per chunk of a bin of 2^log_s rows:
 //                        row sums in LDS, hot slots (head < H) gathered from
 //                        the L2-resident hot colours in head order (a wave's
@@ -51,6 +53,9 @@ constexpr int SIG_THREADS = 512;
 // Zeroed entries past the end of the slot arrays and scatter records (the
 // last aligned group's loads stay in bounds).
 constexpr u64 SLOT_RUN = 512;
+// LDS of the staged scatter: one colour per line of a tile.
+constexpr u64 STAGE_BYTES = 65536;
+constexpr int STAGE_THREADS = 512;
 
 // ---------------------------------------------------------------- build
 // Per stored row: lines into hot heads, into cold heads, and the total
@@ -205,21 +210,42 @@ __global__ void csc_fill_kernel(const u64* __restrict__ k3, const u64* __restric
         for (u64 u = t + 1; u <= n_tiles; ++u) tile_line[u] = n;
 }
 
+// Staged scatter: records of every tile in head order (sorted) -> key
+// (tile, target), value head-order index, to be sorted into slot order.
+__global__ void stage_key_kernel(const u64* __restrict__ k3, const u64* __restrict__ v3, u64 n,
+                                 u64* __restrict__ k4, u64* __restrict__ v4) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    k4[i] = ((k3[i] >> 32) << 32) | (v3[i] >> 32);
+    v4[i] = i;
+}
+
+// Records sorted by (tile, target): targets in slot order, and each
+// head-order record's rank inside its tile's slot order.
+__global__ void stage_fill_kernel(const u64* __restrict__ k4, const u64* __restrict__ v4, u64 n,
+                                  const u64* __restrict__ tile_line, u32* __restrict__ csc_slot,
+                                  unsigned short* __restrict__ csc_perm) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64 t = k4[i] >> 32;
+    csc_slot[i] = (u32)(k4[i] & 0xFFFFFFFFull);
+    csc_perm[v4[i]] = (unsigned short)(i - tile_line[t]);
+}
+
 // ---------------------------------------------------------------- rounds
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 typedef u64 u64x2 __attribute__((ext_vector_type(2)));
 
-// Writes the current colour of every cold line's head into the line's slot,
-// one workgroup per tile (records [tile_line[t], tile_line[t + 1])).  Records
+// Slot-order scatter (RBSC_BIN_SLOT_ORDER): writes the current colour of every
+// cold line's head into the line's slot, one workgroup per tile (records
+// [tile_line[t], tile_line[t + 1]), in slot order).  Records
 // are read 8 at a time with 16-byte loads: thread i takes the aligned groups
 // [8q, 8q + 8) for q = lo / 8 + i, + BLK, ...; records outside the tile (a
 // neighbour's, or the zeroed padding past the end) are loaded, and their
 // heads (valid ids or 0) read, but never stored.  The target is the line's
 // index in the refinement's cold-colour buffer; the colour is the u32 raw
 // colour on the raw layout, the u64 hashed colour on the hashed layout.
-// (Lanes taking 64 consecutive records per load instruction instead
-// measured slower: the stores scatter more.)
 template <typename C>
 __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict__ csc_head,
                                                            const u32* __restrict__ csc_slot,
@@ -243,6 +269,58 @@ __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict
         for (int k = 0; k < 8; ++k) {
             const u64 j = 8 * q + k;
             if (j >= lo && j < hi) out[sl[k]] = c[k];
+        }
+    }
+}
+
+// Staged scatter, one workgroup per tile of at most STAGE_BYTES / sizeof(C)
+// lines: the
+// tile's colours are read in head order (a wave's 64 reads fall into a few
+// cache lines) into LDS at each line's rank in slot order, then written out
+// in slot order.  A tile's lines for one bin are one run of consecutive
+// targets (a bin's cold slots are in head order and a tile is a head range),
+// so a wave's 64 stores cover a few runs instead of 64 scattered lines.
+// Each thread keeps 4 lines in flight per step.
+template <class C>
+__global__ __launch_bounds__(STAGE_THREADS) void cold_stage_kernel(
+    const u32* __restrict__ csc_head, const unsigned short* __restrict__ csc_perm,
+    const u32* __restrict__ csc_slot, const u64* __restrict__ tile_line,
+    const C* __restrict__ col, C* __restrict__ out) {
+    __shared__ C stage[STAGE_BYTES / sizeof(C)];
+    const u64 lo = tile_line[blockIdx.x];
+    const u32 n = (u32)(tile_line[blockIdx.x + 1] - lo);
+    const u32* head = csc_head + lo;
+    const unsigned short* perm = csc_perm + lo;
+    const u32* slot = csc_slot + lo;
+    constexpr u32 STEP = 4 * STAGE_THREADS;
+    for (u32 base = threadIdx.x; base < n; base += STEP) {
+        u32 h[4];
+        unsigned short p[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u32 j = base + k * STAGE_THREADS;
+            h[k] = j < n ? head[j] : 0u;
+            p[k] = j < n ? perm[j] : (unsigned short)0;
+        }
+        C c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = col[h[k]];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (base + k * STAGE_THREADS < n) stage[p[k]] = c[k];
+    }
+    __syncthreads();
+    for (u32 base = threadIdx.x; base < n; base += STEP) {
+        u32 t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u32 j = base + k * STAGE_THREADS;
+            t[k] = j < n ? slot[j] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const u32 j = base + k * STAGE_THREADS;
+            if (j < n) out[t[k]] = stage[j];
         }
     }
 }
@@ -403,6 +481,17 @@ u64 bins_cold_bytes(const rbsc_bins* b) {
 void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, void* cold_buf) {
     const rbsc_bins* b = g->bins;
     if (!b->n_cold) return;
+    if (b->head_order) {
+        if (b->hashed)
+            cold_stage_kernel<u64><<<(unsigned)b->n_tiles, STAGE_THREADS, 0, ctx->stream>>>(
+                b->csc_head, b->csc_perm, b->csc_slot, b->tile_line, static_cast<const u64*>(col),
+                static_cast<u64*>(cold_buf));
+        else
+            cold_stage_kernel<u32><<<(unsigned)b->n_tiles, STAGE_THREADS, 0, ctx->stream>>>(
+                b->csc_head, b->csc_perm, b->csc_slot, b->tile_line, static_cast<const u32*>(col),
+                static_cast<u32*>(cold_buf));
+        return;
+    }
     if (b->hashed)
         cold_scatter_kernel<u64><<<(unsigned)b->n_tiles, BLK, 0, ctx->stream>>>(
             b->csc_head, b->csc_slot, b->tile_line, static_cast<const u64*>(col),
@@ -436,7 +525,7 @@ void bins_release(rbsc_bins* b) {
     for (void* p : {(void*)b->bin_slot, (void*)b->bin_hot, (void*)b->slot_key, (void*)b->slot_row,
                     (void*)b->slot_wt, (void*)b->dsum, (void*)b->csc_head, (void*)b->csc_slot,
                     (void*)b->tile_line, (void*)b->chunks, (void*)b->zero_bins,
-                    (void*)b->cold_pos})
+                    (void*)b->cold_pos, (void*)b->csc_perm})
         if (p) (void)hipFree(p);
     delete b;
 }
@@ -463,6 +552,9 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     const u32* rep = (!g->init && g->U > g->R) ? g->dup_rep : nullptr;
     const u64 n_bins = (R + (1ull << log_s) - 1) >> log_s;
     if (chunk_slots == 0) chunk_slots = 1ull << 19;
+    // the staged scatter holds a tile's colours in LDS
+    if (head_order)
+        scatter_chunk = std::min<u64>(scatter_chunk, STAGE_BYTES / (hashed ? 8 : 4));
     rbsc_bins* b = new rbsc_bins();
     b->log_s = log_s;
     b->H = H;
@@ -689,11 +781,22 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
         B_TRY(hipMalloc(&b->tile_line, sizeof(u64) * (b->n_tiles + 1)));
         csc_fill_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(k3[w3], v3[w3], b->n_cold, b->n_tiles,
                                                            b->csc_head, b->csc_slot, b->tile_line);
+        if (head_order) {
+            u64* k4[2] = {k3[w3 ^ 1], k3[w3]};
+            u64* v4[2] = {v3[w3 ^ 1], v3[w3]};
+            stage_key_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(k3[w3], v3[w3], b->n_cold, k4[0],
+                                                                v4[0]);
+            int w4 = 0;
+            if ((rc = radix_sort_u64(&ctx->sort, k4, v4, b->n_cold, 0, bits, &w4, s))) goto done;
+            B_TRY(hipMalloc(&b->csc_perm, sizeof(unsigned short) * b->n_cold));
+            stage_fill_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(
+                k4[w4], v4[w4], b->n_cold, b->tile_line, b->csc_slot, b->csc_perm);
+        }
     }
     B_TRY(hipGetLastError());
     B_TRY(hipStreamSynchronize(s));
     b->bytes = 16 * (n_bins + 1) + 6 * b->n_slots + (g->wt ? 8 * b->n_slots : 0) + 8 * R +
-               8 * b->n_cold + 8 * (b->n_tiles + 1) + sizeof(BinChunk) * b->n_chunks +
+               (head_order ? 10 : 8) * b->n_cold + 8 * (b->n_tiles + 1) + sizeof(BinChunk) * b->n_chunks +
                4 * b->n_zero + 8 * n_bins;
     if (g->bins) {
         g->bytes -= g->bins->bytes;

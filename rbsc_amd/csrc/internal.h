@@ -61,13 +61,16 @@ struct rbsc_bins {
     u64* slot_wt = nullptr;         // n_slots or null
     u64* dsum = nullptr;            // R: weight of the row's lines into heads >= GL
     // Scatter tiles: tile t = 8c + g is chunk c (scatter_chunk lines in head
-    // order) of the cold lines of the bins in group g; records of a tile in
-    // slot order, or in head order with head_order.
+    // order) of the cold lines of the bins in group g.  With head_order
+    // (staged scatter) csc_head lists a tile's lines in head order, csc_perm
+    // gives each of them its rank in the tile's slot order and csc_slot lists
+    // the targets in slot order; otherwise both arrays are in slot order.
     u64 scatter_chunk = 0;
     bool head_order = false;
     u64 n_tiles = 0;
     u32* csc_head = nullptr;        // n_cold
-    u32* csc_slot = nullptr;        // n_cold: cold-buffer index receiving that line's colour
+    u32* csc_slot = nullptr;        // n_cold: cold-buffer index receiving a line's colour
+    unsigned short* csc_perm = nullptr;  // n_cold with head_order
     u64* tile_line = nullptr;       // n_tiles + 1: first record of each tile
     BinChunk* chunks = nullptr;     // n_chunks
     u32* zero_bins = nullptr;       // n_zero: bins with several chunks

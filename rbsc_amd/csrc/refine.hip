@@ -1614,7 +1614,7 @@ extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts
     if (o.hot_heads == 0) o.hot_heads = 1u << 22;
     if (o.rows_per_bin == 0) o.rows_per_bin = 1u << 13;
     if (o.chunk_slots == 0) o.chunk_slots = 1ull << 19;
-    if (o.scatter_lines == 0) o.scatter_lines = 1u << 11;
+    if (o.scatter_lines == 0) o.scatter_lines = (o.flags & RBSC_BIN_SLOT_ORDER) ? 1u << 11 : 1u << 14;
     if (o.rows_per_bin & (o.rows_per_bin - 1)) {
         rbsc_set_error("rbsc_graph_bin: rows_per_bin must be a power of two");
         return RBSC_ERR_INVALID;

@@ -33,14 +33,16 @@ CASES = {
 # (hot heads, rows per bin, slots per chunk, cold lines per scatter
 # workgroup, scatter records in slot order, hashed form): defaults,
 # everything cold, everything hot, small bins split over many chunks, the
-# largest bins, scatter chunks of a few lines (not multiples of 8); the raw
-# form on all of them, the hashed form on four
+# largest bins, scatter chunks of a few lines (not multiples of 8) or above
+# the staged scatter's LDS capacity (taken as that capacity); the raw form on
+# all of them, the hashed form on five
 SHAPES = [(0, 0, 0, 0, False, False), (1, 256, 0, 0, True, False),
           (1 << 30, 512, 0, 0, False, False), (64, 256, 300, 13, False, False),
           (4096, 16384, 1000, 1 << 20, True, False), (16, 1024, 0, 1, False, False),
-          (3, 256, 0, 77, True, False),
+          (3, 256, 0, 77, True, False), (2, 256, 0, 1 << 20, False, False),
           (0, 0, 0, 0, False, True), (1, 256, 0, 0, True, True),
-          (64, 256, 300, 13, False, True), (3, 256, 0, 77, True, True)]
+          (64, 256, 300, 13, False, True), (3, 256, 0, 77, True, True),
+          (2, 256, 0, 1 << 20, False, True)]
 
 
 def _trace(g, rounds=64):
