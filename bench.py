@@ -288,6 +288,8 @@ def main():
                     help="1: binned signature layout (rbsc_graph_bin) built before timing; "
                          "0: plain line layout")
     ap.add_argument("--bin-hot", type=int, default=0, help="hot heads of the binned layout (0 = default)")
+    ap.add_argument("--bin-dense", type=int, default=0,
+                    help="hot heads gathered inside the bin pass; the others are warm (0 = default)")
     ap.add_argument("--bin-rows", type=int, default=0, help="rows per bin (0 = default)")
     ap.add_argument("--bin-chunk", type=int, default=0, help="slots per chunk (0 = default)")
     ap.add_argument("--bin-scatter", type=int, default=0,
@@ -333,7 +335,7 @@ def main():
     if args.bins and args.parts == 1 and not args.hot:
         t0 = time.perf_counter()
         dg.bin(args.bin_hot, args.bin_rows, args.bin_chunk, args.bin_scatter, args.bin_slot_order,
-               args.bin_hashed)
+               args.bin_hashed, args.bin_dense)
         bin_info = dict(dg.bin_info, build_s=time.perf_counter() - t0)
         log(f"[bench] binned layout in {bin_info['build_s']:.1f}s: {bin_info}")
     flags = (rbsc_amd.FLAG_FULLSORT if args.fullsort else 0) | (rbsc_amd.FLAG_HASHED64 if args.hashed64 else 0) | (rbsc_amd.FLAG_XCD if args.xcd else 0)

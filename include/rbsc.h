@@ -190,7 +190,11 @@ int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_heads);
  * in runs); with RBSC_BIN_SLOT_ORDER (0 = 2048) the records are kept in slot
  * order inside each workgroup's share (colours read out of order); bins with
  * more than chunk_slots lines (0 = 2^19) are split over several
- * workgroups.  opts may be null (all defaults).  Later refinements of g use
+ * workgroups.  Of the hot heads, those below dense_heads (0 = 2^21; at least
+ * hot_heads: none warm) are gathered by the bin pass itself; the lines into
+ * the others (warm) get their colours once per round from a gather pass in
+ * which each XCD reads only its own share of the warm colours (segments of
+ * about equal lines), so they stay in that XCD's L2.  opts may be null (all defaults).  Later refinements of g use
  * the layout unless RBSC_FLAG_NOBINS or streamed lines are asked for (or
  * RBSC_FLAG_HASHED64 on the raw form); relayout, split_hot and
  * lines_to_host drop it.  Signatures and results are identical.  No counterpart in the reference (its
@@ -202,6 +206,8 @@ typedef struct {
     uint64_t chunk_slots;
     uint32_t scatter_lines;
     uint32_t flags;           /* RBSC_BIN_* */
+    uint32_t dense_heads;
+    uint32_t reserved;        /* 0 */
 } rbsc_bin_opts;
 #define RBSC_BIN_SLOT_ORDER 1u
 /* Hashed form of the layout: refinements keep u64 hashed colours per node,
@@ -214,9 +220,9 @@ typedef struct {
 #define RBSC_BIN_HASHED 2u
 int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts* opts);
 int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g);
-/* info[8]: binned (0/1), bins, hot lines, cold lines, workgroup chunks, bins
- * split over several chunks, bytes, hot_heads | log2(rows_per_bin) << 32 |
- * hashed form << 40. */
+/* info[10]: binned (0/1), bins, hot lines (dense and warm), cold lines,
+ * workgroup chunks, bins split over several chunks, bytes, hot_heads |
+ * log2(rows_per_bin) << 32 | hashed form << 40, dense_heads, warm lines. */
 int rbsc_graph_bin_info(const rbsc_dgraph* g, uint64_t* info);
 
 /* Per-phase device timing with hipEvents on the context's stream (off by

@@ -227,16 +227,20 @@ class DeviceGraph:
         check(load().rbsc_graph_split_hot(self.ctx.handle, self.handle, int(hot_heads)))
 
     def bin(self, hot_heads: int = 0, rows_per_bin: int = 0, chunk_slots: int = 0,
-            scatter_lines: int = 0, slot_order: bool = False, hashed: bool = False):
+            scatter_lines: int = 0, slot_order: bool = False, hashed: bool = False,
+            dense_heads: int = 0):
         """Build the binned signature layout (row sums in LDS per bin of
         rows, hot heads gathered in head order, the other colours scattered
         to their lines' slots in chunks of lines); later refinements use it.
         0 picks the defaults (2^22 hot heads, 8192 rows per bin, 2^19 slots
-        per chunk, 16384 lines per scatter workgroup staged through LDS).
+        per chunk, 16384 lines per scatter workgroup staged through LDS, 2^21
+        dense heads).  Hot heads from dense_heads on are warm: their colours
+        are gathered per XCD share in a pass of their own.
         hashed: keep per-node hashed u64 colours (no hashing per line)
         instead of gathering raw u32 colours.  Results are unchanged."""
         o = _lib.BinOpts(int(hot_heads), int(rows_per_bin), int(chunk_slots), int(scatter_lines),
-                         (_lib.BIN_SLOT_ORDER if slot_order else 0) | (_lib.BIN_HASHED if hashed else 0))
+                         (_lib.BIN_SLOT_ORDER if slot_order else 0) | (_lib.BIN_HASHED if hashed else 0),
+                         int(dense_heads), 0)
         check(load().rbsc_graph_bin(self.ctx.handle, self.handle, C.byref(o)))
 
     def unbin(self):
@@ -245,13 +249,14 @@ class DeviceGraph:
     @property
     def bin_info(self) -> dict:
         """Sizes of the binned layout (all zero when the graph has none)."""
-        v = (C.c_uint64 * 8)()
+        v = (C.c_uint64 * 10)()
         check(load().rbsc_graph_bin_info(self.handle, v))
         return {"binned": bool(v[0]), "bins": int(v[1]), "hot_lines": int(v[2]),
                 "cold_lines": int(v[3]), "chunks": int(v[4]), "split_bins": int(v[5]),
                 "bytes": int(v[6]), "hot_heads": int(v[7] & 0xFFFFFFFF),
                 "rows_per_bin": 1 << int((v[7] >> 32) & 0xFF) if v[0] else 0,
-                "hashed": bool((v[7] >> 40) & 1)}
+                "hashed": bool((v[7] >> 40) & 1), "dense_heads": int(v[8]),
+                "warm_lines": int(v[9])}
 
     def lines_to_host(self):
         """Keep the line arrays in pinned host memory; refinements then stream

@@ -40,7 +40,7 @@ BIN_HASHED = 2
 class BinOpts(C.Structure):
     _fields_ = [("hot_heads", C.c_uint32), ("rows_per_bin", C.c_uint32),
                 ("chunk_slots", C.c_uint64), ("scatter_lines", C.c_uint32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("dense_heads", C.c_uint32), ("reserved", C.c_uint32)]
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)

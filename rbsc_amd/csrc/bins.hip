@@ -154,12 +154,13 @@ __global__ void place_hot_kernel(const u64* __restrict__ keys, const u64* __rest
 // Sorted cold records into their slots (after the bin's hot slots), and the
 // scatter records: key (group, head), value slot << 32 | head.
 // A record's target is its index in a refinement's cold-colour buffer
-// (cold_pos[bin] + slot - the bin's first cold slot rounded down to 8), or
-// its slot when cold_pos is null.
+// (cold_pos[bin] + slot - the bin's first buffered (warm or cold) slot
+// rounded down to 8), or its slot when cold_pos is null.
 template <bool HAS_WT>
 __global__ void place_cold_kernel(const u64* __restrict__ keys, const u64* __restrict__ vals,
                                   u64 n, int gb, const u64* __restrict__ bin_slot,
-                                  const u64* __restrict__ bin_hot,
+                                  const u64* __restrict__ hot_all,
+                                  const u64* __restrict__ bin_dense,
                                   const u64* __restrict__ cold_start,
                                   const u32* __restrict__ grp_of_bin,
                                   const u64* __restrict__ cold_pos, const u64* __restrict__ wt,
@@ -172,8 +173,8 @@ __global__ void place_cold_kernel(const u64* __restrict__ keys, const u64* __res
     const u64 k = keys[i], v = vals[i];
     const u64 b = k >> gb;
     const u64 head = k & ((1ull << gb) - 1);
-    const u64 c0 = bin_slot[b] + bin_hot[b];
-    const u64 s = c0 + (i - cold_start[b]);
+    const u64 s = bin_slot[b] + hot_all[b] + (i - cold_start[b]);
+    const u64 c0 = bin_slot[b] + bin_dense[b];
     slot_key[s] = 0;
     slot_row[s] = (unsigned short)(v & 0xFFFF);
     if (HAS_WT) slot_wt[s] = wt[v >> 16];
@@ -208,6 +209,36 @@ __global__ void csc_fill_kernel(const u64* __restrict__ k3, const u64* __restric
     for (u64 u = first; u <= t; ++u) tile_line[u] = i;
     if (i == n - 1)
         for (u64 u = t + 1; u <= n_tiles; ++u) tile_line[u] = n;
+}
+
+// Lines into each warm head (D <= head < H), from the sorted hot records.
+__global__ void warm_hist_kernel(const u64* __restrict__ keys, u64 n, int hb, u32 D, u32 H,
+                                 u32* __restrict__ cnt) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u32 head = (u32)(keys[i] & ((1ull << hb) - 1));
+    if (head >= D && head < H) atomicAdd(cnt + (head - D), 1u);
+}
+
+// Per bin and segment boundary d[j] (j = 0..WARM_SEGS): the first of the
+// bin's hot slots (sorted by head) whose head is >= d[j]; bin_dense = the
+// slots below d[0] = D.
+__global__ void warm_off_kernel(const u32* __restrict__ slot_key, const u64* __restrict__ bin_slot,
+                                const u64* __restrict__ hot_all, u64 n_bins,
+                                const u32* __restrict__ bound, u64* __restrict__ warm_off,
+                                u64* __restrict__ bin_dense) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_bins * (WARM_SEGS + 1)) return;
+    const u64 b = i / (WARM_SEGS + 1), j = i % (WARM_SEGS + 1);
+    u64 lo = bin_slot[b], hi = lo + hot_all[b];
+    const u32 d = bound[j];
+    while (lo < hi) {
+        const u64 mid = lo + (hi - lo) / 2;
+        if (slot_key[mid] < d) lo = mid + 1;
+        else hi = mid;
+    }
+    warm_off[i] = lo;
+    if (j == 0) bin_dense[b] = lo - bin_slot[b];
 }
 
 // Staged scatter: records of every tile in head order (sorted) -> key
@@ -273,6 +304,42 @@ __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict
     }
 }
 
+// Warm gather: workgroup i runs on XCD x = i mod 8 (workgroups are dealt to
+// the XCDs round robin); its waves take x's work items (runs of warm slots
+// whose heads lie in x's segments) one wave per item, so the XCD's gathers
+// fall into its own segments' colours, which its L2 holds.  Keys are read
+// and colours stored in runs; each lane keeps 4 slots in flight per whole
+// step.
+constexpr int WARM_THREADS = 512;
+constexpr int WARM_WAVES = WARM_THREADS / 64;
+template <class C>
+__global__ __launch_bounds__(WARM_THREADS) void warm_gather_kernel(
+    const WarmItem* __restrict__ items, WarmFirst first, const u32* __restrict__ slot_key,
+    const C* __restrict__ col, C* __restrict__ out, u32 per_xcd) {
+    const u32 x = blockIdx.x % BIN_GROUPS;
+    const u32 lane = threadIdx.x & 63;
+    const u32 i1 = first.f[x + 1];
+    const u32 stride = per_xcd * WARM_WAVES;
+    for (u32 i = first.f[x] + (blockIdx.x / BIN_GROUPS) * WARM_WAVES + threadIdx.x / 64; i < i1;
+         i += stride) {
+        const WarmItem it = items[i];
+        const u32* key = slot_key + it.lo;
+        C* o = out + it.dst;
+        u32 base = lane;
+        for (; base + 3 * 64 < it.n; base += 4 * 64) {  // whole steps: straight-line
+            u32 k4[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) k4[k] = key[base + k * 64];
+            C c[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c[k] = col[k4[k]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) o[base + k * 64] = c[k];
+        }
+        for (; base < it.n; base += 64) o[base] = col[key[base]];
+    }
+}
+
 // Staged scatter, one workgroup per tile of at most STAGE_BYTES / sizeof(C)
 // lines: the
 // tile's colours are read in head order (a wave's 64 reads fall into a few
@@ -280,7 +347,7 @@ __global__ __launch_bounds__(BLK) void cold_scatter_kernel(const u32* __restrict
 // in slot order.  A tile's lines for one bin are one run of consecutive
 // targets (a bin's cold slots are in head order and a tile is a head range),
 // so a wave's 64 stores cover a few runs instead of 64 scattered lines.
-// Each thread keeps 4 lines in flight per step.
+// Each thread keeps 4 lines in flight per whole step.
 template <class C>
 __global__ __launch_bounds__(STAGE_THREADS) void cold_stage_kernel(
     const u32* __restrict__ csc_head, const unsigned short* __restrict__ csc_perm,
@@ -294,33 +361,41 @@ __global__ __launch_bounds__(STAGE_THREADS) void cold_stage_kernel(
     const u32* slot = csc_slot + lo;
     constexpr u32 STEP = 4 * STAGE_THREADS;
     for (u32 base = threadIdx.x; base < n; base += STEP) {
-        u32 h[4];
-        unsigned short p[4];
+        if (base + 3 * STAGE_THREADS < n) {  // whole step: straight-line
+            u32 h[4];
+            unsigned short p[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const u32 j = base + k * STAGE_THREADS;
-            h[k] = j < n ? head[j] : 0u;
-            p[k] = j < n ? perm[j] : (unsigned short)0;
+            for (int k = 0; k < 4; ++k) {
+                h[k] = head[base + k * STAGE_THREADS];
+                p[k] = perm[base + k * STAGE_THREADS];
+            }
+            C c[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c[k] = col[h[k]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) stage[p[k]] = c[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const u32 j = base + k * STAGE_THREADS;
+                if (j < n) stage[perm[j]] = col[head[j]];
+            }
         }
-        C c[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) c[k] = col[h[k]];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (base + k * STAGE_THREADS < n) stage[p[k]] = c[k];
     }
     __syncthreads();
     for (u32 base = threadIdx.x; base < n; base += STEP) {
-        u32 t[4];
+        if (base + 3 * STAGE_THREADS < n) {
+            u32 t[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const u32 j = base + k * STAGE_THREADS;
-            t[k] = j < n ? slot[j] : 0u;
-        }
+            for (int k = 0; k < 4; ++k) t[k] = slot[base + k * STAGE_THREADS];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const u32 j = base + k * STAGE_THREADS;
-            if (j < n) out[t[k]] = stage[j];
+            for (int k = 0; k < 4; ++k) out[t[k]] = stage[base + k * STAGE_THREADS];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const u32 j = base + k * STAGE_THREADS;
+                if (j < n) out[slot[j]] = stage[j];
+            }
         }
     }
 }
@@ -385,9 +460,13 @@ __device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0, u64 cbas
             }
         } else {
             const u32* col = static_cast<const u32*>(colv);
-            const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + g0);
-            const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + g0 + 4);
-            u32 key[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
+            u32 key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (g0 < cold0) {  // keys only for groups that hold gathered slots
+                const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + g0);
+                const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + g0 + 4);
+                key[0] = ka.x; key[1] = ka.y; key[2] = ka.z; key[3] = ka.w;
+                key[4] = kb.x; key[5] = kb.y; key[6] = kb.z; key[7] = kb.w;
+            }
             // Cold slots of this chunk in the group: [max(g0, cold0), min(g0 + 8, e)),
             // colours from the refinement's buffer (index s + cbase).
             const u64 c_lo = g0 > cold0 ? g0 : cold0;
@@ -480,6 +559,19 @@ u64 bins_cold_bytes(const rbsc_bins* b) {
 
 void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, void* cold_buf) {
     const rbsc_bins* b = g->bins;
+    if (b->n_warm) {
+        // per XCD: 32 CUs, 4 workgroups of 512 threads each
+        const u32 per_xcd = 128;
+        const unsigned grid = per_xcd * (unsigned)BIN_GROUPS;
+        if (b->hashed)
+            warm_gather_kernel<u64><<<grid, WARM_THREADS, 0, ctx->stream>>>(
+                b->warm_items, b->warm_first, b->slot_key, static_cast<const u64*>(col),
+                static_cast<u64*>(cold_buf), per_xcd);
+        else
+            warm_gather_kernel<u32><<<grid, WARM_THREADS, 0, ctx->stream>>>(
+                b->warm_items, b->warm_first, b->slot_key, static_cast<const u32*>(col),
+                static_cast<u32*>(cold_buf), per_xcd);
+    }
     if (!b->n_cold) return;
     if (b->head_order) {
         if (b->hashed)
@@ -525,12 +617,12 @@ void bins_release(rbsc_bins* b) {
     for (void* p : {(void*)b->bin_slot, (void*)b->bin_hot, (void*)b->slot_key, (void*)b->slot_row,
                     (void*)b->slot_wt, (void*)b->dsum, (void*)b->csc_head, (void*)b->csc_slot,
                     (void*)b->tile_line, (void*)b->chunks, (void*)b->zero_bins,
-                    (void*)b->cold_pos, (void*)b->csc_perm})
+                    (void*)b->cold_pos, (void*)b->csc_perm, (void*)b->warm_items})
         if (p) (void)hipFree(p);
     delete b;
 }
 
-int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
+int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk_slots,
                u64 scatter_chunk, bool head_order, bool hashed) {
     if (!g->dst || g->n_parts != 1) {
         rbsc_set_error("bin: needs the plain layout with lines resident on the device");
@@ -547,6 +639,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     // initial colours are given (their colour is 0 every round).
     const u32 GL = g->init ? g->N : g->U;
     H = std::min(H, GL);
+    D = std::min(D, H);
     // Without initial colours, lines into repeated rows gather their stored
     // rows (refinement then keeps no colours for repeated rows).
     const u32* rep = (!g->init && g->U > g->R) ? g->dup_rep : nullptr;
@@ -558,6 +651,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     rbsc_bins* b = new rbsc_bins();
     b->log_s = log_s;
     b->H = H;
+    b->D = D;
     b->GL = GL;
     b->n_bins = n_bins;
     b->remap = rep != nullptr;
@@ -566,8 +660,11 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     b->hashed = hashed;
     u64 *n_hot = nullptr, *n_cold = nullptr, *tmp = nullptr, *tot = nullptr, *hot_start = nullptr,
         *cold_start = nullptr, *bin_n = nullptr, *keys[2] = {nullptr, nullptr},
-        *vals[2] = {nullptr, nullptr};
+        *vals[2] = {nullptr, nullptr}, *hot_all = nullptr;
     u32* grp_of_bin = nullptr;
+    u32 *warm_cnt = nullptr, *warm_bound = nullptr;
+    u64* warm_off = nullptr;
+    WarmMap xcd_segs{};
     u64* grp_first = nullptr;
     std::vector<u64> h_bin_n, h_bin_hot, h_bin_slot;
     std::vector<u32> h_grp(n_bins + 1, 0), h_zero;
@@ -616,38 +713,19 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     B_TRY(hipMalloc(&hot_start, sizeof(u64) * (n_bins + 1)));
     B_TRY(hipMalloc(&cold_start, sizeof(u64) * (n_bins + 1)));
     B_TRY(hipMalloc(&bin_n, sizeof(u64) * (n_bins + 1)));
+    B_TRY(hipMalloc(&hot_all, sizeof(u64) * (n_bins + 1)));
     B_TRY(hipMalloc(&b->bin_hot, sizeof(u64) * (n_bins + 1)));
     B_TRY(hipMalloc(&b->bin_slot, sizeof(u64) * (n_bins + 1)));
     bin_sizes_kernel<<<grid_for(n_bins + 1), BLK, 0, s>>>(n_hot, n_cold, R, log_s, n_bins, hot_start,
-                                                          cold_start, bin_n, b->bin_hot);
+                                                          cold_start, bin_n, hot_all);
     h_bin_n.resize(n_bins + 1);
     h_bin_hot.resize(n_bins + 1);
     B_TRY(hipMemcpyAsync(h_bin_n.data(), bin_n, sizeof(u64) * (n_bins + 1), hipMemcpyDeviceToHost, s));
-    B_TRY(hipMemcpyAsync(h_bin_hot.data(), b->bin_hot, sizeof(u64) * n_bins, hipMemcpyDeviceToHost, s));
+    B_TRY(hipMemcpyAsync(h_bin_hot.data(), hot_all, sizeof(u64) * n_bins, hipMemcpyDeviceToHost, s));
     if ((rc = scan_excl_u64(bin_n, b->bin_slot, n_bins + 1, nullptr, tmp, s))) goto done;
     B_TRY(hipStreamSynchronize(s));
     h_bin_slot.assign(n_bins + 1, 0);
     for (u64 k = 0; k < n_bins; ++k) h_bin_slot[k + 1] = h_bin_slot[k] + h_bin_n[k];
-    {
-        // Each bin's cold slots [c0, c1) as whole aligned groups of 8 in a
-        // refinement's cold-colour buffer.
-        std::vector<u64> h_pos(n_bins, 0);
-        u64 pos = 0;
-        for (u64 k = 0; k < n_bins; ++k) {
-            h_pos[k] = pos;
-            const u64 c0 = h_bin_slot[k] + h_bin_hot[k], c1 = h_bin_slot[k + 1];
-            if (c1 > c0) pos += ((c1 + 7) & ~7ull) - (c0 & ~7ull);
-        }
-        b->n_cold_pad = pos;
-        if (pos + SLOT_RUN >= (1ull << 32)) {
-            rbsc_set_error("bin: %llu cold slots exceed 2^32", (unsigned long long)pos);
-            rc = RBSC_ERR_INVALID;
-            goto done;
-        }
-        B_TRY(hipMalloc(&b->cold_pos, sizeof(u64) * std::max<u64>(n_bins, 1)));
-        B_TRY(hipMemcpyAsync(b->cold_pos, h_pos.data(), sizeof(u64) * n_bins,
-                             hipMemcpyHostToDevice, s));
-    }
     // Scatter groups: contiguous bin ranges with about equal cold lines.
     {
         u64 seen = 0;
@@ -718,12 +796,112 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
             place_hot_kernel<false><<<grid_for(b->n_hot), BLK, 0, s>>>(
                 keys[w], vals[w], b->n_hot, hb, b->bin_slot, hot_start, nullptr, b->slot_key,
                 b->slot_row, nullptr);
+        if (D < H) {
+            // Warm segments: boundaries at about equal shares of the warm lines.
+            const u64 nw = H - D;
+            B_TRY(hipMalloc(&warm_cnt, sizeof(u32) * nw));
+            B_TRY(hipMemsetAsync(warm_cnt, 0, sizeof(u32) * nw, s));
+            warm_hist_kernel<<<grid_for(b->n_hot), BLK, 0, s>>>(keys[w], b->n_hot, hb, D, H,
+                                                               warm_cnt);
+            std::vector<u32> h_cnt(nw);
+            B_TRY(hipMemcpyAsync(h_cnt.data(), warm_cnt, sizeof(u32) * nw, hipMemcpyDeviceToHost, s));
+            B_TRY(hipStreamSynchronize(s));
+            u64 total = 0;
+            for (u32 c : h_cnt) total += c;
+            u32 h_bound[WARM_SEGS + 1];
+            h_bound[0] = D;
+            h_bound[WARM_SEGS] = H;
+            u64 cum = 0, h = 0;
+            for (int j = 1; j < WARM_SEGS; ++j) {
+                const u64 want = total * (u64)j / WARM_SEGS;
+                while (h < nw && cum + h_cnt[h] <= want) cum += h_cnt[h++];
+                h_bound[j] = (u32)(D + h);
+            }
+            // Segments by width (heads), widest first, dealt to the XCDs in
+            // snake order: each XCD gets WARM_PER_XCD segments of equal line
+            // shares and about equal colour footprints.
+            int order[WARM_SEGS];
+            for (int j = 0; j < WARM_SEGS; ++j) order[j] = j;
+            std::stable_sort(order, order + WARM_SEGS, [&](int a, int c) {
+                return h_bound[a + 1] - h_bound[a] > h_bound[c + 1] - h_bound[c];
+            });
+            for (int r = 0; r < WARM_SEGS; ++r) {
+                const int round = r / (int)BIN_GROUPS, pos = r % (int)BIN_GROUPS;
+                const int x = (round & 1) ? (int)BIN_GROUPS - 1 - pos : pos;
+                xcd_segs.seg[x][round] = (unsigned char)order[r];
+            }
+            B_TRY(hipMalloc(&warm_bound, sizeof(u32) * (WARM_SEGS + 1)));
+            B_TRY(hipMemcpyAsync(warm_bound, h_bound, sizeof(u32) * (WARM_SEGS + 1),
+                                 hipMemcpyHostToDevice, s));
+            B_TRY(hipMalloc(&warm_off, sizeof(u64) * n_bins * (WARM_SEGS + 1)));
+            warm_off_kernel<<<grid_for(n_bins * (WARM_SEGS + 1)), BLK, 0, s>>>(
+                b->slot_key, b->bin_slot, hot_all, n_bins, warm_bound, warm_off, b->bin_hot);
+        } else {
+            B_TRY(hipMemcpyAsync(b->bin_hot, hot_all, sizeof(u64) * n_bins,
+                                 hipMemcpyDeviceToDevice, s));
+        }
         B_TRY(hipStreamSynchronize(s));
         for (int i = 0; i < 2; ++i) {
             (void)hipFree(keys[i]);
             (void)hipFree(vals[i]);
             keys[i] = vals[i] = nullptr;
         }
+    }
+    if (!b->n_hot)
+        B_TRY(hipMemsetAsync(b->bin_hot, 0, sizeof(u64) * n_bins, s));
+    {
+        // Each bin's buffered (warm and cold) slots [c0, c1) as whole aligned
+        // groups of 8 in a refinement's cold-colour buffer.
+        std::vector<u64> h_dense(n_bins, 0), h_pos(n_bins, 0);
+        B_TRY(hipMemcpyAsync(h_dense.data(), b->bin_hot, sizeof(u64) * n_bins,
+                             hipMemcpyDeviceToHost, s));
+        B_TRY(hipStreamSynchronize(s));
+        u64 pos = 0, dense = 0;
+        for (u64 k = 0; k < n_bins; ++k) {
+            h_pos[k] = pos;
+            dense += h_dense[k];
+            const u64 c0 = h_bin_slot[k] + h_dense[k], c1 = h_bin_slot[k + 1];
+            if (c1 > c0) pos += ((c1 + 7) & ~7ull) - (c0 & ~7ull);
+        }
+        b->n_warm = b->n_hot - dense;
+        b->n_cold_pad = pos;
+        if (b->n_warm) {
+            // work items: per XCD, its segments' run in every bin, cut into
+            // pieces of at most WARM_ITEM slots
+            std::vector<u64> h_off(n_bins * (WARM_SEGS + 1));
+            B_TRY(hipMemcpyAsync(h_off.data(), warm_off, sizeof(u64) * h_off.size(),
+                                 hipMemcpyDeviceToHost, s));
+            B_TRY(hipStreamSynchronize(s));
+            std::vector<WarmItem> items;
+            for (u64 x = 0; x < BIN_GROUPS; ++x) {
+                b->warm_first.f[x] = (u32)items.size();
+                for (u64 k = 0; k < n_bins; ++k) {
+                    const u64 c0 = (h_bin_slot[k] + h_dense[k]) & ~7ull;
+                    for (int r = 0; r < WARM_PER_XCD; ++r) {
+                        const int sg = xcd_segs.seg[x][r];
+                        const u64 lo = h_off[k * (WARM_SEGS + 1) + sg];
+                        const u64 hi = h_off[k * (WARM_SEGS + 1) + sg + 1];
+                        for (u64 a = lo; a < hi; a += WARM_ITEM) {
+                            const u32 n = (u32)std::min<u64>(WARM_ITEM, hi - a);
+                            items.push_back({a, h_pos[k] + (a - c0), n, 0});
+                        }
+                    }
+                }
+            }
+            b->warm_first.f[BIN_GROUPS] = (u32)items.size();
+            B_TRY(hipMalloc(&b->warm_items, sizeof(WarmItem) * items.size()));
+            B_TRY(hipMemcpyAsync(b->warm_items, items.data(), sizeof(WarmItem) * items.size(),
+                                 hipMemcpyHostToDevice, s));
+            b->bytes_warm = sizeof(WarmItem) * items.size();
+        }
+        if (pos + SLOT_RUN >= (1ull << 32)) {
+            rbsc_set_error("bin: %llu buffered slots exceed 2^32", (unsigned long long)pos);
+            rc = RBSC_ERR_INVALID;
+            goto done;
+        }
+        B_TRY(hipMalloc(&b->cold_pos, sizeof(u64) * std::max<u64>(n_bins, 1)));
+        B_TRY(hipMemcpyAsync(b->cold_pos, h_pos.data(), sizeof(u64) * n_bins,
+                             hipMemcpyHostToDevice, s));
     }
     // cold slots and the scatter records
     if (b->n_cold) {
@@ -741,12 +919,12 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
         if ((rc = radix_sort_u64(&ctx->sort, keys, vals, b->n_cold, 0, bits, &w, s))) goto done;
         if (g->wt)
             place_cold_kernel<true><<<grid_for(b->n_cold), BLK, 0, s>>>(
-                keys[w], vals[w], b->n_cold, gb, b->bin_slot, b->bin_hot, cold_start, grp_of_bin,
+                keys[w], vals[w], b->n_cold, gb, b->bin_slot, hot_all, b->bin_hot, cold_start, grp_of_bin,
                 b->cold_pos, g->wt, b->slot_key, b->slot_row, b->slot_wt, keys[w ^ 1],
                 vals[w ^ 1]);
         else
             place_cold_kernel<false><<<grid_for(b->n_cold), BLK, 0, s>>>(
-                keys[w], vals[w], b->n_cold, gb, b->bin_slot, b->bin_hot, cold_start, grp_of_bin,
+                keys[w], vals[w], b->n_cold, gb, b->bin_slot, hot_all, b->bin_hot, cold_start, grp_of_bin,
                 b->cold_pos, nullptr, b->slot_key, b->slot_row, nullptr, keys[w ^ 1],
                 vals[w ^ 1]);
         u64* k2[2] = {keys[w ^ 1], keys[w]};
@@ -796,7 +974,8 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     B_TRY(hipGetLastError());
     B_TRY(hipStreamSynchronize(s));
     b->bytes = 16 * (n_bins + 1) + 6 * b->n_slots + (g->wt ? 8 * b->n_slots : 0) + 8 * R +
-               (head_order ? 10 : 8) * b->n_cold + 8 * (b->n_tiles + 1) + sizeof(BinChunk) * b->n_chunks +
+               (head_order ? 10 : 8) * b->n_cold + 8 * (b->n_tiles + 1) +
+               b->bytes_warm + sizeof(BinChunk) * b->n_chunks +
                4 * b->n_zero + 8 * n_bins;
     if (g->bins) {
         g->bytes -= g->bins->bytes;
@@ -807,9 +986,10 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
     b = nullptr;
 done:
     for (u64* p : {n_hot, n_cold, tmp, tot, hot_start, cold_start, bin_n, keys[0], keys[1],
-                   vals[0], vals[1], grp_first})
+                   vals[0], vals[1], grp_first, hot_all, warm_off})
         if (p) (void)hipFree(p);
-    if (grp_of_bin) (void)hipFree(grp_of_bin);
+    for (u32* p : {grp_of_bin, warm_cnt, warm_bound})
+        if (p) (void)hipFree(p);
     if (b) bins_release(b);
     return rc;
 #undef B_TRY

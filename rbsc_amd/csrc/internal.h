@@ -35,8 +35,39 @@ struct BinChunk {
 };                 // bin's only chunk (stores; otherwise adds atomically)
 constexpr u32 CHUNK_FIRST = 1, CHUNK_ONLY = 2;
 constexpr u64 BIN_GROUPS = 8;  // scatter groups, one per XCD
+// Warm heads are cut into WARM_SEGS segments, WARM_PER_XCD per XCD.
+constexpr int WARM_SEGS = 32;
+constexpr int WARM_PER_XCD = WARM_SEGS / (int)BIN_GROUPS;
+struct WarmMap {
+    unsigned char seg[BIN_GROUPS][WARM_PER_XCD];
+};
+// A run of at most WARM_ITEM warm slots of one bin and segment: slots
+// [lo, lo + n) receive their colours at buffer indexes [dst, dst + n).
+struct WarmItem {
+    u64 lo, dst;
+    u32 n, pad;
+};
+constexpr u32 WARM_ITEM = 8192;
+struct WarmFirst {
+    u32 f[BIN_GROUPS + 1];  // items of XCD x: [f[x], f[x + 1])
+};
 struct rbsc_bins {
     u32 log_s = 0, H = 0, GL = 0;
+    // Heads below D (dense) are gathered by the bin pass itself; heads in
+    // [D, H) (warm) are gathered once per round by warm_gather_kernel into
+    // the refinement's buffer, like cold slots receive their colours: the
+    // warm heads are cut into WARM_SEGS segments of about equal lines, dealt
+    // to the XCDs so that each XCD gathers from its own segments only and
+    // their colours stay in its L2.  A bin's slots are its dense, warm and
+    // cold slots, the first two in head order; bin_hot counts the dense ones,
+    // and the buffer covers the warm and cold ones.  The warm slots of a bin
+    // in one segment are one run, cut into work items of at most WARM_ITEM
+    // slots, listed by XCD.
+    u32 D = 0;
+    u64 n_warm = 0;
+    WarmItem* warm_items = nullptr;
+    WarmFirst warm_first{};
+    u64 bytes_warm = 0;
     bool remap = false;             // lines into repeated rows point at their stored rows
     // Hashed layout: refinements on it keep per-node hashed colours
     // randomize(c + r) (u64, written once per node by the rank pass), hot
@@ -46,7 +77,7 @@ struct rbsc_bins {
     bool hashed = false;
     u64 n_bins = 0, n_slots = 0, n_hot = 0, n_cold = 0, n_chunks = 0, n_zero = 0;
     u64* bin_slot = nullptr;        // n_bins + 1: first slot of each bin
-    u64* bin_hot = nullptr;         // n_bins: hot slots of each bin
+    u64* bin_hot = nullptr;         // n_bins: dense slots of each bin
     // The colours of bin b's cold slots [c0, c1) (c0 = bin_slot + bin_hot)
     // change every round; they live in the refinement's own cold-colour
     // buffer (rbsc_ctx::cold, n_cold_pad + 512 entries of u32, or u64 on the
@@ -79,7 +110,7 @@ struct rbsc_bins {
 void bins_release(rbsc_bins* b);
 // Builds g->bins (hot heads below H, 2^log_s rows per bin, chunks of at most
 // chunk_slots slots, scatter tiles of scatter_chunk lines).
-int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 log_s, u64 chunk_slots,
+int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk_slots,
                u64 scatter_chunk, bool head_order, bool hashed);
 // One round on the binned layout: cold colours into their slots, then the
 // signatures of the stored rows into sig.  col: per-node colours by internal

@@ -1610,8 +1610,9 @@ extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts
         rbsc_set_error("rbsc_graph_bin: null argument");
         return RBSC_ERR_INVALID;
     }
-    rbsc_bin_opts o = opts ? *opts : rbsc_bin_opts{0, 0, 0, 0, 0};
+    rbsc_bin_opts o = opts ? *opts : rbsc_bin_opts{0, 0, 0, 0, 0, 0, 0};
     if (o.hot_heads == 0) o.hot_heads = 1u << 22;
+    if (o.dense_heads == 0) o.dense_heads = 1u << 21;
     if (o.rows_per_bin == 0) o.rows_per_bin = 1u << 13;
     if (o.chunk_slots == 0) o.chunk_slots = 1ull << 19;
     if (o.scatter_lines == 0) o.scatter_lines = (o.flags & RBSC_BIN_SLOT_ORDER) ? 1u << 11 : 1u << 14;
@@ -1626,7 +1627,7 @@ extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts
         drop_bins(g);
         return RBSC_OK;
     }
-    return dgraph_bin(ctx, g, o.hot_heads, log_s, o.chunk_slots, o.scatter_lines,
+    return dgraph_bin(ctx, g, o.hot_heads, o.dense_heads, log_s, o.chunk_slots, o.scatter_lines,
                       (o.flags & RBSC_BIN_SLOT_ORDER) == 0, (o.flags & RBSC_BIN_HASHED) != 0);
 }
 
@@ -1643,10 +1644,11 @@ extern "C" int rbsc_graph_bin_info(const rbsc_dgraph* g, uint64_t* info) {
         return RBSC_ERR_INVALID;
     }
     const rbsc_bins* b = g->bins;
-    const uint64_t v[8] = {b ? 1u : 0u, b ? b->n_bins : 0, b ? b->n_hot : 0, b ? b->n_cold : 0,
-                           b ? b->n_chunks : 0, b ? b->n_zero : 0, b ? b->bytes : 0,
-                           b ? ((u64)b->H | ((u64)b->log_s << 32) | ((u64)b->hashed << 40)) : 0};
-    for (int i = 0; i < 8; ++i) info[i] = v[i];
+    const uint64_t v[10] = {b ? 1u : 0u, b ? b->n_bins : 0, b ? b->n_hot : 0, b ? b->n_cold : 0,
+                            b ? b->n_chunks : 0, b ? b->n_zero : 0, b ? b->bytes : 0,
+                            b ? ((u64)b->H | ((u64)b->log_s << 32) | ((u64)b->hashed << 40)) : 0,
+                            b ? b->D : 0, b ? b->n_warm : 0};
+    for (int i = 0; i < 10; ++i) info[i] = v[i];
     return RBSC_OK;
 }
 

@@ -31,18 +31,23 @@ CASES = {
 }
 
 # (hot heads, rows per bin, slots per chunk, cold lines per scatter
-# workgroup, scatter records in slot order, hashed form): defaults,
-# everything cold, everything hot, small bins split over many chunks, the
-# largest bins, scatter chunks of a few lines (not multiples of 8) or above
-# the staged scatter's LDS capacity (taken as that capacity); the raw form on
-# all of them, the hashed form on five
-SHAPES = [(0, 0, 0, 0, False, False), (1, 256, 0, 0, True, False),
-          (1 << 30, 512, 0, 0, False, False), (64, 256, 300, 13, False, False),
-          (4096, 16384, 1000, 1 << 20, True, False), (16, 1024, 0, 1, False, False),
-          (3, 256, 0, 77, True, False), (2, 256, 0, 1 << 20, False, False),
-          (0, 0, 0, 0, False, True), (1, 256, 0, 0, True, True),
-          (64, 256, 300, 13, False, True), (3, 256, 0, 77, True, True),
-          (2, 256, 0, 1 << 20, False, True)]
+# workgroup, scatter records in slot order, hashed form, dense heads):
+# defaults, everything cold, everything hot, small bins split over many
+# chunks, the largest bins, scatter chunks of a few lines (not multiples of
+# 8) or above the staged scatter's LDS capacity (taken as that capacity);
+# warm heads (dense heads below the hot ones): all hot heads warm, a few
+# dense ones with split bins, with slot-order scatter, with cold lines in the
+# same bins; the raw form on all of them, the hashed form on seven
+SHAPES = [(0, 0, 0, 0, False, False, 0), (1, 256, 0, 0, True, False, 0),
+          (1 << 30, 512, 0, 0, False, False, 0), (64, 256, 300, 13, False, False, 0),
+          (4096, 16384, 1000, 1 << 20, True, False, 0), (16, 1024, 0, 1, False, False, 0),
+          (3, 256, 0, 77, True, False, 0), (2, 256, 0, 1 << 20, False, False, 0),
+          (1 << 30, 256, 0, 0, False, False, 1), (4096, 512, 300, 13, False, False, 5),
+          (2048, 256, 0, 0, True, False, 64), (300, 1024, 0, 0, False, False, 40),
+          (0, 0, 0, 0, False, True, 0), (1, 256, 0, 0, True, True, 0),
+          (64, 256, 300, 13, False, True, 0), (3, 256, 0, 77, True, True, 0),
+          (2, 256, 0, 1 << 20, False, True, 0), (1 << 30, 256, 0, 0, False, True, 1),
+          (300, 1024, 300, 0, False, True, 40)]
 
 
 def _trace(g, rounds=64):
@@ -57,17 +62,20 @@ def test_binned_rounds_match_oracle(ctx, name):
     dg = ctx.upload(g)
     plain = dg.refine(colors=True)
     assert (plain.nodes, plain.rounds) == (K, R)
-    for hot, rows, chunk, scatter, slot_order, hashed in SHAPES:
-        dg.bin(hot, rows, chunk, scatter, slot_order, hashed)
+    for shape in SHAPES:
+        hot, rows, chunk, scatter, slot_order, hashed, dense = shape
+        dg.bin(hot, rows, chunk, scatter, slot_order, hashed, dense)
         info = dg.bin_info
         assert info["binned"] and info["hashed"] == hashed
         assert info["hot_lines"] + info["cold_lines"] <= dg.stored_lines
+        assert info["warm_lines"] <= info["hot_lines"]
+        if dense and dense < min(hot, 64):
+            assert info["warm_lines"] > 0
         if chunk and chunk < 1000:
             assert info["split_bins"] > 0
         for r in range(1, R + 1):
             res = dg.refine(max_rounds=r, partition=False, colors=True)
-            assert np.array_equal(res.colors, tr[r - 1]), \
-                f"{(hot, rows, chunk, scatter, slot_order, hashed)} round {r}"
+            assert np.array_equal(res.colors, tr[r - 1]), f"{shape} round {r}"
         full = dg.refine(verify=True, colors=True)
         assert (full.nodes, full.rounds, full.verified) == (K, R, 1)
         assert np.array_equal(full.colors, plain.colors)
@@ -163,11 +171,15 @@ def test_hashed_form_bin_boundaries(ctx):
     g = _coo(n, lines)
     c, K, R, tr = _trace(g)
     dg = ctx.upload(g)
-    for hot, rows, chunk, hashed in ((40, 256, 0, True), (40, 256, 37, True), (5, 256, 9, True),
-                                     (1 << 20, 256, 0, True), (40, 256, 37, False),
-                                     (5, 256, 9, False)):
-        dg.bin(hot, rows, chunk, 3, hashed=hashed)
+    # (dense heads 3 or 17: warm runs that start and end inside groups of 8)
+    for hot, rows, chunk, hashed, dense in ((40, 256, 0, True, 0), (40, 256, 37, True, 0),
+                                            (5, 256, 9, True, 0), (1 << 20, 256, 0, True, 0),
+                                            (40, 256, 37, False, 0), (5, 256, 9, False, 0),
+                                            (40, 256, 0, False, 3), (1 << 20, 256, 37, False, 17),
+                                            (40, 256, 0, True, 17)):
+        dg.bin(hot, rows, chunk, 3, hashed=hashed, dense_heads=dense)
         for r in range(1, R + 1):
             res = dg.refine(max_rounds=r, partition=False, colors=True)
-            assert np.array_equal(res.colors, tr[r - 1]), f"{(hot, rows, chunk, hashed)} round {r}"
+            assert np.array_equal(res.colors, tr[r - 1]), \
+                f"{(hot, rows, chunk, hashed, dense)} round {r}"
     dg.free()
