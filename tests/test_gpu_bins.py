@@ -183,3 +183,25 @@ def test_hashed_form_bin_boundaries(ctx):
             assert np.array_equal(res.colors, tr[r - 1]), \
                 f"{(hot, rows, chunk, hashed, dense)} round {r}"
     dg.free()
+
+
+def test_staged_scatter_sparse_tiles(ctx):
+    # u % 3 lines per node u into random heads among 300,000: a group's cold
+    # lines are so sparse in heads that a tile of 16384 lines spans about
+    # 200,000 heads and touches each bin with a run of a few lines; tiles of
+    # 64 lines, runs of one line; every round against the oracle, raw and
+    # hashed
+    rng = np.random.default_rng(38)
+    n = 300000
+    heads = rng.integers(0, n, n)
+    lines = [(u, int(heads[(u + k) % n])) for u in range(n) for k in range(u % 3)]
+    g = _coo(n, lines)
+    c, K, R, tr = _trace(g)
+    dg = ctx.upload(g)
+    for scatter, hashed in ((0, False), (64, False), (0, True), (64, True)):
+        dg.bin(1, 1024, 0, scatter, hashed=hashed)
+        assert dg.bin_info["cold_lines"] > 0
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True)
+            assert np.array_equal(res.colors, tr[r - 1]), f"{(scatter, hashed)} round {r}"
+    dg.free()
