@@ -369,9 +369,10 @@ def main():
     # The signature pass (the reference's set_values + ReduceByKey): 8 B per
     # stored line (head id + neighbour colour) plus 16 B per stored row (row
     # offset or per-row term, signature write), once per round.  On the
-    # binned layout the pass is two launches, cold_scatter_kernel (phase
-    # "scatter") and bin_sig_kernel (phase "signature"); on the plain layout
-    # it is sig_kernel alone.
+    # binned layout the pass is three launches, warm_gather_kernel and
+    # cold_stage_kernel (phase "scatter": they fill the buffer of warm and
+    # cold colours) and bin_sig_kernel (phase "signature"); on the plain
+    # layout it is sig_kernel alone.
     b_sig = 8 * E_st + 16 * U
     per_round = {k: phases[k][0] / rounds for k in phases if k != "setup"}
     ROUND = ("scatter", "signature", "finish", "gather", "sort", "rank")
@@ -380,7 +381,8 @@ def main():
     binned = bin_info is not None and (bin_info["hashed"] or not (flags & rbsc_amd.FLAG_HASHED64))
     sig_ms = per_round["signature"] + (per_round["scatter"] if binned else 0.0)
     pass_name = "signature_pass" if binned else "sig_kernel"
-    kernels = ({"cold_scatter_kernel": per_round["scatter"], "bin_sig_kernel": per_round["signature"]}
+    kernels = ({"warm_gather_kernel + cold_stage_kernel": per_round["scatter"],
+                "bin_sig_kernel": per_round["signature"]}
                if binned else {"sig_kernel": per_round["signature"]})
     traffic = measured_traffic(args.config, pass_name) if args.parts == 1 and not args.hot else None
     out = {
