@@ -5,30 +5,35 @@
 // the many rarely-used ("cold") heads miss to the fabric, moving a 128-byte
 // line for 4 useful bytes.  This layout computes the same signatures
 //   sig(u) = sum over u's lines (u, v, w) of randomize(c(v) + r) * w  (mod 2^64)
-// (rbsc.cu:80-87 set_values + rbsc.cu:366-377 ReduceByKey) in two passes:
+// (rbsc.cu:80-87 set_values + rbsc.cu:366-377 ReduceByKey) over three
+// classes of gathered heads (internal ids are in gathered in-degree order):
+// dense (head < D), warm (D <= head < H) and cold (H <= head < gather
+// limit).  Each round:
 //
-//   cold_scatter_kernel  writes each cold line's colour (H <= head < gather
-//                        limit) into the refinement's cold-colour buffer, at
-//                        the line's slot position.  Bins are split into 8
-//                        groups, one per XCD; a group's cold lines, in head
-//                        order, are cut into chunks of a fixed number of
-//                        lines, one workgroup per chunk (chunk c of group g is
-//                        workgroup 8c + g: workgroups are dealt to the XCDs
-//                        round robin).  A chunk spans a narrow head range, so
-//                        its colour reads stay in cache.  Default (staged):
-//                        the workgroup reads the colours in head order into
-//                        LDS and writes them out in slot order; a bin's cold
-//                        slots are in head order, so a chunk's lines for one
+//   warm_gather_kernel   copies the colours of the warm lines into the
+//                        refinement's buffer.  The warm heads are cut into
+//                        segments of equal line counts, a few per XCD; each
+//                        XCD gathers only from its own segments, whose
+//                        colours its L2 holds.
+//   cold_stage_kernel    writes each cold line's colour into the buffer.
+//                        Bins are split into 8 groups, one per XCD; a
+//                        group's cold lines, in head order, are cut into
+//                        tiles of a fixed number of lines, one workgroup per
+//                        tile (tile c of group g is workgroup 8c + g:
+//                        workgroups are dealt to the XCDs round robin).  The
+//                        workgroup reads the colours in head order into LDS
+//                        and writes them out in slot order; a bin's cold
+//                        slots are in head order, so a tile's lines for one
 //                        bin fill one run of consecutive positions, and a
-//                        wave's stores cover a few runs.  RBSC_BIN_SLOT_ORDER
-//                        keeps the records in slot order instead (colour
-//                        reads out of order, no LDS).
+//                        wave's stores cover a few runs.  (RBSC_BIN_SLOT_ORDER
+//                        selects cold_scatter_kernel instead: records kept in
+//                        slot order, colour reads out of order, no LDS.)
 //   bin_sig_kernel       one workgroup per chunk of a bin of 2^log_s rows:
-//                        row sums in LDS, hot slots (head < H) gathered from
-//                        the L2-resident hot colours in head order (a wave's
-//                        64 gathers fall into few cache lines), cold slots
-//                        read from the buffer, lines into heads that are never
-//                        gathered folded in per row as dang_h * dsum[row].
+//                        row sums in LDS, dense slots gathered in head order
+//                        (a wave's 64 gathers fall into few cache lines),
+//                        warm and cold slots read from the buffer, lines
+//                        into heads that are never gathered folded in per
+//                        row as dang_h * dsum[row].
 //
 // The layout is read-only during refinement: the per-round cold colours
 // live in the refinement's context, so several contexts may refine one
