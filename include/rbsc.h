@@ -100,6 +100,15 @@ typedef struct rbsc_opts {
  * keeps positions in colour order instead, as with initial colours (results
  * identical). */
 #define RBSC_FLAG_COLOUR_ORDER 128u
+/* Node order only: the sort moves 8 bytes per node (the signature's top 32
+ * bits with the node below them) instead of 16 (signature, colour and node),
+ * and the tie fix-up and rank read the signature's low bits and the colours
+ * by node where neighbouring top bits are equal.  Used by default in rounds
+ * whose previous block count is at least 9/10 of the ranked nodes;
+ * RBSC_FLAG_PACKED uses it in every such round (a test hook),
+ * RBSC_FLAG_UNPACKED in none.  Results are identical. */
+#define RBSC_FLAG_PACKED 512u
+#define RBSC_FLAG_UNPACKED 1024u
 
 /* Result.  nodes / done / time_s are the reference's three output lines
  * (rbsc.cu:492-494). */

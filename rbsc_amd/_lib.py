@@ -33,6 +33,8 @@ FLAG_TIESMALL = 16
 FLAG_FULLSORT = 32
 FLAG_NOBINS = 64
 FLAG_COLOUR_ORDER = 128
+FLAG_PACKED = 512
+FLAG_UNPACKED = 1024
 BIN_SLOT_ORDER = 1
 BIN_HASHED = 2
 

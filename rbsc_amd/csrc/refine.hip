@@ -339,6 +339,9 @@ __global__ __launch_bounds__(256) void gather_keys_hist_kernel(
 // Sorting by (low bits, position) is stable.  A list that overflows raises
 // *overflow and the caller redoes the round's sort on all 64 bits.  `lo` is
 // 32 normally; 48 is a verification hook that makes ties frequent.
+// Packed keys (sig not null): a key holds the signature's top bits and, below
+// `lo`, the node; the low bits of the signature are read from sig[node], only
+// where neighbours' top bits are equal.
 constexpr int TIE_BLK = 256;
 constexpr u32 TIE_LIST = 512;  // small runs a workgroup collects in LDS before spilling
 constexpr unsigned TIE_GRID = 2048;
@@ -349,8 +352,13 @@ constexpr u32 TIE_MID = 4096;
 constexpr u64 TIE_MID_CAP = 65536;   // mid runs listed per round
 constexpr u64 TIE_LONG_CAP = 4096;   // long runs listed per round
 // counts[0..2]: small, mid, long runs listed.
+__device__ __forceinline__ u64 tie_low(u64 k, const u64* __restrict__ sig, u64 lowmask) {
+    return (sig ? sig[k & lowmask] : k) & lowmask;
+}
+
 __global__ __launch_bounds__(TIE_BLK) void tie_find_kernel(
-    const u64* __restrict__ keys, u64 n, u32 lo, u32* __restrict__ mark, u32 epoch,
+    const u64* __restrict__ keys, const u64* __restrict__ sig, u64 n, u32 lo,
+    u32* __restrict__ mark, u32 epoch,
     u64* __restrict__ runs_s, u64 cap_s, u64* __restrict__ runs_m, u64* __restrict__ runs_l,
     u32* __restrict__ counts, u32* __restrict__ overflow) {
     // Small runs are gathered per workgroup in LDS and reserved in the global
@@ -360,6 +368,7 @@ __global__ __launch_bounds__(TIE_BLK) void tie_find_kernel(
     if (threadIdx.x == 0) lcount = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const u64 lowmask = (1ULL << lo) - 1;
     for (u64 t = blockIdx.x; t * TF_TILE < n; t += gridDim.x) {  // uniform per workgroup
         const u64 wbase = t * TF_TILE + (u64)wave * (64 * TF_ITEMS);
         u64 k[TF_ITEMS];
@@ -376,7 +385,9 @@ __global__ __launch_bounds__(TIE_BLK) void tie_find_kernel(
             lk = __shfl(k[j], 63);
             const u64 i = wbase + 64 * j + lane;
             const u64 top = k[j] >> lo;
-            if (!(i < n && i > 0 && top == (pk >> lo) && k[j] != pk)) continue;  // not a break
+            if (!(i < n && i > 0 && top == (pk >> lo))) continue;
+            if (sig ? tie_low(k[j], sig, lowmask) == tie_low(pk, sig, lowmask) : k[j] == pk)
+                continue;  // not a break
             // Run bounds by galloping outward from the break, then bisection,
             // so a run of length L costs O(log L) probes.
             u64 b = i - 1, step = 1;  // keys[b] is in the run
@@ -459,6 +470,7 @@ __global__ __launch_bounds__(TIE_BLK) void tie_find_kernel(
 // member whose position the m-th key names.
 __global__ __launch_bounds__(256) void tie_small_kernel(u64* __restrict__ keys,
                                                         u64* __restrict__ P,
+                                                        const u64* __restrict__ sig,
                                                         const u64* __restrict__ runs,
                                                         const u32* __restrict__ count, u64 cap,
                                                         u32 lo) {
@@ -472,8 +484,8 @@ __global__ __launch_bounds__(256) void tie_small_kernel(u64* __restrict__ keys,
         u64 kv = 0, pv = 0, comp = ~0ULL;
         if (lane < L) {
             kv = keys[a + lane];
-            pv = P[a + lane];
-            comp = ((kv & lowmask) << 6) | lane;
+            pv = P ? P[a + lane] : 0;
+            comp = (tie_low(kv, sig, lowmask) << 6) | lane;
         }
         u32 K = 2;
         while (K < L) K <<= 1;
@@ -490,7 +502,7 @@ __global__ __launch_bounds__(256) void tie_small_kernel(u64* __restrict__ keys,
         const u64 ks = __shfl(kv, src), ps = __shfl(pv, src);
         if (lane < L) {
             keys[a + lane] = ks;
-            P[a + lane] = ps;
+            if (P) P[a + lane] = ps;
         }
     }
 }
@@ -498,6 +510,7 @@ __global__ __launch_bounds__(256) void tie_small_kernel(u64* __restrict__ keys,
 // Mid runs: workgroup g sorts runs g, g + grid, ... in LDS; keys
 // (low bits << 12) | position, a bitonic network over the next power of two.
 __global__ __launch_bounds__(256) void tie_mid_kernel(u64* __restrict__ keys, u64* __restrict__ P,
+                                                      const u64* __restrict__ sig,
                                                       const u64* __restrict__ runs,
                                                       const u32* __restrict__ count, u32 lo) {
     __shared__ u64 s_comp[TIE_MID], s_k[TIE_MID], s_p[TIE_MID];
@@ -512,8 +525,8 @@ __global__ __launch_bounds__(256) void tie_mid_kernel(u64* __restrict__ keys, u6
             if (i < L) {
                 const u64 kv = keys[a + i];
                 s_k[i] = kv;
-                s_p[i] = P[a + i];
-                s_comp[i] = ((kv & lowmask) << 12) | i;
+                s_p[i] = P ? P[a + i] : 0;
+                s_comp[i] = (tie_low(kv, sig, lowmask) << 12) | i;
             } else {
                 s_comp[i] = ~0ULL;
             }
@@ -536,7 +549,7 @@ __global__ __launch_bounds__(256) void tie_mid_kernel(u64* __restrict__ keys, u6
         for (u32 i = threadIdx.x; i < L; i += 256) {
             const u32 src = (u32)(s_comp[i] & (TIE_MID - 1));
             keys[a + i] = s_k[src];
-            P[a + i] = s_p[src];
+            if (P) P[a + i] = s_p[src];
         }
         __syncthreads();
     }
@@ -549,6 +562,7 @@ __global__ void tie_len_kernel(const u64* __restrict__ runs, const u32* __restri
 }
 // Compact element j belongs to run r (largest r with pre[r] <= j).
 __global__ void tie_gather_kernel(const u64* __restrict__ keys, const u64* __restrict__ P,
+                                  const u64* __restrict__ sig,
                                   const u64* __restrict__ runs, const u64* __restrict__ pre, u32 nr,
                                   u64 total, u32 lo, u64* __restrict__ cl, u64* __restrict__ cv,
                                   u64* __restrict__ crun, u64* __restrict__ cfull,
@@ -562,11 +576,11 @@ __global__ void tie_gather_kernel(const u64* __restrict__ keys, const u64* __res
     }
     const u64 i = runs[2 * (u64)a] + (j - pre[a]);
     const u64 k = keys[i];
-    cl[j] = k & ((1ULL << lo) - 1);
+    cl[j] = tie_low(k, sig, (1ULL << lo) - 1);
     cv[j] = j;
     crun[j] = a;
     cfull[j] = k;
-    cP[j] = P[i];
+    cP[j] = P ? P[i] : 0;
     corig[j] = i;
 }
 __global__ void tie_key2_kernel(const u64* __restrict__ cv, const u64* __restrict__ crun, u64 total,
@@ -581,7 +595,7 @@ __global__ void tie_scatter_kernel(const u64* __restrict__ cv, const u64* __rest
     if (m >= total) return;
     const u64 src = cv[m];
     keys[corig[m]] = cfull[src];
-    P[corig[m]] = cP[src];
+    if (P) P[corig[m]] = cP[src];
 }
 
 constexpr int RANK_ITEMS = 16;
@@ -691,6 +705,139 @@ __global__ __launch_bounds__(256) void rank_kernel(
             if (Pout) Pout[i] = (rj << 32) | node;
             if (c32) reinterpret_cast<u32*>(hc)[node] = (u32)rj;
             else hc[node] = rbsc_randomize(rj + salt_next);
+            if (i == n - 1) *K_out = (u32)(rj + 1);
+        }
+    }
+}
+
+// Dense rank over packed keys (node order, raw u32 colours): a key holds
+// the signature's top bits and, below `lo`, the node (the sort moved 8 bytes
+// per node).  A position starts a new (sig, colour) pair when its top bits
+// differ from its predecessor's; only where they are equal are the low bits
+// of both signatures (sig[node]) and both old colours (col[node]) read and
+// compared (and a smaller colour after an equal signature sets *disorder).
+// Writes col_out[node] = rank.  Tiles, waves and the decoupled look-back are
+// those of rank_kernel.
+__global__ __launch_bounds__(256) void rank_packed_kernel(
+    const u64* __restrict__ keys, const u64* __restrict__ sig, const u32* __restrict__ col,
+    u32* __restrict__ col_out, u64 n, u32 lo, u64 rank_offset, u64* status, u64* ticket,
+    u64 ticket_base, u32 epoch, u32* K_out, u32* disorder, u32* err) {
+    __shared__ u64 s_wave[4];
+    __shared__ u64 s_tile, s_prefix;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd((unsigned long long*)ticket, 1ULL) - ticket_base;
+    __syncthreads();
+    const u64 tile = s_tile;
+    const u64 wbase = tile * RANK_TILE + (u64)wave * (64 * RANK_ITEMS);
+    const u64 lowmask = (1ULL << lo) - 1;
+    u64 k[RANK_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RANK_ITEMS; ++j) {
+        const u64 i = wbase + 64 * j + lane;
+        k[j] = i < n ? keys[i] : 0;
+    }
+    // neighbours of the wave's first and last positions
+    const u64 wend = wbase + 64 * RANK_ITEMS;
+    u64 lk = (wbase > 0 && wbase < n) ? keys[wbase - 1] : 0;
+    const u64 nk_last = wend < n ? keys[wend] : 0;
+    // low signature bits and colour of the first position's predecessor,
+    // read only when it shares the first position's top bits
+    u64 ls = 0;
+    u32 lc = 0;
+    if (wbase > 0 && wbase < n && (lk >> lo) == (k[0] >> lo)) {
+        ls = sig[lk & lowmask] & lowmask;
+        lc = col[lk & lowmask];
+    }
+    u64 bal[RANK_ITEMS];
+    u64 wcount = 0;
+#pragma unroll
+    for (int j = 0; j < RANK_ITEMS; ++j) {
+        const u64 i = wbase + 64 * j + lane;
+        const u64 top = k[j] >> lo;
+        u64 pk = __shfl_up(k[j], 1);
+        if (lane == 0) pk = lk;
+        u64 nk = __shfl_down(k[j], 1);
+        if (j + 1 < RANK_ITEMS) {
+            const u64 nx = __shfl(k[j + 1 < RANK_ITEMS ? j + 1 : j], 0);
+            if (lane == 63) nk = nx;
+        } else if (lane == 63) {
+            nk = nk_last;
+        }
+        // a position in a run of equal top bits reads its own low bits and
+        // colour once; its predecessor's come from the neighbouring lane
+        const bool with_pred = i < n && i > 0 && top == (pk >> lo);
+        const bool with_next = i + 1 < n && top == (nk >> lo);
+        u64 so = 0;
+        u32 co = 0;
+        if (with_pred || with_next) {
+            so = sig[k[j] & lowmask] & lowmask;
+            co = col[k[j] & lowmask];
+        }
+        u64 sp = __shfl_up(so, 1);
+        u32 cp = __shfl_up(co, 1);
+        if (lane == 0) {
+            sp = ls;
+            cp = lc;
+        }
+        bool f = false;
+        if (i < n && i > 0) {
+            if (!with_pred) {
+                f = true;
+            } else {
+                f = so != sp || co != cp;
+                if (so == sp && co < cp) atomicOr(disorder, 1u);
+            }
+        }
+        bal[j] = __ballot(f);
+        wcount += (u64)__popcll(bal[j]);
+        lk = __shfl(k[j], 63);  // predecessor of the next step's lane 0
+        ls = __shfl(so, 63);
+        lc = __shfl(co, 63);
+    }
+    if (lane == 0) s_wave[wave] = wcount;
+    __syncthreads();
+    if (tid == 0) {
+        const u64 agg = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        u64* st = status + tile;
+        u64 excl = 0;
+        if (tile == 0) {
+            lb_store(st, lb_pack(LB_PREFIX, epoch, agg));
+        } else {
+            lb_store(st, lb_pack(LB_AGG, epoch, agg));
+            long long q = (long long)tile - 1;
+            u32 spins = 0;
+            while (q >= 0) {
+                const u64 sw = lb_load(status + q);
+                const u64 state = sw >> LB_STATE_SHIFT;
+                const u32 ep = (u32)((sw >> LB_EPOCH_SHIFT) & LB_EPOCH_MASK);
+                if (state == 0 || ep != (epoch & LB_EPOCH_MASK)) {
+                    if (++spins > LB_SPIN_LIMIT) {
+                        atomicOr(err, KERR_SPIN);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += sw & LB_COUNT_MASK;
+                if (state == LB_PREFIX) break;
+                --q;
+            }
+            lb_store(st, lb_pack(LB_PREFIX, epoch, excl + agg));
+        }
+        s_prefix = excl;
+    }
+    __syncthreads();
+    u64 r = s_prefix + rank_offset;
+    for (int w = 0; w < wave; ++w) r += s_wave[w];
+    const u64 below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    const u64 upto = below | (1ULL << lane);
+#pragma unroll
+    for (int j = 0; j < RANK_ITEMS; ++j) {
+        const u64 i = wbase + 64 * j + lane;
+        const u64 rj = r + (u64)__popcll(bal[j] & upto);
+        r += (u64)__popcll(bal[j]);
+        if (i < n) {
+            col_out[k[j] & lowmask] = (u32)rj;
             if (i == n - 1) *K_out = (u32)(rj + 1);
         }
     }
@@ -1161,7 +1308,13 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         // left for fix_long after the round's host sync (d_scalar[10] counts
         // them).  Sets d_scalar[1] when a run list overflows.
         const u32 lo = (prm.flags & RBSC_FLAG_SPLIT16) ? 48 : 32;
-        auto fix_ties = [&](u64* keys, u64* P) {
+        // packed node-order sort this round (see sort_and_rank_nodes); K is
+        // the previous round's block count
+        const bool packed_round =
+            node_order && !(prm.flags & (RBSC_FLAG_FULLSORT | RBSC_FLAG_UNPACKED)) &&
+            ((prm.flags & RBSC_FLAG_PACKED) || (u64)K * 10 >= (u64)n_rank * 9) &&
+            (n_rank >> lo) == 0;
+        auto fix_ties = [&](u64* keys, u64* P, const u64* psig) {
             const u64 cap = (prm.flags & RBSC_FLAG_TIESMALL) ? 2 : w.tie_cap;
             if (++w.tie_epoch == 0) {  // wrapped: clear the stamps
                 fill_u32_kernel<<<grid_for(w.cap_n), BLK, 0, s>>>(w.tie_mark, w.cap_n, 0);
@@ -1169,16 +1322,17 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             }
             fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
             const unsigned tg = (unsigned)std::min<u64>((n_rank + TF_TILE - 1) / TF_TILE, TIE_GRID);
-            tie_find_kernel<<<tg, TIE_BLK, 0, s>>>(keys, n_rank, lo, w.tie_mark, w.tie_epoch,
+            tie_find_kernel<<<tg, TIE_BLK, 0, s>>>(keys, psig, n_rank, lo, w.tie_mark, w.tie_epoch,
                                                    w.tie_runs, cap, w.tie_runs_m, w.tie_runs_l,
                                                    ctx->d_scalar + 8, ctx->d_scalar + 1);
-            tie_small_kernel<<<1024, 256, 0, s>>>(keys, P, w.tie_runs, ctx->d_scalar + 8, cap, lo);
-            tie_mid_kernel<<<256, 256, 0, s>>>(keys, P, w.tie_runs_m, ctx->d_scalar + 9, lo);
+            tie_small_kernel<<<1024, 256, 0, s>>>(keys, P, psig, w.tie_runs, ctx->d_scalar + 8, cap,
+                                                  lo);
+            tie_mid_kernel<<<256, 256, 0, s>>>(keys, P, psig, w.tie_runs_m, ctx->d_scalar + 9, lo);
         };
         // The nl long runs listed this round (host-driven): members gathered
         // into a compact array, sorted by low bits and then by run (both
         // stable), scattered back.
-        auto fix_long = [&](u64* keys, u64* P, u32 nl) -> int {
+        auto fix_long = [&](u64* keys, u64* P, u32 nl, const u64* psig) -> int {
             tie_len_kernel<<<grid_for(TIE_LONG_CAP), BLK, 0, s>>>(w.tie_runs_l, ctx->d_scalar + 10,
                                                                   TIE_LONG_CAP, w.tie_pre);
             int r3 = scan_excl_u64(w.tie_pre, w.tie_pre, TIE_LONG_CAP, w.tie_pre + TIE_LONG_CAP,
@@ -1201,7 +1355,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             u64* cv[2] = {w.tie_buf + 2 * C, w.tie_buf + 3 * C};
             u64 *crun = w.tie_buf + 4 * C, *cfull = w.tie_buf + 5 * C, *cP = w.tie_buf + 6 * C,
                 *corig = w.tie_buf + 7 * C;
-            tie_gather_kernel<<<grid_for(total), BLK, 0, s>>>(keys, P, w.tie_runs_l, w.tie_pre, nl,
+            tie_gather_kernel<<<grid_for(total), BLK, 0, s>>>(keys, P, psig, w.tie_runs_l, w.tie_pre, nl,
                                                              total, lo, cl[0], cv[0], crun, cfull,
                                                              cP, corig);
             int wa = 0;
@@ -1223,14 +1377,22 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         // positions' output (null in node order); kept for a second rank pass
         // after long runs are fixed.
         u64 *rk_keys = nullptr, *rk_P = nullptr, *rk_Pout = nullptr;
+        bool rk_packed = false;  // rk_keys are packed keys (node order)
         auto launch_rank = [&]() {
             const u64 tiles = (n_rank + RANK_TILE - 1) / RANK_TILE;
             const u32 ep = lb_next_epoch(&ctx->sort, s);
             prof_begin(ctx, 3);
-            rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
-                rk_keys, rk_P, rk_Pout, w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
-                w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
-                ctx->d_scalar + 3, ctx->d_err);
+            if (rk_packed)
+                rank_packed_kernel<<<(unsigned)tiles, 256, 0, s>>>(
+                    rk_keys, w.sig, reinterpret_cast<const u32*>(w.hc),
+                    reinterpret_cast<u32*>(w.hc2), n_rank, lo, rank_offset, w.rank_status,
+                    ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar, ctx->d_scalar + 3,
+                    ctx->d_err);
+            else
+                rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
+                    rk_keys, rk_P, rk_Pout, w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
+                    w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
+                    ctx->d_scalar + 3, ctx->d_err);
             rep_colors(w.hc2);
             prof_end(ctx);
             ctx->sort.ticket_total += tiles;
@@ -1256,13 +1418,14 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             prof_begin(ctx, 2);
             int r2 = radix_sort_u64(&ctx->sort, kk, pp, n_rank, bit0, 64, &which, s, fused);
             if (r2) return r2;
-            if (!full) fix_ties(kk[which], pp[which]);
+            if (!full) fix_ties(kk[which], pp[which], nullptr);
             else fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
             prof_end(ctx);
             sorted_pc = (pp[which] == w.P[0]) ? 0 : 1;
             rk_keys = kk[which];
             rk_P = pp[which];
             rk_Pout = pp[which ^ 1];
+            rk_packed = false;
             launch_rank();
             return RBSC_OK;
         };
@@ -1276,19 +1439,38 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         // same 64-bit sum.  The rank pass flags any pair out of colour order
         // and the round's node side is then redone from positions in colour
         // order.
+        //
+        // Packed (rounds whose previous block count is at least 9/10 of the
+        // ranked nodes, so few neighbours share top bits): the sort moves 8
+        // bytes per node, the signature's top bits with the node below them,
+        // and the tie fix-up and rank read the low bits and colours by node
+        // where top bits are equal.
         auto sort_and_rank_nodes = [&](bool full) -> int {
             u64* kk[2] = {w.keys[0], w.keys[1]};
             u64* pp[2] = {w.P[0], w.P[1]};
             prof_begin(ctx, 2);
+            if (!full && packed_round) {
+                int r2 = radix_sort_nodes_packed(&ctx->sort, w.sig, kk, n_rank, (int)lo, 64, s);
+                if (r2) return r2;
+                fix_ties(kk[0], nullptr, w.sig);
+                prof_end(ctx);
+                rk_keys = kk[0];
+                rk_P = nullptr;
+                rk_Pout = nullptr;
+                rk_packed = true;
+                launch_rank();
+                return RBSC_OK;
+            }
             int r2 = radix_sort_nodes(&ctx->sort, w.sig, reinterpret_cast<const u32*>(w.hc), kk,
                                       pp, n_rank, full ? 0 : (int)lo, 64, s);
             if (r2) return r2;
-            if (!full) fix_ties(kk[0], pp[0]);
+            if (!full) fix_ties(kk[0], pp[0], nullptr);
             else fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
             prof_end(ctx);
             rk_keys = kk[0];
             rk_P = pp[0];
             rk_Pout = nullptr;
+            rk_packed = false;
             launch_rank();
             return RBSC_OK;
         };
@@ -1313,7 +1495,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             // array and rank again (the rank pass is a pure function of the
             // sorted arrays).
             ++ctx->long_run_rounds;
-            rc = fix_long(rk_keys, rk_P, ctx->h_scalar[10]);
+            rc = fix_long(rk_keys, rk_P, ctx->h_scalar[10], rk_packed ? w.sig : nullptr);
             if (rc) return rc;
             fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 3, 1, 0);
             launch_rank();

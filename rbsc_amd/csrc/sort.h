@@ -51,6 +51,12 @@ int radix_sort_u64(SortWorkspace* ws, u64* keys[2], u64* vals[2], u64 n, int bit
 int radix_sort_nodes(SortWorkspace* ws, const u64* key0, const u32* col, u64* keys[2],
                      u64* vals[2], u64 n, int bit_begin, int bit_end, hipStream_t s);
 
+// Stable sort of n < 2^bit_begin nodes in id order by key0 bits
+// [bit_begin, bit_end), keys only: element i is (key0[i] with the bits below
+// bit_begin replaced by i); the result is in keys[0].
+int radix_sort_nodes_packed(SortWorkspace* ws, const u64* key0, u64* keys[2], u64 n, int bit_begin,
+                            int bit_end, hipStream_t s);
+
 // Digit histograms in LDS: SORT_HC copies per digit position, copy = lane
 // mod SORT_HC, each padded to 257 words so equal digits of neighbouring lanes
 // fall into different banks; many equal keys (the early rounds of a

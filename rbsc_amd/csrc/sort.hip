@@ -110,7 +110,7 @@ __global__ __launch_bounds__(THREADS, RBSC_SORT_OCC) void onesweep_kernel(
     const u64* __restrict__ kin, const u64* __restrict__ vin, u64* __restrict__ kout,
     u64* __restrict__ vout, u64 n, u32 shift, const u64* __restrict__ digit_base,
     u64* status, u64* ticket, u64 ticket_base, u32 epoch, u32* err) {
-    constexpr bool HAS_VALS = VM != 0;
+    constexpr bool HAS_VALS = VM == 1 || VM == 2;
     __shared__ u64 s_stage[TILE];
     __shared__ unsigned char s_digit[TILE];
     __shared__ u32 s_whist[WAVES][RADIX];
@@ -133,6 +133,8 @@ __global__ __launch_bounds__(THREADS, RBSC_SORT_OCC) void onesweep_kernel(
     for (int j = 0; j < ITEMS; ++j) {
         const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
         key[j] = idx < n ? kin[idx] : ~0ULL;
+        // VM 3: packed key, the bits from `shift` up with the index below
+        if (VM == 3 && idx < n) key[j] = ((key[j] >> shift) << shift) | idx;
     }
     // Values are loaded now so their latency hides behind ranking and look-back.
     u64 val[HAS_VALS ? ITEMS : 1];
@@ -353,6 +355,42 @@ int radix_sort_nodes(SortWorkspace* ws, const u64* key0, const u32* col, u64* ke
             onesweep_kernel<1><<<(unsigned)tiles, THREADS, 0, s>>>(
                 keys[out ^ 1], vals[out ^ 1], keys[out], vals[out], n, shift,
                 ws->digit_base + p * RADIX, ws->status, ws->ticket, ws->ticket_total, ep, ws->err);
+        ws->ticket_total += tiles;
+    }
+    HIP_TRY(hipGetLastError());
+    return RBSC_OK;
+}
+
+int radix_sort_nodes_packed(SortWorkspace* ws, const u64* key0, u64* keys[2], u64 n, int bit_begin,
+                            int bit_end, hipStream_t s) {
+    if (n == 0 || bit_end <= bit_begin) return RBSC_OK;
+    if ((bit_end - bit_begin) % 8 != 0 || bit_end > 64 || bit_begin < 0 ||
+        bit_begin > 63 || (n >> bit_begin) != 0) {
+        rbsc_set_error("radix_sort_nodes_packed: bit range [%d,%d) or %llu keys not supported",
+                       bit_begin, bit_end, (unsigned long long)n);
+        return RBSC_ERR_INVALID;
+    }
+    const u64 tiles = (n + TILE - 1) / TILE;
+    if (tiles + 1 > ws->status_tiles) {
+        rbsc_set_error("radix_sort_nodes_packed: workspace sized for fewer keys");
+        return RBSC_ERR_INVALID;
+    }
+    const int npass = (bit_end - bit_begin) / 8;
+    launch_hist(key0, n, bit_begin, npass, ws->hist, s);
+    scan_hist_kernel<<<1, THREADS, 0, s>>>(ws->hist, npass, ws->digit_base);
+    // pass p writes buffer (npass - 1 - p) & 1, so the last pass writes buffer 0
+    for (int p = 0; p < npass; ++p) {
+        const u32 ep = lb_next_epoch(ws, s);
+        const u32 shift = (u32)(bit_begin + 8 * p);
+        const int out = (npass - 1 - p) & 1;
+        if (p == 0)
+            onesweep_kernel<3><<<(unsigned)tiles, THREADS, 0, s>>>(
+                key0, nullptr, keys[out], nullptr, n, shift, ws->digit_base, ws->status, ws->ticket,
+                ws->ticket_total, ep, ws->err);
+        else
+            onesweep_kernel<0><<<(unsigned)tiles, THREADS, 0, s>>>(
+                keys[out ^ 1], nullptr, keys[out], nullptr, n, shift, ws->digit_base + p * RADIX,
+                ws->status, ws->ticket, ws->ticket_total, ep, ws->err);
         ws->ticket_total += tiles;
     }
     HIP_TRY(hipGetLastError());

@@ -343,14 +343,17 @@ def test_xcd_partitioned_single_launch(ctx, flags):
         dg.free()
 
 
-@pytest.mark.parametrize("flags", [0, 32, 8, 8 | 16, 16, 128, 128 | 32, 128 | 8, 128 | 8 | 16])
+@pytest.mark.parametrize("flags", [0, 32, 8, 8 | 16, 16, 128, 128 | 32, 128 | 8, 128 | 8 | 16,
+                                   512, 512 | 8, 512 | 8 | 16, 1024, 1024 | 8])
 def test_sort_split_and_tie_fixup(ctx, flags):
     """Top-half sort + colliding-run fix-up (default, 0), full 64-bit sort
     (32), frequent collisions via a 16-bit split (8), and the full-sort
     fallback forced by a 2-entry run list (16); on the R-MAT (no initial
     colours) the default sorts nodes from id order, where the forced
     fallback takes the exact redo from colour order, and 128 keeps
-    positions in colour order instead; the BA graph has initial colours."""
+    positions in colour order instead; 512 sorts packed 8-byte keys in
+    every node-order round (low signature bits and colours read by node
+    where top bits tie), 1024 in none; the BA graph has initial colours."""
     for g in (gen.rmat(15, 1 << 19, clone_frac=0.2, seed=15),
               gen.ba(30000, 3, 0.1, 3, seed=9)):
         c, K, R, tr = _oracle_full(g, trace_rounds=64)
@@ -361,14 +364,14 @@ def test_sort_split_and_tie_fixup(ctx, flags):
         dg.free()
 
 
-@pytest.mark.parametrize("flags", [8, 8 | 128])
+@pytest.mark.parametrize("flags", [8, 8 | 128, 8 | 512])
 def test_tie_runs_of_every_length(ctx, flags):
     """A class of 6000 rows with one signature (each row one line into its own
     line-less node) beside an R-MAT with many distinct signatures: under the
     16-bit split (8) the class's run collides with other signatures, so the
     long-run path (sorted in a compact array, rank redone) runs next to the
     in-kernel small and mid sorts; every round equals the oracle's, in node
-    order and in colour order (128)."""
+    order and in colour order (128), with packed keys (512)."""
     base = gen.rmat(16, 1 << 20, clone_frac=0.1, seed=41)
     n0 = base.n_nodes
     src = [base.sources()]
