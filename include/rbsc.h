@@ -104,7 +104,7 @@ typedef struct rbsc_opts {
  * bits with the node below them) instead of 16 (signature, colour and node),
  * and the tie fix-up and rank read the signature's low bits and the colours
  * by node where neighbouring top bits are equal.  Used by default in rounds
- * whose previous block count is at least 9/10 of the ranked nodes;
+ * whose previous block count is at least half the ranked nodes;
  * RBSC_FLAG_PACKED uses it in every such round (a test hook),
  * RBSC_FLAG_UNPACKED in none.  Results are identical. */
 #define RBSC_FLAG_PACKED 512u

@@ -1312,7 +1312,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         // the previous round's block count
         const bool packed_round =
             node_order && !(prm.flags & (RBSC_FLAG_FULLSORT | RBSC_FLAG_UNPACKED)) &&
-            ((prm.flags & RBSC_FLAG_PACKED) || (u64)K * 10 >= (u64)n_rank * 9) &&
+            ((prm.flags & RBSC_FLAG_PACKED) || (u64)K * 2 >= (u64)n_rank) &&
             (n_rank >> lo) == 0;
         auto fix_ties = [&](u64* keys, u64* P, const u64* psig) {
             const u64 cap = (prm.flags & RBSC_FLAG_TIESMALL) ? 2 : w.tie_cap;
@@ -1440,7 +1440,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         // and the round's node side is then redone from positions in colour
         // order.
         //
-        // Packed (rounds whose previous block count is at least 9/10 of the
+        // Packed (rounds whose previous block count is at least half the
         // ranked nodes, so few neighbours share top bits): the sort moves 8
         // bytes per node, the signature's top bits with the node below them,
         // and the tie fix-up and rank read the low bits and colours by node
