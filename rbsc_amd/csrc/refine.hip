@@ -1373,6 +1373,25 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                                                                keys, P);
             return RBSC_OK;
         };
+        // In rounds whose previous block count is below half the ranked
+        // nodes (the early rounds, where a large class sharing top bits with
+        // another signature makes a run too long for the in-kernel sorts),
+        // the long runs are read back and fixed before the rank pass, so it
+        // runs once; in the other rounds they are rare and fixed after the
+        // round's sync, with the rank pass run again.
+        const bool long_before_rank = (u64)K * 2 < (u64)n_rank;
+        auto fix_long_first = [&](u64* keys, u64* P, const u64* psig) -> int {
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 12 * sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            const u32 nl = ctx->h_scalar[10];
+            if (ctx->h_scalar[1] || !nl) return RBSC_OK;  // overflow: the redo handles it
+            ++ctx->long_run_rounds;
+            const int r3 = fix_long(keys, P, nl, psig);
+            if (r3) return r3;
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 10, 1, 0);
+            return RBSC_OK;
+        };
         // Rank input of this round (sorted keys and positions) and the
         // positions' output (null in node order); kept for a second rank pass
         // after long runs are fixed.
@@ -1420,6 +1439,10 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             if (r2) return r2;
             if (!full) fix_ties(kk[which], pp[which], nullptr);
             else fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
+            if (!full && long_before_rank) {
+                r2 = fix_long_first(kk[which], pp[which], nullptr);
+                if (r2) return r2;
+            }
             prof_end(ctx);
             sorted_pc = (pp[which] == w.P[0]) ? 0 : 1;
             rk_keys = kk[which];
@@ -1466,6 +1489,10 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             if (r2) return r2;
             if (!full) fix_ties(kk[0], pp[0], nullptr);
             else fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 8, 3, 0);
+            if (!full && long_before_rank) {
+                r2 = fix_long_first(kk[0], pp[0], nullptr);
+                if (r2) return r2;
+            }
             prof_end(ctx);
             rk_keys = kk[0];
             rk_P = pp[0];
