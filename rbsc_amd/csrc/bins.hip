@@ -416,16 +416,15 @@ __global__ void bin_zero_kernel(const u32* __restrict__ zero_bins, u64 n_zero, u
 }
 
 // Adds the hashed colours of slots [a, e) into the LDS row sums: slots below
-// cold0 (hot) hold heads whose colours are gathered, the others hold
-// colours.  Slots are read 8 at a time with 16-byte loads (aligned groups,
-// slots outside [a, e) loaded but not added).  (Lanes taking 64 consecutive
-// slots per load instruction, so one gather instruction reads 64
-// neighbouring heads, measured no faster.)
-//
-// Cold slots take their colours from the refinement's cold-colour buffer
-// (index s + cbase; aligned groups of 8 stay aligned), which the scatter
-// fills every round; the graph's own arrays are only read, so several
-// refinements of one graph may run at once.
+// cold0 (dense) hold heads whose colours are gathered, the others (warm and
+// cold) take their colours from the refinement's buffer (index s + cbase;
+// aligned groups of 8 stay aligned), which the warm gather and the scatter
+// fill every round; the graph's own arrays are only read, so several
+// refinements of one graph may run at once.  Slots are read 8 at a time with
+// 16-byte loads (aligned groups, slots outside [a, e) loaded but not added).
+// (Measured no faster: lanes taking 64 consecutive slots per load
+// instruction, two groups in flight per thread, dense slots packed to 4
+// bytes.)
 // Raw layout (HASHED false): col and the buffer hold u32 raw colours, and
 // every line hashes randomize(c + salt).  Hashed layout: col and the buffer
 // hold the round's u64 hashed colours, and no line hashes.
@@ -445,7 +444,7 @@ __device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0, u64 cbas
         const u64 g0 = 8 * q;
         if (HASHED) {
             const u64* col = static_cast<const u64*>(colv);
-            if (g0 < cold0) {  // the group holds hot slots
+            if (g0 < cold0) {  // the group holds dense slots
                 const u32x4 ka = *reinterpret_cast<const u32x4*>(slot_key + g0);
                 const u32x4 kb = *reinterpret_cast<const u32x4*>(slot_key + g0 + 4);
                 const u32 key[8] = {ka.x, ka.y, ka.z, ka.w, kb.x, kb.y, kb.z, kb.w};
@@ -484,7 +483,7 @@ __device__ __forceinline__ void bin_accumulate(u64 a, u64 e, u64 cold0, u64 cbas
                 for (int k = 0; k < 8; ++k)
                     if (g0 + k >= cold0) key[k] = c8[k];
             }
-            if (g0 < cold0) {  // the group holds hot slots
+            if (g0 < cold0) {  // the group holds dense slots
                 u32 g[8];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) g[k] = col[g0 + k < cold0 ? key[k] : 0u];

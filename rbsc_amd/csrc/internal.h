@@ -108,8 +108,9 @@ struct rbsc_bins {
     u64 bytes = 0;
 };
 void bins_release(rbsc_bins* b);
-// Builds g->bins (hot heads below H, 2^log_s rows per bin, chunks of at most
-// chunk_slots slots, scatter tiles of scatter_chunk lines).
+// Builds g->bins (hot heads below H, of them dense below D, 2^log_s rows per
+// bin, chunks of at most chunk_slots slots, scatter tiles of scatter_chunk
+// lines).
 int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk_slots,
                u64 scatter_chunk, bool head_order, bool hashed);
 // One round on the binned layout: cold colours into their slots, then the
