@@ -4,6 +4,11 @@
 
 // Largest graph (nodes) refined inside one workgroup's LDS.
 constexpr u32 BATCH_NMAX = 5120;
+// Graphs of at most BATCH_NMID nodes run in a second launch whose LDS layout
+// is sized for them (half of BATCH_NMAX), two workgroups per CU, and graphs
+// of at most BATCH_NLOW nodes in a third, four workgroups per CU.
+constexpr u32 BATCH_NMID = 2560;
+constexpr u32 BATCH_NLOW = 1280;
 
 struct BatchDesc {
     u64 node_base;  // first node of the graph in the batch's node numbering
@@ -17,6 +22,9 @@ struct BatchDevice {
     BatchDesc* desc = nullptr;      // per small graph
     u32* order = nullptr;           // small graphs, largest first
     u32 n_small = 0;
+    u32 n_hi = 0;                   // leading graphs of order with more than BATCH_NMID nodes
+    u32 n_mid = 0;                  // ... with more than BATCH_NLOW nodes
+    int cus = 256;                  // compute units of the device
     u32* rp = nullptr;              // local row offsets
     unsigned short* dst = nullptr;  // local heads
     u64* wt = nullptr;              // or null

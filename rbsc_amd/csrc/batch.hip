@@ -1,6 +1,10 @@
-// batch.hip -- many small graphs refined in one persistent launch.
+// batch.hip -- many small graphs refined in persistent launches, one per size tier.
 //
-// Each workgroup takes graphs from an atomic queue (largest first) and runs
+// Graphs are launched in three tiers by size (above BATCH_NMID nodes with the
+// autotuned shape and the full LDS layout, one workgroup per CU; above
+// BATCH_NLOW with half the layout, two per CU; the rest with a quarter, four
+// per CU), so small graphs do not hold a CU's whole LDS.  In each tier a
+// workgroup takes graphs from an atomic queue (largest first) and runs
 // the whole refinement of one graph inside LDS: colours, signatures, a
 // stable LDS radix sort of the top 32 signature bits over positions kept in
 // colour order (the rare runs colliding in those bits ordered afterwards),
@@ -19,30 +23,33 @@
 
 namespace {
 
-// Workgroup width is a template parameter (picked by the autotuner).
-template <int BT>
+// Workgroup width (picked by the autotuner for the graphs above BATCH_NMID
+// nodes) and node capacity NM (the LDS layout) are template parameters.
+constexpr int MID_THREADS = 512;
+constexpr int LOW_THREADS = 256;
+template <int BT, int NM>
 struct Cfg {
-    static constexpr int BW = BT / 64;                           // waves
-    static constexpr int IPT = (BATCH_NMAX + BT - 1) / BT;       // items per thread (blocked)
-    static constexpr int IPW = (BATCH_NMAX + BW - 1) / BW;       // items per wave chunk
-    static constexpr int WIT = (IPW + 63) / 64;                  // 64-item iterations per wave
+    static constexpr int BW = BT / 64;                   // waves
+    static constexpr int IPT = (NM + BT - 1) / BT;       // items per thread (blocked)
+    static constexpr int IPW = (NM + BW - 1) / BW;       // items per wave chunk
+    static constexpr int WIT = (IPW + 63) / 64;          // 64-item iterations per wave
 };
 
-template <int BT>
+template <int BT, int NM>
 struct Lds {
-    u64 key[2][BATCH_NMAX];
-    u32 val[2][BATCH_NMAX];   // (dense colour << 16) | node
-    u32 col[BATCH_NMAX];      // colour used in the hash (exact c0 in round 0)
-    u32 whist[Cfg<BT>::BW][256];
-    u32 wsum[Cfg<BT>::BW];
+    u64 key[2][NM];
+    u32 val[2][NM];   // (dense colour << 16) | node
+    u32 col[NM];      // colour used in the hash (exact c0 in round 0)
+    u32 whist[Cfg<BT, NM>::BW][256];
+    u32 wsum[Cfg<BT, NM>::BW];
     u32 scal[8];
 };
 
 // Exclusive scan of one u32 per thread over the workgroup; returns the
 // exclusive prefix, total in *tot.  Uses s->wsum.
-template <int BT>
-__device__ u32 block_scan_excl(u32 v, Lds<BT>* s, u32* tot) {
-    constexpr int BW = Cfg<BT>::BW;
+template <int BT, int NM>
+__device__ u32 block_scan_excl(u32 v, Lds<BT, NM>* s, u32* tot) {
+    constexpr int BW = Cfg<BT, NM>::BW;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     u32 x = v;
 #pragma unroll
@@ -65,9 +72,9 @@ __device__ u32 block_scan_excl(u32 v, Lds<BT>* s, u32* tot) {
 
 // One stable LSD pass over n items by digit (key >> shift) & 255, from
 // buffer `in` to buffer 1 - in.  Wave w owns items [w*IPW, (w+1)*IPW).
-template <int BT>
-__device__ void lds_radix_pass(Lds<BT>* s, int in, int n, int shift) {
-    constexpr int BW = Cfg<BT>::BW, IPW = Cfg<BT>::IPW, WIT = Cfg<BT>::WIT;
+template <int BT, int NM>
+__device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
+    constexpr int BW = Cfg<BT, NM>::BW, IPW = Cfg<BT, NM>::IPW, WIT = Cfg<BT, NM>::WIT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < BW * 256; i += BT) (&s->whist[0][0])[i] = 0;
     __syncthreads();
@@ -125,8 +132,8 @@ __device__ void lds_radix_pass(Lds<BT>* s, int in, int n, int shift) {
     __syncthreads();
 }
 
-template <int BT>
-__device__ void lds_sort(Lds<BT>* s, int n, int bit_begin, int bit_end) {
+template <int BT, int NM>
+__device__ void lds_sort(Lds<BT, NM>* s, int n, int bit_begin, int bit_end) {
     int in = 0;
     for (int sh = bit_begin; sh < bit_end; sh += 8) {
         lds_radix_pass(s, in, n, sh);
@@ -135,15 +142,15 @@ __device__ void lds_sort(Lds<BT>* s, int n, int bit_begin, int bit_end) {
     // callers use an even number of passes: result is back in buffer 0
 }
 
-template <int BT, bool HAS_WT, bool HAS_INIT>
+template <int BT, int NM, bool HAS_WT, bool HAS_INIT>
 __global__ __launch_bounds__(BT) void batch_refine_kernel(
     const BatchDesc* __restrict__ desc, const u32* __restrict__ order, u32 n_jobs,
     const u32* __restrict__ rp_all, const unsigned short* __restrict__ dst_all,
     const u64* __restrict__ wt_all, const u32* __restrict__ init_all, u32 max_rounds,
     u32* __restrict__ labels_out, u32* __restrict__ colors_out, u32* __restrict__ blocks_out,
     u32* __restrict__ rounds_out, u32* __restrict__ queue, int split_lo) {
-    constexpr int IPT = Cfg<BT>::IPT;
-    __shared__ Lds<BT> s;
+    constexpr int IPT = Cfg<BT, NM>::IPT;
+    __shared__ Lds<BT, NM> s;
     // Every job runs the whole body: no `continue` back to the thread-0 queue
     // fetch.  A branch that skips the body lets the compiler put the barrier
     // after the fetch at the head of an inner loop that only thread 0 leaves,
@@ -298,7 +305,7 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
 
         // ---- outputs: raw colours, first-occurrence labels, counts
         u32* first = reinterpret_cast<u32*>(&s.key[1][0]);
-        u32* lab = first + BATCH_NMAX;
+        u32* lab = first + NM;
         for (int c = threadIdx.x; c < n; c += BT) first[c] = 0xFFFFFFFFu;
         __syncthreads();
         for (int u = threadIdx.x; u < n; u += BT) atomicMin(&first[s.col[u]], (u32)u);
@@ -334,19 +341,29 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
         __syncthreads();
     }
 }
-static_assert(sizeof(Lds<1024>) <= 160 * 1024, "batch LDS layout exceeds 160 KB");
+static_assert(sizeof(Lds<1024, BATCH_NMAX>) <= 160 * 1024, "batch LDS layout exceeds 160 KB");
+static_assert(sizeof(Lds<MID_THREADS, BATCH_NMID>) <= 80 * 1024,
+              "two mid-size layouts must fit one CU's LDS");
+static_assert(sizeof(Lds<LOW_THREADS, BATCH_NLOW>) <= 40 * 1024,
+              "four small layouts must fit one CU's LDS");
 
-__global__ void zero_queue(u32* q) { *q = 0; }
+__global__ void zero_queue(u32* q) {
+    q[0] = 0;
+    q[1] = 0;
+    q[2] = 0;
+}
 
-template <int BT>
-static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 max_rounds,
-                      u32* d_labels, u32* d_colors, u32* d_blocks, u32* d_rounds, u32* d_queue,
-                      int split_lo) {
-#define BL(W, I)                                                                        \
-    batch_refine_kernel<BT, W, I><<<grid, BT, 0, s>>>(b.desc, b.order, b.n_small, b.rp, b.dst, \
-                                                      b.wt, b.init, max_rounds, d_labels,       \
-                                                      d_colors, d_blocks, d_rounds, d_queue,    \
-                                                      split_lo)
+// Jobs [j0, j1) of the order (largest first) on `grid` workgroups, taken
+// from the queue counter q.
+template <int BT, int NM>
+static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 j0, u32 j1,
+                      u32 max_rounds, u32* d_labels, u32* d_colors, u32* d_blocks, u32* d_rounds,
+                      u32* q, int split_lo) {
+#define BL(W, I)                                                                              \
+    batch_refine_kernel<BT, NM, W, I><<<grid, BT, 0, s>>>(b.desc, b.order + j0, j1 - j0, b.rp, \
+                                                          b.dst, b.wt, b.init, max_rounds,      \
+                                                          d_labels, d_colors, d_blocks,         \
+                                                          d_rounds, q, split_lo)
     if (b.wt) {
         if (b.init) BL(true, true); else BL(true, false);
     } else {
@@ -362,11 +379,28 @@ int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_lab
     hipStream_t s = ctx->stream;
     zero_queue<<<1, 1, 0, s>>>(d_queue);
     if (b.n_small == 0) return RBSC_OK;
-    const unsigned grid = (unsigned)std::max(1, std::min<int>(n_wg, (int)b.n_small));
-    switch (threads) {
-        case 256: launch_bt<256>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
-        case 1024: launch_bt<1024>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
-        default: launch_bt<512>(s, grid, b, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
+    // graphs above BATCH_NMID nodes: the tuned shape and the full layout
+    if (b.n_hi) {
+        const unsigned grid = (unsigned)std::max(1, std::min<int>(n_wg, (int)b.n_hi));
+        switch (threads) {
+            case 256: launch_bt<256, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
+            case 1024: launch_bt<1024, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
+            default: launch_bt<512, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
+        }
+    }
+    // graphs above BATCH_NLOW nodes: half the layout, two workgroups per CU
+    if (b.n_mid > b.n_hi) {
+        const unsigned grid =
+            (unsigned)std::max(1, std::min<int>(2 * b.cus, (int)(b.n_mid - b.n_hi)));
+        launch_bt<MID_THREADS, BATCH_NMID>(s, grid, b, b.n_hi, b.n_mid, max_rounds, d_labels,
+                                           d_colors, d_blocks, d_rounds, d_queue + 1, split_lo);
+    }
+    // the others: a quarter of the layout, four workgroups per CU
+    if (b.n_small > b.n_mid) {
+        const unsigned grid =
+            (unsigned)std::max(1, std::min<int>(4 * b.cus, (int)(b.n_small - b.n_mid)));
+        launch_bt<LOW_THREADS, BATCH_NLOW>(s, grid, b, b.n_mid, b.n_small, max_rounds, d_labels,
+                                           d_colors, d_blocks, d_rounds, d_queue + 2, split_lo);
     }
     HIP_TRY(hipGetLastError());
     return RBSC_OK;
@@ -490,6 +524,11 @@ extern "C" int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* in, rbsc_dbatc
         }
     std::sort(b->small.begin(), b->small.end(),
               [&](u32 x, u32 y) { return desc[x].n > desc[y].n || (desc[x].n == desc[y].n && x < y); });
+    b->dev.n_hi = 0;
+    while (b->dev.n_hi < b->small.size() && desc[b->small[b->dev.n_hi]].n > BATCH_NMID) ++b->dev.n_hi;
+    b->dev.n_mid = b->dev.n_hi;
+    while (b->dev.n_mid < b->small.size() && desc[b->small[b->dev.n_mid]].n > BATCH_NLOW) ++b->dev.n_mid;
+    (void)hipDeviceGetAttribute(&b->dev.cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     // Big graphs go through the single-graph path.
     for (u32 g : b->big) {
         rbsc_graph one{};
@@ -527,7 +566,7 @@ extern "C" int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* in, rbsc_dbatc
     if (!rc) rc = up((void**)&b->d_colors, nullptr, sizeof(u32) * NT);
     if (!rc) rc = up((void**)&b->d_blocks, nullptr, sizeof(u32) * G);
     if (!rc) rc = up((void**)&b->d_rounds, nullptr, sizeof(u32) * G);
-    if (!rc) rc = up((void**)&b->d_queue, nullptr, sizeof(u32));
+    if (!rc) rc = up((void**)&b->d_queue, nullptr, 3 * sizeof(u32));
     if (!rc && hipStreamSynchronize(s) != hipSuccess) {
         rbsc_set_error("batch upload failed");
         rc = RBSC_ERR_HIP;

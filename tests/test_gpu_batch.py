@@ -117,6 +117,31 @@ def test_batch_directed_sizes_and_edges(ctx):
     db.free()
 
 
+def test_batch_size_tiers(ctx):
+    # graphs at both edges of the three launch tiers (above 2560 nodes, above
+    # 1280, the rest), a few of each, weighted and with initial colours too:
+    # every graph's colours, rounds and labels equal the oracle's
+    gs = []
+    for k, n in enumerate((1279, 1280, 1281, 2559, 2560, 2561, 5119, 5120)):
+        for j in range(3):
+            gs.append(gen.er_undirected(n, 2.0 + j, seed=500 + 10 * k + j))
+    rng = np.random.default_rng(12)
+    for g in gs[::4]:
+        g.weight = rng.integers(1, 4, g.n_edges).astype(np.uint64)
+    b = HostBatch.from_graphs(gs)
+    db = ctx.upload_batch(b)
+    res = db.refine(labels=True, colors=True)
+    _check_against_oracle(b, res)
+    db.free()
+    for g in gs:
+        g.init_color = rng.integers(0, 3, g.n_nodes).astype(np.uint32)
+    b = HostBatch.from_graphs(gs)
+    db = ctx.upload_batch(b)
+    res = db.refine(labels=True, colors=True)
+    _check_against_oracle(b, res)
+    db.free()
+
+
 def test_batch_weighted_and_initial_colours(ctx):
     gs = _mixed_graphs(5)
     rng = np.random.default_rng(9)
