@@ -5,10 +5,12 @@
 // Largest graph (nodes) refined inside one workgroup's LDS.
 constexpr u32 BATCH_NMAX = 5120;
 // Graphs of at most BATCH_NMID nodes run in a second launch whose LDS layout
-// is sized for them (half of BATCH_NMAX), two workgroups per CU, and graphs
-// of at most BATCH_NLOW nodes in a third, four workgroups per CU.
+// is sized for them (half of BATCH_NMAX), two workgroups per CU; graphs of
+// at most BATCH_NLOW nodes in a third, four per CU; graphs of at most
+// BATCH_NTINY nodes in a fourth, eight per CU.
 constexpr u32 BATCH_NMID = 2560;
 constexpr u32 BATCH_NLOW = 1280;
+constexpr u32 BATCH_NTINY = 640;
 
 struct BatchDesc {
     u64 node_base;  // first node of the graph in the batch's node numbering
@@ -24,6 +26,7 @@ struct BatchDevice {
     u32 n_small = 0;
     u32 n_hi = 0;                   // leading graphs of order with more than BATCH_NMID nodes
     u32 n_mid = 0;                  // ... with more than BATCH_NLOW nodes
+    u32 n_low = 0;                  // ... with more than BATCH_NTINY nodes
     int cus = 256;                  // compute units of the device
     u32* rp = nullptr;              // local row offsets
     unsigned short* dst = nullptr;  // local heads

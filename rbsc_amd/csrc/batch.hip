@@ -1,9 +1,10 @@
 // batch.hip -- many small graphs refined in persistent launches, one per size tier.
 //
-// Graphs are launched in three tiers by size (above BATCH_NMID nodes with the
+// Graphs are launched in four tiers by size (above BATCH_NMID nodes with the
 // autotuned shape and the full LDS layout, one workgroup per CU; above
-// BATCH_NLOW with half the layout, two per CU; the rest with a quarter, four
-// per CU), so small graphs do not hold a CU's whole LDS.  In each tier a
+// BATCH_NLOW with half the layout, two per CU; above BATCH_NTINY with a
+// quarter, four per CU; the rest with an eighth, eight per CU), so small
+// graphs do not hold a CU's whole LDS.  In each tier a
 // workgroup takes graphs from an atomic queue (largest first) and runs
 // the whole refinement of one graph inside LDS: colours, signatures, a
 // stable LDS radix sort of the top 32 signature bits over positions kept in
@@ -27,6 +28,7 @@ namespace {
 // nodes) and node capacity NM (the LDS layout) are template parameters.
 constexpr int MID_THREADS = 512;
 constexpr int LOW_THREADS = 256;
+constexpr int TINY_THREADS = 128;
 template <int BT, int NM>
 struct Cfg {
     static constexpr int BW = BT / 64;                   // waves
@@ -99,23 +101,35 @@ __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
         rank[it] = old + (u32)__popcll(peers & below);
     }
     __syncthreads();
-    // digit-major offsets: for digit d, waves in order
-    u32 tot_d = 0;
-    if (threadIdx.x < 256) {
-        const int d = threadIdx.x;
+    // digit-major offsets: for digit d, waves in order; thread t takes the
+    // digits [t * DPT, t * DPT + DPT) (one each from 256 threads up)
+    constexpr int DPT = (256 + BT - 1) / BT;
+    u32 tot_d[DPT];
+    u32 tsum = 0;
 #pragma unroll
-        for (int w = 0; w < BW; ++w) {
-            const u32 c = s->whist[w][d];
-            s->whist[w][d] = tot_d;
-            tot_d += c;
+    for (int k = 0; k < DPT; ++k) {
+        const int d = (int)threadIdx.x * DPT + k;
+        tot_d[k] = 0;
+        if (d < 256) {
+#pragma unroll
+            for (int w = 0; w < BW; ++w) {
+                const u32 c = s->whist[w][d];
+                s->whist[w][d] = tot_d[k];
+                tot_d[k] += c;
+            }
         }
+        tsum += tot_d[k];
     }
     u32 total;
-    const u32 base = block_scan_excl(threadIdx.x < 256 ? tot_d : 0u, s, &total);
-    if (threadIdx.x < 256) {
-        const int d = threadIdx.x;
+    u32 run = block_scan_excl(tsum, s, &total);
 #pragma unroll
-        for (int w = 0; w < BW; ++w) s->whist[w][d] += base;
+    for (int k = 0; k < DPT; ++k) {
+        const int d = (int)threadIdx.x * DPT + k;
+        if (d < 256) {
+#pragma unroll
+            for (int w = 0; w < BW; ++w) s->whist[w][d] += run;
+            run += tot_d[k];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -346,11 +360,14 @@ static_assert(sizeof(Lds<MID_THREADS, BATCH_NMID>) <= 80 * 1024,
               "two mid-size layouts must fit one CU's LDS");
 static_assert(sizeof(Lds<LOW_THREADS, BATCH_NLOW>) <= 40 * 1024,
               "four small layouts must fit one CU's LDS");
+static_assert(sizeof(Lds<TINY_THREADS, BATCH_NTINY>) <= 20 * 1024,
+              "eight tiny layouts must fit one CU's LDS");
 
 __global__ void zero_queue(u32* q) {
     q[0] = 0;
     q[1] = 0;
     q[2] = 0;
+    q[3] = 0;
 }
 
 // Jobs [j0, j1) of the order (largest first) on `grid` workgroups, taken
@@ -395,12 +412,19 @@ int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_lab
         launch_bt<MID_THREADS, BATCH_NMID>(s, grid, b, b.n_hi, b.n_mid, max_rounds, d_labels,
                                            d_colors, d_blocks, d_rounds, d_queue + 1, split_lo);
     }
-    // the others: a quarter of the layout, four workgroups per CU
-    if (b.n_small > b.n_mid) {
+    // graphs above BATCH_NTINY nodes: a quarter of the layout, four per CU
+    if (b.n_low > b.n_mid) {
         const unsigned grid =
-            (unsigned)std::max(1, std::min<int>(4 * b.cus, (int)(b.n_small - b.n_mid)));
-        launch_bt<LOW_THREADS, BATCH_NLOW>(s, grid, b, b.n_mid, b.n_small, max_rounds, d_labels,
+            (unsigned)std::max(1, std::min<int>(4 * b.cus, (int)(b.n_low - b.n_mid)));
+        launch_bt<LOW_THREADS, BATCH_NLOW>(s, grid, b, b.n_mid, b.n_low, max_rounds, d_labels,
                                            d_colors, d_blocks, d_rounds, d_queue + 2, split_lo);
+    }
+    // the others: an eighth of the layout, eight workgroups per CU
+    if (b.n_small > b.n_low) {
+        const unsigned grid =
+            (unsigned)std::max(1, std::min<int>(8 * b.cus, (int)(b.n_small - b.n_low)));
+        launch_bt<TINY_THREADS, BATCH_NTINY>(s, grid, b, b.n_low, b.n_small, max_rounds, d_labels,
+                                             d_colors, d_blocks, d_rounds, d_queue + 3, split_lo);
     }
     HIP_TRY(hipGetLastError());
     return RBSC_OK;
@@ -528,6 +552,8 @@ extern "C" int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* in, rbsc_dbatc
     while (b->dev.n_hi < b->small.size() && desc[b->small[b->dev.n_hi]].n > BATCH_NMID) ++b->dev.n_hi;
     b->dev.n_mid = b->dev.n_hi;
     while (b->dev.n_mid < b->small.size() && desc[b->small[b->dev.n_mid]].n > BATCH_NLOW) ++b->dev.n_mid;
+    b->dev.n_low = b->dev.n_mid;
+    while (b->dev.n_low < b->small.size() && desc[b->small[b->dev.n_low]].n > BATCH_NTINY) ++b->dev.n_low;
     (void)hipDeviceGetAttribute(&b->dev.cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     // Big graphs go through the single-graph path.
     for (u32 g : b->big) {
@@ -566,7 +592,7 @@ extern "C" int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* in, rbsc_dbatc
     if (!rc) rc = up((void**)&b->d_colors, nullptr, sizeof(u32) * NT);
     if (!rc) rc = up((void**)&b->d_blocks, nullptr, sizeof(u32) * G);
     if (!rc) rc = up((void**)&b->d_rounds, nullptr, sizeof(u32) * G);
-    if (!rc) rc = up((void**)&b->d_queue, nullptr, 3 * sizeof(u32));
+    if (!rc) rc = up((void**)&b->d_queue, nullptr, 4 * sizeof(u32));
     if (!rc && hipStreamSynchronize(s) != hipSuccess) {
         rbsc_set_error("batch upload failed");
         rc = RBSC_ERR_HIP;

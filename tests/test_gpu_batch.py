@@ -118,11 +118,11 @@ def test_batch_directed_sizes_and_edges(ctx):
 
 
 def test_batch_size_tiers(ctx):
-    # graphs at both edges of the three launch tiers (above 2560 nodes, above
-    # 1280, the rest), a few of each, weighted and with initial colours too:
-    # every graph's colours, rounds and labels equal the oracle's
+    # graphs at both edges of the four launch tiers (above 2560 nodes, above
+    # 1280, above 640, the rest), a few of each, weighted and with initial
+    # colours too: every graph's colours, rounds and labels equal the oracle's
     gs = []
-    for k, n in enumerate((1279, 1280, 1281, 2559, 2560, 2561, 5119, 5120)):
+    for k, n in enumerate((639, 640, 641, 1279, 1280, 1281, 2559, 2560, 2561, 5119, 5120)):
         for j in range(3):
             gs.append(gen.er_undirected(n, 2.0 + j, seed=500 + 10 * k + j))
     rng = np.random.default_rng(12)
