@@ -134,7 +134,7 @@ def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
             "speedup_vs_per_graph_loop": (n_graphs / wall) / loop_rate}
 
 
-def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int) -> dict:
+def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int, workers: int = 16) -> dict:
     """Poisson arrivals of refinement requests over a pool of R-MAT graphs
     (E log-uniform in [1e3, 5e8], N = E/16, colour 0): small graphs as host
     graphs (batched), larger ones resident in HBM (binned from 2^22 lines).  Reports p50/p99 latency
@@ -181,7 +181,7 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int) -> dict:
         gaps = rng.exponential(1.0 / lam, n_req)
         arrivals = np.cumsum(gaps) - gaps[0]
         picks = rng.integers(0, pool_size, n_req)
-        with rbsc_amd.Server(workers=3, hbm_budget=budget, keep_partition=True) as srv:
+        with rbsc_amd.Server(workers=workers, hbm_budget=budget, keep_partition=True) as srv:
             t = time.perf_counter()
             ts = [submit(srv, pool[i], float(a)) for i, a in zip(picks, arrivals)]
             rs = [srv.wait(tk) for tk in ts]
@@ -303,6 +303,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests per arrival rate in the serving leg (0 skips it)")
     ap.add_argument("--serve-rates", default="50,200,1000,2000")
+    ap.add_argument("--serve-workers", type=int, default=16,
+                    help="worker threads (contexts) of the serving loop")
     ap.add_argument("--stream-frac", type=float, default=0.25,
                     help="also time the pinned-host streaming path at this edge-batch fraction (0 skips)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -421,7 +423,7 @@ def main():
             f"(per-graph loop {out['batched']['per_graph_loop_graphs_per_s']:.0f} graphs/s)")
     if args.serve_requests:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
-        out["serving"] = serving_leg(ctx, rates, args.serve_requests, 64, 1005)
+        out["serving"] = serving_leg(ctx, rates, args.serve_requests, 64, 1005, args.serve_workers)
     if args.stream_frac and args.parts == 1 and not args.hot:
         out["streamed"] = streaming_leg(dg, E, args.stream_frac, max(1, min(args.steps, 2)))
         log(f"[bench] streamed at {args.stream_frac}: {out['streamed']['value']:.3g} edge-visits/s, "

@@ -311,7 +311,7 @@ typedef struct rbsc_server rbsc_server;
 typedef struct rbsc_server_opts {
     int device;
     uint64_t hbm_budget;      /* bytes for requests in flight; 0 = 250 GiB */
-    uint32_t workers;         /* concurrent single-graph refinements; 0 = 2, at most 8 */
+    uint32_t workers;         /* concurrent single-graph refinements; 0 = 8, at most 16 */
     uint32_t batch_max;       /* most small graphs per batch launch; 0 = 4096 */
     uint32_t keep_partition;  /* 1: results carry the canonical partition */
 } rbsc_server_opts;

@@ -334,8 +334,8 @@ extern "C" int rbsc_server_create(const rbsc_server_opts* o, rbsc_server** out) 
     auto* s = new rbsc_server();
     s->opts = *o;
     if (!s->opts.hbm_budget) s->opts.hbm_budget = 250ull << 30;
-    if (!s->opts.workers) s->opts.workers = 2;
-    if (s->opts.workers > 8) s->opts.workers = 8;
+    if (!s->opts.workers) s->opts.workers = 8;
+    if (s->opts.workers > 16) s->opts.workers = 16;
     if (!s->opts.batch_max) s->opts.batch_max = 4096;
     for (u32 i = 0; i <= s->opts.workers; ++i) {
         rbsc_ctx* c = nullptr;
