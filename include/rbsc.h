@@ -132,11 +132,12 @@ const char* rbsc_last_error(void);
 /* Library version string. */
 const char* rbsc_version(void);
 
-/* Page-locked host memory for callers' output buffers (labels, colours,
- * partitions): device-to-host copies into it run at the link's rate, where
- * copies into pageable memory are staged through the runtime's own pinned
- * buffers.  No counterpart in the reference (its outputs are host vectors,
- * rbsc.cu:427-445). */
+/* Page-locked, coherent host memory for callers' output buffers (labels,
+ * colours, partitions): device-to-host copies into it run at the link's
+ * rate, where copies into pageable memory are staged through the runtime's
+ * own pinned buffers, and batched refinement writes labels into it from the
+ * kernels directly (the transfer overlaps the refinement).  No counterpart
+ * in the reference (its outputs are host vectors, rbsc.cu:427-445). */
 int rbsc_host_alloc(uint64_t bytes, void** out);
 int rbsc_host_free(void* p);
 

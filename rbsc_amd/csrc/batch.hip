@@ -625,7 +625,12 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     const int split_lo = (opts && (opts->flags & RBSC_FLAG_SPLIT16)) ? 48 : 32;
-    int rc = batch_launch(ctx, b->dev, max_rounds, b->d_labels, b->d_colors, b->d_blocks,
+    // Labels into a buffer from rbsc_host_alloc are written by the kernels
+    // themselves, so their transfer overlaps the refinement.
+    u32* direct = labels_out
+        ? static_cast<u32*>(host_device_ptr(labels_out, sizeof(u32) * b->total_nodes))
+        : nullptr;
+    int rc = batch_launch(ctx, b->dev, max_rounds, direct ? direct : b->d_labels, b->d_colors, b->d_blocks,
                           b->d_rounds, b->d_queue, threads, wgs, split_lo);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev1, s));
@@ -633,7 +638,7 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
     std::vector<u32> blocks(G, 0), rounds(G, 0);
     HIP_TRY(hipMemcpyAsync(blocks.data(), b->d_blocks, sizeof(u32) * G, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(rounds.data(), b->d_rounds, sizeof(u32) * G, hipMemcpyDeviceToHost, s));
-    if (labels_out)
+    if (labels_out && !direct)
         HIP_TRY(hipMemcpyAsync(labels_out, b->d_labels, sizeof(u32) * b->total_nodes,
                                hipMemcpyDeviceToHost, s));
     if (colors_out)

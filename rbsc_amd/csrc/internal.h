@@ -118,6 +118,9 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
 // id, u32 raw colours on the raw layout, u64 hashed colours of the round
 // (randomize(c + salt)) on the hashed layout.
 // cold_buf: the refinement's cold-colour buffer, bins_cold_bytes(b) bytes.
+// Device address of [p, p + bytes) when it lies inside one buffer from
+// rbsc_host_alloc (coherent page-locked memory kernels may write), else null.
+void* host_device_ptr(const void* p, u64 bytes);
 u64 bins_cold_bytes(const rbsc_bins* b);
 void bins_scatter(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, void* cold_buf);
 void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, const void* cold_buf,

@@ -29,6 +29,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 // ---------------------------------------------------------------- errors
@@ -42,25 +44,56 @@ extern "C" void rbsc_set_error(const char* fmt, ...) {
 extern "C" const char* rbsc_last_error(void) { return g_err; }
 extern "C" const char* rbsc_version(void) { return "rbsc-gfx950 0.1"; }
 
+// Host buffers from rbsc_host_alloc: coherent page-locked memory that
+// kernels may write directly (base address -> bytes).
+namespace {
+std::mutex g_host_mu;
+std::map<uintptr_t, u64> g_host_allocs;
+}  // namespace
+
 extern "C" int rbsc_host_alloc(uint64_t bytes, void** out) {
     if (!out) {
         rbsc_set_error("rbsc_host_alloc: null argument");
         return RBSC_ERR_INVALID;
     }
     *out = nullptr;
-    const hipError_t e = hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault);
+    const u64 n = bytes ? bytes : 1;
+    const hipError_t e = hipHostMalloc(out, n, hipHostMallocCoherent);
     if (e != hipSuccess) {
         *out = nullptr;
         rbsc_set_error("hipHostMalloc of %llu bytes failed: %s", (unsigned long long)bytes,
                        hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;
     }
+    std::lock_guard<std::mutex> lk(g_host_mu);
+    g_host_allocs[(uintptr_t)*out] = n;
     return RBSC_OK;
 }
 
 extern "C" int rbsc_host_free(void* p) {
-    if (p) HIP_TRY(hipHostFree(p));
+    if (!p) return RBSC_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_host_mu);
+        g_host_allocs.erase((uintptr_t)p);
+    }
+    HIP_TRY(hipHostFree(p));
     return RBSC_OK;
+}
+
+void* host_device_ptr(const void* p, u64 bytes) {
+    if (!p) return nullptr;
+    uintptr_t base = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_host_mu);
+        auto it = g_host_allocs.upper_bound((uintptr_t)p);
+        if (it == g_host_allocs.begin()) return nullptr;
+        --it;
+        if ((uintptr_t)p + bytes > it->first + it->second) return nullptr;
+        base = it->first;
+    }
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, (void*)base, 0) != hipSuccess || !d) return nullptr;
+    return static_cast<char*>(d) + ((uintptr_t)p - base);
 }
 
 namespace {
