@@ -662,11 +662,9 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
         blocks[g] = o.K;
         rounds[g] = o.rounds;
         if (labels_out || colors_out) {
-            std::vector<u32> col(dg->N);
-            rc = download_colors(ctx, dg, col.data());
+            rc = device_partition(ctx, dg, labels_out ? labels_out + b->node_off[g] : nullptr,
+                                  colors_out ? colors_out + b->node_off[g] : nullptr);
             if (rc) return rc;
-            if (colors_out) memcpy(colors_out + b->node_off[g], col.data(), sizeof(u32) * dg->N);
-            if (labels_out) canonicalize_host(dg->N, col.data(), labels_out + b->node_off[g]);
         }
     }
     auto t1 = std::chrono::steady_clock::now();

@@ -265,6 +265,10 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& p, RefineOut*
 // Raw colours (dense ranks of the last round) of the last refine_core on g,
 // by original node id, to host.
 int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host);
+// Canonical first-occurrence labels and/or raw colours by original id into
+// host memory, computed on the device after a refinement (reuses workspace
+// arrays: call it after anything else that reads the refinement's state).
+int device_partition(rbsc_ctx* ctx, rbsc_dgraph* g, u32* partition_out, u32* colors_out);
 // The same colours by internal id into a device buffer of N words.
 void device_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* d_int);
 // Exact stability check (verify.hip) of internal colours d_col (< K);

@@ -907,6 +907,41 @@ __global__ void map_colors_kernel(const u32* __restrict__ col_int, const u32* __
     if (v < N) out[v] = col_int[pi[v]];
 }
 
+// First-occurrence canonical labels on the device (the relabel of
+// rbsc.cu:431-445): first[c] = least original id of colour c; the first
+// occurrences, flagged in original id order and scanned, number the blocks;
+// a node's label is the number of its colour's first occurrence.
+// (Colours above N are not expected; one sets *bad and is skipped, and the
+// caller then relabels on the host.)
+__global__ void canon_first_kernel(const u32* __restrict__ col, u64 N, u32* __restrict__ first,
+                                   u32* __restrict__ bad) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= N) return;
+    const u32 c = col[u];
+    // read first: most nodes of a large class find an earlier id recorded
+    // and skip the atomic (the class of nodes without lines holds most ids)
+    if (c <= N) {
+        if ((u32)u < __hip_atomic_load(&first[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&first[c], (u32)u);
+    } else {
+        atomicOr(bad, 1u);
+    }
+}
+__global__ void canon_flag_kernel(const u32* __restrict__ col, const u32* __restrict__ first, u64 N,
+                                  u64* __restrict__ flag) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= N) return;
+    const u32 c = col[u];
+    flag[u] = c <= N && first[c] == (u32)u ? 1 : 0;
+}
+__global__ void canon_label_kernel(const u32* __restrict__ col, const u32* __restrict__ first,
+                                   const u64* __restrict__ rank, u64 N, u32* __restrict__ out) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= N) return;
+    const u32 c = col[u];
+    out[u] = c <= N ? (u32)rank[first[c]] : 0u;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -1656,6 +1691,54 @@ int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host) {
     return RBSC_OK;
 }
 
+// Canonical labels (and optionally raw colours) by original id into host
+// memory, computed on the device; reuses workspace arrays the refinement no
+// longer needs (call it last).  Colours are below N + 1.
+int device_partition(rbsc_ctx* ctx, rbsc_dgraph* g, u32* partition_out, u32* colors_out) {
+    const u64 N = g->N;
+    if (N == 0) return RBSC_OK;
+    Workspace& w = ctx->ws;
+    hipStream_t s = ctx->stream;
+    auto on_host = [&]() -> int {
+        std::vector<u32> col;
+        u32* hcol = colors_out;
+        if (!hcol) {
+            col.resize(N);
+            hcol = col.data();
+        }
+        int rc = download_colors(ctx, g, hcol);
+        if (rc) return rc;
+        if (partition_out) canonicalize_host((u32)N, hcol, partition_out);
+        return RBSC_OK;
+    };
+    if (w.cap_n < N || scan_tmp_words(N) > w.cap_n) return on_host();  // workspace too small
+    u32* d_int = reinterpret_cast<u32*>(w.keys[0]);
+    u32* d_col = reinterpret_cast<u32*>(w.keys[1]);  // colours by original id
+    device_colors(ctx, g, d_int);
+    map_colors_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, g->pi, N, d_col);
+    if (colors_out)
+        HIP_TRY(hipMemcpyAsync(colors_out, d_col, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
+    if (partition_out) {
+        u32* first = reinterpret_cast<u32*>(w.P[0]);  // N + 1 entries
+        u64* flag = w.P[1];
+        u32* d_lab = reinterpret_cast<u32*>(w.hc2);
+        fill_u32_kernel<<<grid_for(N + 1), BLK, 0, s>>>(first, N + 1, 0xFFFFFFFFu);
+        fill_u32_kernel<<<1, 32, 0, s>>>(ctx->d_scalar + 12, 1, 0);
+        canon_first_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, N, first, ctx->d_scalar + 12);
+        canon_flag_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, first, N, flag);
+        int rc = scan_excl_u64(flag, flag, N, nullptr, w.sig, s);
+        if (rc) return rc;
+        canon_label_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, first, flag, N, d_lab);
+        HIP_TRY(hipMemcpyAsync(partition_out, d_lab, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
+    }
+    u32 bad = 0;
+    if (partition_out)
+        HIP_TRY(hipMemcpyAsync(&bad, ctx->d_scalar + 12, sizeof(u32), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return bad ? on_host() : RBSC_OK;
+}
+
 static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
                            rbsc_result* res, uint32_t* partition_out, uint32_t* colors_out) {
     const u64 N = g->N;
@@ -1668,17 +1751,6 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
     RefineOut o;
     int rc = refine_core(ctx, g, prm, &o);
     if (rc) return rc;
-    if (N && (partition_out || colors_out)) {
-        std::vector<u32> col;
-        u32* hcol = colors_out;
-        if (!hcol) {
-            col.resize(N);
-            hcol = col.data();
-        }
-        rc = download_colors(ctx, g, hcol);
-        if (rc) return rc;
-        if (partition_out) canonicalize_host((u32)N, hcol, partition_out);
-    }
     int32_t verified = 0;
     if (opts && opts->verify && N) {
         u32* d_int = reinterpret_cast<u32*>(ctx->ws.keys[0]);
@@ -1687,6 +1759,11 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
         rc = verify_partition(ctx, g, d_int, o.K, &bad);
         if (rc) return rc;
         verified = bad ? -1 : 1;
+    }
+    // last: reuses workspace arrays
+    if (N && (partition_out || colors_out)) {
+        rc = device_partition(ctx, g, partition_out, colors_out);
+        if (rc) return rc;
     }
     auto t1 = std::chrono::steady_clock::now();
     rbsc_result r{};
