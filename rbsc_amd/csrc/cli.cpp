@@ -149,10 +149,17 @@ int main(int argc, char** argv) {
            : mode == "aggressive" ? RBSC_MODE_QUOTIENT_AGGRESSIVE : RBSC_MODE_STREAM;
     o.verify = (uint32_t)verify;
     rbsc_result r{};
-    std::vector<uint32_t> part(part_out.empty() ? 0 : N);
-    const int rc = rbsc_refine(ctx, &g, &o, &r, part_out.empty() ? nullptr : part.data());
+    // the partition lands in page-locked memory (a copy at the link's rate)
+    uint32_t* part = nullptr;
+    if (!part_out.empty() && rbsc_host_alloc(sizeof(uint32_t) * (uint64_t)N, (void**)&part) != RBSC_OK) {
+        fprintf(stderr, "rbsc: %s\n", rbsc_last_error());
+        rbsc_ctx_destroy(ctx);
+        return EXIT_FAILURE;
+    }
+    const int rc = rbsc_refine(ctx, &g, &o, &r, part);
     if (rc != RBSC_OK) {
         fprintf(stderr, "rbsc: %s (status %d)\n", rbsc_last_error(), rc);
+        rbsc_host_free(part);
         rbsc_ctx_destroy(ctx);
         return EXIT_FAILURE;
     }
@@ -165,9 +172,13 @@ int main(int argc, char** argv) {
             r.device_ms, r.verified);
     if (!part_out.empty()) {
         FILE* f = fopen(part_out.c_str(), "wb");
-        if (!f) return fail("cannot open --partition-out file");
+        if (!f) {
+            rbsc_host_free(part);
+            return fail("cannot open --partition-out file");
+        }
         for (uint32_t i = 0; i < N; ++i) fprintf(f, "%u\n", part[i]);
         fclose(f);
     }
+    rbsc_host_free(part);
     return 0;
 }
