@@ -109,6 +109,11 @@ typedef struct rbsc_opts {
  * RBSC_FLAG_UNPACKED in none.  Results are identical. */
 #define RBSC_FLAG_PACKED 512u
 #define RBSC_FLAG_UNPACKED 1024u
+/* A round whose previous block count is 1 (the first round from a uniform
+ * start, node order) ranks the distinct signatures through a hash table
+ * instead of sorting all nodes; RBSC_FLAG_SORTRANK sorts as in the other
+ * rounds (a test hook).  Results are identical. */
+#define RBSC_FLAG_SORTRANK 2048u
 
 /* Result.  nodes / done / time_s are the reference's three output lines
  * (rbsc.cu:492-494). */

@@ -35,6 +35,7 @@ FLAG_NOBINS = 64
 FLAG_COLOUR_ORDER = 128
 FLAG_PACKED = 512
 FLAG_UNPACKED = 1024
+FLAG_SORTRANK = 2048
 BIN_SLOT_ORDER = 1
 BIN_HASHED = 2
 

@@ -84,6 +84,7 @@ __device__ __forceinline__ u64 lb_load(u64* p) {
 
 // Error flag bits raised by kernels (read back by the host).
 #define KERR_SPIN 1u
+#define KERR_PROBE 2u  // a hash-table insert found no free slot
 
 // Host-side error check that records the failure and returns from an int
 // function.

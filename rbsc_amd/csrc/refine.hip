@@ -876,6 +876,78 @@ __global__ __launch_bounds__(256) void rank_packed_kernel(
     }
 }
 
+// One old colour (the first round from a uniform start): the dense rank of
+// (sig, colour) is the rank of sig among the distinct signatures, so no sort
+// of all nodes is needed.  Each node finds or claims a slot of an
+// open-addressed table of T = 2n slots (owner = node + 1; keys compared in
+// full through sig[owner - 1]; linear probing, load at most 1/2); the
+// distinct signatures are listed and sorted, each finds its slot again and
+// stores its position there; a node's colour is its slot's position.
+__device__ __forceinline__ u64 sig_slot(u64 s, u64 T) { return __umul64hi(rbsc_randomize(s), T); }
+
+__global__ void sig_insert_kernel(const u64* __restrict__ sig, u64 n, u32* __restrict__ owner,
+                                  u64 T, u32* __restrict__ slot_of, u32* __restrict__ err) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const u64 s = sig[u];
+    u64 h = sig_slot(s, T);
+    for (u64 probes = 0; probes < T; ++probes) {
+        u32 o = __hip_atomic_load(&owner[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (o == 0) {
+            const u32 prev = atomicCAS(&owner[h], 0u, (u32)(u + 1));
+            if (prev == 0) {
+                slot_of[u] = (u32)h;
+                return;
+            }
+            o = prev;
+        }
+        if (sig[o - 1] == s) {
+            slot_of[u] = (u32)h;
+            return;
+        }
+        h = h + 1 == T ? 0 : h + 1;
+    }
+    atomicOr(err, KERR_PROBE);  // not reached while the table is at most half full
+}
+
+// Signatures of the occupied slots, appended in any order.
+__global__ void sig_list_kernel(const u32* __restrict__ owner, u64 T, const u64* __restrict__ sig,
+                                u32* __restrict__ count, u64* __restrict__ keys) {
+    const u64 h = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 o = h < T ? owner[h] : 0u;
+    const u64 bal = __ballot(o != 0);
+    const int lane = threadIdx.x & 63;
+    u32 base = 0;
+    if (lane == 0 && bal) base = atomicAdd(count, (u32)__popcll(bal));
+    base = __shfl(base, 0);
+    if (o) keys[base + (u32)__popcll(bal & ((1ull << lane) - 1))] = sig[o - 1];
+}
+
+// sorted[i] (a listed signature) finds its slot and stores i there.
+__global__ void sig_number_kernel(const u64* __restrict__ sorted, u64 D,
+                                  const u32* __restrict__ owner, u64 T, const u64* __restrict__ sig,
+                                  u32* __restrict__ num, u32* __restrict__ err) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const u64 s = sorted[i];
+    u64 h = sig_slot(s, T);
+    for (u64 probes = 0; probes < T; ++probes) {
+        const u32 o = owner[h];
+        if (o && sig[o - 1] == s) {
+            num[h] = (u32)i;
+            return;
+        }
+        h = h + 1 == T ? 0 : h + 1;
+    }
+    atomicOr(err, KERR_PROBE);  // every listed signature has a slot
+}
+
+__global__ void sig_colour_kernel(const u32* __restrict__ slot_of, const u32* __restrict__ num,
+                                  u64 n, u64 rank_offset, u32* __restrict__ col_out) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) col_out[u] = (u32)(rank_offset + num[slot_of[u]]);
+}
+
 // keys[i] = colour of node i, P[i] = (colour << 32) | i (raw u32 colours).
 __global__ void colour_keys_kernel(const u32* __restrict__ col, u64 n, u64* __restrict__ keys,
                                    u64* __restrict__ P) {
@@ -1000,7 +1072,8 @@ int check_kernel_errors(rbsc_ctx* ctx) {
     HIP_TRY(hipMemcpy(&e, ctx->d_err, sizeof(u32), hipMemcpyDeviceToHost));
     if (e) {
         HIP_TRY(hipMemset(ctx->d_err, 0, sizeof(u32)));
-        rbsc_set_error("kernel error flags 0x%x (look-back wait expired)", e);
+        rbsc_set_error("kernel error flags 0x%x (%s)", e,
+                       (e & KERR_SPIN) ? "look-back wait expired" : "signature table full");
         return RBSC_ERR_KERNEL;
     }
     return RBSC_OK;
@@ -1570,8 +1643,46 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             return RBSC_OK;
         };
         const bool full_sort = (prm.flags & RBSC_FLAG_FULLSORT) != 0;
+        // Node order with one old colour (see sig_insert_kernel): ranks from
+        // the distinct signatures through a hash table.
+        auto rank_one_class = [&]() -> int {
+            const u64 T = 2 * n_rank;
+            u32* owner = reinterpret_cast<u32*>(w.P[0]);    // T slots
+            u32* num = reinterpret_cast<u32*>(w.P[1]);      // T slots
+            u32* slot_of = reinterpret_cast<u32*>(w.hc);    // the old colours, all equal
+            prof_begin(ctx, 2);
+            fill_u32_kernel<<<grid_for(T), BLK, 0, s>>>(owner, T, 0);
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 13, 1, 0);
+            sig_insert_kernel<<<grid_for(n_rank), BLK, 0, s>>>(w.sig, n_rank, owner, T, slot_of,
+                                                                ctx->d_err);
+            sig_list_kernel<<<grid_for(T), BLK, 0, s>>>(owner, T, w.sig, ctx->d_scalar + 13,
+                                                        w.keys[0]);
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 13, ctx->d_scalar + 13, sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            const u64 D = ctx->h_scalar[13];
+            u64* kk[2] = {w.keys[0], w.keys[1]};
+            int wh = 0;
+            int r2 = radix_sort_u64(&ctx->sort, kk, nullptr, D, 0, 64, &wh, s);
+            if (r2) return r2;
+            sig_number_kernel<<<grid_for(D), BLK, 0, s>>>(kk[wh], D, owner, T, w.sig, num,
+                                                          ctx->d_err);
+            prof_end(ctx);
+            prof_begin(ctx, 3);
+            sig_colour_kernel<<<grid_for(n_rank), BLK, 0, s>>>(slot_of, num, n_rank, rank_offset,
+                                                               reinterpret_cast<u32*>(w.hc2));
+            rep_colors(w.hc2);
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 1, 11, 0);  // no overflow, disorder, runs
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar, 1, (u32)(rank_offset + D));
+            prof_end(ctx);
+            return RBSC_OK;
+        };
+        const bool one_class = node_order && K == 1 && !full_sort &&
+                               !(prm.flags & RBSC_FLAG_SORTRANK) &&
+                               n_rank < (1ull << 30);
         if (n_rank) {
-            rc = node_order ? sort_and_rank_nodes(full_sort) : sort_and_rank(full_sort, true);
+            rc = one_class ? rank_one_class()
+               : node_order ? sort_and_rank_nodes(full_sort) : sort_and_rank(full_sort, true);
             if (rc) return rc;
         }
         HIP_TRY(hipGetLastError());

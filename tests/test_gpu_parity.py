@@ -344,7 +344,7 @@ def test_xcd_partitioned_single_launch(ctx, flags):
 
 
 @pytest.mark.parametrize("flags", [0, 32, 8, 8 | 16, 16, 128, 128 | 32, 128 | 8, 128 | 8 | 16,
-                                   512, 512 | 8, 512 | 8 | 16, 1024, 1024 | 8])
+                                   512, 512 | 8, 512 | 8 | 16, 1024, 1024 | 8, 2048, 2048 | 8])
 def test_sort_split_and_tie_fixup(ctx, flags):
     """Top-half sort + colliding-run fix-up (default, 0), full 64-bit sort
     (32), frequent collisions via a 16-bit split (8), and the full-sort
@@ -353,7 +353,8 @@ def test_sort_split_and_tie_fixup(ctx, flags):
     fallback takes the exact redo from colour order, and 128 keeps
     positions in colour order instead; 512 sorts packed 8-byte keys in
     every node-order round (low signature bits and colours read by node
-    where top bits tie), 1024 in none; the BA graph has initial colours."""
+    where top bits tie), 1024 in none; 2048 sorts in the first round too
+    (instead of the hash-table rank); the BA graph has initial colours."""
     for g in (gen.rmat(15, 1 << 19, clone_frac=0.2, seed=15),
               gen.ba(30000, 3, 0.1, 3, seed=9)):
         c, K, R, tr = _oracle_full(g, trace_rounds=64)
@@ -389,3 +390,24 @@ def test_tie_runs_of_every_length(ctx, flags):
     st = ctx.stats(reset=True)
     assert st["long_run_rounds"] > 0, st
     dg.free()
+
+
+@pytest.mark.parametrize("flags", [0, rbsc_amd.FLAG_SORTRANK, 8])
+def test_first_round_hash_rank(ctx, flags):
+    """The first round from a uniform start (one old colour) ranks the
+    distinct signatures through a hash table (default, also under the
+    16-bit split 8) or sorts all nodes (FLAG_SORTRANK): every round equals
+    the oracle's on a weighted R-MAT (first-round signatures nearly all
+    distinct), a sparse R-MAT (many line-less nodes, colour 0 kept apart)
+    and a 3-regular graph (one signature: the fixpoint after one round)."""
+    for g in (_weighted(gen.rmat(14, 1 << 18, clone_frac=0.2, seed=21), 1 << 20),
+              gen.rmat(12, 1 << 13, clone_frac=0.3, seed=22),
+              gen.regular_undirected(3000, 3, seed=4)):
+        c, K, R, tr = _oracle_full(g, trace_rounds=64)
+        dg = ctx.upload(g)
+        for r in range(1, R + 1):
+            res = dg.refine(max_rounds=r, partition=False, colors=True, flags=flags)
+            assert np.array_equal(res.colors, tr[r - 1]), f"flags={flags} round {r}"
+        res = dg.refine(partition=True, flags=flags)
+        assert (res.nodes, res.rounds) == (K, R)
+        dg.free()
