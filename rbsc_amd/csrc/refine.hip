@@ -885,29 +885,65 @@ __global__ __launch_bounds__(256) void rank_packed_kernel(
 // stores its position there; a node's colour is its slot's position.
 __device__ __forceinline__ u64 sig_slot(u64 s, u64 T) { return __umul64hi(rbsc_randomize(s), T); }
 
-__global__ void sig_insert_kernel(const u64* __restrict__ sig, u64 n, u32* __restrict__ owner,
-                                  u64 T, u32* __restrict__ slot_of, u32* __restrict__ err) {
-    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
-    const u64 s = sig[u];
-    u64 h = sig_slot(s, T);
-    for (u64 probes = 0; probes < T; ++probes) {
-        u32 o = __hip_atomic_load(&owner[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (o == 0) {
-            const u32 prev = atomicCAS(&owner[h], 0u, (u32)(u + 1));
-            if (prev == 0) {
-                slot_of[u] = (u32)h;
-                return;
+// Rows of one workgroup that share a signature first meet in an LDS table
+// (INS_LDS slots for INS_BLK rows, so at most half full); only the first of
+// them probes the global table, and the others take its slot.  Rows sharing
+// a first-round signature (millions here) then reach a global slot's line
+// once per workgroup instead of once per row.
+constexpr int INS_BLK = 256;
+constexpr u32 INS_LDS = 512;
+
+__global__ __launch_bounds__(INS_BLK) void sig_insert_kernel(
+    const u64* __restrict__ sig, u64 n, u32* __restrict__ owner, u64 T,
+    u32* __restrict__ slot_of, u32* __restrict__ err) {
+    __shared__ u64 l_sig[INS_BLK];
+    __shared__ u32 l_lead[INS_LDS];  // row (in the workgroup) + 1 owning the slot; 0 = empty
+    __shared__ u32 l_slot[INS_BLK];
+    const int t = threadIdx.x;
+    const u64 u = (u64)blockIdx.x * INS_BLK + t;
+    const bool valid = u < n;
+    const u64 s = valid ? sig[u] : 0;
+    l_sig[t] = s;
+    for (u32 i = t; i < INS_LDS; i += INS_BLK) l_lead[i] = 0;
+    __syncthreads();
+    const u64 r = rbsc_randomize(s);
+    int lead = t;
+    if (valid) {
+        u32 h = (u32)r & (INS_LDS - 1);
+        for (u32 p = 0; p < INS_LDS; ++p) {
+            const u32 o = atomicCAS(&l_lead[h], 0u, (u32)(t + 1));
+            if (o == 0) break;
+            if (l_sig[o - 1] == s) {
+                lead = (int)(o - 1);
+                break;
             }
-            o = prev;
+            h = (h + 1) & (INS_LDS - 1);
         }
-        if (sig[o - 1] == s) {
-            slot_of[u] = (u32)h;
-            return;
-        }
-        h = h + 1 == T ? 0 : h + 1;
     }
-    atomicOr(err, KERR_PROBE);  // not reached while the table is at most half full
+    if (valid && lead == t) {
+        u64 h = __umul64hi(r, T);
+        u32 found = 0;
+        for (u64 probes = 0; probes < T; ++probes) {
+            u32 o = __hip_atomic_load(&owner[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (o == 0) {
+                const u32 prev = atomicCAS(&owner[h], 0u, (u32)(u + 1));
+                if (prev == 0) {
+                    found = 1;
+                    break;
+                }
+                o = prev;
+            }
+            if (sig[o - 1] == s) {
+                found = 1;
+                break;
+            }
+            h = h + 1 == T ? 0 : h + 1;
+        }
+        if (!found) atomicOr(err, KERR_PROBE);  // not reached while the table is at most half full
+        l_slot[t] = (u32)h;
+    }
+    __syncthreads();
+    if (valid) slot_of[u] = l_slot[lead];
 }
 
 // Signatures of the occupied slots, appended in any order.
@@ -1653,8 +1689,8 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             prof_begin(ctx, 2);
             fill_u32_kernel<<<grid_for(T), BLK, 0, s>>>(owner, T, 0);
             fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 13, 1, 0);
-            sig_insert_kernel<<<grid_for(n_rank), BLK, 0, s>>>(w.sig, n_rank, owner, T, slot_of,
-                                                                ctx->d_err);
+            sig_insert_kernel<<<(unsigned)((n_rank + INS_BLK - 1) / INS_BLK), INS_BLK, 0, s>>>(
+                w.sig, n_rank, owner, T, slot_of, ctx->d_err);
             sig_list_kernel<<<grid_for(T), BLK, 0, s>>>(owner, T, w.sig, ctx->d_scalar + 13,
                                                         w.keys[0]);
             HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 13, ctx->d_scalar + 13, sizeof(u32),
