@@ -946,17 +946,47 @@ __global__ __launch_bounds__(INS_BLK) void sig_insert_kernel(
     if (valid) slot_of[u] = l_slot[lead];
 }
 
-// Signatures of the occupied slots, appended in any order.
-__global__ void sig_list_kernel(const u32* __restrict__ owner, u64 T, const u64* __restrict__ sig,
-                                u32* __restrict__ count, u64* __restrict__ keys) {
-    const u64 h = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    const u32 o = h < T ? owner[h] : 0u;
-    const u64 bal = __ballot(o != 0);
-    const int lane = threadIdx.x & 63;
-    u32 base = 0;
-    if (lane == 0 && bal) base = atomicAdd(count, (u32)__popcll(bal));
-    base = __shfl(base, 0);
-    if (o) keys[base + (u32)__popcll(bal & ((1ull << lane) - 1))] = sig[o - 1];
+// Signatures of the occupied slots, appended in any order.  A workgroup
+// covers LIST_ITEMS * 256 slots and reserves its range of the output with
+// one atomic (the occupied slots are few and scattered, and one atomic per
+// wave that met any serialised on the counter's address); inside it,
+// positions come from ballots, so stores stay packed.
+constexpr int LIST_ITEMS = 64;
+constexpr u64 LIST_TILE = 256 * LIST_ITEMS;
+
+__global__ __launch_bounds__(256) void sig_list_kernel(const u32* __restrict__ owner, u64 T,
+                                                       const u64* __restrict__ sig,
+                                                       u32* __restrict__ count,
+                                                       u64* __restrict__ keys) {
+    __shared__ u32 s_wave[4];
+    __shared__ u32 s_base;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const u64 base = (u64)blockIdx.x * LIST_TILE + (u64)wave * (64 * LIST_ITEMS);
+    u32 o[LIST_ITEMS];
+    u32 wn = 0;
+#pragma unroll
+    for (int j = 0; j < LIST_ITEMS; ++j) {
+        const u64 h = base + 64 * j + lane;
+        o[j] = h < T ? owner[h] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < LIST_ITEMS; ++j) wn += (u32)__popcll(__ballot(o[j] != 0));
+    if (lane == 0) s_wave[wave] = wn;
+    __syncthreads();
+    if (t == 0) {
+        const u32 tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        s_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    u32 off = s_base;
+    for (int w = 0; w < wave; ++w) off += s_wave[w];
+    const u64 below = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+#pragma unroll
+    for (int j = 0; j < LIST_ITEMS; ++j) {
+        const u64 bal = __ballot(o[j] != 0);
+        if (o[j]) keys[off + (u32)__popcll(bal & below)] = sig[o[j] - 1];
+        off += (u32)__popcll(bal);
+    }
 }
 
 // sorted[i] (a listed signature) finds its slot and stores i there.
@@ -1691,8 +1721,8 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 13, 1, 0);
             sig_insert_kernel<<<(unsigned)((n_rank + INS_BLK - 1) / INS_BLK), INS_BLK, 0, s>>>(
                 w.sig, n_rank, owner, T, slot_of, ctx->d_err);
-            sig_list_kernel<<<grid_for(T), BLK, 0, s>>>(owner, T, w.sig, ctx->d_scalar + 13,
-                                                        w.keys[0]);
+            sig_list_kernel<<<(unsigned)((T + LIST_TILE - 1) / LIST_TILE), 256, 0, s>>>(
+                owner, T, w.sig, ctx->d_scalar + 13, w.keys[0]);
             HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 13, ctx->d_scalar + 13, sizeof(u32),
                                    hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
