@@ -227,10 +227,41 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
+def pinned_copy_peak(nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Host-to-device rate (bytes/s) of hipMemcpyAsync from page-locked
+    memory on this box: the attainable peak of the streamed path."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    h, d, st = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipHostMalloc(C.byref(h), C.c_size_t(nbytes), 0) == 0
+    assert hip.hipMalloc(C.byref(d), C.c_size_t(nbytes)) == 0
+    assert hip.hipStreamCreate(C.byref(st)) == 0
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    C.memset(h, 1, nbytes)
+    assert hip.hipMemcpyAsync(d, h, C.c_size_t(nbytes), 1, st) == 0  # warm
+    assert hip.hipEventRecord(e0, st) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpyAsync(d, h, C.c_size_t(nbytes), 1, st) == 0
+    assert hip.hipEventRecord(e1, st) == 0
+    assert hip.hipEventSynchronize(e1) == 0
+    ms = C.c_float()
+    assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipStreamDestroy(st)
+    hip.hipFree(d)
+    hip.hipHostFree(h)
+    return reps * nbytes / (ms.value / 1e3)
+
+
 def streaming_leg(dg, E: int, frac: float, steps: int) -> dict:
     """The same graph with its lines in pinned host memory, refined in edge
     batches of frac * E lines streamed over PCIe each round (double-buffered
-    copies on a side stream, overlapped with the signature kernel)."""
+    copies on a side stream, overlapped with the signature kernel).  The
+    rate over the link is priced against the pinned host-to-device copy
+    rate measured on the same box."""
+    peak = pinned_copy_peak()
     t0 = time.perf_counter()
     dg.lines_to_host()
     log(f"[bench] lines moved to pinned host memory in {time.perf_counter() - t0:.1f}s")
@@ -244,10 +275,34 @@ def streaming_leg(dg, E: int, frac: float, steps: int) -> dict:
         dev_ms += r.device_ms
     wall = time.perf_counter() - t
     pcie = 4 * dg.stored_lines  # every round streams the stored line array (unweighted heads)
+    rate = pcie * rounds / (dev_ms / 1e3)
     return {"workload": "same graph, lines in pinned host memory", "batch_fraction": frac,
             "max_batch_edges": B, "value": rounds * E / wall, "unit": "edge-visits/s",
             "ms_per_round": dev_ms / rounds, "pcie_bytes_per_round": pcie,
-            "pcie_GBps": pcie * rounds / (dev_ms / 1e3) / 1e9, "blocks": r.nodes}
+            "pcie_GBps": rate / 1e9,
+            "roofline": {"bound": "pcie", "achieved": rate / 1e9, "peak": peak / 1e9,
+                         "unit": "GB/s", "frac": rate / peak,
+                         "peak_source": "hipMemcpyAsync of 1 GiB from page-locked memory, "
+                                        "measured in this run"},
+            "blocks": r.nodes}
+
+
+def one_shot_leg(ctx, dg) -> dict:
+    """The drop-in's whole run on the same graph from host arrays
+    (rbsc_refine: upload, identical-row merge and relabelling, layout,
+    rounds, canonical partition), the span the reference's timer covers
+    after parsing (rbsc.cu:272-489).  The graph is first copied back to the
+    host in CSR form."""
+    t = time.perf_counter()
+    hg = dg.download()
+    dl = time.perf_counter() - t
+    ctx.refine(hg)  # warm: device allocations of this size exist afterwards
+    t = time.perf_counter()
+    r = ctx.refine(hg)
+    wall = time.perf_counter() - t
+    return {"workload": "same graph from host CSR arrays (rbsc_refine)", "time_s": r.time_s,
+            "wall_s": wall, "rounds": r.rounds, "blocks": r.nodes, "device_ms": r.device_ms,
+            "download_s": dl}
 
 
 def copy_probe(nbytes: int) -> None:
@@ -307,6 +362,8 @@ def main():
                     help="worker threads (contexts) of the serving loop")
     ap.add_argument("--stream-frac", type=float, default=0.25,
                     help="also time the pinned-host streaming path at this edge-batch fraction (0 skips)")
+    ap.add_argument("--one-shot", type=int, default=1,
+                    help="also time the drop-in's whole run (rbsc_refine from host arrays)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -345,6 +402,19 @@ def main():
         r = dg.refine(partition=False, flags=flags)
         log(f"[bench] warmup {i}: rounds={r.rounds} blocks={r.nodes} device_ms={r.device_ms:.2f}")
 
+    # Exactness of the measured path, outside the timed region: the exact
+    # stability check on the final partition, and the same refinement with
+    # the first round ranked by a full sort instead of the hash table.
+    chk = dg.refine(verify=True, flags=flags)
+    srt = dg.refine(partition=False, flags=flags | rbsc_amd.FLAG_SORTRANK)
+    checks = {"verified": chk.verified, "blocks": chk.nodes, "rounds": chk.rounds,
+              "sortrank_blocks": srt.nodes, "sortrank_rounds": srt.rounds,
+              "sortrank_same": (srt.nodes, srt.rounds) == (chk.nodes, chk.rounds)}
+    log(f"[bench] checks: {checks}")
+    if chk.verified != 1 or not checks["sortrank_same"]:
+        raise SystemExit(f"bench: the measured refinement failed its checks: {checks}")
+    del chk
+
     ctx.profile(True)
     ctx.profile_read(reset=True)
     rounds = 0
@@ -365,9 +435,13 @@ def main():
     ms_round = dev_ms / rounds
     U = dg.rows            # rows the signature pass visits (identical rows once)
     E_st = dg.stored_lines  # their lines
-    # Whole round: 8 B per input line (head id + neighbour colour) and 32 B
-    # per node (row offset, signature write and read, old and new colour).
-    b_round = 8 * E + 32 * N
+    # Whole round: 8 B per line the pass reads (head id + neighbour colour;
+    # identical rows are stored once, so E_st lines) and 32 B per node (row
+    # offset, signature write and read, old and new colour).  Beside it the
+    # reference-equivalent figure over every input line (8 E + 32 N), the
+    # bytes a pass over all lines would move.
+    b_round = 8 * E_st + 32 * N
+    b_round_ref = 8 * E + 32 * N
     # The signature pass (the reference's set_values + ReduceByKey): 8 B per
     # stored line (head id + neighbour colour) plus 16 B per stored row (row
     # offset or per-row term, signature write), once per round.  On the
@@ -415,8 +489,18 @@ def main():
                      "round": {"alg_bytes": b_round, "ms": round_ms,
                                "achieved": b_round / (round_ms / 1e3) / 1e9,
                                "frac": b_round / (round_ms / 1e3) / HBM_PEAK,
-                               "phase_ms": per_round}},
+                               "phase_ms": per_round,
+                               "reference_equivalent": {
+                                   "alg_bytes": b_round_ref,
+                                   "achieved": b_round_ref / (round_ms / 1e3) / 1e9,
+                                   "frac": b_round_ref / (round_ms / 1e3) / HBM_PEAK}}},
+        "checks": checks,
     }
+    if bin_info is not None:
+        out["config"]["layout_build_s"] = bin_info["build_s"]
+    if args.one_shot:
+        out["one_shot"] = one_shot_leg(ctx, dg)
+        log(f"[bench] one-shot: {out['one_shot']}")
     if args.batch_graphs:
         out["batched"] = batch_leg(ctx, args.batch_graphs, 1004, 500)
         log(f"[bench] batched: {out['batched']['graphs_per_s']:.0f} graphs/s "

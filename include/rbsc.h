@@ -295,6 +295,25 @@ int rbsc_batch_free(rbsc_ctx* ctx, rbsc_dbatch* b);
 int rbsc_batch_refine(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, uint32_t* blocks_out,
                       uint32_t* rounds_out, uint32_t* labels_out, uint32_t* colors_out,
                       rbsc_result* total);
+/* Outputs of rbsc_batch_refine_ex, all nullable.  blocks, rounds: per graph;
+ * labels, colors: per batch node as for rbsc_batch_refine; block_sizes: per
+ * batch node, the final colour histogram of each graph: graph g's entries
+ * [node_offsets[g], node_offsets[g+1]) hold its block sizes in descending
+ * order (blocks[g] nonzero entries, summing to n_g), then zeros.  Block sizes
+ * do not depend on how blocks are numbered, so they compare across graphs and
+ * tools.  Buffers from rbsc_host_alloc are written by the kernels directly. */
+typedef struct rbsc_batch_outputs {
+    uint32_t* blocks;
+    uint32_t* rounds;
+    uint32_t* labels;
+    uint32_t* colors;
+    uint32_t* block_sizes;
+} rbsc_batch_outputs;
+/* rbsc_batch_refine with the per-graph block-size histogram as a further
+ * output.  No counterpart in the reference (one graph per process; its
+ * output is the block count, rbsc.cu:492-494). */
+int rbsc_batch_refine_ex(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts,
+                         const rbsc_batch_outputs* out, rbsc_result* total);
 /* Times the persistent launch at workgroup widths 256/512/1024 and 1/2/4
  * workgroups per CU on this batch and keeps the fastest for later refines. */
 int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* threads_out, uint32_t* wgs_out,

@@ -359,29 +359,56 @@ class BatchResult:
     labels: Optional[np.ndarray]
     colors: Optional[np.ndarray]
     total: RefineResult
+    block_sizes: Optional[np.ndarray] = None
+    node_offsets: Optional[np.ndarray] = None
+
+    def histogram(self, g: int) -> np.ndarray:
+        """Graph g's final colour histogram: its block sizes in descending
+        order (one entry per block)."""
+        if self.block_sizes is None:
+            raise ValueError("refine(histogram=True) computes the histograms")
+        a = int(self.node_offsets[g])
+        return self.block_sizes[a:a + int(self.blocks[g])]
+
+
+class _PinnedBlock:
+    """Owner of one ``rbsc_host_alloc`` block, exposed to numpy through
+    ``__array_interface__``: every array made from it (and every view of
+    those) keeps the block as its base, so the block is released by
+    ``rbsc_host_free`` only when the last array over it is gone."""
+
+    def __init__(self, n: int):
+        n = max(int(n), 1)
+        p = C.c_void_p()
+        check(load().rbsc_host_alloc(4 * n, C.byref(p)))
+        self._p = p
+        self.__array_interface__ = {"shape": (n,), "typestr": "<u4", "data": (p.value, False),
+                                    "version": 3}
+
+    def __del__(self):
+        try:
+            if self._p:
+                load().rbsc_host_free(self._p)
+                self._p = None
+        except Exception:
+            pass
 
 
 class PinnedArray:
     """A numpy uint32 array over page-locked host memory (``rbsc_host_alloc``):
-    device-to-host copies into it are not staged through pageable memory."""
+    device-to-host copies into it are not staged through pageable memory.
+
+    ``free()`` drops this object's hold on the memory; arrays handed out
+    earlier (``.array`` and its views) keep it alive until they are gone."""
 
     def __init__(self, n: int):
-        self._p = C.c_void_p()
-        check(load().rbsc_host_alloc(4 * max(int(n), 1), C.byref(self._p)))
-        buf = (C.c_uint32 * max(int(n), 1)).from_address(self._p.value)
-        self.array = np.ctypeslib.as_array(buf)
+        self.array = np.asarray(_PinnedBlock(n))
 
     def free(self):
-        if self._p:
-            self.array = None
-            load().rbsc_host_free(self._p)
-            self._p = C.c_void_p()
+        self.array = None
 
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        self.free()
 
 
 class DeviceBatch:
@@ -392,13 +419,16 @@ class DeviceBatch:
         self.handle = handle
         self.n_graphs = host.n_graphs
         self.n_nodes = host.n_nodes
+        self.node_offsets = host.node_offsets
         self._pinned = None
 
     def refine(self, max_rounds=0, labels=True, colors=False, pinned=False,
-               flags=0) -> BatchResult:
-        """pinned: download the labels into page-locked memory owned by the
-        batch (faster); the returned labels array is then reused, and
-        overwritten, by the next pinned refine of this batch."""
+               flags=0, histogram=False) -> BatchResult:
+        """pinned: download the labels into page-locked memory shared by the
+        batch and the returned array (faster); that array is reused, and
+        overwritten, by the next pinned refine of this batch, and stays
+        valid after the batch is freed (the memory is released with the
+        last array over it)."""
         blocks = np.empty(max(self.n_graphs, 1), np.uint32)
         rounds = np.empty(max(self.n_graphs, 1), np.uint32)
         if labels and pinned:
@@ -408,14 +438,18 @@ class DeviceBatch:
         else:
             lab = np.empty(max(self.n_nodes, 1), np.uint32) if labels else None
         col = np.empty(max(self.n_nodes, 1), np.uint32) if colors else None
+        sizes = np.empty(max(self.n_nodes, 1), np.uint32) if histogram else None
         r = _lib.Result()
         o = _opts(max_rounds=max_rounds, flags=flags)
-        check(load().rbsc_batch_refine(self.ctx.handle, self.handle, C.byref(o),
-                                       _ptr(blocks, C.c_uint32), _ptr(rounds, C.c_uint32),
-                                       _ptr(lab, C.c_uint32), _ptr(col, C.c_uint32), C.byref(r)))
+        out = _lib.BatchOutputs(_ptr(blocks, C.c_uint32), _ptr(rounds, C.c_uint32),
+                                _ptr(lab, C.c_uint32), _ptr(col, C.c_uint32),
+                                _ptr(sizes, C.c_uint32))
+        check(load().rbsc_batch_refine_ex(self.ctx.handle, self.handle, C.byref(o), C.byref(out),
+                                          C.byref(r)))
         n, g = self.n_nodes, self.n_graphs
         return BatchResult(blocks[:g], rounds[:g], None if lab is None else lab[:n],
-                           None if col is None else col[:n], _result(r))
+                           None if col is None else col[:n], _result(r),
+                           None if sizes is None else sizes[:n], self.node_offsets)
 
     def autotune(self):
         t = C.c_uint32()

@@ -107,6 +107,11 @@ class Batch(C.Structure):
     ]
 
 
+class BatchOutputs(C.Structure):
+    _fields_ = [("blocks", u32p), ("rounds", u32p), ("labels", u32p), ("colors", u32p),
+                ("block_sizes", u32p)]
+
+
 class HostBatch(C.Structure):
     _fields_ = [
         ("n_graphs", C.c_uint32),
@@ -187,6 +192,8 @@ SIGNATURES = {
     "rbsc_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rbsc_batch_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), u32p, u32p, u32p, u32p,
                                     C.POINTER(Result)]),
+    "rbsc_batch_refine_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts),
+                                       C.POINTER(BatchOutputs), C.POINTER(Result)]),
     "rbsc_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, u32p, u32p, C.POINTER(C.c_double)]),
     "rbsc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rbsc_refine_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Opts), u32p, u32p, u32p,

@@ -37,6 +37,8 @@ struct BatchDevice {
 struct rbsc_ctx;
 // split_lo: the signatures are sorted by their bits [split_lo, 64) and runs
 // colliding there ordered afterwards (32; 48 is a verification hook).
+// d_sizes (nullable): per batch node, each graph's block sizes in
+// descending order, then zeros.
 int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
-                 u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg,
+                 u32* d_sizes, u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg,
                  int split_lo = 32);

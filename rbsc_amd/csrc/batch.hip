@@ -161,8 +161,9 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
     const BatchDesc* __restrict__ desc, const u32* __restrict__ order, u32 n_jobs,
     const u32* __restrict__ rp_all, const unsigned short* __restrict__ dst_all,
     const u64* __restrict__ wt_all, const u32* __restrict__ init_all, u32 max_rounds,
-    u32* __restrict__ labels_out, u32* __restrict__ colors_out, u32* __restrict__ blocks_out,
-    u32* __restrict__ rounds_out, u32* __restrict__ queue, int split_lo) {
+    u32* __restrict__ labels_out, u32* __restrict__ colors_out, u32* __restrict__ sizes_out,
+    u32* __restrict__ blocks_out, u32* __restrict__ rounds_out, u32* __restrict__ queue,
+    int split_lo) {
     constexpr int IPT = Cfg<BT, NM>::IPT;
     __shared__ Lds<BT, NM> s;
     // Every job runs the whole body: no `continue` back to the thread-0 queue
@@ -353,6 +354,52 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
             rounds_out[g] = it;
         }
         __syncthreads();
+        if (sizes_out) {
+            // Block sizes in descending order, then zeros (n entries), by a
+            // counting sort: size[c] nodes per colour c < Knew, then bys[i]
+            // blocks of size n - i, scanned, so block j of the descending
+            // order has the size n - i for the last i whose scan is <= j.
+            u32* size = first;                                  // n entries
+            u32* bys = reinterpret_cast<u32*>(&s.key[0][0]);    // n entries
+            for (int i = threadIdx.x; i < n; i += BT) {
+                size[i] = 0;
+                bys[i] = 0;
+            }
+            __syncthreads();
+            for (int u = threadIdx.x; u < n; u += BT) atomicAdd(&size[s.col[u]], 1u);
+            __syncthreads();
+            for (int c = threadIdx.x; c < (int)Knew; c += BT) atomicAdd(&bys[n - (int)size[c]], 1u);
+            __syncthreads();
+            u32 v[IPT];
+            u32 cnt = 0;
+            const int b0 = threadIdx.x * IPT;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                v[j] = b0 + j < n ? bys[b0 + j] : 0u;
+                cnt += v[j];
+            }
+            u32 tot;
+            u32 r = block_scan_excl(cnt, &s, &tot);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (b0 + j < n) bys[b0 + j] = r;
+                r += v[j];
+            }
+            __syncthreads();
+            for (int j = threadIdx.x; j < n; j += BT) {
+                u32 out = 0;
+                if (j < (int)Knew) {
+                    int lo = 0, hi = n - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (bys[mid] <= (u32)j) lo = mid; else hi = mid - 1;
+                    }
+                    out = (u32)(n - lo);
+                }
+                sizes_out[d.node_base + j] = out;
+            }
+            __syncthreads();
+        }
     }
 }
 static_assert(sizeof(Lds<1024, BATCH_NMAX>) <= 160 * 1024, "batch LDS layout exceeds 160 KB");
@@ -374,13 +421,13 @@ __global__ void zero_queue(u32* q) {
 // from the queue counter q.
 template <int BT, int NM>
 static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 j0, u32 j1,
-                      u32 max_rounds, u32* d_labels, u32* d_colors, u32* d_blocks, u32* d_rounds,
-                      u32* q, int split_lo) {
+                      u32 max_rounds, u32* d_labels, u32* d_colors, u32* d_sizes, u32* d_blocks,
+                      u32* d_rounds, u32* q, int split_lo) {
 #define BL(W, I)                                                                              \
     batch_refine_kernel<BT, NM, W, I><<<grid, BT, 0, s>>>(b.desc, b.order + j0, j1 - j0, b.rp, \
                                                           b.dst, b.wt, b.init, max_rounds,      \
-                                                          d_labels, d_colors, d_blocks,         \
-                                                          d_rounds, q, split_lo)
+                                                          d_labels, d_colors, d_sizes,          \
+                                                          d_blocks, d_rounds, q, split_lo)
     if (b.wt) {
         if (b.init) BL(true, true); else BL(true, false);
     } else {
@@ -392,7 +439,8 @@ static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 j0
 }  // namespace
 
 int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
-                 u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg, int split_lo) {
+                 u32* d_sizes, u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg,
+                 int split_lo) {
     hipStream_t s = ctx->stream;
     zero_queue<<<1, 1, 0, s>>>(d_queue);
     if (b.n_small == 0) return RBSC_OK;
@@ -400,9 +448,9 @@ int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_lab
     if (b.n_hi) {
         const unsigned grid = (unsigned)std::max(1, std::min<int>(n_wg, (int)b.n_hi));
         switch (threads) {
-            case 256: launch_bt<256, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
-            case 1024: launch_bt<1024, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
-            default: launch_bt<512, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_blocks, d_rounds, d_queue, split_lo); break;
+            case 256: launch_bt<256, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo); break;
+            case 1024: launch_bt<1024, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo); break;
+            default: launch_bt<512, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo); break;
         }
     }
     // graphs above BATCH_NLOW nodes: half the layout, two workgroups per CU
@@ -410,21 +458,24 @@ int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_lab
         const unsigned grid =
             (unsigned)std::max(1, std::min<int>(2 * b.cus, (int)(b.n_mid - b.n_hi)));
         launch_bt<MID_THREADS, BATCH_NMID>(s, grid, b, b.n_hi, b.n_mid, max_rounds, d_labels,
-                                           d_colors, d_blocks, d_rounds, d_queue + 1, split_lo);
+                                           d_colors, d_sizes, d_blocks, d_rounds, d_queue + 1,
+                                           split_lo);
     }
     // graphs above BATCH_NTINY nodes: a quarter of the layout, four per CU
     if (b.n_low > b.n_mid) {
         const unsigned grid =
             (unsigned)std::max(1, std::min<int>(4 * b.cus, (int)(b.n_low - b.n_mid)));
         launch_bt<LOW_THREADS, BATCH_NLOW>(s, grid, b, b.n_mid, b.n_low, max_rounds, d_labels,
-                                           d_colors, d_blocks, d_rounds, d_queue + 2, split_lo);
+                                           d_colors, d_sizes, d_blocks, d_rounds, d_queue + 2,
+                                           split_lo);
     }
     // the others: an eighth of the layout, eight workgroups per CU
     if (b.n_small > b.n_low) {
         const unsigned grid =
             (unsigned)std::max(1, std::min<int>(8 * b.cus, (int)(b.n_small - b.n_low)));
         launch_bt<TINY_THREADS, BATCH_NTINY>(s, grid, b, b.n_low, b.n_small, max_rounds, d_labels,
-                                             d_colors, d_blocks, d_rounds, d_queue + 3, split_lo);
+                                             d_colors, d_sizes, d_blocks, d_rounds, d_queue + 3,
+                                             split_lo);
     }
     HIP_TRY(hipGetLastError());
     return RBSC_OK;
@@ -440,6 +491,7 @@ struct rbsc_dbatch {
     BatchDevice dev;
     u32 *d_labels = nullptr, *d_colors = nullptr, *d_blocks = nullptr, *d_rounds = nullptr,
         *d_queue = nullptr;
+    u32* d_sizes = nullptr;             // block sizes, allocated by the first refine asking for them
     int threads = 512, wgs = 0;
     u64 bytes = 0;
 };
@@ -448,7 +500,7 @@ static void dbatch_release(rbsc_ctx* ctx, rbsc_dbatch* b) {
     for (rbsc_dgraph* g : b->big_g) rbsc_graph_free(ctx, g);
     for (void* p : {(void*)b->dev.desc, (void*)b->dev.order, (void*)b->dev.rp, (void*)b->dev.dst,
                     (void*)b->dev.wt, (void*)b->dev.init, (void*)b->d_labels, (void*)b->d_colors,
-                    (void*)b->d_blocks, (void*)b->d_rounds, (void*)b->d_queue})
+                    (void*)b->d_blocks, (void*)b->d_rounds, (void*)b->d_queue, (void*)b->d_sizes})
         if (p) (void)hipFree(p);
 }
 
@@ -618,20 +670,26 @@ extern "C" int rbsc_batch_free(rbsc_ctx* ctx, rbsc_dbatch* b) {
 }
 
 static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* blocks_out,
-                     u32* rounds_out, u32* labels_out, u32* colors_out, rbsc_result* total,
-                     int threads, int wgs) {
+                     u32* rounds_out, u32* labels_out, u32* colors_out, u32* sizes_out,
+                     rbsc_result* total, int threads, int wgs) {
     hipStream_t s = ctx->stream;
     const u32 max_rounds = opts ? opts->max_rounds : 0;
     auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(hipEventRecord(ctx->ev0, s));
-    const int split_lo = (opts && (opts->flags & RBSC_FLAG_SPLIT16)) ? 48 : 32;
-    // Labels into a buffer from rbsc_host_alloc are written by the kernels
-    // themselves, so their transfer overlaps the refinement.
+    // Labels (and block sizes) into buffers from rbsc_host_alloc are written
+    // by the kernels themselves, so their transfer overlaps the refinement.
     u32* direct = labels_out
         ? static_cast<u32*>(host_device_ptr(labels_out, sizeof(u32) * b->total_nodes))
         : nullptr;
-    int rc = batch_launch(ctx, b->dev, max_rounds, direct ? direct : b->d_labels, b->d_colors, b->d_blocks,
-                          b->d_rounds, b->d_queue, threads, wgs, split_lo);
+    u32* direct_sizes = sizes_out
+        ? static_cast<u32*>(host_device_ptr(sizes_out, sizeof(u32) * b->total_nodes))
+        : nullptr;
+    if (sizes_out && !direct_sizes && !b->d_sizes)
+        HIP_TRY(hipMalloc(&b->d_sizes, sizeof(u32) * std::max<u64>(b->total_nodes, 1)));
+    u32* d_sizes = sizes_out ? (direct_sizes ? direct_sizes : b->d_sizes) : nullptr;
+    HIP_TRY(hipEventRecord(ctx->ev0, s));
+    const int split_lo = (opts && (opts->flags & RBSC_FLAG_SPLIT16)) ? 48 : 32;
+    int rc = batch_launch(ctx, b->dev, max_rounds, direct ? direct : b->d_labels, b->d_colors,
+                          d_sizes, b->d_blocks, b->d_rounds, b->d_queue, threads, wgs, split_lo);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(ctx->ev1, s));
     const u32 G = b->n_graphs;
@@ -643,6 +701,9 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
                                hipMemcpyDeviceToHost, s));
     if (colors_out)
         HIP_TRY(hipMemcpyAsync(colors_out, b->d_colors, sizeof(u32) * b->total_nodes,
+                               hipMemcpyDeviceToHost, s));
+    if (sizes_out && !direct_sizes)
+        HIP_TRY(hipMemcpyAsync(sizes_out, b->d_sizes, sizeof(u32) * b->total_nodes,
                                hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0;
@@ -661,6 +722,10 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
         dev_ms += o.device_ms;
         blocks[g] = o.K;
         rounds[g] = o.rounds;
+        if (sizes_out) {  // before the partition, which reuses the colours' arrays
+            rc = device_block_sizes(ctx, dg, o.K, sizes_out + b->node_off[g]);
+            if (rc) return rc;
+        }
         if (labels_out || colors_out) {
             rc = device_partition(ctx, dg, labels_out ? labels_out + b->node_off[g] : nullptr,
                                   colors_out ? colors_out + b->node_off[g] : nullptr);
@@ -695,7 +760,20 @@ extern "C" int rbsc_batch_refine(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts*
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    return batch_run(ctx, b, opts, blocks_out, rounds_out, labels_out, colors_out, total,
+    return batch_run(ctx, b, opts, blocks_out, rounds_out, labels_out, colors_out, nullptr, total,
+                     b->threads, b->wgs);
+}
+
+extern "C" int rbsc_batch_refine_ex(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts,
+                                    const rbsc_batch_outputs* out, rbsc_result* total) {
+    if (!ctx || !b) {
+        rbsc_set_error("rbsc_batch_refine_ex: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const rbsc_batch_outputs none{};
+    const rbsc_batch_outputs& o = out ? *out : none;
+    return batch_run(ctx, b, opts, o.blocks, o.rounds, o.labels, o.colors, o.block_sizes, total,
                      b->threads, b->wgs);
 }
 
@@ -716,7 +794,7 @@ extern "C" int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* thre
             double ms_sum = 0;
             for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
                 HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-                int rc = batch_launch(ctx, b->dev, 0, b->d_labels, nullptr, b->d_blocks,
+                int rc = batch_launch(ctx, b->dev, 0, b->d_labels, nullptr, nullptr, b->d_blocks,
                                       b->d_rounds, b->d_queue, threads, wgs);
                 if (rc) return rc;
                 HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));

@@ -269,6 +269,10 @@ int download_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* host);
 // host memory, computed on the device after a refinement (reuses workspace
 // arrays: call it after anything else that reads the refinement's state).
 int device_partition(rbsc_ctx* ctx, rbsc_dgraph* g, u32* partition_out, u32* colors_out);
+// Block sizes of g's final colouring (K colours) in descending order, then
+// zeros: N entries into host memory sizes_out.  Uses workspace arrays that
+// hold no colours, so device_partition may follow it (not precede it).
+int device_block_sizes(rbsc_ctx* ctx, rbsc_dgraph* g, u32 K, u32* sizes_out);
 // The same colours by internal id into a device buffer of N words.
 void device_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* d_int);
 // Exact stability check (verify.hip) of internal colours d_col (< K);

@@ -32,6 +32,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 
 // ---------------------------------------------------------------- errors
 static thread_local char g_err[512] = "";
@@ -1080,6 +1081,36 @@ __global__ void canon_label_kernel(const u32* __restrict__ col, const u32* __res
     out[u] = c <= N ? (u32)rank[first[c]] : 0u;
 }
 
+
+// Block sizes of a colouring with K dense colours over N nodes, in
+// descending order, then zeros (N entries): size[c] nodes per colour, then
+// bys[i] = blocks of size N - i, exclusively scanned, so entry j < K of the
+// descending order is N - i for the last i whose scan is <= j.
+__global__ void colour_size_kernel(const u32* __restrict__ col, u64 N, u32 K,
+                                   u32* __restrict__ size) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < N && col[u] < K) atomicAdd(&size[col[u]], 1u);
+}
+__global__ void size_count_kernel(const u32* __restrict__ size, u32 K, u64 N,
+                                  u64* __restrict__ bys) {
+    const u64 c = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < K && size[c] >= 1 && size[c] <= N) atomicAdd((unsigned long long*)&bys[N - size[c]], 1ull);
+}
+__global__ void sizes_desc_kernel(const u64* __restrict__ bys, u64 N, u32 K, u32* __restrict__ out) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    u32 v = 0;
+    if (j < K) {
+        u64 lo = 0, hi = N - 1;
+        while (lo < hi) {
+            const u64 mid = (lo + hi + 1) >> 1;
+            if (bys[mid] <= j) lo = mid; else hi = mid - 1;
+        }
+        v = (u32)(N - lo);
+    }
+    out[j] = v;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -1916,6 +1947,35 @@ int device_partition(rbsc_ctx* ctx, rbsc_dgraph* g, u32* partition_out, u32* col
     return bad ? on_host() : RBSC_OK;
 }
 
+int device_block_sizes(rbsc_ctx* ctx, rbsc_dgraph* g, u32 K, u32* sizes_out) {
+    const u64 N = g->N;
+    if (N == 0 || !sizes_out) return RBSC_OK;
+    Workspace& w = ctx->ws;
+    hipStream_t s = ctx->stream;
+    if (w.cap_n < N || scan_tmp_words(N) > w.cap_n || K > N) {
+        rbsc_set_error("block sizes: workspace smaller than the graph");
+        return RBSC_ERR_INVALID;
+    }
+    // Scratch only in arrays that hold no colours (device_colors reads hc
+    // or P), so the canonical partition can follow.
+    u32* d_int = reinterpret_cast<u32*>(w.keys[0]);
+    u32* size = reinterpret_cast<u32*>(w.keys[1]);   // K <= N entries
+    u32* d_out = size + N;                            // N entries (keys[1] holds 2 cap)
+    u64* bys = w.sig;                                 // N entries
+    device_colors(ctx, g, d_int);
+    fill_u32_kernel<<<grid_for(K), BLK, 0, s>>>(size, K, 0);
+    fill_u64<<<grid_for(N), BLK, 0, s>>>(bys, N, 0);
+    colour_size_kernel<<<grid_for(N), BLK, 0, s>>>(d_int, N, K, size);
+    size_count_kernel<<<grid_for(K), BLK, 0, s>>>(size, K, N, bys);
+    int rc = scan_excl_u64(bys, bys, N, nullptr, w.hc2, s);
+    if (rc) return rc;
+    sizes_desc_kernel<<<grid_for(N), BLK, 0, s>>>(bys, N, K, d_out);
+    HIP_TRY(hipMemcpyAsync(sizes_out, d_out, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return RBSC_OK;
+}
+
 static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
                            rbsc_result* res, uint32_t* partition_out, uint32_t* colors_out) {
     const u64 N = g->N;
@@ -2252,17 +2312,32 @@ extern "C" int rbsc_graph_download(rbsc_ctx* ctx, const rbsc_dgraph* g, uint64_t
     }
     for (u64 v = 0; v < N; ++v) inv[pi[v]] = (u32)v;
     // Rows come back in original node order (the input is assumed sorted by
-    // source, as the generators produce it).
-    u64 o = 0;
-    if (row_offsets) row_offsets[0] = 0;
-    for (u64 v = 0; v < N; ++v) {
+    // source, as the generators produce it): offsets serially, then the
+    // lines by node ranges on several threads.
+    auto stored = [&](u64 v) -> u64 {
         u64 j = pi[v];
-        if (j < U) {
-            if (j >= R) j = dup[j - R];  // a repeated row: the stored row's lines
-            for (u64 p = rp[j]; p < rp[j + 1]; ++p, ++o)
-                if (dst) dst[o] = inv[d[p]];
-        }
-        if (row_offsets) row_offsets[v + 1] = o;
+        return j < U && j >= R ? dup[j - R] : j;  // a repeated row: the stored row's lines
+    };
+    std::vector<u64> ro(N + 1, 0);
+    for (u64 v = 0; v < N; ++v) {
+        const u64 j = stored(v);
+        ro[v + 1] = ro[v] + (pi[v] < U ? rp[j + 1] - rp[j] : 0);
+    }
+    if (row_offsets) memcpy(row_offsets, ro.data(), sizeof(u64) * (N + 1));
+    if (dst && N) {
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                const u64 v0 = N * t / nt, v1 = N * (t + 1) / nt;
+                for (u64 v = v0; v < v1; ++v) {
+                    if (pi[v] >= U) continue;
+                    const u64 j = stored(v);
+                    u64 o = ro[v];
+                    for (u64 p = rp[j]; p < rp[j + 1]; ++p, ++o) dst[o] = inv[d[p]];
+                }
+            });
+        for (auto& x : th) x.join();
     }
     return RBSC_OK;
 }

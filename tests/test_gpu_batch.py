@@ -40,6 +40,12 @@ def _check_against_oracle(b: HostBatch, res, max_rounds=0, rounds_exact=True):
             assert np.array_equal(res.colors[a:z], c), f"graph {g} (n={hg.n_nodes}): colours"
         if res.labels is not None:
             assert np.array_equal(res.labels[a:z], oracle.canonical(c)), f"graph {g}: labels"
+        if res.block_sizes is not None:
+            # final colour histogram: block sizes in descending order, then zeros
+            want = np.zeros(z - a, np.uint32)
+            want[:K] = np.sort(np.bincount(c))[::-1][:K] if K else []
+            assert np.array_equal(res.block_sizes[a:z], want), f"graph {g}: block sizes"
+            assert np.array_equal(res.histogram(g), want[:K])
 
 
 def _mixed_graphs(seed):
@@ -64,7 +70,7 @@ def _mixed_graphs(seed):
 def test_batch_mixed_matches_oracle(ctx):
     b = gen.batch_mixed(400, 10, 5000, seed=21)
     db = ctx.upload_batch(b)
-    res = db.refine(labels=True, colors=True)
+    res = db.refine(labels=True, colors=True, histogram=True)
     _check_against_oracle(b, res)
     assert res.total.nodes == int(res.blocks.sum())
     assert res.total.rounds == int(res.rounds.max())
@@ -99,6 +105,17 @@ def test_batch_labels_into_pinned_memory(ctx):
     db.free()
 
 
+def test_pinned_labels_valid_after_batch_is_freed(ctx):
+    # the batch is a temporary: its pinned labels must outlive it
+    b = gen.batch_mixed(120, 10, 5000, seed=24)
+    want = ctx.upload_batch(b).refine(labels=True).labels.copy()
+    res = ctx.upload_batch(b).refine(labels=True, pinned=True)
+    import gc
+    gc.collect()
+    assert int(res.labels.sum()) == int(want.sum())
+    assert np.array_equal(res.labels, want)
+
+
 def test_batch_directed_sizes_and_edges(ctx):
     gs = _mixed_graphs(3)
     gs += [HostGraph(0, np.zeros(0, np.uint32), src=np.zeros(0, np.uint32)),
@@ -112,7 +129,7 @@ def test_batch_directed_sizes_and_edges(ctx):
            gen.rmat(14, 1 << 17, clone_frac=0.2, seed=79)]
     b = HostBatch.from_graphs(gs)
     db = ctx.upload_batch(b)
-    res = db.refine(labels=True, colors=True)
+    res = db.refine(labels=True, colors=True, histogram=True)
     _check_against_oracle(b, res)
     db.free()
 
@@ -137,7 +154,7 @@ def test_batch_size_tiers(ctx):
         g.init_color = rng.integers(0, 3, g.n_nodes).astype(np.uint32)
     b = HostBatch.from_graphs(gs)
     db = ctx.upload_batch(b)
-    res = db.refine(labels=True, colors=True)
+    res = db.refine(labels=True, colors=True, histogram=True)
     _check_against_oracle(b, res)
     db.free()
 
@@ -153,7 +170,7 @@ def test_batch_weighted_and_initial_colours(ctx):
     gs.append(big)
     b = HostBatch.from_graphs(gs)
     db = ctx.upload_batch(b)
-    res = db.refine(labels=True, colors=True)
+    res = db.refine(labels=True, colors=True, histogram=True)
     _check_against_oracle(b, res)
     db.free()
 

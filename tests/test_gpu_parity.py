@@ -209,6 +209,40 @@ def test_cli_end_to_end(tmp_path):
     assert np.array_equal(got, oracle.canonical(c))
 
 
+@pytest.mark.parametrize("name", ["ba_n5000_m3_dup10_init3", "ba_8m_m6_init3"])
+def test_cli_init_colors(tmp_path, name):
+    """Initial colours through the drop-in command (``--init-colors``, the
+    counterpart of Valmari's initial blocks, MDP_read.cc:140-150): the BA
+    multigraphs with three initial labels give the block counts Valmari
+    printed for them (tests/golden/fixtures.json) and the oracle's
+    partition."""
+    g = gen.ba(5000, 3, 0.10, 3, seed=13) if name == "ba_n5000_m3_dup10_init3" \
+        else gen.ba_8m_m6_init3()
+    with open(os.path.join(HERE, "golden", "fixtures.json")) as f:
+        want_nodes = {r["name"]: r for r in json.load(f)["fixtures"]}[name]["valmari_nodes"]
+    text, init, part = tmp_path / "g.txt", tmp_path / "init.txt", tmp_path / "part.txt"
+    g.write_text(str(text), "lines")
+    init.write_text("\n".join(map(str, g.init_color.tolist())) + "\n")
+    with open(text, "rb") as f:
+        p = subprocess.run([CLI_PATH, str(g.n_edges), "1", "--init-colors", str(init),
+                            "--partition-out", str(part), "--verify"], stdin=f,
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=240)
+    assert p.returncode == 0, p.stderr.decode()
+    lines = p.stdout.decode().split("\n")
+    float(lines[0])
+    assert int(lines[1]) == 1 and int(lines[2]) == want_nodes
+    assert "verified=1" in p.stderr.decode()
+    c, K, _, _ = _oracle_full(g)
+    assert K == want_nodes
+    got = np.fromfile(part, dtype=np.uint32, sep="\n")
+    assert np.array_equal(got, oracle.canonical(c))
+    # without the initial colours the count differs: the file is read
+    with open(text, "rb") as f:
+        q = subprocess.run([CLI_PATH, str(g.n_edges), "1"], stdin=f, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, timeout=240)
+    assert q.returncode == 0 and int(q.stdout.decode().split("\n")[2]) != want_nodes
+
+
 @pytest.mark.parametrize("frac", [0.5, 0.25, 0.1])
 def test_streaming_equals_resident(ctx, frac):
     """Edge-batch streaming (lines in pinned host memory, slices of

@@ -161,3 +161,40 @@ def test_write_text_matches_python_formats(tmp_path):
     p = tmp_path / "h.lines"
     h.write_text(str(p))
     assert p.read_bytes() == h.to_text()
+
+
+def test_pinned_labels_outlive_their_batch(monkeypatch):
+    """Labels handed out over page-locked memory stay readable after the
+    batch (the memory's other holder) is freed: the memory is released with
+    the last array over it, not before.  The allocator is replaced by libc
+    malloc/free so this runs without a device."""
+    import ctypes as C
+    libc = C.CDLL(None)
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    libc.free.argtypes = [C.c_void_p]
+    freed = []
+
+    class FakeLib:
+        def rbsc_host_alloc(self, nbytes, ref):
+            ref._obj.value = libc.malloc(int(nbytes))
+            return 0
+
+        def rbsc_host_free(self, p):
+            freed.append(p.value)
+            libc.free(p)
+            return 0
+
+    fake = FakeLib()
+    monkeypatch.setattr(rbsc_amd, "load", lambda: fake)
+    pa = rbsc_amd.PinnedArray(1000)
+    pa.array[:] = np.arange(1000, dtype=np.uint32)
+    view = pa.array[10:20]
+    pa.free()                   # what DeviceBatch.free() does
+    del pa
+    assert freed == []          # the view still holds the memory
+    assert np.array_equal(view, np.arange(10, 20, dtype=np.uint32))
+    del view
+    import gc
+    gc.collect()
+    assert len(freed) == 1      # released with the last array
