@@ -314,6 +314,24 @@ typedef struct rbsc_batch_outputs {
  * output is the block count, rbsc.cu:492-494). */
 int rbsc_batch_refine_ex(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts,
                          const rbsc_batch_outputs* out, rbsc_result* total);
+/* Batched refinement of a batch already in device memory, allocating
+ * nothing.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The fields of `batch` after
+ * n_graphs are device pointers (weight and init_color nullable), with each
+ * graph's lines sorted by source; n_nodes and n_lines are the totals
+ * node_offsets[n_graphs] and line_offsets[n_graphs].  Outputs are device
+ * arrays: blocks and rounds (n_graphs entries, required), labels and colors
+ * (n_nodes entries, nullable) as for rbsc_batch_refine; each graph gets
+ * exactly those results.  The launch layout is built in a fixed array of
+ * the library on each device (about 17 MB), so a call holds at most 32768
+ * graphs, 2^21 nodes and 2^22 lines, each graph at most 5120 nodes; larger
+ * batches are rejected (RBSC_ERR_INVALID) before any device work.  Calls on
+ * one device share that array: calls that overlap on one device race.
+ * Synchronous on the context's stream. */
+int rbsc_batch_refine_device(rbsc_ctx* ctx, const rbsc_batch* batch, uint64_t n_nodes,
+                             uint64_t n_lines, const rbsc_opts* opts, uint32_t* blocks,
+                             uint32_t* rounds, uint32_t* labels, uint32_t* colors);
 /* Times the persistent launch at workgroup widths 256/512/1024 and 1/2/4
  * workgroups per CU on this batch and keeps the fastest for later refines. */
 int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* threads_out, uint32_t* wgs_out,

@@ -194,6 +194,9 @@ SIGNATURES = {
                                     C.POINTER(Result)]),
     "rbsc_batch_refine_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts),
                                        C.POINTER(BatchOutputs), C.POINTER(Result)]),
+    "rbsc_batch_refine_device": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.c_uint64, C.c_uint64,
+                                           C.POINTER(Opts), C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
     "rbsc_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, u32p, u32p, C.POINTER(C.c_double)]),
     "rbsc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "rbsc_refine_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Opts), u32p, u32p, u32p,

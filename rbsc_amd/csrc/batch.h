@@ -42,3 +42,6 @@ struct rbsc_ctx;
 int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
                  u32* d_sizes, u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg,
                  int split_lo = 32);
+// Looks up, once per device (that device current), the device address of
+// the fixed scratch array of rbsc_batch_refine_device.
+int batch_device_scratch_init(int device);

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -159,7 +160,7 @@ __device__ void lds_sort(Lds<BT, NM>* s, int n, int bit_begin, int bit_end) {
 template <int BT, int NM, bool HAS_WT, bool HAS_INIT>
 __global__ __launch_bounds__(BT) void batch_refine_kernel(
     const BatchDesc* __restrict__ desc, const u32* __restrict__ order, u32 n_jobs,
-    const u32* __restrict__ rp_all, const unsigned short* __restrict__ dst_all,
+    const u32* __restrict__ range, const u32* __restrict__ skip, const u32* __restrict__ rp_all, const unsigned short* __restrict__ dst_all,
     const u64* __restrict__ wt_all, const u32* __restrict__ init_all, u32 max_rounds,
     u32* __restrict__ labels_out, u32* __restrict__ colors_out, u32* __restrict__ sizes_out,
     u32* __restrict__ blocks_out, u32* __restrict__ rounds_out, u32* __restrict__ queue,
@@ -171,6 +172,17 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
     // after the fetch at the head of an inner loop that only thread 0 leaves,
     // and the rest of wave 0 then waits there for ever.  Graphs without nodes
     // are therefore answered on the host and never queued.
+    // A layout built on the device gives the tier's jobs as a device range
+    // [range[0], range[1]) of the order, and a nonzero *skip (a layout
+    // error found by the build) cancels the launch.
+    // (Read through readfirstlane: the values are the same for every thread,
+    // and the loop's exit below must stay a uniform branch.)
+    if (skip && __builtin_amdgcn_readfirstlane(*skip)) return;
+    if (range) {
+        const u32 j0 = __builtin_amdgcn_readfirstlane(range[0]);
+        n_jobs = __builtin_amdgcn_readfirstlane(range[1]) - j0;
+        order += j0;
+    }
     for (;;) {
         if (threadIdx.x == 0) s.scal[0] = atomicAdd(queue, 1u);
         __syncthreads();
@@ -422,9 +434,11 @@ __global__ void zero_queue(u32* q) {
 template <int BT, int NM>
 static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 j0, u32 j1,
                       u32 max_rounds, u32* d_labels, u32* d_colors, u32* d_sizes, u32* d_blocks,
-                      u32* d_rounds, u32* q, int split_lo) {
+                      u32* d_rounds, u32* q, int split_lo, const u32* range = nullptr,
+                      const u32* skip = nullptr) {
 #define BL(W, I)                                                                              \
-    batch_refine_kernel<BT, NM, W, I><<<grid, BT, 0, s>>>(b.desc, b.order + j0, j1 - j0, b.rp, \
+    batch_refine_kernel<BT, NM, W, I><<<grid, BT, 0, s>>>(b.desc, b.order + j0, j1 - j0, range, \
+                                                          skip, b.rp,                           \
                                                           b.dst, b.wt, b.init, max_rounds,      \
                                                           d_labels, d_colors, d_sizes,          \
                                                           d_blocks, d_rounds, q, split_lo)
@@ -841,4 +855,229 @@ extern "C" int rbsc_refine_batch(rbsc_ctx* ctx, const rbsc_batch* batch, const r
     if (!rc && total)
         total->time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+// ------------------------------------------- batches in device memory, no allocation
+// rbsc_batch_refine_device refines a batch whose arrays the caller already
+// holds in device memory and allocates nothing: the launch layout (graph
+// descriptors, the tier order, local row offsets, 16-bit heads) and the
+// queue counters are built in `scratch`, a fixed array of this library on
+// every device it runs on.  Calls on one device share that array, so calls
+// that overlap on one device race (one at a time per device).
+namespace {
+constexpr u32 ND_MAX_GRAPHS = 1u << 15;
+constexpr u64 ND_MAX_NODES = 1ull << 21;
+constexpr u64 ND_MAX_LINES = 1ull << 22;
+// Byte offsets inside the scratch array.
+constexpr u64 ND_CTL = 0;                                      // 64 u32 of counters
+constexpr u64 ND_DESC = 256;
+constexpr u64 ND_ORDER = ND_DESC + sizeof(BatchDesc) * ND_MAX_GRAPHS;
+constexpr u64 ND_RP = ND_ORDER + 4ull * ND_MAX_GRAPHS;
+constexpr u64 ND_DST = ND_RP + 4ull * (ND_MAX_NODES + ND_MAX_GRAPHS);
+constexpr u64 ND_BYTES = ND_DST + 2ull * ND_MAX_LINES + 256;
+// Control words: [0..3] tier queues, [4..7] graphs per tier, [8..15] tier
+// ranges (begin, end), [16..19] per-tier fill counters, [20] error flags.
+constexpr int NDC_QUEUE = 0, NDC_COUNT = 4, NDC_RANGE = 8, NDC_FILL = 16, NDC_ERR = 20;
+constexpr u32 NDE_OFFSETS = 1u, NDE_TOO_BIG = 2u, NDE_ID = 4u, NDE_ORDER = 8u;
+
+static __device__ __align__(256) u64 scratch[ND_BYTES / 8];
+
+std::mutex g_nd_mu;
+void* g_nd_ptr[64] = {};  // scratch's device address, per device
+
+__device__ __forceinline__ u32 nd_tier(u32 n) {
+    return n > BATCH_NMID ? 0u : n > BATCH_NLOW ? 1u : n > BATCH_NTINY ? 2u : 3u;
+}
+
+__global__ void nd_clear_kernel(u32* ctl) {
+    if (threadIdx.x < 64) ctl[threadIdx.x] = 0;
+}
+
+// One thread per graph: descriptor, tier count; graphs without nodes are
+// answered here (0 blocks, 0 rounds) and never queued.
+__global__ void nd_desc_kernel(const u64* __restrict__ node_off, const u64* __restrict__ line_off,
+                               u32 G, u64 n_nodes, u64 n_lines, BatchDesc* __restrict__ desc,
+                               u32* __restrict__ ctl, u32* __restrict__ blocks,
+                               u32* __restrict__ rounds) {
+    const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const u64 a = node_off[g], z = node_off[g + 1], la = line_off[g], lz = line_off[g + 1];
+    u32 err = 0;
+    if (z < a || lz < la || z > n_nodes || lz > n_lines) err |= NDE_OFFSETS;
+    if (g == 0 && (a != 0 || la != 0)) err |= NDE_OFFSETS;
+    if (g == G - 1 && (z != n_nodes || lz != n_lines)) err |= NDE_OFFSETS;
+    const u64 n = err ? 0 : z - a;
+    if (n > BATCH_NMAX) err |= NDE_TOO_BIG;
+    BatchDesc d;
+    d.node_base = a;
+    d.line_base = la;
+    d.rp_base = a + g;
+    d.n = err ? 0u : (u32)n;
+    d.pad = 0;
+    desc[g] = d;
+    if (err) atomicOr(&ctl[NDC_ERR], err);
+    if (d.n == 0) {
+        blocks[g] = 0;
+        rounds[g] = 0;
+    } else {
+        atomicAdd(&ctl[NDC_COUNT + nd_tier(d.n)], 1u);
+    }
+}
+
+__global__ void nd_ranges_kernel(u32* ctl) {
+    if (threadIdx.x != 0) return;
+    u32 b = 0;
+    for (int t = 0; t < 4; ++t) {
+        ctl[NDC_RANGE + 2 * t] = b;
+        b += ctl[NDC_COUNT + t];
+        ctl[NDC_RANGE + 2 * t + 1] = b;
+    }
+}
+
+// Each graph with nodes takes a place in its tier's range (order inside a
+// tier does not change any graph's result).
+__global__ void nd_order_kernel(const BatchDesc* __restrict__ desc, u32 G, u32* __restrict__ ctl,
+                                u32* __restrict__ order) {
+    const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const u32 n = desc[g].n;
+    if (n == 0) return;
+    const u32 t = nd_tier(n);
+    order[ctl[NDC_RANGE + 2 * t] + atomicAdd(&ctl[NDC_FILL + t], 1u)] = g;
+}
+
+// One workgroup per graph: local row offsets from lines sorted by source
+// (rp[u] = first local line of u, rp[n] = line count), heads as 16-bit
+// local ids, and the ids and order validated.  Writes stay inside the
+// graph's own rp range and line range whatever the input holds.
+__global__ __launch_bounds__(256) void nd_rows_kernel(const BatchDesc* __restrict__ desc,
+                                                      const u64* __restrict__ line_off,
+                                                      const u32* __restrict__ src,
+                                                      const u32* __restrict__ dst,
+                                                      u32* __restrict__ rp_all,
+                                                      unsigned short* __restrict__ dst16,
+                                                      u32* __restrict__ ctl) {
+    const u32 g = blockIdx.x;
+    const BatchDesc d = desc[g];
+    const u32 n = d.n;
+    if (n == 0) return;
+    const u64 la = d.line_base, L = line_off[g + 1] - la;
+    u32* rp = rp_all + d.rp_base;
+    u32 err = 0;
+    for (u64 i = threadIdx.x; i < L; i += blockDim.x) {
+        const u64 e = la + i;
+        const u32 sv = src[e], dv = dst[e];
+        const long long ps = i ? (long long)src[e - 1] : -1;
+        if (sv >= n || dv >= n) err |= NDE_ID;
+        if ((long long)sv < ps) err |= NDE_ORDER;
+        dst16[e] = (unsigned short)(dv < n ? dv : 0u);
+        const long long s_hi = sv < n ? (long long)sv : (long long)n - 1;
+        if (i == 0 || (long long)sv != ps)
+            for (long long u = ps + 1; u <= s_hi; ++u) rp[u] = (u32)i;
+        if (i == L - 1)
+            for (long long u = s_hi + 1; u <= (long long)n; ++u) rp[u] = (u32)L;
+    }
+    if (L == 0)
+        for (u32 u = threadIdx.x; u <= n; u += blockDim.x) rp[u] = 0;
+    if (err) atomicOr(&ctl[NDC_ERR], err);
+}
+}  // namespace
+
+int batch_device_scratch_init(int device) {
+    if (device < 0 || device >= 64) {
+        rbsc_set_error("device %d beyond the library's device table", device);
+        return RBSC_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(g_nd_mu);
+    if (g_nd_ptr[device]) return RBSC_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        rbsc_set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                       hipGetErrorString(e));
+        return RBSC_ERR_HIP;
+    }
+    g_nd_ptr[device] = p;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_batch_refine_device(rbsc_ctx* ctx, const rbsc_batch* batch, uint64_t n_nodes,
+                                        uint64_t n_lines, const rbsc_opts* opts,
+                                        uint32_t* blocks, uint32_t* rounds, uint32_t* labels,
+                                        uint32_t* colors) {
+    if (!ctx || !batch || !blocks || !rounds || !batch->node_offsets || !batch->line_offsets ||
+        (n_lines && (!batch->src || !batch->dst))) {
+        rbsc_set_error("rbsc_batch_refine_device: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    const u32 G = batch->n_graphs;
+    if (G == 0) return RBSC_OK;
+    if (G > ND_MAX_GRAPHS || n_nodes > ND_MAX_NODES || n_lines > ND_MAX_LINES) {
+        rbsc_set_error("rbsc_batch_refine_device: batch of %u graphs, %llu nodes, %llu lines "
+                       "exceeds the fixed scratch (at most %u graphs, %llu nodes, %llu lines)",
+                       G, (unsigned long long)n_nodes, (unsigned long long)n_lines, ND_MAX_GRAPHS,
+                       (unsigned long long)ND_MAX_NODES, (unsigned long long)ND_MAX_LINES);
+        return RBSC_ERR_INVALID;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_nd_mu);
+        if (ctx->device >= 0 && ctx->device < 64) base = g_nd_ptr[ctx->device];
+    }
+    if (!base) {
+        const int rc = batch_device_scratch_init(ctx->device);
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lk(g_nd_mu);
+        base = g_nd_ptr[ctx->device];
+    }
+    unsigned char* sp = static_cast<unsigned char*>(base);
+    u32* ctl = reinterpret_cast<u32*>(sp + ND_CTL);
+    BatchDevice b;
+    b.desc = reinterpret_cast<BatchDesc*>(sp + ND_DESC);
+    b.order = reinterpret_cast<u32*>(sp + ND_ORDER);
+    b.rp = reinterpret_cast<u32*>(sp + ND_RP);
+    b.dst = reinterpret_cast<unsigned short*>(sp + ND_DST);
+    b.wt = const_cast<u64*>(batch->weight);
+    b.init = const_cast<u32*>(batch->init_color);
+    b.n_small = G;
+    (void)hipDeviceGetAttribute(&b.cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    hipStream_t s = ctx->stream;
+    const u32 max_rounds = opts ? opts->max_rounds : 0;
+    const int split_lo = (opts && (opts->flags & RBSC_FLAG_SPLIT16)) ? 48 : 32;
+    const unsigned gg = (G + 255) / 256;
+    nd_clear_kernel<<<1, 64, 0, s>>>(ctl);
+    nd_desc_kernel<<<gg, 256, 0, s>>>(batch->node_offsets, batch->line_offsets, G, n_nodes, n_lines,
+                                      b.desc, ctl, blocks, rounds);
+    nd_ranges_kernel<<<1, 64, 0, s>>>(ctl);
+    nd_order_kernel<<<gg, 256, 0, s>>>(b.desc, G, ctl, b.order);
+    nd_rows_kernel<<<G, 256, 0, s>>>(b.desc, batch->line_offsets, batch->src, batch->dst, b.rp,
+                                     b.dst, ctl);
+    // every tier launched at its full grid; its jobs come from the device
+    // ranges, so nothing is read back before the refinement
+    const u32* err = ctl + NDC_ERR;
+    u32* q = ctl + NDC_QUEUE;
+    const u32* rg = ctl + NDC_RANGE;
+    const int cus = b.cus;
+    launch_bt<512, BATCH_NMAX>(s, cus, b, 0, 0, max_rounds, labels, colors, nullptr, blocks,
+                               rounds, q, split_lo, rg, err);
+    launch_bt<MID_THREADS, BATCH_NMID>(s, 2 * cus, b, 0, 0, max_rounds, labels, colors, nullptr,
+                                       blocks, rounds, q + 1, split_lo, rg + 2, err);
+    launch_bt<LOW_THREADS, BATCH_NLOW>(s, 4 * cus, b, 0, 0, max_rounds, labels, colors, nullptr,
+                                       blocks, rounds, q + 2, split_lo, rg + 4, err);
+    launch_bt<TINY_THREADS, BATCH_NTINY>(s, 8 * cus, b, 0, 0, max_rounds, labels, colors, nullptr,
+                                         blocks, rounds, q + 3, split_lo, rg + 6, err);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 14, err, sizeof(u32), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const u32 e = ctx->h_scalar[14];
+    if (e) {
+        rbsc_set_error("rbsc_batch_refine_device: %s",
+                       (e & NDE_OFFSETS) ? "offsets not monotone or not ending at the totals"
+                       : (e & NDE_TOO_BIG) ? "a graph has more than 5120 nodes"
+                       : (e & NDE_ID) ? "a node id is out of its graph's range"
+                       : "lines of a graph are not sorted by source");
+        return (e & (NDE_OFFSETS | NDE_TOO_BIG)) ? RBSC_ERR_INVALID : RBSC_ERR_INPUT;
+    }
+    return RBSC_OK;
 }

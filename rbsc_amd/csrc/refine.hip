@@ -24,6 +24,7 @@
 // pass because randomize is a bijection; it is not repeated here.
 #include "internal.h"
 #include "scan.h"
+#include "batch.h"
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
@@ -2033,6 +2034,10 @@ extern "C" int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out) 
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(device));
+    {
+        const int rc = batch_device_scratch_init(device);
+        if (rc) return rc;
+    }
     rbsc_ctx* c = new rbsc_ctx();
     c->device = device;
     c->budget = hbm_budget;
