@@ -31,29 +31,39 @@ def build():
     subprocess.run(["make", "-C", HERE, "all"], check=True, capture_output=True)
 
 
+def load_lib(path: str) -> C.CDLL:
+    """The oracle's functions from the shared library at `path` (the built
+    oracle, or a copy a test compiled)."""
+    L = C.CDLL(path)
+    _declare(L)
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.oracle_randomize.restype = C.c_uint64
-        L.oracle_randomize.argtypes = [C.c_uint64]
-        L.oracle_rbsc_refine.restype = C.c_int
-        L.oracle_rbsc_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, u32p, C.c_uint32,
-                                         u32p, u32p, u32p, u32p, C.c_uint32]
-        L.oracle_canonicalize.restype = C.c_uint32
-        L.oracle_canonicalize.argtypes = [C.c_uint32, u32p, u32p]
-        L.oracle_rsc_refine.restype = C.c_int
-        L.oracle_rsc_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u32p, u32p, u32p, u32p, u32p]
-        L.oracle_exact_refine.restype = C.c_int
-        L.oracle_exact_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, u32p, u32p,
-                                          u32p, u32p]
-        L.oracle_rbsc_multibatch.restype = C.c_int
-        L.oracle_rbsc_multibatch.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, C.c_uint64,
-                                             C.c_int, u32p, u32p, u32p]
-        _lib = L
+        _lib = load_lib(LIB)
     return _lib
+
+
+def _declare(L):
+    L.oracle_randomize.restype = C.c_uint64
+    L.oracle_randomize.argtypes = [C.c_uint64]
+    L.oracle_rbsc_refine.restype = C.c_int
+    L.oracle_rbsc_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, u32p, C.c_uint32,
+                                     u32p, u32p, u32p, u32p, C.c_uint32]
+    L.oracle_canonicalize.restype = C.c_uint32
+    L.oracle_canonicalize.argtypes = [C.c_uint32, u32p, u32p]
+    L.oracle_rsc_refine.restype = C.c_int
+    L.oracle_rsc_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u32p, u32p, u32p, u32p, u32p]
+    L.oracle_exact_refine.restype = C.c_int
+    L.oracle_exact_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, u32p, u32p,
+                                      u32p, u32p]
+    L.oracle_rbsc_multibatch.restype = C.c_int
+    L.oracle_rbsc_multibatch.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, C.c_uint64,
+                                         C.c_int, u32p, u32p, u32p]
 
 
 def _p(a, t):
@@ -79,19 +89,21 @@ def canonical(colors: np.ndarray) -> np.ndarray:
     return out
 
 
-def rbsc_refine(n, src, dst, weight=None, init=None, max_rounds=0, trace_rounds=0):
+def rbsc_refine(n, src, dst, weight=None, init=None, max_rounds=0, trace_rounds=0, _from=None):
     """Restatement of rbsc.cu single-batch refinement.  Returns
-    (colors, K, rounds, trace) where trace[r] = colours after round r+1."""
+    (colors, K, rounds, trace) where trace[r] = colours after round r+1.
+    _from: another build of the oracle (load_lib), for tests of the checks."""
     src, dst, w = _arrs(src, dst, weight)
     init = None if init is None else np.ascontiguousarray(init, np.uint32)
     colors = np.empty(max(n, 1), np.uint32)
     K = C.c_uint32()
     R = C.c_uint32()
     trace = np.zeros((trace_rounds, max(n, 1)), np.uint32) if trace_rounds else None
-    rc = lib().oracle_rbsc_refine(n, len(dst), _p(src, C.c_uint32), _p(w, C.c_uint64),
-                                  _p(dst, C.c_uint32), _p(init, C.c_uint32), max_rounds,
-                                  _p(colors, C.c_uint32), C.byref(K), C.byref(R),
-                                  _p(trace, C.c_uint32), trace_rounds)
+    L = _from or lib()
+    rc = L.oracle_rbsc_refine(n, len(dst), _p(src, C.c_uint32), _p(w, C.c_uint64),
+                              _p(dst, C.c_uint32), _p(init, C.c_uint32), max_rounds,
+                              _p(colors, C.c_uint32), C.byref(K), C.byref(R),
+                              _p(trace, C.c_uint32), trace_rounds)
     if rc:
         raise MemoryError("oracle_rbsc_refine failed")
     return colors[:n], int(K.value), int(R.value), trace

@@ -16,7 +16,9 @@ parse (time-out, crash) records ``time = TIMEOUT, done = 0, nodes = 0``
 (bench.py:41-54).  Tools run as the same shell pipelines (bench.py:7-12):
 ``{input} | timeout T ./rbsc {edges} {gpu}``, ``./rsc``, ``./part_ref``;
 ``rbsc`` is this build's command (rbsc_amd/bin/rbsc), ``rsc`` and
-``part_ref`` the reference's CPU tools compiled from source.  The ``cuda``
+``part_ref`` the reference's CPU tools, which the caller provides
+(``--rsc``, ``--part-ref``; by default ``./rsc`` and ``./part_ref`` in the
+working directory, as the reference's harness runs them).  The ``cuda``
 columns run at edge batches of 100 %, 50 % and 25 % (bench.py:83-89).
 
 Inputs: ``tool = 'file'`` reads ``datasets/{name}.txt`` as the reference
@@ -55,10 +57,12 @@ BENCH_COLUMNS = ("name TEXT, tool TEXT, nodes INT, edges INT, valmari TEXT, cpu 
 LARGE_COLUMNS = "name TEXT, tool TEXT, nodes INT, edges INT, valmari TEXT, cpu TEXT, cuda TEXT, status INT"
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.dirname(HERE)
 RBSC = os.path.join(HERE, "bin", "rbsc")
-RSC = os.path.join(ROOT, "oracle", "_ref", "rsc")
-PART_REF = os.path.join(ROOT, "oracle", "_ref", "part_ref")
+# The CPU tools are the caller's: like the reference's harness, which runs
+# ./rsc and ./part_ref from its working directory (bench.py:11-12), unless
+# the caller names their paths.
+RSC = "./rsc"
+PART_REF = "./part_ref"
 
 
 def _workloads() -> Dict[str, Callable]:
@@ -252,12 +256,15 @@ def main(argv=None) -> int:
     b.add_argument("--datasets", default="datasets")
     b.add_argument("--timeout", type=int, default=None)
     b.add_argument("--law-dir", default=".", help="directory holding NAME/NAME.graph")
+    b.add_argument("--rsc", default=RSC, help="the RSC CPU tool (default ./rsc)")
+    b.add_argument("--part-ref", default=PART_REF, help="Valmari's tool (default ./part_ref)")
+    b.add_argument("--rbsc", default=RBSC, help="the rbsc command (default: this build's)")
     args = ap.parse_args(argv)
     if args.cmd == "init":
         init_db(args.db, args.large, tuple(args.only) if args.only else WORKLOADS)
         return 0
-    run_db(args.db, args.gpus, args.large, args.batch, args.datasets, timeout=args.timeout,
-           law_dir=args.law_dir)
+    run_db(args.db, args.gpus, args.large, args.batch, args.datasets, rbsc=args.rbsc,
+           rsc=args.rsc, part_ref=args.part_ref, timeout=args.timeout, law_dir=args.law_dir)
     return 0
 
 

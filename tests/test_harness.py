@@ -92,7 +92,8 @@ def test_real_reference_tools_in_the_harness(tmp_path):
     db = str(tmp_path / "b.db")
     harness.init_db(db, True, names=("rmat_s12_d16_c30_seed1011",))
     rbsc = _tool(tmp_path, "rbsc", "cat > /dev/null; printf '0\\n1\\n0\\n'")
-    harness.run_db(db, 1, large=True, rbsc=rbsc, timeout=600)
+    harness.run_db(db, 1, large=True, rbsc=rbsc, rsc=oracle.REF_RSC, part_ref=oracle.REF_VALMARI,
+                   timeout=600)
     con = sqlite3.connect(db)
     r = con.execute("SELECT valmari, cpu, status FROM results").fetchone()
     from rbsc_amd import gen
@@ -137,7 +138,8 @@ def test_law_row_through_the_bvgraph_decoder(tmp_path):
     con.execute("INSERT INTO results VALUES ('cnr-2000', 'law', 1, 1, '', '', '', 0)")
     con.commit()
     rbsc = _tool(tmp_path, "rbsc", "cat > /dev/null; printf '0\\n1\\n0\\n'")
-    assert harness.run_db(db, 1, large=True, rbsc=rbsc, timeout=600, law_dir=str(tmp_path)) == 1
+    assert harness.run_db(db, 1, large=True, rbsc=rbsc, rsc=oracle.REF_RSC,
+                          part_ref=oracle.REF_VALMARI, timeout=600, law_dir=str(tmp_path)) == 1
     r = con.execute("SELECT nodes, edges, valmari, cpu, status FROM results").fetchone()
     assert r[:2] == (325557, 3216152) and r[4] == 1
     assert json.loads(r[2])["nodes"] == 85418 and json.loads(r[3])["nodes"] == 85418

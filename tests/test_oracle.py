@@ -6,7 +6,12 @@
   block counts equal to the reference's own compiled rsc and Valmari tools on
   the committed fixtures (tests/golden/fixtures.json).
 * closed forms: cycle, path, star, regular graphs, clones, disjoint unions.
-* a planted mistake in the oracle is caught by these checks.
+* per-round colours: equal to a second restatement that follows the
+  reference's own steps (rbsc.cu:352-425) with numpy's stable sorts.
+* multi-batch quotient passes: hand-worked cases (a group larger than the
+  batch, a group straddling a batch boundary, the aggressive stop).
+* mistakes planted in compiled copies of the oracle (colour-before-signature
+  rank order, no dangling split) are caught by these checks.
 """
 import json
 import os
@@ -214,23 +219,190 @@ def test_multibatch_hand_worked():
     nodes, done, passes = oracle.rbsc_multibatch(n, s, d, None, 2)
     assert (nodes, done, passes) == (3, 0, 2)
     assert oracle.rbsc_refine(n, s, d)[1] == 3
+    # The aggressive stop (rbsc_aggressive.cu:487) also needs the batch count
+    # to shrink: after pass 1 the quotient has 3 lines, ceil(3 / 2) = 2 is
+    # not below the 2 batches of pass 1, so it stops after one pass.
+    assert oracle.rbsc_multibatch(n, s, d, None, 2, aggressive=True) == (3, 0, 1)
     # with a batch of 4 lines everything is one batch
     assert oracle.rbsc_multibatch(n, s, d, None, 4) == (3, 1, 1)
-    # a group larger than the batch is never processed: it stays a singleton
-    # and the run does not reach one batch (rbsc.cu:315-334, 460)
+
+
+def test_multibatch_group_larger_than_the_batch():
+    # Lines 0->1, 0->1, 0->2, 1->2, batch = 2 lines (rbsc.cu:302-336).
+    # Batch 0 = lines [0, 2): node 0's group starts at 0 but does not end
+    # inside it (line 2 is node 0's too), so no end is found and the batch is
+    # skipped (332).  Batch 1 = lines [2, 4): line 2 continues node 0's
+    # group, so the batch starts at line 3 (node 1's group) and ends at 4;
+    # node 1 is the only node in the batch, nodes 0 and 2 are off-batch
+    # singletons (colours 1 and 3, signature UINT64_MAX), node 1 starts at
+    # N+1 = 4 with signature randomize(3 + 0): the pairs (H, 4), (MAX, 1),
+    # (MAX, 3) are 3 distinct = the 3 of the start, so one round; node 1 gets
+    # label 0 (431-445).  Nodes 0 and 2 were never labelled and get 1 and 2
+    # (457-461): 3 nodes from 3, so the pass loop stops (487) with
+    # batches = 2: (nodes, done, passes) = (3, 0, 1).
     n, s, d = _coo(3, [(0, 1), (0, 2), (0, 1), (1, 2)])
-    nodes, done, passes = oracle.rbsc_multibatch(n, s, d, None, 2)
-    assert done in (0, 1) and 1 <= nodes <= 3
+    assert oracle.rbsc_multibatch(n, s, d, None, 2) == (3, 0, 1)
+    assert oracle.rbsc_multibatch(n, s, d, None, 2, aggressive=True) == (3, 0, 1)
 
 
-def test_planted_mistake_is_caught():
-    """A restatement that ranks by signature only (dropping the colour term,
-    as rsc does) but starts from rbsc's dangling split would still agree with
-    the exact refiner here; dropping a line of the signature does not."""
+def test_multibatch_group_straddling_a_batch_boundary():
+    # Lines 0->1, 1->2, 1->3, 2->3, batch = 2 lines.  Batch 0 = lines
+    # [0, 2): node 0's group ends at line 1, node 1's continues past the
+    # batch, so the batch is [0, 1) and node 1 is marked off-batch
+    # (326-328); node 3 stays dangling (batch 0 starts all nodes dangling).
+    # Colours: node 0 at N+1 = 5, nodes 1 and 2 off-batch at 2 and 3, node 3
+    # at 0.  Round 0: node 0 has signature randomize(2 + 0), giving 4
+    # distinct pairs against the 3 of the start; round 1 keeps 4, so the
+    # batch labels node 0 -> 0 and node 3 -> 1.  Batch 1 = lines [2, 4):
+    # line 2 continues node 1's group, so the batch is [3, 4), node 2 alone,
+    # in one round (4 distinct pairs from 4), label 2.  Node 1 was in no
+    # batch and gets 3 (457-461): 4 nodes from 4, stop.  (4, 0, 1).
+    n, s, d = _coo(4, [(0, 1), (1, 2), (1, 3), (2, 3)])
+    assert oracle.rbsc_multibatch(n, s, d, None, 2) == (4, 0, 1)
+    # the coarsest partition has 4 blocks too (a path of length 3 into a
+    # dangling node, and node 1 with lines into both)
+    assert oracle.exact_refine(n, s, d)[1] == 4
+
+
+def _reference_round_steps(n, src, dst, weight=None, max_rounds=64):
+    """rbsc.cu:352-425 for one batch holding every line, step by step, with
+    numpy's stable sort standing in for CUB's stable radix sorts:
+    init_partition (51-63: N+1 for nodes with a line, 0 for the dangling
+    ones), set_values (80-87), ReduceByKey (366-377) + reorder (379) +
+    mend_partition (382), SortPairs of the indexes by the colour's low 32
+    bits (386), gather (388), stable SortPairs by the 64-bit signature
+    (390), compute_thruths_lex (118-123: full 64-bit colours compared),
+    InclusiveSum (394) and reorder (396); the salt advances once per round
+    (404-419 run once); stop when the count repeats (425).  Returns the
+    colour vector after every round."""
+    src = np.asarray(src, np.int64)
+    dst = np.asarray(dst, np.int64)
+    wt = np.ones(len(dst), np.uint64) if weight is None else np.asarray(weight, np.uint64)
+    has = np.zeros(n, bool)
+    has[src] = True
+    col = np.where(has, n + 1, 0).astype(np.uint64)
+    K, it, trace = 1, 0, []
+    while True:
+        K_old = K
+        vals = np.array([(oracle.randomize(int(col[v]) + it) * int(w)) & M64
+                         for v, w in zip(dst.tolist(), wt.tolist())], np.uint64)
+        sig = np.zeros(n, np.uint64)
+        np.add.at(sig, src, vals)                      # mod 2^64; dangling stay 0
+        o1 = np.argsort(col & np.uint64(0xFFFFFFFF), kind="stable")
+        z1 = sig[o1]
+        o2 = np.argsort(z1, kind="stable")
+        idx, z = o1[o2], z1[o2]
+        truths = np.zeros(n, np.int64)
+        truths[1:] = (z[1:] != z[:-1]) | (col[idx[1:]] != col[idx[:-1]])
+        cs = np.cumsum(truths)
+        new = np.empty(n, np.uint64)
+        new[idx] = cs
+        K = int(cs[-1]) + 1 if n else 0
+        col = new
+        it += 1
+        trace.append(col.astype(np.uint32))
+        if K == K_old or it >= max_rounds:
+            return trace
+
+
+def _step_graphs():
+    path = _coo(6, [(i, i + 1) for i in range(5)])
+    w = gen.rmat(7, 1 << 10, clone_frac=0.3, seed=61)
+    rng = np.random.default_rng(62)
+    wts = rng.integers(1, 4, w.n_edges).astype(np.uint64)
+    return [path[:3] + (None,),
+            (w.n_nodes, w.sources(), w.dst, wts),
+            (lambda g: (g.n_nodes, g.sources(), g.dst, None))(gen.rmat(8, 1 << 11, 0.57, 0.19,
+                                                                       0.19, 0.2, seed=63)),
+            (lambda g: (g.n_nodes, g.sources(), g.dst, None))(gen.ba(400, 2, 0.1, 0, seed=64))]
+
+
+def _trace_of(n, s, d, w, lib=None):
+    _, _, R, tr = oracle.rbsc_refine(n, s, d, w, trace_rounds=64, _from=lib)
+    return [tr[r] for r in range(R)]
+
+
+def test_per_round_colours_follow_the_reference_steps():
+    """The oracle's colour vector after every round equals the reference's
+    own sequence of steps restated with numpy's stable sorts (a second,
+    independent restatement of the per-round colours, not a retyping of the
+    oracle's comparator)."""
+    for n, s, d, w in _step_graphs():
+        want = _reference_round_steps(n, s, d, w)
+        got = _trace_of(n, s, d, w)
+        assert len(got) == len(want)
+        for r, (a, b) in enumerate(zip(got, want)):
+            assert np.array_equal(a, b), f"round {r + 1}"
+
+
+MUTATIONS = {
+    # the rank order with the colour before the signature
+    "colour_before_sig": (
+        """    if (x->sig != y->sig) return x->sig < y->sig ? -1 : 1;
+    u32 cx = (u32)x->col, cy = (u32)y->col;
+    if (cx != cy) return cx < cy ? -1 : 1;""",
+        """    u32 cx = (u32)x->col, cy = (u32)y->col;
+    if (cx != cy) return cx < cy ? -1 : 1;
+    if (x->sig != y->sig) return x->sig < y->sig ? -1 : 1;"""),
+    # no dangling split: every node starts at N+1 (as rsc starts all at 1)
+    "no_dangling_split": (
+        "col[i] = init ? (u64)init[i] : (has[i] ? (u64)N + 1 : 0);",
+        "col[i] = init ? (u64)init[i] : (u64)N + 1;"),
+}
+
+
+def _build_copy(tmp_path, name, mutation):
+    import shutil
+    import subprocess
+    src = os.path.join(os.path.dirname(oracle.__file__), "rbsc_oracle.c")
+    text = open(src).read()
+    if mutation is not None:
+        old, new = mutation
+        assert text.count(old) == 1, f"mutation {name} no longer matches the oracle source"
+        text = text.replace(old, new)
+    c = tmp_path / f"{name}.c"
+    c.write_text(text)
+    so = tmp_path / f"lib{name}.so"
+    subprocess.run([shutil.which("gcc") or "gcc", "-O2", "-shared", "-fPIC", "-o", str(so),
+                    str(c), "-lm"], check=True, capture_output=True)
+    return oracle.load_lib(str(so))
+
+
+def _caught(lib):
+    """True iff the per-round check above rejects this build of the oracle on
+    one of the graphs (its final partitions may still all be right)."""
+    for n, s, d, w in _step_graphs():
+        want = _reference_round_steps(n, s, d, w)
+        got = _trace_of(n, s, d, w, lib)
+        if len(got) != len(want) or any(not np.array_equal(a, b) for a, b in zip(got, want)):
+            return True
+    return False
+
+
+def test_planted_mistakes_in_a_copy_of_the_oracle_are_caught(tmp_path):
+    """Copies of the oracle with a mistake planted (the rank order with the
+    colour first; no dangling split) still give the right final partitions
+    -- dense ranks of equal pairs and a start that the first round refines
+    anyway -- so block counts and the exact refiner cannot see them.  The
+    per-round check against the reference's steps must reject both, and
+    accept the unchanged copy."""
+    assert not _caught(_build_copy(tmp_path, "unchanged", None))
+    for name, mutation in MUTATIONS.items():
+        lib = _build_copy(tmp_path, name, mutation)
+        for n, s, d, w in _step_graphs():  # the final partitions still agree
+            a = oracle.rbsc_refine(n, s, d, w, _from=lib)
+            b = oracle.rbsc_refine(n, s, d, w)
+            assert a[1] == b[1] and np.array_equal(oracle.canonical(a[0]), oracle.canonical(b[0]))
+        assert _caught(lib), f"planted mistake {name} not caught"
+
+
+def test_dropped_line_changes_the_partition():
+    """Dropping a line of the input (not a mistake in the oracle) moves the
+    result away from the exact refiner's partition of the full graph."""
     g = gen.rmat(10, 1 << 13, clone_frac=0.3, seed=14)
     src, dst = g.sources(), g.dst
     K = oracle.rbsc_refine(g.n_nodes, src, dst)[1]
-    bad_src, bad_dst = src[:-1], dst[:-1]          # the planted mistake
+    bad_src, bad_dst = src[:-1], dst[:-1]          # one line fewer
     Kbad = oracle.rbsc_refine(g.n_nodes, bad_src, bad_dst)[1]
     eK = oracle.exact_refine(g.n_nodes, src, dst)[1]
     assert K == eK
