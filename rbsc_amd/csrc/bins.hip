@@ -43,6 +43,7 @@
 // exact in Z/2^64: signatures are identical.
 #include "internal.h"
 #include "scan.h"
+#include "tiles.h"
 #include <algorithm>
 #include <vector>
 
@@ -73,48 +74,125 @@ __device__ __forceinline__ u32 gathered_head(u32 v, const u32* __restrict__ rep,
     return (rep && v >= R && v < U) ? rep[v - R] : v;
 }
 
+// Per stored row, in tiles of LT_LINES lines (rows are skewed): lines into
+// hot heads, into cold heads, and the total weight of lines into heads that
+// are never gathered, added to n_hot / n_cold / dsum (zeroed) with one
+// atomic per row and tile; and per tile its hot and cold line counts, whose
+// scans place each tile's records of a class (records go in line order).
 template <bool HAS_WT>
-__global__ void bin_count_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                 const u64* __restrict__ wt, u64 R, u32 H, u32 GL,
-                                 const u32* __restrict__ rep, u32 U, u64* __restrict__ n_hot,
-                                 u64* __restrict__ n_cold, u64* __restrict__ dsum) {
-    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u > R) return;
-    if (u == R) {
-        n_hot[R] = 0;
-        n_cold[R] = 0;
-        return;
+__global__ __launch_bounds__(LT_THREADS) void bin_count_tile_kernel(
+    const u64* __restrict__ row_ptr, const u32* __restrict__ dst, const u64* __restrict__ wt,
+    u64 R, u64 E, u32 H, u32 GL, const u32* __restrict__ rep, u32 U,
+    const u32* __restrict__ first, unsigned long long* __restrict__ n_hot,
+    unsigned long long* __restrict__ n_cold, unsigned long long* __restrict__ dsum,
+    u64* __restrict__ tile_hot, u64* __restrict__ tile_cold) {
+    __shared__ LtLds s;
+    __shared__ u32 acc_h[LT_LINES + 1], acc_c[LT_LINES + 1];
+    __shared__ unsigned long long acc_d[LT_LINES + 1];
+    __shared__ u32 tot[2];
+    const u64 t = blockIdx.x;
+    u32 row[LT_PER];
+    const u32 last = lt_rows(row_ptr, R, E, first, t, &s, row);
+    const u32 r0 = first[t], nr = last - r0 + 1;
+    for (u32 i = threadIdx.x; i < nr; i += LT_THREADS) {
+        acc_h[i] = acc_c[i] = 0;
+        acc_d[i] = 0;
     }
-    u64 h = 0, c = 0, d = 0;
-    for (u64 e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
-        const u32 v = gathered_head(dst[e], rep, (u32)R, U);
-        if (v < H) ++h;
-        else if (v < GL) ++c;
-        else d += HAS_WT ? wt[e] : 1;
+    if (threadIdx.x < 2) tot[threadIdx.x] = 0;
+    __syncthreads();
+    const u64 b0 = t * LT_LINES + (u64)threadIdx.x * LT_PER;
+    u32 h = 0, c = 0, th = 0, tc = 0;
+    u64 d = 0;
+    u32 cur = row[0];
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j) {
+        const u64 e = b0 + j;
+        if (e < E) {
+            if (row[j] != cur) {
+                if (h) atomicAdd(&acc_h[cur - r0], h);
+                if (c) atomicAdd(&acc_c[cur - r0], c);
+                if (d) atomicAdd(&acc_d[cur - r0], (unsigned long long)d);
+                h = c = 0;
+                d = 0;
+                cur = row[j];
+            }
+            const u32 v = gathered_head(dst[e], rep, (u32)R, U);
+            if (v < H) {
+                ++h;
+                ++th;
+            } else if (v < GL) {
+                ++c;
+                ++tc;
+            } else {
+                d += HAS_WT ? wt[e] : 1;
+            }
+        }
     }
-    n_hot[u] = h;
-    n_cold[u] = c;
-    dsum[u] = d;
+    if (h) atomicAdd(&acc_h[cur - r0], h);
+    if (c) atomicAdd(&acc_c[cur - r0], c);
+    if (d) atomicAdd(&acc_d[cur - r0], (unsigned long long)d);
+    if (th) atomicAdd(&tot[0], th);
+    if (tc) atomicAdd(&tot[1], tc);
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nr; i += LT_THREADS) {
+        if (acc_h[i]) atomicAdd(&n_hot[r0 + i], (unsigned long long)acc_h[i]);
+        if (acc_c[i]) atomicAdd(&n_cold[r0 + i], (unsigned long long)acc_c[i]);
+        if (acc_d[i]) atomicAdd(&dsum[r0 + i], acc_d[i]);
+    }
+    if (threadIdx.x == 0) {
+        tile_hot[t] = tot[0];
+        tile_cold[t] = tot[1];
+    }
 }
 
-// Line records of one class in row-major order: key (bin, head), value
-// (line << 16 | row in bin).
+// Line records of one class in line (row-major) order: key (bin, head),
+// value (line << 16 | row in bin); a tile's records start at toff[t] and
+// follow in line order inside it (a workgroup prefix count).
 template <bool HOT>
-__global__ void bin_fill_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                u64 R, u32 H, u32 GL, const u32* __restrict__ rep, u32 U,
-                                u32 log_s, int hb, const u64* __restrict__ off,
-                                u64* __restrict__ keys, u64* __restrict__ vals) {
-    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= R) return;
-    u64 p = off[u];
-    const u64 bin = u >> log_s;
-    const u64 rib = u & ((1ull << log_s) - 1);
-    for (u64 e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
-        const u32 v = gathered_head(dst[e], rep, (u32)R, U);
-        const bool take = HOT ? v < H : (v >= H && v < GL);
-        if (take) {
-            keys[p] = (bin << hb) | v;
-            vals[p] = (e << 16) | rib;
+__global__ __launch_bounds__(LT_THREADS) void bin_fill_tile_kernel(
+    const u64* __restrict__ row_ptr, const u32* __restrict__ dst, u64 R, u64 E, u32 H, u32 GL,
+    const u32* __restrict__ rep, u32 U, u32 log_s, int hb, const u32* __restrict__ first,
+    const u64* __restrict__ toff, u64* __restrict__ keys, u64* __restrict__ vals) {
+    __shared__ LtLds s;
+    const u64 t = blockIdx.x;
+    u32 row[LT_PER];
+    lt_rows(row_ptr, R, E, first, t, &s, row);
+    const u64 b0 = t * LT_LINES + (u64)threadIdx.x * LT_PER;
+    u32 take[LT_PER], vv[LT_PER], cnt = 0;
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j) {
+        const u64 e = b0 + j;
+        take[j] = 0;
+        vv[j] = 0;
+        if (e < E) {
+            const u32 v = gathered_head(dst[e], rep, (u32)R, U);
+            vv[j] = v;
+            take[j] = (HOT ? v < H : (v >= H && v < GL)) ? 1u : 0u;
+        }
+        cnt += take[j];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    u32 x = cnt;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const u32 y = __shfl_up(x, dd);
+        if (lane >= dd) x += y;
+    }
+    if (lane == 63) s.wsum[wave] = x;
+    __syncthreads();
+    u32 off = 0;
+#pragma unroll
+    for (int w = 0; w < LT_THREADS / 64; ++w)
+        if (w < wave) off += s.wsum[w];
+    u64 p = toff[t] + off + x - cnt;
+    const u64 mask = (1ull << log_s) - 1;
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j) {
+        if (take[j]) {
+            const u64 e = b0 + j;
+            const u64 u = row[j];
+            keys[p] = ((u >> log_s) << hb) | vv[j];
+            vals[p] = (e << 16) | (u & mask);
             ++p;
         }
     }
@@ -670,6 +748,10 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
     u64* warm_off = nullptr;
     WarmMap xcd_segs{};
     u64* grp_first = nullptr;
+    // line tiles of the stored rows (every stored row has a line)
+    const u64 n_lt = lt_tiles(g->E);
+    u32* lt_first = nullptr;
+    u64 *lt_hot = nullptr, *lt_cold = nullptr;
     std::vector<u64> h_bin_n, h_bin_hot, h_bin_slot;
     std::vector<u32> h_grp(n_bins + 1, 0), h_zero;
     std::vector<BinChunk> h_chunks;
@@ -693,16 +775,35 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
     B_TRY(hipMalloc(&n_hot, sizeof(u64) * (R + 1)));
     B_TRY(hipMalloc(&n_cold, sizeof(u64) * (R + 1)));
     B_TRY(hipMalloc(&b->dsum, sizeof(u64) * std::max<u64>(R, 1)));
-    B_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(std::max<u64>(R, n_bins) + 1)));
+    B_TRY(hipMalloc(&tmp, sizeof(u64) *
+                              scan_tmp_words(std::max<u64>(std::max<u64>(R, n_bins), n_lt) + 1)));
     B_TRY(hipMalloc(&tot, sizeof(u64) * 2));
-    if (g->wt)
-        bin_count_kernel<true><<<grid_for(R + 1), BLK, 0, s>>>(g->row_ptr, g->dst, g->wt, R, H, GL,
-                                                             rep, g->U, n_hot, n_cold, b->dsum);
-    else
-        bin_count_kernel<false><<<grid_for(R + 1), BLK, 0, s>>>(g->row_ptr, g->dst, nullptr, R, H,
-                                                              GL, rep, g->U, n_hot, n_cold, b->dsum);
+    B_TRY(hipMalloc(&lt_first, sizeof(u32) * (n_lt + 1)));
+    B_TRY(hipMalloc(&lt_hot, sizeof(u64) * (n_lt + 1)));
+    B_TRY(hipMalloc(&lt_cold, sizeof(u64) * (n_lt + 1)));
+    B_TRY(hipMemsetAsync(n_hot, 0, sizeof(u64) * (R + 1), s));
+    B_TRY(hipMemsetAsync(n_cold, 0, sizeof(u64) * (R + 1), s));
+    B_TRY(hipMemsetAsync(b->dsum, 0, sizeof(u64) * std::max<u64>(R, 1), s));
+    B_TRY(hipMemsetAsync(lt_hot + n_lt, 0, sizeof(u64), s));
+    B_TRY(hipMemsetAsync(lt_cold + n_lt, 0, sizeof(u64), s));
+    lt_first_row_kernel<<<grid_for(n_lt + 1), BLK, 0, s>>>(g->row_ptr, R, g->E, n_lt, lt_first);
+    {
+        auto* nh = reinterpret_cast<unsigned long long*>(n_hot);
+        auto* nc = reinterpret_cast<unsigned long long*>(n_cold);
+        auto* ds = reinterpret_cast<unsigned long long*>(b->dsum);
+        if (g->wt)
+            bin_count_tile_kernel<true><<<(unsigned)n_lt, LT_THREADS, 0, s>>>(
+                g->row_ptr, g->dst, g->wt, R, g->E, H, GL, rep, g->U, lt_first, nh, nc, ds, lt_hot,
+                lt_cold);
+        else
+            bin_count_tile_kernel<false><<<(unsigned)n_lt, LT_THREADS, 0, s>>>(
+                g->row_ptr, g->dst, nullptr, R, g->E, H, GL, rep, g->U, lt_first, nh, nc, ds,
+                lt_hot, lt_cold);
+    }
     if ((rc = scan_excl_u64(n_hot, n_hot, R + 1, tot, tmp, s))) goto done;
     if ((rc = scan_excl_u64(n_cold, n_cold, R + 1, tot + 1, tmp, s))) goto done;
+    if ((rc = scan_excl_u64(lt_hot, lt_hot, n_lt + 1, nullptr, tmp, s))) goto done;
+    if ((rc = scan_excl_u64(lt_cold, lt_cold, n_lt + 1, nullptr, tmp, s))) goto done;
     B_TRY(hipMemcpyAsync(h_tot, tot, sizeof(u64) * 2, hipMemcpyDeviceToHost, s));
     B_TRY(hipStreamSynchronize(s));
     b->n_hot = h_tot[0];
@@ -787,8 +888,9 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
             B_TRY(hipMalloc(&keys[i], sizeof(u64) * b->n_hot));
             B_TRY(hipMalloc(&vals[i], sizeof(u64) * b->n_hot));
         }
-        bin_fill_kernel<true><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, g->dst, R, H, GL, rep, g->U,
-                                                          log_s, hb, n_hot, keys[0], vals[0]);
+        bin_fill_tile_kernel<true><<<(unsigned)n_lt, LT_THREADS, 0, s>>>(
+            g->row_ptr, g->dst, R, g->E, H, GL, rep, g->U, log_s, hb, lt_first, lt_hot, keys[0],
+            vals[0]);
         int w = 0;
         const int bits = (hb + bb + 7) / 8 * 8;
         if ((rc = radix_sort_u64(&ctx->sort, keys, vals, b->n_hot, 0, bits, &w, s))) goto done;
@@ -916,8 +1018,9 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
         B_TRY(hipMalloc(&grp_of_bin, sizeof(u32) * (n_bins + 1)));
         B_TRY(hipMemcpyAsync(grp_of_bin, h_grp.data(), sizeof(u32) * (n_bins + 1),
                              hipMemcpyHostToDevice, s));
-        bin_fill_kernel<false><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, g->dst, R, H, GL, rep, g->U,
-                                                           log_s, gb, n_cold, keys[0], vals[0]);
+        bin_fill_tile_kernel<false><<<(unsigned)n_lt, LT_THREADS, 0, s>>>(
+            g->row_ptr, g->dst, R, g->E, H, GL, rep, g->U, log_s, gb, lt_first, lt_cold, keys[0],
+            vals[0]);
         int w = 0;
         int bits = (gb + bb + 7) / 8 * 8;
         if ((rc = radix_sort_u64(&ctx->sort, keys, vals, b->n_cold, 0, bits, &w, s))) goto done;
@@ -990,9 +1093,9 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
     b = nullptr;
 done:
     for (u64* p : {n_hot, n_cold, tmp, tot, hot_start, cold_start, bin_n, keys[0], keys[1],
-                   vals[0], vals[1], grp_first, hot_all, warm_off})
+                   vals[0], vals[1], grp_first, hot_all, warm_off, lt_hot, lt_cold})
         if (p) (void)hipFree(p);
-    for (u32* p : {grp_of_bin, warm_cnt, warm_bound})
+    for (u32* p : {grp_of_bin, warm_cnt, warm_bound, lt_first})
         if (p) (void)hipFree(p);
     if (b) bins_release(b);
     return rc;

@@ -27,6 +27,7 @@
 //     dup_rep[j - R] names the stored row whose signature they copy.
 #include "internal.h"
 #include "scan.h"
+#include "tiles.h"
 #include <algorithm>
 
 namespace {
@@ -46,6 +47,24 @@ __global__ void fill_ones_kernel(u64* p, u64 n) {
 __global__ void indeg_kernel(const u32* __restrict__ dst, u64 E, u32* __restrict__ indeg) {
     const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < E) atomicAdd(&indeg[dst[e]], 1u);
+}
+// In-degrees of a large graph into INDEG_COPIES private copies (copy =
+// workgroup mod INDEG_COPIES), summed afterwards: an R-MAT hub receives
+// millions of lines, and atomics on one address serialise, so each copy's
+// address takes a share of them.
+constexpr u32 INDEG_COPIES = 8;
+__global__ void indeg_priv_kernel(const u32* __restrict__ dst, u64 E, u64 N,
+                                  u32* __restrict__ copies) {
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) atomicAdd(&copies[(u64)(blockIdx.x % INDEG_COPIES) * N + dst[e]], 1u);
+}
+__global__ void indeg_sum_kernel(const u32* __restrict__ copies, u64 N, u32* __restrict__ indeg) {
+    const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    u32 d = 0;
+#pragma unroll
+    for (u32 k = 0; k < INDEG_COPIES; ++k) d += copies[(u64)k * N + v];
+    indeg[v] = d;
 }
 
 // keys: (~in-degree, original id) so an ascending sort gives in-degree
@@ -125,46 +144,100 @@ __device__ __forceinline__ u64 mix_a(u64 x) {
     x *= 0xc4ceb9fe1a85ec53ULL;
     return x ^ (x >> 33);
 }
-template <bool HAS_WT>
-__global__ void row_hash_kernel(const u64* __restrict__ row_ptr, const u32* __restrict__ dst,
-                                const u64* __restrict__ wt, u64 U, u64* __restrict__ h1,
-                                u64* __restrict__ h2, u64* __restrict__ idx) {
-    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= U) return;
-    u64 a = 0, b = 0;
-    const u64 e0 = row_ptr[k];
-    for (u64 e = e0; e < row_ptr[k + 1]; ++e) {
-        const u64 w = HAS_WT ? wt[e] : 1;
-        const u64 j = (e - e0) * 0xbf58476d1ce4e5b9ULL;
-        a += mix_a(((u64)dst[e] << 1) ^ (w * 0x9e3779b97f4a7c15ULL) ^ j);
-        b += mix_a(((u64)dst[e] + 0x632be59bd9b4e019ULL) ^ (j >> 7)) * (2 * w + 1);
-    }
-    h1[k] = a;
-    h2[k] = b ^ ((row_ptr[k + 1] - row_ptr[k]) * 0xd6e8feb86659fd93ULL);
-    idx[k] = k;
+// Row fingerprints summed over lines in tiles of LT_LINES lines (rows are
+// skewed): each line adds its terms, position-aware through its index in
+// its row, to per-row sums in LDS, which go to h1/h2 (zeroed) with one
+// atomic per row and tile.  Sums in Z/2^64 do not depend on the order.
+__device__ __forceinline__ void fp_terms(u64 d, u64 w, u64 pos, u64* a, u64* b) {
+    const u64 j = pos * 0xbf58476d1ce4e5b9ULL;
+    *a += mix_a((d << 1) ^ (w * 0x9e3779b97f4a7c15ULL) ^ j);
+    *b += mix_a((d + 0x632be59bd9b4e019ULL) ^ (j >> 7)) * (2 * w + 1);
 }
-// After sorting rows by h1: eq[i] = 1 if the row at sorted position i has
-// exactly the lines (heads and weights, in order) of the row at i - 1.
 template <bool HAS_WT>
-__global__ void row_eq_kernel(const u64* __restrict__ h1s, const u64* __restrict__ rows,
-                              const u64* __restrict__ h2, const u64* __restrict__ row_ptr,
-                              const u32* __restrict__ dst, const u64* __restrict__ wt, u64 U,
-                              u64* __restrict__ head) {
-    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= U) return;
-    bool eq = false;
-    if (i > 0 && h1s[i] == h1s[i - 1]) {
-        const u64 x = rows[i], y = rows[i - 1];
-        const u64 ax = row_ptr[x], ay = row_ptr[y], n = row_ptr[x + 1] - ax;
-        if (h2[x] == h2[y] && n == row_ptr[y + 1] - ay) {
-            eq = true;
-            for (u64 j = 0; j < n && eq; ++j) {
-                if (dst[ax + j] != dst[ay + j]) eq = false;
-                if (HAS_WT && wt[ax + j] != wt[ay + j]) eq = false;
+__global__ __launch_bounds__(LT_THREADS) void row_hash_tile_kernel(
+    const u64* __restrict__ row_ptr, const u32* __restrict__ dst, const u64* __restrict__ wt,
+    u64 U, u64 E, const u32* __restrict__ first, unsigned long long* __restrict__ h1,
+    unsigned long long* __restrict__ h2) {
+    __shared__ LtLds s;
+    __shared__ unsigned long long acc[2][LT_LINES + 1];
+    const u64 t = blockIdx.x;
+    u32 row[LT_PER];
+    const u32 last = lt_rows(row_ptr, U, E, first, t, &s, row);
+    const u32 r0 = first[t], nr = last - r0 + 1;
+    for (u32 i = threadIdx.x; i < nr; i += LT_THREADS) acc[0][i] = acc[1][i] = 0;
+    __syncthreads();
+    const u64 b0 = t * LT_LINES + (u64)threadIdx.x * LT_PER;
+    u64 a = 0, b = 0;
+    u32 cur = row[0];
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j) {
+        const u64 e = b0 + j;
+        if (e < E) {
+            if (row[j] != cur) {
+                atomicAdd(&acc[0][cur - r0], (unsigned long long)a);
+                atomicAdd(&acc[1][cur - r0], (unsigned long long)b);
+                a = b = 0;
+                cur = row[j];
             }
+            fp_terms(dst[e], HAS_WT ? wt[e] : 1, e - s.start[cur - r0], &a, &b);
         }
     }
-    head[i] = eq ? 0 : 1;
+    if (b0 < E) {
+        atomicAdd(&acc[0][cur - r0], (unsigned long long)a);
+        atomicAdd(&acc[1][cur - r0], (unsigned long long)b);
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nr; i += LT_THREADS) {
+        atomicAdd(&h1[r0 + i], acc[0][i]);
+        atomicAdd(&h2[r0 + i], acc[1][i]);
+    }
+}
+__global__ void iota_u64_kernel(u64* __restrict__ p, u64 n) {
+    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = k;
+}
+// After sorting rows by h1: head[i] = 0 if the row at sorted position i has
+// exactly the lines (heads and weights, in order) of the row at i - 1.  A
+// lane compares a candidate pair of up to 64 lines itself; longer pairs are
+// compared by the whole wave in turn, 64 lines a step.
+template <bool HAS_WT>
+__global__ __launch_bounds__(256) void row_eq_wave_kernel(
+    const u64* __restrict__ h1s, const u64* __restrict__ rows, const u64* __restrict__ h2,
+    const u64* __restrict__ row_ptr, const u32* __restrict__ dst, const u64* __restrict__ wt,
+    u64 U, u64* __restrict__ head) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool cand = false;
+    u64 ax = 0, ay = 0, n = 0;
+    if (i < U && i > 0 && h1s[i] == h1s[i - 1]) {
+        const u64 x = rows[i], y = rows[i - 1];
+        ax = row_ptr[x];
+        ay = row_ptr[y];
+        n = row_ptr[x + 1] - ax;
+        cand = h2[x] == h2[y] && n == row_ptr[y + 1] - ay;
+    }
+    bool eq = cand;
+    const bool longrow = cand && n > 64;
+    if (cand && !longrow) {
+        for (u64 j = 0; j < n && eq; ++j) {
+            if (dst[ax + j] != dst[ay + j]) eq = false;
+            if (HAS_WT && wt[ax + j] != wt[ay + j]) eq = false;
+        }
+    }
+    u64 todo = __ballot(longrow);
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        const u64 lx = __shfl(ax, l), ly = __shfl(ay, l), ln = __shfl(n, l);
+        bool ok = true;
+        for (u64 j = lane; j < ln; j += 64) {
+            if (dst[lx + j] != dst[ly + j]) ok = false;
+            if (HAS_WT && wt[lx + j] != wt[ly + j]) ok = false;
+        }
+        const bool same = __ballot(!ok) == 0;
+        if (lane == l) eq = same;
+    }
+    if (i < U) head[i] = eq ? 0 : 1;
 }
 // Heads of runs of identical rows record their raw row per run.
 __global__ void run_leader_kernel(const u64* __restrict__ head, const u64* __restrict__ hscan,
@@ -187,24 +260,44 @@ __global__ void dup_rep_kernel(const u32* __restrict__ src_row, const u32* __res
     if (j < U) dup_rep[j - R] = pi[row_node[rep_raw[src_row[j]]]];
 }
 
-// One thread per new row: copy its lines with heads renumbered.
+// The stored rows' lines in their new order, heads renumbered, in tiles of
+// LT_LINES output lines: line e of new row r comes from raw line
+// e + (row_ptr_raw[src_row[r]] - row_ptr_new[r]).  Each tile also writes its
+// 64 words of the row-end bitmap (tiles are word aligned, so no word is
+// shared).
 template <bool HAS_WT>
-__global__ void copy_rows_kernel(const u64* __restrict__ row_ptr_new, const u32* __restrict__ src_row,
-                                 const u64* __restrict__ row_ptr_raw, const u32* __restrict__ dst_raw,
-                                 const u64* __restrict__ wt_raw, const u32* __restrict__ pi, u64 U,
-                                 u32* __restrict__ dst_new, u64* __restrict__ wt_new,
-                                 u32* __restrict__ end_bits) {
-    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= U) return;
-    const u64 k = src_row[j];
-    const u64 a = row_ptr_raw[k], b = row_ptr_raw[k + 1];
-    u64 o = row_ptr_new[j];
-    for (u64 p = a; p < b; ++p, ++o) {
-        dst_new[o] = pi[dst_raw[p]];
-        if (HAS_WT) wt_new[o] = wt_raw[p];
+__global__ __launch_bounds__(LT_THREADS) void copy_rows_tile_kernel(
+    const u64* __restrict__ row_ptr_new, const u32* __restrict__ src_row,
+    const u64* __restrict__ row_ptr_raw, const u32* __restrict__ dst_raw,
+    const u64* __restrict__ wt_raw, const u32* __restrict__ pi, u64 R, u64 E,
+    const u32* __restrict__ first, u32* __restrict__ dst_new, u64* __restrict__ wt_new,
+    u32* __restrict__ end_bits) {
+    __shared__ LtLds s;
+    __shared__ u64 base[LT_LINES + 1];
+    __shared__ u32 endw[LT_LINES / 32];
+    const u64 t = blockIdx.x;
+    u32 row[LT_PER];
+    const u32 last = lt_rows(row_ptr_new, R, E, first, t, &s, row);
+    const u32 r0 = first[t], nr = last - r0 + 1;
+    for (u32 i = threadIdx.x; i < nr; i += LT_THREADS)
+        base[i] = row_ptr_raw[src_row[r0 + i]] - s.start[i];
+    for (u32 i = threadIdx.x; i < LT_LINES / 32; i += LT_THREADS) endw[i] = 0;
+    __syncthreads();
+    const u64 e0 = t * LT_LINES, b0 = e0 + (u64)threadIdx.x * LT_PER;
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j) {
+        const u64 e = b0 + j;
+        if (e < E) {
+            const u32 k = row[j] - r0;
+            const u64 src = e + base[k];
+            dst_new[e] = pi[dst_raw[src]];
+            if (HAS_WT) wt_new[e] = wt_raw[src];
+            if (e + 1 == s.start[k + 1])
+                atomicOr(&endw[(e - e0) >> 5], 1u << ((e - e0) & 31));
+        }
     }
-    const u64 last = row_ptr_new[j + 1] - 1;
-    atomicOr(&end_bits[last >> 5], 1u << (last & 31));
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < LT_LINES / 32; i += LT_THREADS) end_bits[e0 / 32 + i] = endw[i];
 }
 
 __global__ void permute_init_kernel(const u32* __restrict__ init_raw, const u32* __restrict__ pi,
@@ -259,7 +352,8 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     g->U = (u32)U;
     u64 *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr}, *dang = nullptr,
         *deg = nullptr, *tmp = nullptr, *h2 = nullptr, *d_tot = nullptr;
-    u32 *indeg = nullptr, *src_row = nullptr, *rep_raw = nullptr;
+    u32 *indeg = nullptr, *src_row = nullptr, *rep_raw = nullptr, *tfirst = nullptr,
+        *indeg_copies = nullptr;
     u64 R = U, E = E_in, h_tot[2] = {0, 0};
     int rc = RBSC_OK;
 #define B_TRY(expr)                                                                  \
@@ -279,6 +373,7 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     B_TRY(hipMalloc(&src_row, sizeof(u32) * std::max<u64>(U, 1)));
     B_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(std::max(N, U) + 1)));
     B_TRY(hipMalloc(&d_tot, sizeof(u64) * 2));
+    B_TRY(hipMalloc(&tfirst, sizeof(u32) * (lt_tiles(E_in) + 1)));
     for (int i = 0; i < 2; ++i) {
         B_TRY(hipMalloc(&keys[i], sizeof(u64) * std::max<u64>(U, 1)));
         B_TRY(hipMalloc(&vals[i], sizeof(u64) * std::max<u64>(U, 1)));
@@ -287,8 +382,15 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
         rc = sort_ws_init(&ctx->sort, U, ctx->d_err, s);
         if (rc) goto done;
     }
-    zero_u32<<<grid_for(N), BLK, 0, s>>>(indeg, N);
-    if (E_in) indeg_kernel<<<grid_for(E_in), BLK, 0, s>>>(raw->dst, E_in, indeg);
+    if (E_in >= (1ull << 24)) {
+        B_TRY(hipMalloc(&indeg_copies, sizeof(u32) * INDEG_COPIES * N));
+        B_TRY(hipMemsetAsync(indeg_copies, 0, sizeof(u32) * INDEG_COPIES * N, s));
+        indeg_priv_kernel<<<grid_for(E_in), BLK, 0, s>>>(raw->dst, E_in, N, indeg_copies);
+        indeg_sum_kernel<<<grid_for(N), BLK, 0, s>>>(indeg_copies, N, indeg);
+    } else {
+        zero_u32<<<grid_for(N), BLK, 0, s>>>(indeg, N);
+        if (E_in) indeg_kernel<<<grid_for(E_in), BLK, 0, s>>>(raw->dst, E_in, indeg);
+    }
     fill_ones_kernel<<<grid_for(N), BLK, 0, s>>>(dang, N);
 
     // Identical rows: sort rows by fingerprint, compare neighbours exactly,
@@ -296,23 +398,31 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     if (U > 1 && ctx->dedup) {
         B_TRY(hipMalloc(&h2, sizeof(u64) * U));
         B_TRY(hipMalloc(&rep_raw, sizeof(u32) * U));
+        const u64 nt = lt_tiles(E_in);
+        lt_first_row_kernel<<<grid_for(nt + 1), BLK, 0, s>>>(raw->row_ptr, U, E_in, nt, tfirst);
+        B_TRY(hipMemsetAsync(keys[0], 0, sizeof(u64) * U, s));
+        B_TRY(hipMemsetAsync(h2, 0, sizeof(u64) * U, s));
+        iota_u64_kernel<<<grid_for(U), BLK, 0, s>>>(vals[0], U);
+        auto* h1p = reinterpret_cast<unsigned long long*>(keys[0]);
+        auto* h2p = reinterpret_cast<unsigned long long*>(h2);
         if (raw->wt)
-            row_hash_kernel<true><<<grid_for(U), BLK, 0, s>>>(raw->row_ptr, raw->dst, raw->wt, U,
-                                                              keys[0], h2, vals[0]);
+            row_hash_tile_kernel<true><<<(unsigned)nt, LT_THREADS, 0, s>>>(
+                raw->row_ptr, raw->dst, raw->wt, U, E_in, tfirst, h1p, h2p);
         else
-            row_hash_kernel<false><<<grid_for(U), BLK, 0, s>>>(raw->row_ptr, raw->dst, nullptr, U,
-                                                               keys[0], h2, vals[0]);
+            row_hash_tile_kernel<false><<<(unsigned)nt, LT_THREADS, 0, s>>>(
+                raw->row_ptr, raw->dst, nullptr, U, E_in, tfirst, h1p, h2p);
         int w = 0;
         rc = radix_sort_u64(&ctx->sort, keys, vals, U, 0, 64, &w, s);
         if (rc) goto done;
         u64* head = keys[w ^ 1];
         u64* hscan = vals[w ^ 1];
         if (raw->wt)
-            row_eq_kernel<true><<<grid_for(U), BLK, 0, s>>>(keys[w], vals[w], h2, raw->row_ptr,
-                                                            raw->dst, raw->wt, U, head);
+            row_eq_wave_kernel<true><<<grid_for(U), BLK, 0, s>>>(keys[w], vals[w], h2, raw->row_ptr,
+                                                                 raw->dst, raw->wt, U, head);
         else
-            row_eq_kernel<false><<<grid_for(U), BLK, 0, s>>>(keys[w], vals[w], h2, raw->row_ptr,
-                                                             raw->dst, nullptr, U, head);
+            row_eq_wave_kernel<false><<<grid_for(U), BLK, 0, s>>>(keys[w], vals[w], h2,
+                                                                  raw->row_ptr, raw->dst, nullptr,
+                                                                  U, head);
         rc = scan_excl_u64(head, hscan, U, nullptr, tmp, s);
         if (rc) goto done;
         run_leader_kernel<<<grid_for(U), BLK, 0, s>>>(head, hscan, vals[w], U, h2);  // h2 free now
@@ -366,14 +476,16 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     set_last_kernel<<<1, 1, 0, s>>>(g->row_ptr, R, E);
     zero_u32<<<grid_for(g->end_words), BLK, 0, s>>>(g->end_bits, g->end_words);
     if (R) {
+        const u64 nt = lt_tiles(E);
+        lt_first_row_kernel<<<grid_for(nt + 1), BLK, 0, s>>>(g->row_ptr, R, E, nt, tfirst);
         if (raw->wt)
-            copy_rows_kernel<true><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
-                                                               raw->dst, raw->wt, g->pi, R, g->dst,
-                                                               g->wt, g->end_bits);
+            copy_rows_tile_kernel<true><<<(unsigned)nt, LT_THREADS, 0, s>>>(
+                g->row_ptr, src_row, raw->row_ptr, raw->dst, raw->wt, g->pi, R, E, tfirst, g->dst,
+                g->wt, g->end_bits);
         else
-            copy_rows_kernel<false><<<grid_for(R), BLK, 0, s>>>(g->row_ptr, src_row, raw->row_ptr,
-                                                                raw->dst, nullptr, g->pi, R, g->dst,
-                                                                nullptr, g->end_bits);
+            copy_rows_tile_kernel<false><<<(unsigned)nt, LT_THREADS, 0, s>>>(
+                g->row_ptr, src_row, raw->row_ptr, raw->dst, nullptr, g->pi, R, E, tfirst, g->dst,
+                nullptr, g->end_bits);
     }
     if (raw->init) permute_init_kernel<<<grid_for(N), BLK, 0, s>>>(raw->init, g->pi, N, g->init);
     if (g->n_tiles)
@@ -388,7 +500,7 @@ done:
         if (vals[i]) (void)hipFree(vals[i]);
     }
     for (void* p : {(void*)dang, (void*)deg, (void*)tmp, (void*)indeg, (void*)src_row, (void*)h2,
-                    (void*)rep_raw, (void*)d_tot})
+                    (void*)rep_raw, (void*)d_tot, (void*)tfirst, (void*)indeg_copies})
         if (p) (void)hipFree(p);
     return rc;
 #undef B_TRY

@@ -25,6 +25,7 @@
 #include "internal.h"
 #include "scan.h"
 #include "batch.h"
+#include "tiles.h"
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
@@ -2083,6 +2084,187 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
     return RBSC_OK;
 }
 
+// ------------------------------------------------------ upload: rows on the device
+// The lines of a host graph are checked and compressed into rows on the
+// device after the copy (the host loops over every line cost seconds at
+// 2^31 lines): rows are the nodes with lines, in input order; ids below N,
+// offsets monotone, each source's lines contiguous.  A flagged error is
+// described by the host check, which finds its first line.
+namespace {
+constexpr u32 UPE_BAD = 1u;
+// CSR input: flag[u] = 1 for nodes with lines; bad offsets set the flag.
+__global__ void csr_flag_kernel(const u64* __restrict__ ro, u64 N, u64 E, u64* __restrict__ flag,
+                                u32* __restrict__ err) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u == 0 && (ro[0] != 0 || ro[N] != E)) atomicOr(err, UPE_BAD);
+    if (u >= N) return;
+    const u64 a = ro[u], b = ro[u + 1];
+    if (b < a || b > E) atomicOr(err, UPE_BAD);
+    flag[u] = b > a ? 1 : 0;
+}
+__global__ void csr_rows_kernel(const u64* __restrict__ ro, const u64* __restrict__ pos, u64 N,
+                                u64* __restrict__ row_ptr, u32* __restrict__ row_node) {
+    const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= N || ro[u + 1] <= ro[u]) return;
+    row_ptr[pos[u]] = ro[u];
+    row_node[pos[u]] = (u32)u;
+}
+// COO input, tiles of LT_LINES lines: group starts per tile (and source ids
+// checked), then each start's row slot from the tile's offset and its rank.
+__device__ __forceinline__ u32 coo_starts(const u32* __restrict__ src, u64 E, u64 b0,
+                                          u32 f[LT_PER]) {
+    u32 c = 0;
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j) {
+        const u64 e = b0 + j;
+        f[j] = (e < E && (e == 0 || src[e] != src[e - 1])) ? 1u : 0u;
+        c += f[j];
+    }
+    return c;
+}
+__global__ __launch_bounds__(LT_THREADS) void coo_count_kernel(const u32* __restrict__ src, u64 E,
+                                                               u64 N, u64* __restrict__ tcount,
+                                                               u32* __restrict__ err) {
+    __shared__ u32 tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    const u64 b0 = (u64)blockIdx.x * LT_LINES + (u64)threadIdx.x * LT_PER;
+    u32 f[LT_PER];
+    const u32 c = coo_starts(src, E, b0, f);
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j)
+        if (b0 + j < E && src[b0 + j] >= N) bad = true;
+    if (bad) atomicOr(err, UPE_BAD);
+    if (c) atomicAdd(&tot, c);
+    __syncthreads();
+    if (threadIdx.x == 0) tcount[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(LT_THREADS) void coo_rows_kernel(const u32* __restrict__ src, u64 E,
+                                                              u64 N, const u64* __restrict__ toff,
+                                                              u32* __restrict__ seen,
+                                                              u64* __restrict__ row_ptr,
+                                                              u32* __restrict__ row_node,
+                                                              u32* __restrict__ err) {
+    __shared__ u32 wsum[LT_THREADS / 64];
+    const u64 b0 = (u64)blockIdx.x * LT_LINES + (u64)threadIdx.x * LT_PER;
+    u32 f[LT_PER];
+    const u32 c = coo_starts(src, E, b0, f);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    u32 x = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    u32 off = 0;
+#pragma unroll
+    for (int w = 0; w < LT_THREADS / 64; ++w)
+        if (w < wave) off += wsum[w];
+    u64 p = toff[blockIdx.x] + off + x - c;
+#pragma unroll
+    for (int j = 0; j < LT_PER; ++j) {
+        if (!f[j]) continue;
+        const u64 e = b0 + j;
+        const u32 sv = src[e];
+        row_ptr[p] = e;
+        row_node[p] = sv;
+        ++p;
+        // a source starting two groups: its lines are not contiguous
+        if (sv < N && atomicAdd(&seen[sv], 1u) != 0) atomicOr(err, UPE_BAD);
+    }
+}
+__global__ void dst_check_kernel(const u32* __restrict__ dst, u64 E, u64 N, u32* __restrict__ err) {
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E && dst[e] >= N) atomicOr(err, UPE_BAD);
+}
+__global__ void set_u64_kernel(u64* p, u64 v) { *p = v; }
+}  // namespace
+
+// Rows of the uploaded lines into raw (row_ptr, row_node, U); on a bad
+// input, RBSC_ERR_INPUT with the host check's message.
+static int rows_on_device(rbsc_ctx* ctx, const rbsc_graph* gr, RawGraph* raw) {
+    hipStream_t s = ctx->stream;
+    const u64 N = gr->n_nodes, E = gr->n_edges;
+    u64 *d_ro = nullptr, *flag = nullptr, *tmp = nullptr, *d_tot = nullptr;
+    u32 *d_src = nullptr, *seen = nullptr, *d_err = nullptr;
+    u64 h_tot = 0;
+    u32 h_err = 0;
+    int rc = RBSC_OK;
+#define U_TRY(expr)                                                                  \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            rbsc_set_error("%s failed: %s", #expr, hipGetErrorString(_e));           \
+            rc = _e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;          \
+            goto done;                                                               \
+        }                                                                            \
+    } while (0)
+    U_TRY(hipMalloc(&d_err, sizeof(u32)));
+    U_TRY(hipMalloc(&d_tot, sizeof(u64)));
+    U_TRY(hipMemsetAsync(d_err, 0, sizeof(u32), s));
+    if (gr->row_offsets) {
+        U_TRY(hipMalloc(&d_ro, sizeof(u64) * (N + 1)));
+        U_TRY(hipMalloc(&flag, sizeof(u64) * std::max<u64>(N, 1)));
+        U_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(N + 1)));
+        U_TRY(hipMemcpyAsync(d_ro, gr->row_offsets, sizeof(u64) * (N + 1), hipMemcpyHostToDevice, s));
+        csr_flag_kernel<<<grid_for(std::max<u64>(N, 1)), BLK, 0, s>>>(d_ro, N, E, flag, d_err);
+        if ((rc = scan_excl_u64(flag, flag, N, d_tot, tmp, s))) goto done;
+        U_TRY(hipMemcpyAsync(&h_tot, d_tot, sizeof(u64), hipMemcpyDeviceToHost, s));
+        U_TRY(hipMemcpyAsync(&h_err, d_err, sizeof(u32), hipMemcpyDeviceToHost, s));
+        U_TRY(hipStreamSynchronize(s));
+        if (h_err) goto done;
+        raw->U = h_tot;
+        U_TRY(hipMalloc(&raw->row_ptr, sizeof(u64) * (raw->U + 1)));
+        U_TRY(hipMalloc(&raw->row_node, sizeof(u32) * std::max<u64>(raw->U, 1)));
+        if (N) csr_rows_kernel<<<grid_for(N), BLK, 0, s>>>(d_ro, flag, N, raw->row_ptr, raw->row_node);
+    } else {
+        const u64 nt = lt_tiles(E);
+        U_TRY(hipMalloc(&d_src, sizeof(u32) * std::max<u64>(E, 1)));
+        U_TRY(hipMalloc(&flag, sizeof(u64) * (nt + 1)));
+        U_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(nt + 1)));
+        U_TRY(hipMalloc(&seen, sizeof(u32) * std::max<u64>(N, 1)));
+        if (E) U_TRY(hipMemcpyAsync(d_src, gr->src, sizeof(u32) * E, hipMemcpyHostToDevice, s));
+        U_TRY(hipMemsetAsync(seen, 0, sizeof(u32) * std::max<u64>(N, 1), s));
+        U_TRY(hipMemsetAsync(flag + nt, 0, sizeof(u64), s));
+        if (nt) coo_count_kernel<<<(unsigned)nt, LT_THREADS, 0, s>>>(d_src, E, N, flag, d_err);
+        if ((rc = scan_excl_u64(flag, flag, nt + 1, nullptr, tmp, s))) goto done;
+        U_TRY(hipMemcpyAsync(&h_tot, flag + nt, sizeof(u64), hipMemcpyDeviceToHost, s));
+        U_TRY(hipMemcpyAsync(&h_err, d_err, sizeof(u32), hipMemcpyDeviceToHost, s));
+        U_TRY(hipStreamSynchronize(s));
+        if (h_err) goto done;
+        raw->U = h_tot;
+        U_TRY(hipMalloc(&raw->row_ptr, sizeof(u64) * (raw->U + 1)));
+        U_TRY(hipMalloc(&raw->row_node, sizeof(u32) * std::max<u64>(raw->U, 1)));
+        if (nt)
+            coo_rows_kernel<<<(unsigned)nt, LT_THREADS, 0, s>>>(d_src, E, N, flag, seen,
+                                                                raw->row_ptr, raw->row_node, d_err);
+    }
+    set_u64_kernel<<<1, 1, 0, s>>>(raw->row_ptr + raw->U, E);
+    if (E) dst_check_kernel<<<grid_for(E), BLK, 0, s>>>(raw->dst, E, N, d_err);
+    U_TRY(hipMemcpyAsync(&h_err, d_err, sizeof(u32), hipMemcpyDeviceToHost, s));
+    U_TRY(hipGetLastError());
+    U_TRY(hipStreamSynchronize(s));
+done:
+    for (void* p : {(void*)d_ro, (void*)flag, (void*)tmp, (void*)d_tot, (void*)d_src, (void*)seen,
+                    (void*)d_err})
+        if (p) (void)hipFree(p);
+    if (!rc && h_err) {
+        // the host check names the first bad line
+        std::vector<u64> row_ptr;
+        std::vector<u32> row_node;
+        rc = compress_host(gr, row_ptr, row_node);
+        if (!rc) {
+            rbsc_set_error("rbsc_graph_upload: malformed lines");
+            rc = RBSC_ERR_INPUT;
+        }
+    }
+    return rc;
+#undef U_TRY
+}
+
 int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out) {
     if (!ctx || !gr || !out) {
         rbsc_set_error("rbsc_graph_upload: null argument");
@@ -2097,15 +2279,16 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
         rbsc_set_error("rbsc_graph_upload: dst is null");
         return RBSC_ERR_INVALID;
     }
+    if (gr->n_edges && !gr->row_offsets && !gr->src) {
+        rbsc_set_error("graph needs src or row_offsets");
+        return RBSC_ERR_INVALID;
+    }
     HIP_TRY(hipSetDevice(ctx->device));
-    std::vector<u64> row_ptr;
-    std::vector<u32> row_node;
-    int rc = compress_host(gr, row_ptr, row_node);
-    if (rc) return rc;
-    const u64 U = row_node.size();
+    int rc = RBSC_OK;
     const u64 N = gr->n_nodes, E = gr->n_edges;
     const u64 line_bytes = lines_on_host ? 0 : 4ull * E + (gr->weight ? 8ull * E : 0);
-    const u64 need = 8 * (U + 1) + 8 * N + line_bytes + E / 8 + 48 * N;
+    // rows are at most the nodes
+    const u64 need = 8 * (N + 1) + 8 * N + line_bytes + E / 8 + 48 * N;
     if (ctx->budget && need > ctx->budget) {
         rbsc_set_error("graph needs %llu bytes, budget is %llu", (unsigned long long)need,
                        (unsigned long long)ctx->budget);
@@ -2115,26 +2298,26 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
     RawGraph raw;
     raw.N = N;
     raw.E = E;
-    raw.U = U;
     auto fail = [&](hipError_t e, const char* what) {
         rbsc_set_error("%s failed: %s", what, hipGetErrorString(e));
         raw_release(&raw);
         return e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;
     };
     hipError_t e;
-    if ((e = hipMalloc(&raw.row_ptr, sizeof(u64) * (U + 1))) != hipSuccess) return fail(e, "hipMalloc");
-    if ((e = hipMalloc(&raw.row_node, sizeof(u32) * std::max<u64>(U, 1))) != hipSuccess) return fail(e, "hipMalloc");
     if ((e = hipMalloc(&raw.dst, sizeof(u32) * std::max<u64>(E, 1))) != hipSuccess) return fail(e, "hipMalloc");
     if (gr->weight && (e = hipMalloc(&raw.wt, sizeof(u64) * std::max<u64>(E, 1))) != hipSuccess)
         return fail(e, "hipMalloc");
     if (gr->init_color && (e = hipMalloc(&raw.init, sizeof(u32) * std::max<u64>(N, 1))) != hipSuccess)
         return fail(e, "hipMalloc");
-    if ((e = hipMemcpyAsync(raw.row_ptr, row_ptr.data(), 8 * (U + 1), hipMemcpyHostToDevice, s)) ||
-        (U && (e = hipMemcpyAsync(raw.row_node, row_node.data(), 4 * U, hipMemcpyHostToDevice, s))) ||
-        (E && (e = hipMemcpyAsync(raw.dst, gr->dst, 4 * E, hipMemcpyHostToDevice, s))) ||
+    if ((E && (e = hipMemcpyAsync(raw.dst, gr->dst, 4 * E, hipMemcpyHostToDevice, s))) ||
         (E && gr->weight && (e = hipMemcpyAsync(raw.wt, gr->weight, 8 * E, hipMemcpyHostToDevice, s))) ||
         (N && gr->init_color && (e = hipMemcpyAsync(raw.init, gr->init_color, 4 * N, hipMemcpyHostToDevice, s))))
         return fail(e, "hipMemcpyAsync");
+    rc = rows_on_device(ctx, gr, &raw);
+    if (rc) {
+        raw_release(&raw);
+        return rc;
+    }
     rbsc_dgraph* g = new rbsc_dgraph();
     rc = dgraph_build(ctx, &raw, g);
     raw_release(&raw);

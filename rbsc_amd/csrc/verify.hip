@@ -48,9 +48,16 @@ __global__ void vfill_u32(u32* p, u64 n, u32 v) {
     if (i < n) p[i] = v;
 }
 
+// rep[c] = least node of colour c.  The recorded value is read first: the
+// nodes of a large class (all nodes without lines share one colour) mostly
+// find a smaller id there and skip the atomic instead of queuing on one
+// address.
 __global__ void vrep_kernel(const u32* __restrict__ col, u64 N, u32* __restrict__ rep) {
     const u64 u = (u64)blockIdx.x * VB + threadIdx.x;
-    if (u < N) atomicMin(&rep[col[u]], (u32)u);
+    if (u >= N) return;
+    u32* r = &rep[col[u]];
+    if ((u32)u < __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(r, (u32)u);
 }
 
 // Nodes with and without lines never share a block; nor do initial colours.
