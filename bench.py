@@ -101,7 +101,8 @@ def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
     log(f"[bench] batch: {n_graphs} graphs, {hb.n_nodes} nodes, {len(hb.dst)} lines, "
         f"built and uploaded in {time.perf_counter() - t0:.1f}s")
     threads, wgs, tune_ms = db.autotune()
-    log(f"[bench] batch autotune: {threads} threads x {wgs} workgroups ({tune_ms:.1f} ms)")
+    launch = db.launch_info
+    log(f"[bench] batch autotune: {launch} ({tune_ms:.1f} ms)")
     db.refine(labels=True, pinned=True)
     reps = 5
     t = time.perf_counter()
@@ -128,7 +129,10 @@ def batch_leg(ctx, n_graphs: int, seed: int, loop_sample: int) -> dict:
             "graphs": n_graphs, "nodes": hb.n_nodes, "lines": int(len(hb.dst)),
             "graphs_per_s": n_graphs / wall, "ms_per_batch": wall * 1e3,
             "device_ms_per_batch": dev / reps, "blocks": blocks,
-            "launch": {"threads": threads, "workgroups": wgs},
+            "launch": {"threads": threads, "workgroups": wgs,
+                       "lds_sort_digit_bits_per_tier": launch["digit_bits"],
+                       "graphs_per_tier": launch["tier_graphs"],
+                       "autotuned": True},
             "per_graph_loop_graphs_per_s": loop_rate,
             "per_graph_loop_sample": loop_sample,
             "speedup_vs_per_graph_loop": (n_graphs / wall) / loop_rate}

@@ -332,11 +332,20 @@ int rbsc_batch_refine_ex(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts,
 int rbsc_batch_refine_device(rbsc_ctx* ctx, const rbsc_batch* batch, uint64_t n_nodes,
                              uint64_t n_lines, const rbsc_opts* opts, uint32_t* blocks,
                              uint32_t* rounds, uint32_t* labels, uint32_t* colors);
-/* Times the persistent launch at workgroup widths 256/512/1024 and 1/2/4
- * workgroups per CU on this batch and keeps the fastest for later refines. */
+/* Tunes the persistent launches on this batch and keeps the fastest for
+ * later refines: the largest graphs' workgroup width (256/512/1024) and
+ * count (1/2/4 per CU), then for each of the four size tiers the digit
+ * width of its LDS radix sort (8 bits: 4 passes over the 32 sorted
+ * signature bits; or 4 bits: 8 passes with 16-digit histograms).
+ * best_ms_out: device time of one launch with the chosen parameters. */
 int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* threads_out, uint32_t* wgs_out,
                         double* best_ms_out);
 int rbsc_batch_set_launch(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t threads, uint32_t wgs);
+/* info[10]: workgroup width and count of the largest tier, digit bits of
+ * tiers 0..3 (largest graphs first), graphs in tiers 0..3. */
+int rbsc_batch_launch_info(const rbsc_dbatch* b, uint32_t* info);
+/* Digit bits (4 or 8) of the LDS radix sort of tiers 0..3. */
+int rbsc_batch_set_digits(rbsc_dbatch* b, const uint32_t* digit_bits);
 /* One-shot: upload, refine, free. */
 int rbsc_refine_batch(rbsc_ctx* ctx, const rbsc_batch* batch, const rbsc_opts* opts,
                       uint32_t* blocks_out, uint32_t* rounds_out, uint32_t* labels_out,

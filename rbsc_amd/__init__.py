@@ -462,6 +462,19 @@ class DeviceBatch:
     def set_launch(self, threads: int, wgs: int):
         check(load().rbsc_batch_set_launch(self.ctx.handle, self.handle, threads, wgs))
 
+    def set_digits(self, digit_bits):
+        """LDS radix digit width (4 or 8 bits) of each size tier, largest first."""
+        v = (C.c_uint32 * 4)(*[int(x) for x in digit_bits])
+        check(load().rbsc_batch_set_digits(self.handle, v))
+
+    @property
+    def launch_info(self) -> dict:
+        """The persistent launches' parameters (set or autotuned)."""
+        v = (C.c_uint32 * 10)()
+        check(load().rbsc_batch_launch_info(self.handle, v))
+        return {"threads": int(v[0]), "workgroups": int(v[1]),
+                "digit_bits": [int(x) for x in v[2:6]], "tier_graphs": [int(x) for x in v[6:10]]}
+
     def free(self):
         if self._pinned is not None:
             self._pinned.free()

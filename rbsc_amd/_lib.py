@@ -199,6 +199,8 @@ SIGNATURES = {
                                            C.c_void_p]),
     "rbsc_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, u32p, u32p, C.POINTER(C.c_double)]),
     "rbsc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "rbsc_batch_launch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "rbsc_batch_set_digits": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "rbsc_refine_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(Opts), u32p, u32p, u32p,
                                     C.POINTER(Result)]),
     "rbsc_bvgraph_load": (C.c_int, [C.c_char_p, C.POINTER(HostGraph)]),

@@ -32,16 +32,18 @@ struct BatchDevice {
     unsigned short* dst = nullptr;  // local heads
     u64* wt = nullptr;              // or null
     u32* init = nullptr;            // per node (batch numbering) or null
+    int rb[4] = {8, 8, 8, 8};       // LDS radix digit bits per tier (8 or 4; autotuned)
 };
 
 struct rbsc_ctx;
 // split_lo: the signatures are sorted by their bits [split_lo, 64) and runs
 // colliding there ordered afterwards (32; 48 is a verification hook).
+// tiers: bit t launches size tier t (the autotuner times them one by one).
 // d_sizes (nullable): per batch node, each graph's block sizes in
 // descending order, then zeros.
 int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
                  u32* d_sizes, u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg,
-                 int split_lo = 32);
+                 int split_lo = 32, u32 tiers = 0xFu);
 // Looks up, once per device (that device current), the device address of
 // the fixed scratch array of rbsc_batch_refine_device.
 int batch_device_scratch_init(int device);

@@ -73,11 +73,14 @@ __device__ u32 block_scan_excl(u32 v, Lds<BT, NM>* s, u32* tot) {
     return off + x - v;
 }
 
-// One stable LSD pass over n items by digit (key >> shift) & 255, from
-// buffer `in` to buffer 1 - in.  Wave w owns items [w*IPW, (w+1)*IPW).
-template <int BT, int NM>
+// One stable LSD pass over n items by digit (key >> shift) & (2^RB - 1)
+// (RB = 8 or 4 bits: the autotuner picks per size tier), from buffer `in`
+// to buffer 1 - in.  Wave w owns items [w*IPW, (w+1)*IPW).
+template <int BT, int NM, int RB>
 __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
     constexpr int BW = Cfg<BT, NM>::BW, IPW = Cfg<BT, NM>::IPW, WIT = Cfg<BT, NM>::WIT;
+    constexpr int ND = 1 << RB;
+    constexpr u32 DM = (u32)ND - 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < BW * 256; i += BT) (&s->whist[0][0])[i] = 0;
     __syncthreads();
@@ -87,10 +90,10 @@ __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
     for (int it = 0; it < WIT; ++it) {
         const int idx = wave * IPW + it * 64 + lane;
         const bool valid = idx < n && (it * 64 + lane) < IPW;
-        const u32 d = valid ? (u32)(s->key[in][idx] >> shift) & 255u : 0u;
+        const u32 d = valid ? (u32)(s->key[in][idx] >> shift) & DM : 0u;
         u64 peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const u64 bal = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
@@ -103,15 +106,15 @@ __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
     }
     __syncthreads();
     // digit-major offsets: for digit d, waves in order; thread t takes the
-    // digits [t * DPT, t * DPT + DPT) (one each from 256 threads up)
-    constexpr int DPT = (256 + BT - 1) / BT;
+    // digits [t * DPT, t * DPT + DPT) (one each from ND threads up)
+    constexpr int DPT = (ND + BT - 1) / BT;
     u32 tot_d[DPT];
     u32 tsum = 0;
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
         const int d = (int)threadIdx.x * DPT + k;
         tot_d[k] = 0;
-        if (d < 256) {
+        if (d < ND) {
 #pragma unroll
             for (int w = 0; w < BW; ++w) {
                 const u32 c = s->whist[w][d];
@@ -126,7 +129,7 @@ __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
         const int d = (int)threadIdx.x * DPT + k;
-        if (d < 256) {
+        if (d < ND) {
 #pragma unroll
             for (int w = 0; w < BW; ++w) s->whist[w][d] += run;
             run += tot_d[k];
@@ -138,7 +141,7 @@ __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
         const int idx = wave * IPW + it * 64 + lane;
         if (idx < n && (it * 64 + lane) < IPW) {
             const u64 k = s->key[in][idx];
-            const u32 d = (u32)(k >> shift) & 255u;
+            const u32 d = (u32)(k >> shift) & DM;
             const u32 pos = s->whist[wave][d] + rank[it];
             s->key[1 - in][pos] = k;
             s->val[1 - in][pos] = s->val[in][idx];
@@ -147,17 +150,18 @@ __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
     __syncthreads();
 }
 
-template <int BT, int NM>
+template <int BT, int NM, int RB>
 __device__ void lds_sort(Lds<BT, NM>* s, int n, int bit_begin, int bit_end) {
     int in = 0;
-    for (int sh = bit_begin; sh < bit_end; sh += 8) {
-        lds_radix_pass(s, in, n, sh);
+    for (int sh = bit_begin; sh < bit_end; sh += RB) {
+        lds_radix_pass<BT, NM, RB>(s, in, n, sh);
         in ^= 1;
     }
-    // callers use an even number of passes: result is back in buffer 0
+    // callers sort 32 or 16 bits, an even number of passes for either digit
+    // width: the result is back in buffer 0
 }
 
-template <int BT, int NM, bool HAS_WT, bool HAS_INIT>
+template <int BT, int NM, bool HAS_WT, bool HAS_INIT, int RB>
 __global__ __launch_bounds__(BT) void batch_refine_kernel(
     const BatchDesc* __restrict__ desc, const u32* __restrict__ order, u32 n_jobs,
     const u32* __restrict__ range, const u32* __restrict__ skip, const u32* __restrict__ rp_all, const unsigned short* __restrict__ dst_all,
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
                 s.val[0][u] = (u32)u;
             }
             __syncthreads();
-            lds_sort(&s, n, 0, 32);
+            lds_sort<BT, NM, RB>(&s, n, 0, 32);
             // dense rank of the initial colours, in position order
             u32 f[IPT];
             u32 cnt = 0;
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(BT) void batch_refine_kernel(
             // pairs a round -- are put in order by thread 0 with a stable
             // insertion pass over the full keys (inversions exist only
             // inside such runs).
-            lds_sort(&s, n, split_lo, 64);
+            lds_sort<BT, NM, RB>(&s, n, split_lo, 64);
             for (int i = threadIdx.x + 1; i < n; i += BT) {
                 const u64 a = s.key[0][i - 1], b = s.key[0][i];
                 if ((a >> split_lo) == (b >> split_lo) && a != b) s.scal[1] = 1;
@@ -434,19 +438,23 @@ __global__ void zero_queue(u32* q) {
 template <int BT, int NM>
 static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 j0, u32 j1,
                       u32 max_rounds, u32* d_labels, u32* d_colors, u32* d_sizes, u32* d_blocks,
-                      u32* d_rounds, u32* q, int split_lo, const u32* range = nullptr,
+                      u32* d_rounds, u32* q, int split_lo, int rb, const u32* range = nullptr,
                       const u32* skip = nullptr) {
-#define BL(W, I)                                                                              \
-    batch_refine_kernel<BT, NM, W, I><<<grid, BT, 0, s>>>(b.desc, b.order + j0, j1 - j0, range, \
-                                                          skip, b.rp,                           \
-                                                          b.dst, b.wt, b.init, max_rounds,      \
-                                                          d_labels, d_colors, d_sizes,          \
-                                                          d_blocks, d_rounds, q, split_lo)
+#define BL(W, I, R)                                                                           \
+    batch_refine_kernel<BT, NM, W, I, R><<<grid, BT, 0, s>>>(                                   \
+        b.desc, b.order + j0, j1 - j0, range, skip, b.rp, b.dst, b.wt, b.init, max_rounds,      \
+        d_labels, d_colors, d_sizes, d_blocks, d_rounds, q, split_lo)
+#define BLR(W, I)                   \
+    do {                            \
+        if (rb == 4) BL(W, I, 4);   \
+        else BL(W, I, 8);           \
+    } while (0)
     if (b.wt) {
-        if (b.init) BL(true, true); else BL(true, false);
+        if (b.init) BLR(true, true); else BLR(true, false);
     } else {
-        if (b.init) BL(false, true); else BL(false, false);
+        if (b.init) BLR(false, true); else BLR(false, false);
     }
+#undef BLR
 #undef BL
 }
 
@@ -454,42 +462,42 @@ static void launch_bt(hipStream_t s, unsigned grid, const BatchDevice& b, u32 j0
 
 int batch_launch(rbsc_ctx* ctx, const BatchDevice& b, u32 max_rounds, u32* d_labels, u32* d_colors,
                  u32* d_sizes, u32* d_blocks, u32* d_rounds, u32* d_queue, int threads, int n_wg,
-                 int split_lo) {
+                 int split_lo, u32 tiers) {
     hipStream_t s = ctx->stream;
     zero_queue<<<1, 1, 0, s>>>(d_queue);
     if (b.n_small == 0) return RBSC_OK;
     // graphs above BATCH_NMID nodes: the tuned shape and the full layout
-    if (b.n_hi) {
+    if (b.n_hi && (tiers & 1u)) {
         const unsigned grid = (unsigned)std::max(1, std::min<int>(n_wg, (int)b.n_hi));
         switch (threads) {
-            case 256: launch_bt<256, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo); break;
-            case 1024: launch_bt<1024, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo); break;
-            default: launch_bt<512, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo); break;
+            case 256: launch_bt<256, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo, b.rb[0]); break;
+            case 1024: launch_bt<1024, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo, b.rb[0]); break;
+            default: launch_bt<512, BATCH_NMAX>(s, grid, b, 0, b.n_hi, max_rounds, d_labels, d_colors, d_sizes, d_blocks, d_rounds, d_queue, split_lo, b.rb[0]); break;
         }
     }
     // graphs above BATCH_NLOW nodes: half the layout, two workgroups per CU
-    if (b.n_mid > b.n_hi) {
+    if (b.n_mid > b.n_hi && (tiers & 2u)) {
         const unsigned grid =
             (unsigned)std::max(1, std::min<int>(2 * b.cus, (int)(b.n_mid - b.n_hi)));
         launch_bt<MID_THREADS, BATCH_NMID>(s, grid, b, b.n_hi, b.n_mid, max_rounds, d_labels,
                                            d_colors, d_sizes, d_blocks, d_rounds, d_queue + 1,
-                                           split_lo);
+                                           split_lo, b.rb[1]);
     }
     // graphs above BATCH_NTINY nodes: a quarter of the layout, four per CU
-    if (b.n_low > b.n_mid) {
+    if (b.n_low > b.n_mid && (tiers & 4u)) {
         const unsigned grid =
             (unsigned)std::max(1, std::min<int>(4 * b.cus, (int)(b.n_low - b.n_mid)));
         launch_bt<LOW_THREADS, BATCH_NLOW>(s, grid, b, b.n_mid, b.n_low, max_rounds, d_labels,
                                            d_colors, d_sizes, d_blocks, d_rounds, d_queue + 2,
-                                           split_lo);
+                                           split_lo, b.rb[2]);
     }
     // the others: an eighth of the layout, eight workgroups per CU
-    if (b.n_small > b.n_low) {
+    if (b.n_small > b.n_low && (tiers & 8u)) {
         const unsigned grid =
             (unsigned)std::max(1, std::min<int>(8 * b.cus, (int)(b.n_small - b.n_low)));
         launch_bt<TINY_THREADS, BATCH_NTINY>(s, grid, b, b.n_low, b.n_small, max_rounds, d_labels,
                                              d_colors, d_sizes, d_blocks, d_rounds, d_queue + 3,
-                                             split_lo);
+                                             split_lo, b.rb[3]);
     }
     HIP_TRY(hipGetLastError());
     return RBSC_OK;
@@ -791,6 +799,25 @@ extern "C" int rbsc_batch_refine_ex(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_op
                      b->threads, b->wgs);
 }
 
+// Time one launch of the tiers in `tiers` (second of two runs; the first
+// warms the caches).
+static int time_tiers(rbsc_ctx* ctx, rbsc_dbatch* b, int threads, int wgs, u32 tiers, double* ms_out) {
+    double sum = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        const int rc = batch_launch(ctx, b->dev, 0, b->d_labels, nullptr, nullptr, b->d_blocks,
+                                    b->d_rounds, b->d_queue, threads, wgs, 32, tiers);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (rep) sum += ms;
+    }
+    *ms_out = sum;
+    return RBSC_OK;
+}
+
 extern "C" int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* threads_out,
                                    uint32_t* wgs_out, double* best_ms_out) {
     if (!ctx || !b) {
@@ -800,35 +827,73 @@ extern "C" int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* thre
     HIP_TRY(hipSetDevice(ctx->device));
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    // 1. the largest tier's workgroup width and count
     double best = 1e30;
     int bt = b->threads, bw = b->wgs;
     for (int threads : {256, 512, 1024}) {
         for (int mult : {1, 2, 4}) {
-            const int wgs = cus * mult;
-            double ms_sum = 0;
-            for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
-                HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-                int rc = batch_launch(ctx, b->dev, 0, b->d_labels, nullptr, nullptr, b->d_blocks,
-                                      b->d_rounds, b->d_queue, threads, wgs);
-                if (rc) return rc;
-                HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-                HIP_TRY(hipEventSynchronize(ctx->ev1));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-                if (rep) ms_sum += ms;
-            }
-            if (ms_sum < best) {
-                best = ms_sum;
+            double ms = 0;
+            const int rc = time_tiers(ctx, b, threads, cus * mult, 1u, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
                 bt = threads;
-                bw = wgs;
+                bw = cus * mult;
             }
         }
     }
     b->threads = bt;
     b->wgs = bw;
+    // 2. the LDS sort's digit width (8 or 4 bits) of each tier
+    for (int t = 0; t < 4; ++t) {
+        double ms_best = 1e30;
+        int rb_best = b->dev.rb[t];
+        for (int rb : {8, 4}) {
+            b->dev.rb[t] = rb;
+            double ms = 0;
+            const int rc = time_tiers(ctx, b, bt, bw, 1u << t, &ms);
+            if (rc) return rc;
+            if (ms < ms_best) {
+                ms_best = ms;
+                rb_best = rb;
+            }
+        }
+        b->dev.rb[t] = rb_best;
+    }
+    double total = 0;
+    const int rc = time_tiers(ctx, b, bt, bw, 0xFu, &total);
+    if (rc) return rc;
     if (threads_out) *threads_out = (uint32_t)bt;
     if (wgs_out) *wgs_out = (uint32_t)bw;
-    if (best_ms_out) *best_ms_out = best;
+    if (best_ms_out) *best_ms_out = total;
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_batch_launch_info(const rbsc_dbatch* b, uint32_t* info) {
+    if (!b || !info) {
+        rbsc_set_error("rbsc_batch_launch_info: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    info[0] = (uint32_t)b->threads;
+    info[1] = (uint32_t)b->wgs;
+    for (int t = 0; t < 4; ++t) info[2 + t] = (uint32_t)b->dev.rb[t];
+    const u32 n[4] = {b->dev.n_hi, b->dev.n_mid - b->dev.n_hi, b->dev.n_low - b->dev.n_mid,
+                      b->dev.n_small - b->dev.n_low};
+    for (int t = 0; t < 4; ++t) info[6 + t] = n[t];
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_batch_set_digits(rbsc_dbatch* b, const uint32_t* digit_bits) {
+    if (!b || !digit_bits) {
+        rbsc_set_error("rbsc_batch_set_digits: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    for (int t = 0; t < 4; ++t)
+        if (digit_bits[t] != 4 && digit_bits[t] != 8) {
+            rbsc_set_error("rbsc_batch_set_digits: digit widths must be 4 or 8 bits");
+            return RBSC_ERR_INVALID;
+        }
+    for (int t = 0; t < 4; ++t) b->dev.rb[t] = (int)digit_bits[t];
     return RBSC_OK;
 }
 
@@ -1060,13 +1125,13 @@ extern "C" int rbsc_batch_refine_device(rbsc_ctx* ctx, const rbsc_batch* batch, 
     const u32* rg = ctl + NDC_RANGE;
     const int cus = b.cus;
     launch_bt<512, BATCH_NMAX>(s, cus, b, 0, 0, max_rounds, labels, colors, nullptr, blocks,
-                               rounds, q, split_lo, rg, err);
+                               rounds, q, split_lo, b.rb[0], rg, err);
     launch_bt<MID_THREADS, BATCH_NMID>(s, 2 * cus, b, 0, 0, max_rounds, labels, colors, nullptr,
-                                       blocks, rounds, q + 1, split_lo, rg + 2, err);
+                                       blocks, rounds, q + 1, split_lo, b.rb[1], rg + 2, err);
     launch_bt<LOW_THREADS, BATCH_NLOW>(s, 4 * cus, b, 0, 0, max_rounds, labels, colors, nullptr,
-                                       blocks, rounds, q + 2, split_lo, rg + 4, err);
+                                       blocks, rounds, q + 2, split_lo, b.rb[2], rg + 4, err);
     launch_bt<TINY_THREADS, BATCH_NTINY>(s, 8 * cus, b, 0, 0, max_rounds, labels, colors, nullptr,
-                                         blocks, rounds, q + 3, split_lo, rg + 6, err);
+                                         blocks, rounds, q + 3, split_lo, b.rb[3], rg + 6, err);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 14, err, sizeof(u32), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

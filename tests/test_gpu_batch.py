@@ -206,9 +206,39 @@ def test_batch_autotune_keeps_results(ctx):
     ref = db.refine(labels=True, colors=True)
     t, w, ms = db.autotune()
     assert t in (256, 512, 1024) and w >= 1 and ms > 0
+    info = db.launch_info
+    assert (info["threads"], info["workgroups"]) == (t, w)
+    assert all(d in (4, 8) for d in info["digit_bits"])
+    assert sum(info["tier_graphs"]) == b.n_graphs
     res = db.refine(labels=True, colors=True)
     assert np.array_equal(res.colors, ref.colors)
     assert np.array_equal(res.labels, ref.labels)
+    db.free()
+
+
+@pytest.mark.parametrize("digits", [(4, 4, 4, 4), (8, 4, 8, 4), (4, 8, 4, 8)])
+def test_batch_lds_sort_digit_widths(ctx, digits):
+    # the LDS radix sort with 4-bit digits (8 passes) in some tiers and
+    # 8-bit digits (4 passes) in others: every graph's colours, rounds and
+    # labels equal the oracle's, also under the 16-bit split (flag 8), and
+    # with initial colours and weights (their first sort is by colour)
+    b = gen.batch_mixed(300, 10, 5000, seed=42)
+    db = ctx.upload_batch(b)
+    db.set_digits(digits)
+    assert db.launch_info["digit_bits"] == list(digits)
+    for flags in (0, 8):
+        res = db.refine(labels=True, colors=True, flags=flags)
+        _check_against_oracle(b, res)
+    db.free()
+    gs = _mixed_graphs(13)[:40]
+    rng = np.random.default_rng(14)
+    for g in gs:
+        g.weight = rng.integers(1, 4, g.n_edges).astype(np.uint64)
+        g.init_color = rng.integers(0, 3, g.n_nodes).astype(np.uint32)
+    hb = HostBatch.from_graphs(gs)
+    db = ctx.upload_batch(hb)
+    db.set_digits(digits)
+    _check_against_oracle(hb, db.refine(labels=True, colors=True))
     db.free()
 
 
