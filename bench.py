@@ -78,6 +78,32 @@ def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
     return out
 
 
+# The reference's rsc (src/rsc/rsc.cpp, compiled from source, one core) on
+# the full workload, once: its own timer, its block count and the host, as
+# recorded in profiles/rsc_reference_rmat_s27_d31_c20.md.  Too long (about 15
+# minutes of refinement plus 40 GB of text) for every bench run, so the live
+# leg above times a bounded sample of the same family.
+RSC_FULL_SIZE = {
+    "rmat_s27_d31_c20": {"seconds": 862.642872, "blocks": 31824702, "done": 1, "cores": 1,
+                         "cpu_model": "Intel(R) Xeon(R) Processor @ 2.70GHz",
+                         "record": "profiles/rsc_reference_rmat_s27_d31_c20.md"},
+}
+
+
+def full_size_reference(config: str, blocks: int, refine_s: float, one_shot_s) -> dict | None:
+    """The recorded full-size rsc run beside this run's time to the fixpoint
+    (one refinement from resident lines, and the drop-in's one-shot span)."""
+    rec = RSC_FULL_SIZE.get(config)
+    if rec is None:
+        return None
+    out = dict(rec)
+    out["blocks_agree"] = rec["blocks"] == blocks
+    out["speedup_refine"] = rec["seconds"] / refine_s
+    if one_shot_s:
+        out["speedup_one_shot"] = rec["seconds"] / one_shot_s
+    return out
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -519,6 +545,10 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_scale + 4, 0.20, seed)
+        full = full_size_reference(args.config, r.nodes, wall / args.steps,
+                                   out.get("one_shot", {}).get("time_s"))
+        if full is not None:
+            out["cpu_baseline"]["full_size"] = full
     dg.free()
     ctx.close()
     print(json.dumps(out), flush=True)
