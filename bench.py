@@ -335,6 +335,29 @@ def one_shot_leg(ctx, dg) -> dict:
             "download_s": dl}
 
 
+def quotient_leg(ctx) -> dict:
+    """The reference's multi-batch quotient mode (rbsc.cu:274-487 and the
+    stop rule of rbsc_aggressive.cu:487, its cuda_50 / cuda_25 columns) on
+    the 2^20-node R-MAT with 30% clones, from host arrays (rbsc_refine: the
+    one upload, every pass's batches and contraction on the device, the
+    canonical partition back)."""
+    from rbsc_amd import gen
+    g = gen.rmat_2p20_clone30()
+    ctx.refine(g, max_batch_edges=g.n_edges // 2, mode="quotient")  # warm
+    out = {"workload": f"rmat_2p20_clone30: {g.n_nodes} nodes, {g.n_edges} lines", "runs": []}
+    for frac in (2, 4):
+        for mode in ("quotient", "aggressive"):
+            B = g.n_edges // frac
+            t = time.perf_counter()
+            r = ctx.refine(g, max_batch_edges=B, mode=mode)
+            wall = time.perf_counter() - t
+            out["runs"].append({"batch_frac": 1 / frac, "mode": mode, "max_batch_edges": B,
+                                "time_s": r.time_s, "wall_s": wall, "device_ms": r.device_ms,
+                                "nodes": r.nodes, "done": r.done, "passes": r.passes,
+                                "rounds": r.rounds})
+    return out
+
+
 def copy_probe(nbytes: int) -> None:
     """One plain device-to-device copy of nbytes through the HIP runtime, so a
     counter run can read FETCH_SIZE / WRITE_SIZE of a kernel of known size."""
@@ -392,6 +415,8 @@ def main():
                     help="worker threads (contexts) of the serving loop")
     ap.add_argument("--stream-frac", type=float, default=0.25,
                     help="also time the pinned-host streaming path at this edge-batch fraction (0 skips)")
+    ap.add_argument("--quotient", type=int, default=1,
+                    help="time the multi-batch quotient mode at 50%% / 25%% batches (0 = skip)")
     ap.add_argument("--one-shot", type=int, default=1,
                     help="also time the drop-in's whole run (rbsc_refine from host arrays)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -531,6 +556,9 @@ def main():
     if args.one_shot:
         out["one_shot"] = one_shot_leg(ctx, dg)
         log(f"[bench] one-shot: {out['one_shot']}")
+    if args.quotient:
+        out["quotient"] = quotient_leg(ctx)
+        log(f"[bench] quotient: {out['quotient']}")
     if args.batch_graphs:
         out["batched"] = batch_leg(ctx, args.batch_graphs, 1004, 500)
         log(f"[bench] batched: {out['batched']['graphs_per_s']:.0f} graphs/s "

@@ -63,7 +63,7 @@ def _declare(L):
                                       u32p, u32p]
     L.oracle_rbsc_multibatch.restype = C.c_int
     L.oracle_rbsc_multibatch.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, C.c_uint64,
-                                         C.c_int, u32p, u32p, u32p]
+                                         C.c_int, u32p, u32p, u32p, u32p]
 
 
 def _p(a, t):
@@ -137,19 +137,23 @@ def exact_refine(n, src, dst, weight=None, init=None):
     return colors[:n], int(K.value), int(R.value)
 
 
-def rbsc_multibatch(n, src, dst, weight, max_batch, aggressive=False):
+def rbsc_multibatch(n, src, dst, weight, max_batch, aggressive=False, partition=False):
     """Restatement of rbsc.cu's multi-batch quotient passes.
-    Returns (nodes, done, passes)."""
+    Returns (nodes, done, passes), and with partition=True also the
+    canonical partition of the original nodes into their final classes."""
     src, dst, w = _arrs(src, dst, weight)
     nodes = C.c_uint32()
     done = C.c_uint32()
     passes = C.c_uint32()
+    part = np.empty(max(n, 1), np.uint32) if partition else None
     rc = lib().oracle_rbsc_multibatch(n, len(dst), _p(src, C.c_uint32), _p(w, C.c_uint64),
                                       _p(dst, C.c_uint32), max_batch, int(aggressive),
-                                      C.byref(nodes), C.byref(done), C.byref(passes))
+                                      C.byref(nodes), C.byref(done), C.byref(passes),
+                                      _p(part, C.c_uint32))
     if rc:
         raise MemoryError("oracle_rbsc_multibatch failed")
-    return int(nodes.value), int(done.value), int(passes.value)
+    out = (int(nodes.value), int(done.value), int(passes.value))
+    return out + (part[:n],) if partition else out
 
 
 def have_ref() -> bool:

@@ -316,13 +316,18 @@ int oracle_exact_refine(u32 N, u64 E, const u32* src, const u64* wt, const u32* 
  * repeat while batches > 1 && node_n > new_node_n (487); the `aggressive`
  * variant also requires batches > ceilf(new_edge_n / (float)B)
  * (rbsc_aggressive.cu:487).  done = (batches == 1) (493).
+ * partition_out (nullable): the first-occurrence canonical form of each
+ * original node's class, the composition of its labels over the passes
+ * (the reference keeps only the node count; the composition is what a
+ * caller of the quotient would read back).
  * Inputs are copied; weights are u64 (rbsc.cu:173).  Returns 0 or -1. */
 #define MB_OFF 0
 #define MB_IN 1
 #define MB_DANG 2
 int oracle_rbsc_multibatch(u32 N0, u64 E0, const u32* src0, const u64* wt0, const u32* dst0,
                            u64 max_batch, int aggressive,
-                           u32* nodes_out, u32* done_out, u32* passes_out) {
+                           u32* nodes_out, u32* done_out, u32* passes_out,
+                           u32* partition_out) {
     u32* src = (u32*)malloc(sizeof(u32) * (E0 ? E0 : 1));
     u32* dst = (u32*)malloc(sizeof(u32) * (E0 ? E0 : 1));
     u64* wt = (u64*)malloc(sizeof(u64) * (E0 ? E0 : 1));
@@ -335,13 +340,15 @@ int oracle_rbsc_multibatch(u32 N0, u64 E0, const u32* src0, const u64* wt0, cons
     u64* col = (u64*)malloc(sizeof(u64) * NN);
     u64* sig = (u64*)malloc(sizeof(u64) * NN);
     trip_t* tmp = (trip_t*)malloc(sizeof(trip_t) * NN);
+    u32* comp = (u32*)malloc(sizeof(u32) * NN);
     if (!src || !dst || !wt || !w || !mask || !considered || !windex || !label_map ||
-        !col || !sig || !tmp) {
+        !col || !sig || !tmp || !comp) {
         free(src); free(dst); free(wt); free(w); free(mask); free(considered);
-        free(windex); free(label_map); free(col); free(sig); free(tmp);
+        free(windex); free(label_map); free(col); free(sig); free(tmp); free(comp);
         return -1;
     }
     for (u64 e = 0; e < E0; ++e) { src[e] = src0[e]; dst[e] = dst0[e]; wt[e] = wt0 ? wt0[e] : 1ULL; }
+    for (u32 i = 0; i < N0; ++i) comp[i] = i;
     u32 node_n, new_node_n = N0;
     u64 edge_n, new_edge_n = E0, batches = 1;
     u32 passes = 0;
@@ -428,6 +435,8 @@ int oracle_rbsc_multibatch(u32 N0, u64 E0, const u32* src0, const u64* wt0, cons
                 }
             }
         }
+        /* the class of every original node: its label in this pass */
+        for (u32 i = 0; i < N0; ++i) comp[i] = comp[i] < node_n ? w[comp[i]] : comp[i];
         if (!(batches > 1 && node_n > new_node_n)) break;
         if (aggressive) {
             float q = ceilf((float)new_edge_n / (float)max_batch);
@@ -437,7 +446,9 @@ int oracle_rbsc_multibatch(u32 N0, u64 E0, const u32* src0, const u64* wt0, cons
     if (nodes_out) *nodes_out = new_node_n;
     if (done_out) *done_out = batches == 1;
     if (passes_out) *passes_out = passes;
+    int rc = 0;
+    if (partition_out && N0 && oracle_canonicalize(N0, comp, partition_out) == 0) rc = -1;
     free(src); free(dst); free(wt); free(w); free(mask); free(considered);
-    free(windex); free(label_map); free(col); free(sig); free(tmp);
-    return 0;
+    free(windex); free(label_map); free(col); free(sig); free(tmp); free(comp);
+    return rc;
 }

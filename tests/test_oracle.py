@@ -204,6 +204,10 @@ def test_multibatch_reduces_to_single_batch():
     K = oracle.rbsc_refine(g.n_nodes, src, g.dst)[1]
     nodes, done, passes = oracle.rbsc_multibatch(g.n_nodes, src, g.dst, None, g.n_edges)
     assert (nodes, done, passes) == (K, 1, 1)
+    # one batch: the classes are the single-batch refinement's
+    c = oracle.rbsc_refine(g.n_nodes, src, g.dst)[0]
+    part = oracle.rbsc_multibatch(g.n_nodes, src, g.dst, None, g.n_edges, partition=True)[3]
+    assert np.array_equal(part, oracle.canonical(c))
     nodes, done, passes = oracle.rbsc_multibatch(g.n_nodes, src, g.dst, None, 10 * g.n_edges)
     assert (nodes, done) == (K, 1)
 
@@ -225,6 +229,12 @@ def test_multibatch_hand_worked():
     assert oracle.rbsc_multibatch(n, s, d, None, 2, aggressive=True) == (3, 0, 1)
     # with a batch of 4 lines everything is one batch
     assert oracle.rbsc_multibatch(n, s, d, None, 4) == (3, 1, 1)
+    # The classes: pass 1 labels node 0 -> 0, nodes 2 and 3 -> 1 (dangling,
+    # both colour 0 and signature 0 in batch 0), node 1 -> 2; pass 2 keeps
+    # every quotient node apart, so the composition is [0, 2, 1, 1],
+    # canonical [0, 1, 2, 2].
+    part = oracle.rbsc_multibatch(n, s, d, None, 2, partition=True)[3]
+    assert part.tolist() == [0, 1, 2, 2]
 
 
 def test_multibatch_group_larger_than_the_batch():
@@ -242,6 +252,8 @@ def test_multibatch_group_larger_than_the_batch():
     # batches = 2: (nodes, done, passes) = (3, 0, 1).
     n, s, d = _coo(3, [(0, 1), (0, 2), (0, 1), (1, 2)])
     assert oracle.rbsc_multibatch(n, s, d, None, 2) == (3, 0, 1)
+    # labels [1, 0, 2] in node order, canonical [0, 1, 2]
+    assert oracle.rbsc_multibatch(n, s, d, None, 2, partition=True)[3].tolist() == [0, 1, 2]
     assert oracle.rbsc_multibatch(n, s, d, None, 2, aggressive=True) == (3, 0, 1)
 
 
@@ -259,6 +271,8 @@ def test_multibatch_group_straddling_a_batch_boundary():
     # batch and gets 3 (457-461): 4 nodes from 4, stop.  (4, 0, 1).
     n, s, d = _coo(4, [(0, 1), (1, 2), (1, 3), (2, 3)])
     assert oracle.rbsc_multibatch(n, s, d, None, 2) == (4, 0, 1)
+    # labels [0, 3, 2, 1], canonical [0, 1, 2, 3]
+    assert oracle.rbsc_multibatch(n, s, d, None, 2, partition=True)[3].tolist() == [0, 1, 2, 3]
     # the coarsest partition has 4 blocks too (a path of length 3 into a
     # dangling node, and node 1 with lines into both)
     assert oracle.exact_refine(n, s, d)[1] == 4
