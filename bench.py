@@ -84,7 +84,8 @@ def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
 # minutes of refinement plus 40 GB of text) for every bench run, so the live
 # leg above times a bounded sample of the same family.
 RSC_FULL_SIZE = {
-    "rmat_s27_d31_c20": {"seconds": 862.642872, "blocks": 31824702, "done": 1, "cores": 1,
+    "rmat_s27_d31_c20": {"seconds": 862.642872, "blocks": 31824702, "done": 1, "rounds": 7,
+                         "edge_visits_per_s": 7 * 2139161279 / 862.642872, "cores": 1,
                          "cpu_model": "Intel(R) Xeon(R) Processor @ 2.70GHz",
                          "record": "profiles/rsc_reference_rmat_s27_d31_c20.md"},
 }

@@ -20,49 +20,238 @@
 //   --init-colors FILE     N initial colours, whitespace separated
 //   --verify               exact stability check of the final partition
 //   --device K             HIP device index
+//   --check-input          parse and check stdin only (no device): print
+//       "N E line-digest csr|coo unit|weighted offsets-digest" and the parse
+//       time on stderr; the digests are sums mod 2^64 of
+//       src*A + w*B + dst*C + e*D over lines e and of ro[u]*(u+1) over the
+//       row offsets (0 when the lines go over as COO)
 #include "../../include/rbsc.h"
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <thread>
+#include <unistd.h>
 #include <vector>
 
 namespace {
 
-struct Input {
-    std::vector<char> buf;
-    size_t pos = 0;
-    bool read_all(FILE* f) {
-        buf.clear();
-        const size_t chunk = 1 << 24;
+// The whole of a file descriptor's bytes: mapped when it is a regular file
+// (stdin redirected from a file: no copy, pages fault in on the parsing
+// threads), else read into an anonymous mapping grown by mremap (a pipe;
+// growth moves page tables, not bytes).
+struct Text {
+    char* p = nullptr;
+    size_t n = 0, cap = 0;
+    bool mapped_file = false;
+    bool load(int fd) {
+        struct stat st;
+        if (fstat(fd, &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0) {
+            const off_t at = lseek(fd, 0, SEEK_CUR);
+            if (at >= 0 && at < st.st_size) {
+                void* m = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+                if (m != MAP_FAILED) {
+                    p = (char*)m + at;
+                    n = (size_t)(st.st_size - at);
+                    cap = (size_t)st.st_size;
+                    mapped_file = true;
+                    return true;
+                }
+            }
+        }
+        cap = (size_t)1 << 30;
+        void* m = mmap(nullptr, cap, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+        if (m == MAP_FAILED) return false;
+        p = (char*)m;
         for (;;) {
-            const size_t old = buf.size();
-            buf.resize(old + chunk);
-            const size_t got = fread(buf.data() + old, 1, chunk, f);
-            buf.resize(old + got);
-            if (got < chunk) break;
+            if (n == cap) {
+                void* g = mremap(p, cap, cap * 2, MREMAP_MAYMOVE);
+                if (g == MAP_FAILED) return false;
+                p = (char*)g;
+                cap *= 2;
+            }
+            const ssize_t got = read(fd, p + n, cap - n);
+            if (got < 0) {
+                if (errno == EINTR) continue;
+                return false;
+            }
+            if (got == 0) return true;
+            n += (size_t)got;
         }
-        buf.push_back('\0');
-        return !ferror(f);
     }
-    // Next unsigned integer; false at end of input or on a malformed token.
-    bool next(uint64_t* out) {
-        const char* p = buf.data() + pos;
-        while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') ++p;
-        if (*p < '0' || *p > '9') return false;
-        uint64_t v = 0;
-        while (*p >= '0' && *p <= '9') {
-            const uint64_t d = (uint64_t)(*p - '0');
-            if (v > (UINT64_MAX - d) / 10) return false;
-            v = v * 10 + d;
-            ++p;
-        }
-        pos = p - buf.data();
-        *out = v;
-        return true;
+    ~Text() {
+        if (!p) return;
+        if (mapped_file) munmap(p - (cap - n), cap);
+        else munmap(p, cap);
     }
 };
+
+inline bool is_digit(char c) { return (unsigned char)(c - '0') < 10; }
+inline bool is_space(char c) { return c == ' ' || c == '\n' || c == '\r' || c == '\t'; }
+
+// Next unsigned integer at or after *pos in [0, n); false at the end, on a
+// byte that is neither a digit nor white space, or on overflow.
+bool next_u64(const char* p, size_t n, size_t* pos, uint64_t* out) {
+    size_t i = *pos;
+    while (i < n && is_space(p[i])) ++i;
+    if (i >= n || !is_digit(p[i])) return false;
+    uint64_t v = 0;
+    while (i < n && is_digit(p[i])) {
+        const uint64_t d = (uint64_t)(p[i] - '0');
+        if (v > (UINT64_MAX - d) / 10) return false;
+        v = v * 10 + d;
+        ++i;
+    }
+    *pos = i;
+    *out = v;
+    return true;
+}
+
+// The E lines "src weight dst" after the header (rbsc.cu:156-180), parsed
+// by T threads in two passes over T byte ranges cut at token boundaries:
+// the first counts each range's tokens (and notes its first byte that is
+// neither a digit nor white space), a prefix sum gives every range its
+// first token index, the second parses token t of the body into field t % 3
+// of line t / 3.  Bytes past the 3E-th token are ignored, as a reader that
+// stops after E lines ignores them.
+struct Lines {
+    // left uninitialised: the parsing threads write every entry first
+    std::unique_ptr<uint32_t[]> src, dst;
+    std::unique_ptr<uint64_t[]> wt;
+    bool unit = true;    // every weight is 1
+    bool sorted = true;  // sources never decrease
+};
+
+const char* parse_lines(const char* p, size_t b0, size_t n, uint64_t N, uint64_t E, Lines* L) {
+    unsigned T = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    if (n - b0 < ((size_t)1 << 22)) T = 1;
+    std::vector<size_t> cut(T + 1);
+    cut[0] = b0;
+    cut[T] = n;
+    for (unsigned k = 1; k < T; ++k) {
+        size_t c = b0 + (n - b0) / T * k;
+        while (c < n && is_digit(p[c])) ++c;  // never inside a token
+        cut[k] = std::max(c, cut[k - 1]);
+    }
+    std::vector<uint64_t> tok(T + 1, 0), bad_tok(T, UINT64_MAX);
+    auto count = [&](unsigned k) {
+        const char* q = p + cut[k];
+        const size_t len = cut[k + 1] - cut[k];
+        uint64_t c = 0, bad = UINT64_MAX;
+        bool prev = false;  // a range starts after a non-digit
+        for (size_t i = 0; i < len; i += 4096) {
+            const size_t stop = std::min(len, i + 4096);
+            const uint64_t c0 = c;
+            const bool prev0 = prev;
+            unsigned stray = 0;
+            for (size_t j = i; j < stop; ++j) {  // branch-free over the block
+                const bool d = is_digit(q[j]);
+                c += d & !prev;
+                prev = d;
+                stray |= !(d | is_space(q[j]));
+            }
+            if (stray && bad == UINT64_MAX) {
+                // tokens read before the block's first stray byte
+                uint64_t cb = c0;
+                bool pv = prev0;
+                for (size_t j = i; j < stop; ++j) {
+                    const bool d = is_digit(q[j]);
+                    if (!(d || is_space(q[j]))) break;
+                    cb += d & !pv;
+                    pv = d;
+                }
+                bad = cb;
+            }
+        }
+        tok[k + 1] = c;
+        bad_tok[k] = bad;
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned k = 0; k < T; ++k) th.emplace_back(count, k);
+        for (auto& t : th) t.join();
+    }
+    for (unsigned k = 0; k < T; ++k) {
+        if (bad_tok[k] != UINT64_MAX && tok[k] + bad_tok[k] < 3 * E)
+            return "truncated or malformed line";
+        tok[k + 1] += tok[k];
+    }
+    if (tok[T] < 3 * E) return "truncated or malformed line";
+    L->src.reset(new uint32_t[E ? E : 1]);
+    L->dst.reset(new uint32_t[E ? E : 1]);
+    L->wt.reset(new uint64_t[E ? E : 1]);
+    std::atomic<int> err{0};  // 1 overflow, 2 id out of range
+    std::vector<unsigned char> unit(T, 1), sorted(T, 1);
+    auto parse = [&](unsigned k) {
+        uint64_t t = tok[k];
+        if (t >= 3 * E) return;
+        size_t i = cut[k];
+        const size_t end = cut[k + 1];
+        bool u1 = true;
+        while (t < 3 * E) {
+            while (i < end && !is_digit(p[i])) ++i;
+            if (i >= end) break;
+            uint64_t v = 0;
+            while (i < end && is_digit(p[i])) {
+                const uint64_t d = (uint64_t)(p[i] - '0');
+                if (v > (UINT64_MAX - d) / 10) {
+                    err.fetch_or(1);
+                    return;
+                }
+                v = v * 10 + d;
+                ++i;
+            }
+            const uint64_t e = t / 3;
+            switch (t % 3) {
+                case 0:
+                    if (v >= N) err.fetch_or(2);
+                    L->src[e] = (uint32_t)v;
+                    break;
+                case 1:
+                    L->wt[e] = v;
+                    u1 = u1 && v == 1;
+                    break;
+                default:
+                    if (v >= N) err.fetch_or(2);
+                    L->dst[e] = (uint32_t)v;
+            }
+            ++t;
+        }
+        unit[k] = u1;
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned k = 0; k < T; ++k) th.emplace_back(parse, k);
+        for (auto& t : th) t.join();
+    }
+    if (err.load() & 1) return "truncated or malformed line";
+    if (err.load() & 2) return "node id out of range";
+    for (unsigned k = 0; k < T; ++k) L->unit = L->unit && unit[k];
+    // sources in order?  each thread checks a slice of lines against the line before
+    {
+        std::vector<std::thread> th;
+        for (unsigned k = 0; k < T; ++k)
+            th.emplace_back([&, k]() {
+                const uint64_t a0 = std::max<uint64_t>(1, E / T * k);
+                const uint64_t z = k + 1 == T ? E : E / T * (k + 1);
+                for (uint64_t e = a0; e < z; ++e)
+                    if (L->src[e] < L->src[e - 1]) {
+                        sorted[k] = 0;
+                        return;
+                    }
+            });
+        for (auto& t : th) t.join();
+    }
+    for (unsigned k = 0; k < T; ++k) L->sorted = L->sorted && sorted[k];
+    return nullptr;
+}
 
 int fail(const char* what) {
     fprintf(stderr, "rbsc: %s\n", what);
@@ -74,7 +263,7 @@ int fail(const char* what) {
 int main(int argc, char** argv) {
     std::vector<std::string> pos;
     std::string mode = "quotient", part_out, init_file;
-    int verify = 0, device = 0;
+    int verify = 0, device = 0, check_input = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto need = [&](const char* flag) -> const char* {
@@ -88,6 +277,7 @@ int main(int argc, char** argv) {
         else if (a == "--partition-out") part_out = need("--partition-out");
         else if (a == "--init-colors") init_file = need("--init-colors");
         else if (a == "--verify") verify = 1;
+        else if (a == "--check-input") check_input = 1;
         else if (a == "--device") device = atoi(need("--device"));
         else if (a.rfind("--", 0) == 0) {
             fprintf(stderr, "rbsc: unknown option %s\n", a.c_str());
@@ -95,35 +285,68 @@ int main(int argc, char** argv) {
         } else pos.push_back(a);
     }
 
-    Input in;
-    if (!in.read_all(stdin)) return fail("error reading stdin");
+    const auto t_read = std::chrono::steady_clock::now();
+    Text in;
+    if (!in.load(0)) return fail("error reading stdin");
     uint64_t N64 = 0, E = 0;
-    if (!in.next(&N64) || !in.next(&E)) return fail("input must start with N and E");
+    size_t at = 0;
+    if (!next_u64(in.p, in.n, &at, &N64) || !next_u64(in.p, in.n, &at, &E))
+        return fail("input must start with N and E");
     if (N64 >= 0xFFFFFFFFull) return fail("N must be below 2^32-1");
     const uint32_t N = (uint32_t)N64;
-    std::vector<uint32_t> src(E), dst(E);
-    std::vector<uint64_t> wt(E);
-    bool unit = true;
-    for (uint64_t e = 0; e < E; ++e) {
-        uint64_t u, w, v;
-        if (!in.next(&u) || !in.next(&w) || !in.next(&v)) return fail("truncated or malformed line");
-        if (u >= N || v >= N) return fail("node id out of range");
-        src[e] = (uint32_t)u;
-        dst[e] = (uint32_t)v;
-        wt[e] = w;
-        unit = unit && w == 1;
+    Lines L;
+    if (const char* why = parse_lines(in.p, at, in.n, N, E, &L)) return fail(why);
+    uint64_t line_digest = 0;
+    if (check_input)
+        for (uint64_t e = 0; e < E; ++e)
+            line_digest += (uint64_t)L.src[e] * 0x9e3779b97f4a7c15ULL + L.wt[e] * 0xbf58476d1ce4e5b9ULL +
+                           (uint64_t)L.dst[e] * 0x94d049bb133111ebULL + e * 0x2545f4914f6cdd1dULL;
+    // sources in order: hand the lines over as CSR (row offsets instead of
+    // one source per line: 8 (N + 1) bytes to upload instead of 4 E)
+    std::vector<uint64_t> row_offsets;
+    if (L.sorted && E) {
+        row_offsets.assign((size_t)N + 1, 0);
+        const unsigned T = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned k = 0; k < T; ++k)
+            th.emplace_back([&, k]() {
+                // line e starting a row (or e = E) ends the rows before it
+                const uint64_t a = E / T * k, z = k + 1 == T ? E + 1 : E / T * (k + 1);
+                for (uint64_t e = a; e < z; ++e) {
+                    const uint64_t lo = e == 0 ? 0 : (uint64_t)L.src[e - 1] + 1;
+                    const uint64_t hi = e == E ? (uint64_t)N : (uint64_t)L.src[e];
+                    if (e == 0 || e == E || L.src[e] != L.src[e - 1])
+                        for (uint64_t u = lo; u <= hi && u <= N; ++u) row_offsets[u] = e;
+                }
+            });
+        for (auto& t : th) t.join();
+        row_offsets[N] = E;
+        L.src.reset();
+    }
+    if (check_input) {
+        uint64_t ro_digest = 0;
+        for (size_t u = 0; u < row_offsets.size(); ++u) ro_digest += row_offsets[u] * (uint64_t)(u + 1);
+        const double dt =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t_read).count();
+        printf("%u %llu %016llx %s %s %016llx\n", N, (unsigned long long)E,
+               (unsigned long long)line_digest, row_offsets.empty() ? "coo" : "csr",
+               L.unit ? "unit" : "weighted", (unsigned long long)ro_digest);
+        fprintf(stderr, "rbsc: read and parsed %zu bytes in %.3f s\n", in.n, dt);
+        return 0;
     }
     std::vector<uint32_t> init;
     if (!init_file.empty()) {
         FILE* f = fopen(init_file.c_str(), "rb");
         if (!f) return fail("cannot open --init-colors file");
-        Input ic;
-        ic.read_all(f);
+        Text ic;
+        const bool ok = ic.load(fileno(f));
         fclose(f);
+        if (!ok) return fail("cannot read --init-colors file");
         init.resize(N);
+        size_t q = 0;
         for (uint32_t i = 0; i < N; ++i) {
             uint64_t c;
-            if (!ic.next(&c) || c > 0xFFFFFFFEull) return fail("bad --init-colors file");
+            if (!next_u64(ic.p, ic.n, &q, &c) || c > 0xFFFFFFFEull) return fail("bad --init-colors file");
             init[i] = (uint32_t)c;
         }
     }
@@ -139,9 +362,10 @@ int main(int argc, char** argv) {
     rbsc_graph g{};
     g.n_nodes = N;
     g.n_edges = E;
-    g.src = src.data();
-    g.dst = dst.data();
-    g.weight = unit ? nullptr : wt.data();
+    g.row_offsets = row_offsets.empty() ? nullptr : row_offsets.data();
+    g.src = row_offsets.empty() ? L.src.get() : nullptr;
+    g.dst = L.dst.get();
+    g.weight = L.unit ? nullptr : L.wt.get();
     g.init_color = init.empty() ? nullptr : init.data();
     rbsc_opts o{};
     o.max_batch_edges = max_batch;
