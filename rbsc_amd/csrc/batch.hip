@@ -91,18 +91,14 @@ __device__ void lds_radix_pass(Lds<BT, NM>* s, int in, int n, int shift) {
         const int idx = wave * IPW + it * 64 + lane;
         const bool valid = idx < n && (it * 64 + lane) < IPW;
         const u32 d = valid ? (u32)(s->key[in][idx] >> shift) & DM : 0u;
-        u64 peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < RB; ++b) {
-            const u64 bal = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bal : ~bal;
-        }
-        const int leader = valid ? (__ffsll((long long)peers) - 1) : lane;
+        // the group's count is read by every lane, then written by its
+        // lowest lane (in-order LDS, as in wave_rank)
+        const u64 peers = wave_match<RB>(d, __ballot(valid));
+        const u32 bl = (u32)__popcll(peers & below);
         u32 old = 0;
-        if (valid && lane == leader) old = s->whist[wave][d];
-        old = __shfl(old, leader);
-        if (valid && lane == leader) s->whist[wave][d] = old + (u32)__popcll(peers);
-        rank[it] = old + (u32)__popcll(peers & below);
+        if (valid) old = s->whist[wave][d];
+        if (valid && bl == 0) s->whist[wave][d] = old + (u32)__popcll(peers);
+        rank[it] = old + bl;
     }
     __syncthreads();
     // digit-major offsets: for digit d, waves in order; thread t takes the

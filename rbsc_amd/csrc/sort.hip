@@ -152,26 +152,8 @@ __global__ __launch_bounds__(THREADS, RBSC_SORT_OCC) void onesweep_kernel(
             val[j] = idx < n ? ((u64)col[idx] << 32) | idx : 0;
         }
     }
-    const u64 below_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const u64 idx = wave_base + (u64)j * RBSC_WAVE + lane;
-        const bool valid = idx < n;
-        const u32 d = (u32)(key[j] >> shift) & 255u;
-        u64 peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const u64 bal = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bal : ~bal;
-        }
-        const int leader = valid ? (__ffsll((long long)peers) - 1) : lane;
-        const u32 below = (u32)__popcll(peers & below_mask);
-        u32 old = 0;
-        if (valid && lane == leader) old = s_whist[wave][d];
-        old = __shfl(old, leader);
-        if (valid && lane == leader) s_whist[wave][d] = old + (u32)__popcll(peers);
-        rank[j] = old + below;
-    }
+    if (tile_base + TILE <= n) wave_rank<8, ITEMS, true>(key, rank, shift, wave_base, n, s_whist[wave]);
+    else wave_rank<8, ITEMS, false>(key, rank, shift, wave_base, n, s_whist[wave]);
     __syncthreads();
 
     // Per digit (thread == digit): counts across waves, tile-local starts.
