@@ -91,9 +91,12 @@ RSC_FULL_SIZE = {
 }
 
 
-def full_size_reference(config: str, blocks: int, refine_s: float, one_shot_s) -> dict | None:
+def full_size_reference(config: str, blocks: int, refine_s: float, one_shot_s,
+                        command_s=None) -> dict | None:
     """The recorded full-size rsc run beside this run's time to the fixpoint
-    (one refinement from resident lines, and the drop-in's one-shot span)."""
+    (one refinement from resident lines, the drop-in's one-shot span from
+    host arrays, and the rbsc command's own reported time), each the span
+    the tool's own timer covers."""
     rec = RSC_FULL_SIZE.get(config)
     if rec is None:
         return None
@@ -102,6 +105,8 @@ def full_size_reference(config: str, blocks: int, refine_s: float, one_shot_s) -
     out["speedup_refine"] = rec["seconds"] / refine_s
     if one_shot_s:
         out["speedup_one_shot"] = rec["seconds"] / one_shot_s
+    if command_s:
+        out["speedup_rbsc_command"] = rec["seconds"] / command_s
     return out
 
 
@@ -331,9 +336,55 @@ def one_shot_leg(ctx, dg) -> dict:
     t = time.perf_counter()
     r = ctx.refine(hg)
     wall = time.perf_counter() - t
-    return {"workload": "same graph from host CSR arrays (rbsc_refine)", "time_s": r.time_s,
-            "wall_s": wall, "rounds": r.rounds, "blocks": r.nodes, "device_ms": r.device_ms,
-            "download_s": dl}
+    # the same from page-locked arrays, as the rbsc command parses into
+    import numpy as np
+    from rbsc_amd import HostGraph, PinnedArray
+    pd = PinnedArray(hg.n_edges).array
+    pro = PinnedArray(hg.n_nodes + 1, np.uint64).array
+    pd[:] = hg.dst
+    pro[:] = hg.row_offsets
+    hp = HostGraph(hg.n_nodes, pd[:hg.n_edges], row_offsets=pro[:hg.n_nodes + 1])
+    t = time.perf_counter()
+    rp = ctx.refine(hp)
+    wall_p = time.perf_counter() - t
+    del hp, pd, pro
+    assert (rp.nodes, rp.rounds) == (r.nodes, r.rounds)
+    return {"workload": "same graph from host CSR arrays (rbsc_refine)", "time_s": rp.time_s,
+            "wall_s": wall_p, "host_memory": "page-locked", "rounds": rp.rounds,
+            "blocks": rp.nodes, "device_ms": rp.device_ms,
+            "pageable": {"time_s": r.time_s, "wall_s": wall}, "download_s": dl}
+
+
+def cli_leg(dg) -> dict:
+    """The rbsc command (rbsc_amd/bin/rbsc, the reference binary's drop-in)
+    on the workload's text in the reference's stdin format, piped from the
+    text writer: its three output lines (rbsc.cu:492-494; `time` spans
+    upload, rounds and canonical partition, as the reference's timer) and
+    its parse time, once plain and once with --verify."""
+    import subprocess
+    from rbsc_amd._lib import CLI_PATH
+    hg = dg.download()
+    out = {"workload": f"the same graph as text ({hg.n_edges} lines), piped into rbsc_amd/bin/rbsc"}
+    for name, extra in (("plain", []), ("verify", ["--verify"])):
+        p = subprocess.Popen([CLI_PATH, str(hg.n_edges), "1"] + extra, stdin=subprocess.PIPE,
+                             stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+        t = time.perf_counter()
+        hg.write_text(f"/proc/self/fd/{p.stdin.fileno()}")
+        p.stdin.close()
+        p.stdin = None
+        so, se = p.communicate(timeout=900)
+        wall = time.perf_counter() - t
+        if p.returncode:
+            raise SystemExit(f"bench: rbsc failed: {se.decode()}")
+        lines = so.decode().split("\n")
+        err = se.decode()
+        parse = [x for x in err.split("\n") if "read and parsed" in x]
+        ver = [x for x in err.split("\n") if "verified=" in x]
+        out[name] = {"time_s": float(lines[0]), "done": int(lines[1]), "nodes": int(lines[2]),
+                     "wall_s": wall,
+                     "parse_s": float(parse[0].split(" in ")[1].split(" s")[0]) if parse else None,
+                     "verified": int(ver[0].split("verified=")[1]) if ver else None}
+    return out
 
 
 def quotient_leg(ctx) -> dict:
@@ -416,6 +467,8 @@ def main():
                     help="worker threads (contexts) of the serving loop")
     ap.add_argument("--stream-frac", type=float, default=0.25,
                     help="also time the pinned-host streaming path at this edge-batch fraction (0 skips)")
+    ap.add_argument("--cli", type=int, default=1,
+                    help="run the rbsc command on the workload's text (0 = skip)")
     ap.add_argument("--quotient", type=int, default=1,
                     help="time the multi-batch quotient mode at 50%% / 25%% batches (0 = skip)")
     ap.add_argument("--one-shot", type=int, default=1,
@@ -557,6 +610,9 @@ def main():
     if args.one_shot:
         out["one_shot"] = one_shot_leg(ctx, dg)
         log(f"[bench] one-shot: {out['one_shot']}")
+    if args.cli and args.parts == 1 and not args.hot:
+        out["cli"] = cli_leg(dg)
+        log(f"[bench] rbsc command: {out['cli']}")
     if args.quotient:
         out["quotient"] = quotient_leg(ctx)
         log(f"[bench] quotient: {out['quotient']}")
@@ -575,7 +631,8 @@ def main():
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_scale + 4, 0.20, seed)
         full = full_size_reference(args.config, r.nodes, wall / args.steps,
-                                   out.get("one_shot", {}).get("time_s"))
+                                   out.get("one_shot", {}).get("time_s"),
+                                   out.get("cli", {}).get("plain", {}).get("time_s"))
         if full is not None:
             out["cpu_baseline"]["full_size"] = full
     dg.free()

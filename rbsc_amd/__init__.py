@@ -377,12 +377,13 @@ class _PinnedBlock:
     those) keeps the block as its base, so the block is released by
     ``rbsc_host_free`` only when the last array over it is gone."""
 
-    def __init__(self, n: int):
+    def __init__(self, n: int, dtype=np.uint32):
         n = max(int(n), 1)
+        dt = np.dtype(dtype)
         p = C.c_void_p()
-        check(load().rbsc_host_alloc(4 * n, C.byref(p)))
+        check(load().rbsc_host_alloc(dt.itemsize * n, C.byref(p)))
         self._p = p
-        self.__array_interface__ = {"shape": (n,), "typestr": "<u4", "data": (p.value, False),
+        self.__array_interface__ = {"shape": (n,), "typestr": dt.str, "data": (p.value, False),
                                     "version": 3}
 
     def __del__(self):
@@ -395,14 +396,15 @@ class _PinnedBlock:
 
 
 class PinnedArray:
-    """A numpy uint32 array over page-locked host memory (``rbsc_host_alloc``):
-    device-to-host copies into it are not staged through pageable memory.
+    """A numpy array (uint32 unless ``dtype`` says otherwise) over page-locked
+    host memory (``rbsc_host_alloc``): copies between it and the device are
+    not staged through pageable memory.
 
     ``free()`` drops this object's hold on the memory; arrays handed out
     earlier (``.array`` and its views) keep it alive until they are gone."""
 
-    def __init__(self, n: int):
-        self.array = np.asarray(_PinnedBlock(n))
+    def __init__(self, n: int, dtype=np.uint32):
+        self.array = np.asarray(_PinnedBlock(n, dtype))
 
     def free(self):
         self.array = None

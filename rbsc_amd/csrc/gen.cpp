@@ -448,16 +448,40 @@ extern "C" int rbsc_host_graph_write_text(const rbsc_host_graph* g, const char* 
         p = put_u64(p, blocks);
         *p++ = '\n';
     }
-    for (uint32_t u = 0; u < g->n_nodes && ok; ++u)
-        for (uint64_t e = g->row_offsets[u]; e < g->row_offsets[u + 1]; ++e) {
-            p = put_u64(p, (uint64_t)u + one);
-            *p++ = ' ';
-            p = put_u64(p, g->weight ? g->weight[e] : 1);
-            *p++ = ' ';
-            p = put_u64(p, (uint64_t)g->dst[e] + one);
-            *p++ = '\n';
-            if (p >= end) flush();
+    flush();
+    // The lines, formatted by T threads in blocks of BL consecutive lines
+    // (block b by thread b mod T, into its own buffer) and written in order.
+    {
+        const uint64_t E = g->n_edges, BL = 1 << 20;
+        const unsigned T = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        std::vector<std::vector<char>> tb(T, std::vector<char>(BL * 64));
+        std::vector<size_t> used(T, 0);
+        auto fmt = [&](unsigned k, uint64_t b) {
+            const uint64_t e0 = b * BL, e1 = std::min(E, e0 + BL);
+            // the row holding line e0: last u with row_offsets[u] <= e0
+            const uint64_t* ro = g->row_offsets;
+            uint64_t u = (uint64_t)(std::upper_bound(ro, ro + g->n_nodes + 1, e0) - ro) - 1;
+            char* q = tb[k].data();
+            for (uint64_t e = e0; e < e1; ++e) {
+                while (ro[u + 1] <= e) ++u;
+                q = put_u64(q, u + one);
+                *q++ = ' ';
+                q = put_u64(q, g->weight ? g->weight[e] : 1);
+                *q++ = ' ';
+                q = put_u64(q, (uint64_t)g->dst[e] + one);
+                *q++ = '\n';
+            }
+            used[k] = (size_t)(q - tb[k].data());
+        };
+        const uint64_t nb = (E + BL - 1) / BL;
+        for (uint64_t b0 = 0; b0 < nb && ok; b0 += T) {
+            std::vector<std::thread> th;
+            for (unsigned k = 0; k < T && b0 + k < nb; ++k) th.emplace_back(fmt, k, b0 + k);
+            for (auto& t : th) t.join();
+            for (unsigned k = 0; k < T && b0 + k < nb && ok; ++k)
+                if (fwrite(tb[k].data(), 1, used[k], f) != used[k]) ok = false;
         }
+    }
     if (format == RBSC_TEXT_VALMARI && g->init_color) {
         // MDP_read.cc:140-150: every initial block but the first, members then 0
         std::vector<std::pair<uint32_t, uint32_t>> cu(g->n_nodes);

@@ -37,10 +37,13 @@ def _text(n, src, w, dst, sep=" ", eol="\n", tail=""):
     return f"{n}{eol}{len(dst)}{eol}{body}{tail}".encode()
 
 
-def _run(data, tmp_path=None):
+def _run(data, tmp_path=None, chunk=None):
     if tmp_path is None:
+        env = dict(os.environ)
+        if chunk:
+            env["RBSC_READ_CHUNK"] = str(chunk)
         return subprocess.run([CLI_PATH, "--check-input"], input=data, capture_output=True,
-                              timeout=300)
+                              timeout=300, env=env)
     f = tmp_path / "in.txt"
     f.write_bytes(data)
     with open(f, "rb") as fh:
@@ -69,6 +72,9 @@ def test_large_text_pipe_and_file(tmp_path, sort, weights):
     data = _text(n, src, w, dst)
     _check(_run(data), n, src, w, dst)
     _check(_run(data, tmp_path), n, src, w, dst)
+    # a pipe read in chunks much smaller than the text: tokens cut at chunk ends
+    for chunk in (4096, 65537, 1 << 20):
+        _check(_run(data, chunk=chunk), n, src, w, dst)
 
 
 def test_any_white_space_and_trailing_bytes(tmp_path):
@@ -82,6 +88,7 @@ def test_any_white_space_and_trailing_bytes(tmp_path):
         data = _text(n, src, w, dst, sep, eol, tail)
         _check(_run(data), n, src, w, dst)
         _check(_run(data, tmp_path), n, src, w, dst)
+        _check(_run(data, chunk=5000), n, src, w, dst)
 
 
 def test_small_and_empty():
@@ -110,5 +117,6 @@ def test_malformed_text_is_refused():
     cases.append(b"4294967295\n0\n")                                     # N too large
     cases.append(b"x\n")
     for data in cases:
-        out = _run(data)
-        assert out.returncode != 0 and out.stdout == b"", data[:40]
+        for chunk in (None, 4096):
+            out = _run(data, chunk=chunk)
+            assert out.returncode != 0 and out.stdout == b"", (data[:40], chunk)
