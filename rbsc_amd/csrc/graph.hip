@@ -48,12 +48,12 @@ __global__ void fill_ones_kernel(u64* p, u64 n) {
 // only order the internal ids (most-gathered colours first, for cache
 // locality and the hot / dense / warm classes of the binned layout); colours
 // are dense ranks of (signature, colour) pairs and do not depend on the
-// numbering, so a sample serves as well as the exact counts.  Graphs of
-// 2^28 lines and more count every stride-th line, a power of two leaving
-// 2^27 to 2^28 samples (every 8th line of the 2^31-line R-MAT): the counts
-// are random atomics into an array of N words (88 ms for all 2.1 G lines
-// of that graph), and the sample ranks the heads all but identically where
-// it matters, among heads of hundreds of lines and more.
+// numbering.  With RawGraph::sample_indeg, graphs of 2^28 lines and more
+// count every stride-th line, a power of two leaving 2^27 to 2^28 samples
+// (every 8th line of the 2^31-line R-MAT: 11 instead of 88 ms of random
+// atomics into N words).  Measured on that graph, the sample's order moves
+// 5.3 M lines from the hot to the cold class and costs the binned round
+// 0.54 GB more gather traffic, so resident graphs count every line.
 __global__ void indeg_kernel(const u32* __restrict__ dst, u64 E, u64 stride,
                              u32* __restrict__ indeg) {
     const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -383,7 +383,7 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     }
     zero_u32<<<grid_for(N), BLK, 0, s>>>(indeg, N);
     if (E_in) {
-        const u64 st = indeg_stride(E_in);
+        const u64 st = raw->sample_indeg ? indeg_stride(E_in) : 1;
         indeg_kernel<<<grid_for((E_in + st - 1) / st), BLK, 0, s>>>(raw->dst, E_in, st, indeg);
     }
     fill_ones_kernel<<<grid_for(N), BLK, 0, s>>>(dang, N);

@@ -21,6 +21,12 @@ struct RawGraph {
     u32* dst = nullptr;         // E
     u64* wt = nullptr;          // E or null
     u32* init = nullptr;        // N or null
+    // In-degrees (they only order the internal ids) from every line, or from
+    // a sample of the lines of a large graph: the sample saves most of the
+    // count's cost but orders heads of similar in-degree less well, which
+    // costs the binned layout's gathers a little locality every round, so
+    // only a graph refined once (the one-shot rbsc_refine) takes it.
+    bool sample_indeg = false;
 };
 
 // Binned signature layout (bins.hip): stored rows cut into bins of S rows
@@ -278,7 +284,8 @@ void device_colors(rbsc_ctx* ctx, rbsc_dgraph* g, u32* d_int);
 // Exact stability check (verify.hip) of internal colours d_col (< K);
 // *bad_out = number of violations, 0 iff stable.
 int verify_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* d_col, u32 K, u64* bad_out);
-int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out);
+int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out,
+                      bool sample_indeg = false);
 int quotient_refine(rbsc_ctx* ctx, const rbsc_graph* gr, const rbsc_opts* opts, rbsc_result* res,
                     uint32_t* partition_out);
 int check_kernel_errors(rbsc_ctx* ctx);

@@ -518,6 +518,7 @@ int quotient_refine(rbsc_ctx* ctx, const rbsc_graph* gr, const rbsc_opts* opts, 
             raw.dst = bdst;
             raw.wt = const_cast<u64*>(bw);
             raw.init = init;
+            raw.sample_indeg = true;  // refined once
             rbsc_dgraph* g = new rbsc_dgraph();
             int rc = dgraph_build(ctx, &raw, g);
             RefineOut o;

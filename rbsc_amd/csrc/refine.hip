@@ -2265,7 +2265,8 @@ done:
 #undef U_TRY
 }
 
-int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out) {
+int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out,
+                      bool sample_indeg) {
     if (!ctx || !gr || !out) {
         rbsc_set_error("rbsc_graph_upload: null argument");
         return RBSC_ERR_INVALID;
@@ -2298,6 +2299,7 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
     RawGraph raw;
     raw.N = N;
     raw.E = E;
+    raw.sample_indeg = sample_indeg;
     auto fail = [&](hipError_t e, const char* what) {
         rbsc_set_error("%s failed: %s", what, hipGetErrorString(e));
         raw_release(&raw);
@@ -2560,7 +2562,8 @@ extern "C" int rbsc_refine(rbsc_ctx* ctx, const rbsc_graph* graph, const rbsc_op
         return quotient_refine(ctx, graph, opts, result, partition_out);
     auto t0 = std::chrono::steady_clock::now();
     rbsc_dgraph* g = nullptr;
-    int rc = graph_upload_host(ctx, graph, batched, &g);
+    // refined once: sampled in-degrees are enough for the plain layout's order
+    int rc = graph_upload_host(ctx, graph, batched, &g, true);
     if (rc) return rc;
     rbsc_result r{};
     rc = refine_resident(ctx, g, opts, &r, partition_out, nullptr);

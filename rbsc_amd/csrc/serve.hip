@@ -178,7 +178,7 @@ void worker_main(rbsc_server* s, rbsc_ctx* ctx) {
             gr.dst = r->dst.data();
             gr.weight = r->wt.empty() ? nullptr : r->wt.data();
             gr.init_color = r->init.empty() ? nullptr : r->init.data();
-            rc = rbsc_graph_upload(ctx, &gr, &g);
+            rc = graph_upload_host(ctx, &gr, false, &g, true);  // refined once
         } else if (r->kind == REQ_RMAT) {
             rc = rbsc_gen_rmat_device(ctx, r->scale, r->draws, r->a, r->b, r->c, r->clone, r->seed, &g);
         } else {
