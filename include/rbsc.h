@@ -405,6 +405,14 @@ int rbsc_server_submit_resident(rbsc_server* s, rbsc_dgraph* g, double delay_s, 
  * keep_partition is set. */
 int rbsc_server_wait(rbsc_server* s, uint64_t ticket, double timeout_s, rbsc_request_result* out,
                      uint32_t* partition_out);
+/* The same without copying the partition: *labels (n_nodes words, or null
+ * without keep_partition) is page-locked memory the refinement wrote its
+ * labels into, the caller's until rbsc_server_labels_free(labels) hands it
+ * back for reuse; it stays valid after the server is destroyed.  No
+ * counterpart in the reference (its result is the block count only). */
+int rbsc_server_wait_labels(rbsc_server* s, uint64_t ticket, double timeout_s,
+                            rbsc_request_result* out, uint32_t** labels);
+int rbsc_server_labels_free(uint32_t* labels);
 int rbsc_server_get_stats(rbsc_server* s, rbsc_server_stats* out);
 
 /* Seeded R-MAT generated on the device (bench-scale inputs without a host

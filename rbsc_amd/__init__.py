@@ -591,6 +591,26 @@ class RequestResult:
     partition: Optional[np.ndarray] = None
 
 
+class _ServerLabels:
+    """Owner of a partition buffer from ``rbsc_server_wait_labels`` (page-locked
+    memory the refinement wrote the labels into), exposed to numpy through
+    ``__array_interface__``: the buffer goes back to the library's pool with
+    ``rbsc_server_labels_free`` once the last array over it is gone."""
+
+    def __init__(self, p: int, n: int):
+        self._p = p
+        self.__array_interface__ = {"shape": (n,), "typestr": "<u4", "data": (p, False),
+                                    "version": 3}
+
+    def __del__(self):
+        try:
+            if self._p:
+                load().rbsc_server_labels_free(C.c_void_p(self._p))
+                self._p = None
+        except Exception:
+            pass
+
+
 class Server:
     """Continuous-batching request loop on one device (``rbsc_server``):
     FIFO admission under an HBM budget, small graphs batched into persistent
@@ -632,20 +652,26 @@ class Server:
 
     def wait(self, ticket: int, timeout_s=-1.0) -> Optional[RequestResult]:
         """The request's result (raises on its error), or None if it is not
-        finished within timeout_s."""
+        finished within timeout_s.  The partition is the page-locked buffer
+        the refinement wrote, not a copy; it returns to the library's pool
+        when the last array over it is gone."""
         r = _lib.RequestResult()
-        n = self._nodes.get(ticket, 0)
-        part = np.empty(max(n, 1), np.uint32) if self.keep_partition else None
-        rc = load().rbsc_server_wait(self.handle, ticket, float(timeout_s), C.byref(r),
-                                     _ptr(part, C.c_uint32))
+        lab = C.c_void_p()
+        rc = load().rbsc_server_wait_labels(self.handle, ticket, float(timeout_s), C.byref(r),
+                                            C.byref(lab))
         if rc == _lib.PENDING:
             return None
         self._nodes.pop(ticket, None)
         check(rc)
+        n = int(r.n_nodes)
+        part = None
+        if self.keep_partition:
+            part = (np.asarray(_ServerLabels(lab.value, n)) if lab.value
+                    else np.empty(0, np.uint32))
         return RequestResult(int(r.ticket), int(r.status), int(r.nodes), int(r.rounds),
-                             bool(r.batched), int(r.n_nodes), int(r.n_edges), float(r.latency_s),
+                             bool(r.batched), n, int(r.n_edges), float(r.latency_s),
                              float(r.queue_s), float(r.service_s), float(r.arrival_s),
-                             float(r.done_s), None if part is None else part[:n])
+                             float(r.done_s), part)
 
     def stats(self) -> dict:
         st = _lib.ServerStats()

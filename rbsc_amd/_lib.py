@@ -187,6 +187,9 @@ SIGNATURES = {
                                               C.POINTER(C.c_uint64)]),
     "rbsc_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double,
                                    C.POINTER(RequestResult), u32p]),
+    "rbsc_server_wait_labels": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double,
+                                          C.POINTER(RequestResult), C.POINTER(C.c_void_p)]),
+    "rbsc_server_labels_free": (C.c_int, [C.c_void_p]),
     "rbsc_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "rbsc_batch_upload": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(C.c_void_p)]),
     "rbsc_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -1931,7 +1931,10 @@ int device_partition(rbsc_ctx* ctx, rbsc_dgraph* g, u32* partition_out, u32* col
     if (partition_out) {
         u32* first = reinterpret_cast<u32*>(w.P[0]);  // N + 1 entries
         u64* flag = w.P[1];
-        u32* d_lab = reinterpret_cast<u32*>(w.hc2);
+        // labels straight into the caller's page-locked buffer when it is
+        // one from rbsc_host_alloc (no staging copy), else through hc2
+        u32* direct = static_cast<u32*>(host_device_ptr(partition_out, sizeof(u32) * N));
+        u32* d_lab = direct ? direct : reinterpret_cast<u32*>(w.hc2);
         fill_u32_kernel<<<grid_for(N + 1), BLK, 0, s>>>(first, N + 1, 0xFFFFFFFFu);
         fill_u32_kernel<<<1, 32, 0, s>>>(ctx->d_scalar + 12, 1, 0);
         canon_first_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, N, first, ctx->d_scalar + 12);
@@ -1939,7 +1942,8 @@ int device_partition(rbsc_ctx* ctx, rbsc_dgraph* g, u32* partition_out, u32* col
         int rc = scan_excl_u64(flag, flag, N, nullptr, w.sig, s);
         if (rc) return rc;
         canon_label_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, first, flag, N, d_lab);
-        HIP_TRY(hipMemcpyAsync(partition_out, d_lab, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
+        if (!direct)
+            HIP_TRY(hipMemcpyAsync(partition_out, d_lab, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
     }
     u32 bad = 0;
     if (partition_out)

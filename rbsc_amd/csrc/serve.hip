@@ -57,8 +57,69 @@ struct Req {
     int status = RBSC_OK;
     std::string error;
     u32 blocks = 0, rounds = 0, batched = 0;
-    std::vector<u32> partition;
+    std::vector<u32> partition;  // batched requests
+    u32* part = nullptr;         // single-graph requests: a page-locked pool buffer
 };
+
+// Page-locked buffers for partitions, kept for reuse by size class (powers
+// of two, from 4096 labels): a refinement writes its labels straight into
+// them (rbsc_host_alloc memory, no staging copy), a large graph's buffer is
+// not pinned again for every request, and rbsc_server_wait_labels hands a
+// buffer to the caller instead of copying it.  One pool serves the process
+// (servers come and go; buffers handed out may outlive their server); it
+// keeps at most POOL_KEEP bytes of free buffers and is never destroyed, so
+// no buffer is freed after the runtime shuts down.
+class PartPool {
+  public:
+    static constexpr u64 POOL_KEEP = 8ull << 30;
+    u32* get(u64 n) {
+        u64 c = 4096;
+        while (c < n) c *= 2;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto& v = free_[c];
+            if (!v.empty()) {
+                u32* p = v.back();
+                v.pop_back();
+                kept_ -= 4 * c;
+                out_[p] = c;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (rbsc_host_alloc(sizeof(u32) * c, &p) != RBSC_OK) return nullptr;
+        std::lock_guard<std::mutex> lk(mu_);
+        out_[static_cast<u32*>(p)] = c;
+        return static_cast<u32*>(p);
+    }
+    // false: p is not a buffer of this pool that is out
+    bool put(u32* p) {
+        if (!p) return true;
+        std::unique_lock<std::mutex> lk(mu_);
+        auto it = out_.find(p);
+        if (it == out_.end()) return false;
+        const u64 c = it->second;
+        out_.erase(it);
+        if (kept_ + 4 * c <= POOL_KEEP) {
+            free_[c].push_back(p);
+            kept_ += 4 * c;
+            return true;
+        }
+        lk.unlock();
+        rbsc_host_free(p);
+        return true;
+    }
+
+  private:
+    std::mutex mu_;
+    std::map<u64, std::vector<u32*>> free_;
+    std::map<u32*, u64> out_;
+    u64 kept_ = 0;
+};
+PartPool& part_pool() {
+    static PartPool* p = new PartPool();
+    return *p;
+}
 
 struct ByArrival {
     bool operator()(const Req* x, const Req* y) const {
@@ -187,9 +248,11 @@ void worker_main(rbsc_server* s, rbsc_ctx* ctx) {
         rbsc_result res{};
         if (!rc) {
             rbsc_opts o{};
-            if (s->opts.keep_partition) r->partition.resize(r->n);
-            rc = rbsc_graph_refine(ctx, g, &o, &res,
-                                   s->opts.keep_partition ? r->partition.data() : nullptr, nullptr);
+            if (s->opts.keep_partition && r->n) {
+                r->part = part_pool().get(r->n);
+                if (!r->part) rc = RBSC_ERR_NOMEM;
+            }
+            if (!rc) rc = rbsc_graph_refine(ctx, g, &o, &res, r->part, nullptr);
         }
         const std::string err = rc ? rbsc_last_error() : "";
         if (g && r->kind != REQ_RESIDENT) rbsc_graph_free(ctx, g);
@@ -367,6 +430,11 @@ extern "C" int rbsc_server_destroy(rbsc_server* s) {
     s->cv_small.notify_all();
     for (auto& t : s->threads) t.join();
     for (rbsc_ctx* c : s->ctxs) rbsc_ctx_destroy(c);
+    // partitions never collected go back to the pool
+    for (auto& kv : s->reqs) {
+        part_pool().put(kv.second->part);
+        kv.second->part = nullptr;
+    }
     delete s;
     return RBSC_OK;
 }
@@ -453,16 +521,20 @@ extern "C" int rbsc_server_submit_resident(rbsc_server* s, rbsc_dgraph* g, doubl
     return RBSC_OK;
 }
 
-extern "C" int rbsc_server_wait(rbsc_server* s, uint64_t ticket, double timeout_s,
-                                rbsc_request_result* out, uint32_t* partition_out) {
+namespace {
+// The request's result once it has finished (or RBSC_PENDING); the request
+// leaves the table and is handed to the caller, whose copy of its labels
+// then runs without the server's lock.
+int take_result(rbsc_server* s, uint64_t ticket, double timeout_s, rbsc_request_result* out,
+                std::unique_ptr<Req>* own, const char* who) {
     if (!s || !out) {
-        rbsc_set_error("rbsc_server_wait: null argument");
+        rbsc_set_error("%s: null argument", who);
         return RBSC_ERR_INVALID;
     }
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) {
-        rbsc_set_error("rbsc_server_wait: unknown ticket %llu", (unsigned long long)ticket);
+        rbsc_set_error("%s: unknown ticket %llu", who, (unsigned long long)ticket);
         return RBSC_ERR_INVALID;
     }
     Req* r = it->second.get();
@@ -470,7 +542,7 @@ extern "C" int rbsc_server_wait(rbsc_server* s, uint64_t ticket, double timeout_
     if (timeout_s < 0) {
         s->cv_done.wait(lk, ready);
     } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
-        rbsc_set_error("rbsc_server_wait: request %llu not finished", (unsigned long long)ticket);
+        rbsc_set_error("%s: request %llu not finished", who, (unsigned long long)ticket);
         return RBSC_PENDING;
     }
     rbsc_request_result o{};
@@ -488,12 +560,68 @@ extern "C" int rbsc_server_wait(rbsc_server* s, uint64_t ticket, double timeout_
     o.arrival_s = sec(r->t_arrival.time_since_epoch());
     o.done_s = sec(r->t_done.time_since_epoch());
     *out = o;
-    if (partition_out && !r->partition.empty())
-        std::memcpy(partition_out, r->partition.data(), sizeof(u32) * r->partition.size());
-    const int st = r->status;
-    if (st) rbsc_set_error("%s", r->error.c_str());
+    *own = std::move(it->second);
     s->reqs.erase(it);
+    return RBSC_OK;
+}
+}  // namespace
+
+extern "C" int rbsc_server_wait(rbsc_server* s, uint64_t ticket, double timeout_s,
+                                rbsc_request_result* out, uint32_t* partition_out) {
+    std::unique_ptr<Req> own;
+    const int rc = take_result(s, ticket, timeout_s, out, &own, "rbsc_server_wait");
+    if (rc) return rc;
+    const int st = own->status;
+    if (partition_out && !st) {
+        if (own->part)
+            std::memcpy(partition_out, own->part, sizeof(u32) * own->n);
+        else if (!own->partition.empty())
+            std::memcpy(partition_out, own->partition.data(), sizeof(u32) * own->partition.size());
+    }
+    part_pool().put(own->part);
+    own->part = nullptr;
+    if (st) rbsc_set_error("%s", own->error.c_str());
     return st ? st : RBSC_OK;
+}
+
+extern "C" int rbsc_server_wait_labels(rbsc_server* s, uint64_t ticket, double timeout_s,
+                                       rbsc_request_result* out, uint32_t** labels) {
+    if (!labels) {
+        rbsc_set_error("rbsc_server_wait_labels: null argument");
+        return RBSC_ERR_INVALID;
+    }
+    *labels = nullptr;
+    std::unique_ptr<Req> own;
+    const int rc = take_result(s, ticket, timeout_s, out, &own, "rbsc_server_wait_labels");
+    if (rc) return rc;
+    const int st = own->status;
+    if (st) {
+        part_pool().put(own->part);
+        own->part = nullptr;
+        rbsc_set_error("%s", own->error.c_str());
+        return st;
+    }
+    if (own->part) {
+        *labels = own->part;  // the caller's until rbsc_server_labels_free
+        own->part = nullptr;
+    } else if (!own->partition.empty()) {
+        u32* p = part_pool().get(own->partition.size());
+        if (!p) {
+            rbsc_set_error("rbsc_server_wait_labels: page-locked memory exhausted");
+            return RBSC_ERR_NOMEM;
+        }
+        std::memcpy(p, own->partition.data(), sizeof(u32) * own->partition.size());
+        *labels = p;
+    }
+    return RBSC_OK;
+}
+
+extern "C" int rbsc_server_labels_free(uint32_t* labels) {
+    if (!part_pool().put(labels)) {
+        rbsc_set_error("rbsc_server_labels_free: not a buffer from rbsc_server_wait_labels");
+        return RBSC_ERR_INVALID;
+    }
+    return RBSC_OK;
 }
 
 extern "C" int rbsc_server_get_stats(rbsc_server* s, rbsc_server_stats* out) {

@@ -126,6 +126,33 @@ def test_serve_rmat_and_resident(ctx):
     assert np.array_equal(rs[1].partition, hl.partition)
 
 
+def test_served_partitions_outlive_the_server(ctx):
+    """Partitions are the page-locked buffers the refinements wrote (no copy);
+    they stay valid after the server is destroyed and go back to the
+    library's pool when the last array over them is gone, for both the
+    batched and the single-graph path; a later server reuses the pool."""
+    graphs = [gen.rmat(9, 1 << 13, clone_frac=0.3, seed=41),
+              gen.rmat(14, 1 << 18, clone_frac=0.2, seed=42)]
+    want = [_alone(ctx, g) for g in graphs]
+    for rep in range(2):
+        with rbsc_amd.Server(workers=2) as srv:
+            ts = [srv.submit(g) for g in graphs]
+            rs = [srv.wait(t) for t in ts]
+        import gc
+        gc.collect()
+        assert rs[0].batched and not rs[1].batched
+        for r, a in zip(rs, want):
+            assert (r.nodes, r.rounds) == (a.nodes, a.rounds)
+            assert np.array_equal(r.partition, a.partition)
+        views = [r.partition[1:5] for r in rs]
+        del rs
+        gc.collect()
+        for v, a in zip(views, want):
+            assert np.array_equal(v, a.partition[1:5])
+        del views
+        gc.collect()
+
+
 @pytest.mark.parametrize("binned", [False, True])
 def test_serve_same_resident_graph_concurrently(ctx, binned):
     # rows longer than a 512-line tile carry partial sums between kernels,

@@ -198,3 +198,12 @@ def test_pinned_labels_outlive_their_batch(monkeypatch):
     import gc
     gc.collect()
     assert len(freed) == 1      # released with the last array
+
+
+def test_server_labels_free_checks_its_argument():
+    """rbsc_server_labels_free takes back only buffers rbsc_server_wait_labels
+    handed out (a null pointer is a no-op); no device is touched."""
+    import ctypes as C
+    L = rbsc_amd.load()
+    assert L.rbsc_server_labels_free(None) == 0
+    assert L.rbsc_server_labels_free(C.c_void_p(0x1000)) == -1  # RBSC_ERR_INVALID
