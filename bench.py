@@ -336,19 +336,22 @@ def one_shot_leg(ctx, dg) -> dict:
     t = time.perf_counter()
     r = ctx.refine(hg)
     wall = time.perf_counter() - t
-    # the same from page-locked arrays, as the rbsc command parses into
+    # the same from page-locked arrays, as the rbsc command parses into, the
+    # partition too (the device writes the labels into it directly)
     import numpy as np
     from rbsc_amd import HostGraph, PinnedArray
     pd = PinnedArray(hg.n_edges).array
     pro = PinnedArray(hg.n_nodes + 1, np.uint64).array
+    ppart = PinnedArray(hg.n_nodes).array
     pd[:] = hg.dst
     pro[:] = hg.row_offsets
     hp = HostGraph(hg.n_nodes, pd[:hg.n_edges], row_offsets=pro[:hg.n_nodes + 1])
     t = time.perf_counter()
-    rp = ctx.refine(hp)
+    rp = ctx.refine(hp, partition_out=ppart)
     wall_p = time.perf_counter() - t
-    del hp, pd, pro
     assert (rp.nodes, rp.rounds) == (r.nodes, r.rounds)
+    assert np.array_equal(rp.partition, r.partition)
+    del hp, pd, pro, ppart
     return {"workload": "same graph from host CSR arrays (rbsc_refine)", "time_s": rp.time_s,
             "wall_s": wall_p, "host_memory": "page-locked", "rounds": rp.rounds,
             "blocks": rp.nodes, "device_ms": rp.device_ms,

@@ -538,10 +538,19 @@ class Context:
         return {"full_sort_redos": int(v[0]), "long_run_rounds": int(v[1])}
 
     def refine(self, g: HostGraph, max_batch_edges=0, mode="stream", max_rounds=0,
-               verify=False, partition=True, flags=0) -> RefineResult:
+               verify=False, partition=True, flags=0, partition_out=None) -> RefineResult:
+        """The drop-in's whole run (rbsc_refine).  partition_out: an array of
+        at least n_nodes uint32 to receive the partition (a PinnedArray's is
+        written by the device directly), else one is allocated."""
         st = g.as_struct()
         o = _opts(max_batch_edges, mode, max_rounds, verify, flags)
-        part = np.empty(g.n_nodes, np.uint32) if partition else None
+        if partition and partition_out is not None:
+            if (partition_out.dtype != np.uint32 or not partition_out.flags.c_contiguous
+                    or len(partition_out) < g.n_nodes):
+                raise ValueError("partition_out: contiguous uint32, n_nodes entries")
+            part = partition_out[:g.n_nodes]
+        else:
+            part = np.empty(g.n_nodes, np.uint32) if partition else None
         r = _lib.Result()
         check(load().rbsc_refine(self.handle, C.byref(st), C.byref(o), C.byref(r),
                                  _ptr(part, C.c_uint32)))

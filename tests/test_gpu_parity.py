@@ -263,6 +263,22 @@ def test_streaming_equals_resident(ctx, frac):
     dg.free()
 
 
+def test_partition_into_page_locked_memory(ctx):
+    """rbsc_refine writing its labels straight into an rbsc_host_alloc buffer
+    (no staging copy) gives the same partition as into pageable memory, and
+    leaves the buffer's tail untouched."""
+    g = gen.rmat(12, 1 << 16, clone_frac=0.3, seed=19)
+    want = ctx.refine(g)
+    out = rbsc_amd.PinnedArray(g.n_nodes + 7).array
+    out[:] = 0xDEADBEEF
+    r = ctx.refine(g, partition_out=out)
+    assert (r.nodes, r.rounds) == (want.nodes, want.rounds)
+    assert np.array_equal(out[:g.n_nodes], want.partition)
+    assert np.all(out[g.n_nodes:] == 0xDEADBEEF)
+    c, K, _, _ = _oracle_full(g)
+    assert np.array_equal(want.partition, oracle.canonical(c))
+
+
 @pytest.mark.parametrize("frac,aggr", [(0.5, False), (0.25, False), (0.25, True), (0.1, False)])
 def test_quotient_mode_matches_oracle(ctx, frac, aggr):
     """Reference multi-batch quotient passes (rbsc.cu:274-487): nodes, done and
