@@ -634,7 +634,10 @@ __global__ void tie_scatter_kernel(const u64* __restrict__ cv, const u64* __rest
     if (P) P[corig[m]] = cP[src];
 }
 
-constexpr int RANK_ITEMS = 16;
+#ifndef RBSC_RANK_ITEMS
+#define RBSC_RANK_ITEMS 16
+#endif
+constexpr int RANK_ITEMS = RBSC_RANK_ITEMS;  // keys per thread of a rank tile
 constexpr u64 RANK_TILE = 256 * RANK_ITEMS;
 
 // Dense rank of (sig, colour) over the sorted sequence (rbsc.cu:392-396):
