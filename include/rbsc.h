@@ -114,6 +114,15 @@ typedef struct rbsc_opts {
  * instead of sorting all nodes; RBSC_FLAG_SORTRANK sorts as in the other
  * rounds (a test hook).  Results are identical. */
 #define RBSC_FLAG_SORTRANK 2048u
+/* Node order: the last round (the one whose block count repeats the
+ * previous one's) is confirmed without sorting or ranking -- the previous
+ * round's rank pass lists the nodes of its blocks of two or more nodes, and
+ * when none of those blocks has two signatures this round, no block split.
+ * The final colours are then the previous round's, which name the same
+ * blocks (the partition, block count and rounds are those of the ranked
+ * round).  RBSC_FLAG_FULLROUNDS ranks the last round too (a test hook); so
+ * do a round limit and a colours_out request. */
+#define RBSC_FLAG_FULLROUNDS 4096u
 
 /* Result.  nodes / done / time_s are the reference's three output lines
  * (rbsc.cu:492-494). */
@@ -252,8 +261,9 @@ int rbsc_ctx_profile_read(rbsc_ctx* ctx, double* ms, uint64_t* launches, int res
 /* Counters of the node side's slow paths since the context was made (or
  * last reset): out[0] rounds whose node side was redone with a full sort
  * (a tie run list overflowed, or equal signatures out of colour order),
- * out[1] rounds with tie runs too long for the in-kernel sorts.  No
- * counterpart in the reference. */
+ * out[1] rounds with tie runs too long for the in-kernel sorts, out[2]
+ * last rounds confirmed without ranking (RBSC_FLAG_FULLROUNDS); out holds 3
+ * words.  No counterpart in the reference. */
 int rbsc_ctx_stats(rbsc_ctx* ctx, uint64_t* out, int reset);
 
 /* Exact, hash-free stability check of a colouring of g (colors: one u32

@@ -23,14 +23,14 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (FLAG_COLOUR_ORDER, FLAG_PACKED, FLAG_UNPACKED, FLAG_SORTRANK, FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
+from ._lib import (FLAG_COLOUR_ORDER, FLAG_PACKED, FLAG_UNPACKED, FLAG_SORTRANK, FLAG_FULLROUNDS, FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
                    RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
            "PinnedArray",
            "Server", "RequestResult",
            "RefineResult", "RbscError", "load",
-           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "FLAG_COLOUR_ORDER", "FLAG_PACKED", "FLAG_UNPACKED", "FLAG_SORTRANK", "gen"]
+           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "FLAG_COLOUR_ORDER", "FLAG_PACKED", "FLAG_UNPACKED", "FLAG_SORTRANK", "FLAG_FULLROUNDS", "gen"]
 
 _MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
           "aggressive": MODE_QUOTIENT_AGGRESSIVE}
@@ -531,11 +531,13 @@ class Context:
         return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(self.PHASES)}
 
     def stats(self, reset: bool = False) -> dict:
-        """Node-side slow-path counters: rounds redone with a full sort, and
-        rounds with tie runs too long for the in-kernel sorts."""
-        v = (C.c_uint64 * 2)()
+        """Node-side counters: rounds redone with a full sort, rounds with tie
+        runs too long for the in-kernel sorts, and last rounds confirmed
+        without ranking (see FLAG_FULLROUNDS)."""
+        v = (C.c_uint64 * 3)()
         check(load().rbsc_ctx_stats(self.handle, v, 1 if reset else 0))
-        return {"full_sort_redos": int(v[0]), "long_run_rounds": int(v[1])}
+        return {"full_sort_redos": int(v[0]), "long_run_rounds": int(v[1]),
+                "confirmed_rounds": int(v[2])}
 
     def refine(self, g: HostGraph, max_batch_edges=0, mode="stream", max_rounds=0,
                verify=False, partition=True, flags=0, partition_out=None) -> RefineResult:

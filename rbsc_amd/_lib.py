@@ -36,6 +36,7 @@ FLAG_COLOUR_ORDER = 128
 FLAG_PACKED = 512
 FLAG_UNPACKED = 1024
 FLAG_SORTRANK = 2048
+FLAG_FULLROUNDS = 4096
 BIN_SLOT_ORDER = 1
 BIN_HASHED = 2
 

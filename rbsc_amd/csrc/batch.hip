@@ -734,6 +734,7 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
         rbsc_dgraph* dg = b->big_g[i];
         RefineParams prm;
         prm.max_rounds = max_rounds;
+        prm.exact_colours = colors_out != nullptr;  // raw colours of the last round
         RefineOut o;
         rc = refine_core(ctx, dg, prm, &o);
         if (rc) return rc;

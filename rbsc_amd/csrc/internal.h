@@ -212,6 +212,7 @@ struct RefineParams {
     u32 max_rounds = 0;               // 0 = to the fixpoint
     u64 slice_lines = 0;              // streaming slice size; 0 = lines resident
     u32 flags = 0;                    // RBSC_FLAG_*
+    bool exact_colours = false;       // the caller reads the last round's raw colours
 };
 struct RefineOut {
     u32 K = 0;
@@ -235,6 +236,7 @@ struct rbsc_ctx {
     u64 full_sort_redos = 0;   // rounds whose node side was redone (run list overflow, or
                                // equal signatures out of colour order)
     u64 long_run_rounds = 0;   // rounds with runs too long for the in-kernel tie sorts
+    u64 confirmed_rounds = 0;  // last rounds confirmed from the previous round's blocks
     u32* h_scalar = nullptr;   // pinned mirror
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // streaming double buffer

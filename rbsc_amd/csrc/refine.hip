@@ -756,11 +756,16 @@ __global__ __launch_bounds__(256) void rank_kernel(
 // of both signatures (sig[node]) and both old colours (col[node]) read and
 // compared (and a smaller colour after an equal signature sets *disorder).
 // Writes col_out[node] = rank.  Tiles, waves and the decoupled look-back are
-// those of rank_kernel.
+// those of rank_kernel.  With `members` (else null): every position whose
+// pair equals its predecessor's appends its node and the predecessor's to
+// members[0, *members_n) (wave-aggregated), so the list holds every node of
+// every block of two or more nodes, some twice -- what the next round
+// needs to see whether any block split (confirm_check_kernel).
 __global__ __launch_bounds__(256) void rank_packed_kernel(
     const u64* __restrict__ keys, const u64* __restrict__ sig, const u32* __restrict__ col,
     u32* __restrict__ col_out, u64 n, u32 lo, u64 rank_offset, u64* status, u64* ticket,
-    u64 ticket_base, u32 epoch, u32* K_out, u32* disorder, u32* err) {
+    u64 ticket_base, u32 epoch, u32* K_out, u32* disorder, u32* err, u32* __restrict__ members,
+    u32* __restrict__ members_n) {
     __shared__ u64 s_wave[4];
     __shared__ u64 s_tile, s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -778,6 +783,7 @@ __global__ __launch_bounds__(256) void rank_packed_kernel(
     // neighbours of the wave's first and last positions
     const u64 wend = wbase + 64 * RANK_ITEMS;
     u64 lk = (wbase > 0 && wbase < n) ? keys[wbase - 1] : 0;
+    const u64 lk0 = lk;
     const u64 nk_last = wend < n ? keys[wend] : 0;
     // low signature bits and colour of the first position's predecessor,
     // read only when it shares the first position's top bits
@@ -879,6 +885,63 @@ __global__ __launch_bounds__(256) void rank_packed_kernel(
             col_out[k[j] & lowmask] = (u32)rj;
             if (i == n - 1) *K_out = (u32)(rj + 1);
         }
+    }
+    if (members) {
+        // positions sharing their predecessor's pair: counted over the whole
+        // wave first, so the wave reserves its output with one atomic
+        u32 total = 0;
+#pragma unroll
+        for (int j = 0; j < RANK_ITEMS; ++j) {
+            const u64 i = wbase + 64 * j + lane;
+            total += (u32)__popcll(__ballot(i < n && i > 0 && !((bal[j] >> lane) & 1ULL)));
+        }
+        if (total) {
+            u32 at = 0;
+            if (lane == 0) at = atomicAdd(members_n, 2u * total);
+            at = __shfl(at, 0);
+            u64 pk = lk0;  // predecessor of the step's lane 0
+#pragma unroll
+            for (int j = 0; j < RANK_ITEMS; ++j) {
+                const u64 i = wbase + 64 * j + lane;
+                u64 pred = __shfl_up(k[j], 1);
+                if (lane == 0) pred = pk;
+                pk = __shfl(k[j], 63);
+                const bool same = i < n && i > 0 && !((bal[j] >> lane) & 1ULL);
+                const u64 m = __ballot(same);
+                if (same) {
+                    const u32 o = at + 2u * (u32)__popcll(m & below);
+                    members[o] = (u32)(k[j] & lowmask);
+                    members[o + 1] = (u32)(pred & lowmask);
+                }
+                at += 2u * (u32)__popcll(m);
+            }
+        }
+    }
+}
+
+// Did any block of the previous round split?  The previous round's
+// rank_packed_kernel listed every node of every block of two or more nodes
+// (members, some twice; singletons cannot split).  Each listed node names
+// itself its block's representative (any member will do: the racing stores
+// leave one of them), then compares its signature with the
+// representative's; any difference sets *split.  No difference means the
+// round's (signature, colour) pairs are as many as the old colours: the
+// block count repeats and the refinement has reached its fixpoint.
+__global__ void confirm_rep_kernel(const u32* __restrict__ members, u64 m,
+                                   const u32* __restrict__ col, u32* __restrict__ rep) {
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        const u32 v = members[t];
+        rep[col[v]] = v;
+    }
+}
+__global__ void confirm_check_kernel(const u32* __restrict__ members, u64 m,
+                                     const u32* __restrict__ col, const u64* __restrict__ sig,
+                                     const u32* __restrict__ rep, u32* __restrict__ split) {
+    const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        const u32 v = members[t];
+        if (sig[v] != sig[rep[col[v]]]) atomicOr(split, 1u);
     }
 }
 
@@ -1485,6 +1548,17 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                                                                     g->U - g->R, hcbuf, c32 ? 0 : 1);
     };
     rep_colors(w.hc);
+    // The last round (the one whose block count repeats) is confirmed from
+    // the previous round's blocks of two or more nodes instead of being
+    // sorted and ranked, when only the partition is read afterwards: its
+    // colours would name the same blocks as the previous round's (see
+    // confirm_check_kernel).  Callers that read raw colours or stop after a
+    // given round get the ranked round.
+    const bool confirm_ok = node_order && !prm.max_rounds && !prm.exact_colours &&
+                            !(prm.flags & RBSC_FLAG_FULLROUNDS);
+    bool have_members = false;  // members lists the previous round's multi-node blocks
+    u64 members_n = 0;
+    u32* members = reinterpret_cast<u32*>(w.keys[1]);  // 2 cap_n words; free between rounds
     do {
         K = Knew;
         const u64 dang_h = rbsc_randomize(0 + (u64)it);
@@ -1543,6 +1617,29 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             dup_sig_kernel<<<grid_for(g->U - g->R), BLK, 0, s>>>(g->dup_rep, g->R, g->U - g->R,
                                                                  w.sig);
         prof_end(ctx);
+        if (confirm_ok && have_members && members_n <= n_rank / 4) {
+            // no block split: the block count repeats, fixpoint (K stays)
+            prof_begin(ctx, 3);
+            u32* rep = reinterpret_cast<u32*>(w.P[0]);  // one word per old colour (< K)
+            const u32* hc32 = reinterpret_cast<const u32*>(w.hc);
+            fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 14, 1, 0);
+            if (members_n) {
+                confirm_rep_kernel<<<grid_for(members_n), BLK, 0, s>>>(members, members_n, hc32, rep);
+                confirm_check_kernel<<<grid_for(members_n), BLK, 0, s>>>(members, members_n, hc32,
+                                                                         w.sig, rep,
+                                                                         ctx->d_scalar + 14);
+            }
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 14, ctx->d_scalar + 14, sizeof(u32),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            prof_end(ctx);
+            if (ctx->h_scalar[14] == 0) {
+                ++ctx->confirmed_rounds;
+                ++it;
+                break;
+            }
+        }
+        have_members = false;
         // Node side: keys in colour order, stable sort by signature, dense rank.
         // sorted_pc is the P buffer holding the sorted order (input of rank).
         int sorted_pc = pc;
@@ -1644,13 +1741,14 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             const u64 tiles = (n_rank + RANK_TILE - 1) / RANK_TILE;
             const u32 ep = lb_next_epoch(&ctx->sort, s);
             prof_begin(ctx, 3);
-            if (rk_packed)
+            if (rk_packed) {
+                if (confirm_ok) fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 15, 1, 0);
                 rank_packed_kernel<<<(unsigned)tiles, 256, 0, s>>>(
                     rk_keys, w.sig, reinterpret_cast<const u32*>(w.hc),
                     reinterpret_cast<u32*>(w.hc2), n_rank, lo, rank_offset, w.rank_status,
                     ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar, ctx->d_scalar + 3,
-                    ctx->d_err);
-            else
+                    ctx->d_err, confirm_ok ? members : nullptr, ctx->d_scalar + 15);
+            } else
                 rank_kernel<<<(unsigned)tiles, 256, 0, s>>>(
                     rk_keys, rk_P, rk_Pout, w.hc2, n_rank, rank_offset, (u64)it + 1, c32,
                     w.rank_status, ctx->sort.ticket, ctx->sort.ticket_total, ep, ctx->d_scalar,
@@ -1789,7 +1887,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         }
         HIP_TRY(hipGetLastError());
         auto read_scalars = [&]() -> int {
-            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 12 * sizeof(u32),
+            HIP_TRY(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, 16 * sizeof(u32),
                                    hipMemcpyDeviceToHost, s));
             HIP_TRY(hipMemcpyAsync(ctx->h_scalar + 4, ctx->d_err, sizeof(u32),
                                    hipMemcpyDeviceToHost, s));
@@ -1798,6 +1896,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         };
         if ((rc = read_scalars())) return rc;
         if (ctx->h_scalar[4]) return check_kernel_errors(ctx);  // K of a failed round is void
+        bool redone = false;
         if (n_rank && !ctx->h_scalar[1] && ctx->h_scalar[10]) {
             // Runs too long for the in-kernel sorts: sort them in a compact
             // array and rank again (the rank pass is a pure function of the
@@ -1815,6 +1914,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             // side from positions sorted by the (unchanged) old colours, with
             // the stable full 64-bit sort by signature.
             ++ctx->full_sort_redos;
+            redone = true;
             fill_u32<<<1, 32, 0, s>>>(ctx->d_scalar + 1, 3, 0);
             colour_keys_kernel<<<grid_for(n_rank), BLK, 0, s>>>(
                 reinterpret_cast<const u32*>(w.hc), n_rank, w.keys[0], w.P[0]);
@@ -1838,6 +1938,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             // colour order, so the full stable sort gives (signature, colour)
             // order.
             ++ctx->full_sort_redos;
+            redone = true;
             fill_u32<<<1, 1, 0, s>>>(ctx->d_scalar + 1, 1, 0);
             pc = sorted_pc;
             rc = sort_and_rank(true, false);
@@ -1845,6 +1946,10 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             if ((rc = read_scalars())) return rc;
             if (ctx->h_scalar[4]) return check_kernel_errors(ctx);
         }
+        // the member list stands if this round's ranks came from the packed
+        // rank pass (the redo paths below it sort through keys[1])
+        have_members = confirm_ok && rk_packed && !redone && n_rank;
+        members_n = have_members ? ctx->h_scalar[15] : 0;
         if (n_rank) pc = sorted_pc ^ 1;
         std::swap(w.hc, w.hc2);  // next round reads the colours ranked here
         Knew = n_rank ? ctx->h_scalar[0] : (u32)rank_offset;
@@ -1991,6 +2096,7 @@ static int refine_resident(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_opts* opts,
     auto t0 = std::chrono::steady_clock::now();
     RefineParams prm;
     prm.max_rounds = opts ? opts->max_rounds : 0;
+    prm.exact_colours = colors_out != nullptr;
     prm.flags = opts ? opts->flags : 0;
     if (opts && opts->max_batch_edges && opts->max_batch_edges < g->E_in)
         prm.slice_lines = opts->max_batch_edges;
@@ -2600,7 +2706,8 @@ extern "C" int rbsc_ctx_stats(rbsc_ctx* ctx, uint64_t* out, int reset) {
     }
     out[0] = ctx->full_sort_redos;
     out[1] = ctx->long_run_rounds;
-    if (reset) ctx->full_sort_redos = ctx->long_run_rounds = 0;
+    out[2] = ctx->confirmed_rounds;
+    if (reset) ctx->full_sort_redos = ctx->long_run_rounds = ctx->confirmed_rounds = 0;
     return RBSC_OK;
 }
 

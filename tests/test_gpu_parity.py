@@ -263,6 +263,35 @@ def test_streaming_equals_resident(ctx, frac):
     dg.free()
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_last_round_confirmed_equals_ranked(ctx, name):
+    """The last round confirmed from the previous round's multi-node blocks
+    (no sort, no rank) gives the partition, block count and rounds of the
+    ranked last round (RBSC_FLAG_FULLROUNDS), which equal the oracle's; the
+    confirmation fires on the node-order cases with a packed previous round,
+    and asking for colours gets the ranked round's colours."""
+    g = CASES[name]()
+    dg = ctx.upload(g)
+    a = dg.refine(flags=rbsc_amd.FLAG_FULLROUNDS)
+    ctx.stats(reset=True)
+    b = dg.refine()
+    st = ctx.stats(reset=True)
+    assert (a.nodes, a.rounds) == (b.nodes, b.rounds)
+    assert np.array_equal(a.partition, b.partition)
+    c, K, R, _ = _oracle_full(g)
+    assert (b.nodes, b.rounds) == (K, R)
+    assert np.array_equal(b.partition, oracle.canonical(c))
+    if name.startswith("rmat"):
+        assert st["confirmed_rounds"] == 1, st
+    col = dg.refine(colors=True)
+    assert np.array_equal(col.colors, c)
+    dg.bin()
+    bb = dg.refine(verify=True)
+    assert (bb.nodes, bb.rounds, bb.verified) == (K, R, 1)
+    assert np.array_equal(bb.partition, b.partition)
+    dg.free()
+
+
 def test_partition_into_page_locked_memory(ctx):
     """rbsc_refine writing its labels straight into an rbsc_host_alloc buffer
     (no staging copy) gives the same partition as into pageable memory, and
