@@ -123,6 +123,14 @@ typedef struct rbsc_opts {
  * round).  RBSC_FLAG_FULLROUNDS ranks the last round too (a test hook); so
  * do a round limit and a colours_out request. */
 #define RBSC_FLAG_FULLROUNDS 4096u
+/* Binned graph without initial colours: in the first round every gathered
+ * head has colour N+1 and every other head colour 0 (rbsc.cu:51-63), so each
+ * row's signature is randomize(N+1) * (weight of its gathered lines) +
+ * randomize(0) * (weight of its other lines) in Z/2^64, computed per row
+ * from the layout's per-row weights without reading the lines.
+ * RBSC_FLAG_GATHER_FIRST gathers the first round's colours like every other
+ * round's (a test hook).  Signatures and results are identical. */
+#define RBSC_FLAG_GATHER_FIRST 8192u
 
 /* Result.  nodes / done / time_s are the reference's three output lines
  * (rbsc.cu:492-494). */

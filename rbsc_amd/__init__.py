@@ -23,14 +23,14 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (FLAG_COLOUR_ORDER, FLAG_PACKED, FLAG_UNPACKED, FLAG_SORTRANK, FLAG_FULLROUNDS, FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
+from ._lib import (FLAG_COLOUR_ORDER, FLAG_GATHER_FIRST, FLAG_PACKED, FLAG_UNPACKED, FLAG_SORTRANK, FLAG_FULLROUNDS, FLAG_FULLSORT, FLAG_HASHED64, FLAG_NOBINS, FLAG_SPLIT, FLAG_XCD, MODE_QUOTIENT, MODE_QUOTIENT_AGGRESSIVE, MODE_STREAM,
                    RbscError, check, load)
 
 __all__ = ["Context", "DeviceGraph", "HostGraph", "HostBatch", "DeviceBatch", "BatchResult",
            "PinnedArray",
            "Server", "RequestResult",
            "RefineResult", "RbscError", "load",
-           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "FLAG_COLOUR_ORDER", "FLAG_PACKED", "FLAG_UNPACKED", "FLAG_SORTRANK", "FLAG_FULLROUNDS", "gen"]
+           "MODE_STREAM", "MODE_QUOTIENT", "MODE_QUOTIENT_AGGRESSIVE", "FLAG_HASHED64", "FLAG_XCD", "FLAG_SPLIT", "FLAG_FULLSORT", "FLAG_NOBINS", "FLAG_COLOUR_ORDER", "FLAG_PACKED", "FLAG_UNPACKED", "FLAG_SORTRANK", "FLAG_FULLROUNDS", "FLAG_GATHER_FIRST", "gen"]
 
 _MODES = {"stream": MODE_STREAM, "quotient": MODE_QUOTIENT,
           "aggressive": MODE_QUOTIENT_AGGRESSIVE}

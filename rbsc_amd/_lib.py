@@ -37,6 +37,7 @@ FLAG_PACKED = 512
 FLAG_UNPACKED = 1024
 FLAG_SORTRANK = 2048
 FLAG_FULLROUNDS = 4096
+FLAG_GATHER_FIRST = 8192
 BIN_SLOT_ORDER = 1
 BIN_HASHED = 2
 

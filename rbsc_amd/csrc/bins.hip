@@ -611,7 +611,8 @@ __global__ __launch_bounds__(SIG_THREADS) void bin_sig_kernel(
     const bool only = (ch.flags & CHUNK_ONLY) != 0;
     for (u32 i = threadIdx.x; i < nrows; i += SIG_THREADS) {
         if (only) sig[row0 + i] = sums[i];
-        else atomicAdd((unsigned long long*)&sig[row0 + i], (unsigned long long)sums[i]);
+        else if (sums[i])  // adding 0 changes nothing (rows without lines in this chunk)
+            atomicAdd((unsigned long long*)&sig[row0 + i], (unsigned long long)sums[i]);
     }
 }
 

@@ -264,40 +264,72 @@ __global__ void dup_rep_kernel(const u32* __restrict__ src_row, const u32* __res
 // LT_LINES output lines: line e of new row r comes from raw line
 // e + (row_ptr_raw[src_row[r]] - row_ptr_new[r]).  Each tile also writes its
 // 64 words of the row-end bitmap (tiles are word aligned, so no word is
-// shared).
+// shared).  With dang_w (no initial colours) it also adds up, per row, the
+// weight of the lines into nodes without lines (new ids >= U) into dang_w
+// and, with weights, the weight of all lines into row_w (both zeroed; one
+// atomic per row and tile).
 template <bool HAS_WT>
 __global__ __launch_bounds__(LT_THREADS) void copy_rows_tile_kernel(
     const u64* __restrict__ row_ptr_new, const u32* __restrict__ src_row,
     const u64* __restrict__ row_ptr_raw, const u32* __restrict__ dst_raw,
     const u64* __restrict__ wt_raw, const u32* __restrict__ pi, u64 R, u64 E,
     const u32* __restrict__ first, u32* __restrict__ dst_new, u64* __restrict__ wt_new,
-    u32* __restrict__ end_bits) {
+    u32* __restrict__ end_bits, u32 U, unsigned long long* __restrict__ dang_w,
+    unsigned long long* __restrict__ row_w) {
     __shared__ LtLds s;
     __shared__ u64 base[LT_LINES + 1];
     __shared__ u32 endw[LT_LINES / 32];
+    __shared__ unsigned long long acc_d[LT_LINES + 1];
+    __shared__ unsigned long long acc_w[HAS_WT ? LT_LINES + 1 : 1];
     const u64 t = blockIdx.x;
     u32 row[LT_PER];
     const u32 last = lt_rows(row_ptr_new, R, E, first, t, &s, row);
     const u32 r0 = first[t], nr = last - r0 + 1;
-    for (u32 i = threadIdx.x; i < nr; i += LT_THREADS)
+    for (u32 i = threadIdx.x; i < nr; i += LT_THREADS) {
         base[i] = row_ptr_raw[src_row[r0 + i]] - s.start[i];
+        acc_d[i] = 0;
+        if (HAS_WT) acc_w[i] = 0;
+    }
     for (u32 i = threadIdx.x; i < LT_LINES / 32; i += LT_THREADS) endw[i] = 0;
     __syncthreads();
     const u64 e0 = t * LT_LINES, b0 = e0 + (u64)threadIdx.x * LT_PER;
+    u64 d = 0, wsum = 0;  // this thread's run of lines of row cur
+    u32 cur = row[0];
 #pragma unroll
     for (int j = 0; j < LT_PER; ++j) {
         const u64 e = b0 + j;
         if (e < E) {
             const u32 k = row[j] - r0;
             const u64 src = e + base[k];
-            dst_new[e] = pi[dst_raw[src]];
-            if (HAS_WT) wt_new[e] = wt_raw[src];
+            const u32 v = pi[dst_raw[src]];
+            dst_new[e] = v;
+            const u64 wv = HAS_WT ? wt_raw[src] : 1;
+            if (HAS_WT) wt_new[e] = wv;
             if (e + 1 == s.start[k + 1])
                 atomicOr(&endw[(e - e0) >> 5], 1u << ((e - e0) & 31));
+            if (dang_w) {
+                if (row[j] != cur) {
+                    if (d) atomicAdd(&acc_d[cur - r0], (unsigned long long)d);
+                    if (HAS_WT && wsum) atomicAdd(&acc_w[cur - r0], (unsigned long long)wsum);
+                    d = wsum = 0;
+                    cur = row[j];
+                }
+                if (v >= U) d += wv;
+                if (HAS_WT) wsum += wv;
+            }
         }
+    }
+    if (dang_w) {
+        if (d) atomicAdd(&acc_d[cur - r0], (unsigned long long)d);
+        if (HAS_WT && wsum) atomicAdd(&acc_w[cur - r0], (unsigned long long)wsum);
     }
     __syncthreads();
     for (u32 i = threadIdx.x; i < LT_LINES / 32; i += LT_THREADS) end_bits[e0 / 32 + i] = endw[i];
+    if (dang_w)
+        for (u32 i = threadIdx.x; i < nr; i += LT_THREADS) {
+            if (acc_d[i]) atomicAdd(&dang_w[r0 + i], acc_d[i]);
+            if (HAS_WT && acc_w[i]) atomicAdd(&row_w[r0 + i], acc_w[i]);
+        }
 }
 
 __global__ void permute_init_kernel(const u32* __restrict__ init_raw, const u32* __restrict__ pi,
@@ -466,21 +498,32 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
         dup_rep_kernel<<<grid_for(U - R), BLK, 0, s>>>(src_row, rep_raw, raw->row_node, g->pi, R, U,
                                                        g->dup_rep);
     }
+    if (!raw->init && R) {  // per-row weights of the first round's closed form
+        B_TRY(hipMalloc(&g->dang_w, sizeof(u64) * R));
+        B_TRY(hipMemsetAsync(g->dang_w, 0, sizeof(u64) * R, s));
+        if (raw->wt) {
+            B_TRY(hipMalloc(&g->row_w, sizeof(u64) * R));
+            B_TRY(hipMemsetAsync(g->row_w, 0, sizeof(u64) * R, s));
+        }
+    }
     g->bytes = sizeof(u64) * (R + 1) + 4 * N + 4 * E + (raw->wt ? 8 * E : 0) +
-               (raw->init ? 4 * N : 0) + 4 * g->end_words + 16 * (g->n_tiles + 1) + 4 * (U - R);
+               (raw->init ? 4 * N : 0) + 4 * g->end_words + 16 * (g->n_tiles + 1) + 4 * (U - R) +
+               (g->dang_w ? 8 * (u64)R : 0) + (g->row_w ? 8 * (u64)R : 0);
     set_last_kernel<<<1, 1, 0, s>>>(g->row_ptr, R, E);
     zero_u32<<<grid_for(g->end_words), BLK, 0, s>>>(g->end_bits, g->end_words);
     if (R) {
         const u64 nt = lt_tiles(E);
         lt_first_row_kernel<<<grid_for(nt + 1), BLK, 0, s>>>(g->row_ptr, R, E, nt, tfirst);
+        auto* dw = reinterpret_cast<unsigned long long*>(g->dang_w);
+        auto* rw = reinterpret_cast<unsigned long long*>(g->row_w);
         if (raw->wt)
             copy_rows_tile_kernel<true><<<(unsigned)nt, LT_THREADS, 0, s>>>(
                 g->row_ptr, src_row, raw->row_ptr, raw->dst, raw->wt, g->pi, R, E, tfirst, g->dst,
-                g->wt, g->end_bits);
+                g->wt, g->end_bits, (u32)U, dw, rw);
         else
             copy_rows_tile_kernel<false><<<(unsigned)nt, LT_THREADS, 0, s>>>(
                 g->row_ptr, src_row, raw->row_ptr, raw->dst, nullptr, g->pi, R, E, tfirst, g->dst,
-                nullptr, g->end_bits);
+                nullptr, g->end_bits, (u32)U, dw, nullptr);
     }
     if (raw->init) permute_init_kernel<<<grid_for(N), BLK, 0, s>>>(raw->init, g->pi, N, g->init);
     if (g->n_tiles)

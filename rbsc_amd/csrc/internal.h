@@ -150,6 +150,12 @@ struct rbsc_dgraph {
     u32* dst = nullptr;         // E, internal ids (null when streamed)
     u64* wt = nullptr;          // E or null (all 1)
     u32* init = nullptr;        // N (internal order) or null
+    // Without initial colours: per stored row, the weight of its lines into
+    // nodes without lines (internal ids >= U) and, with weights, the weight of
+    // all its lines (without weights: its line count); the first round's
+    // signatures follow from them (first_round_sig).
+    u64* dang_w = nullptr;      // R or null
+    u64* row_w = nullptr;       // R or null
     u32* end_bits = nullptr;    // bit p: line p is the last line of its row
     u64 end_words = 0;
     u64 n_tiles = 0;            // wave tiles of SIG_TW lines

@@ -105,6 +105,24 @@ constexpr int BLK = 256;
 inline unsigned grid_for(u64 n, int blk = BLK) { return (unsigned)((n + blk - 1) / blk); }
 
 // ---------------------------------------------------------------- kernels
+// Signatures of the first round from a uniform start (rbsc.cu:51-63): every
+// node with a line has colour N+1 (hot_h = randomize(N + 1 + 0)) and every
+// other node colour 0 (dang_h = randomize(0 + 0)), so a stored row's
+// signature, the sum over its lines of randomize(c(v)) * w
+// (rbsc.cu:80-87, 366-377), is hot_h times the weight of its lines into
+// nodes with lines plus dang_h times the weight of the others, in Z/2^64.
+// row_w: the rows' total weights, or null without weights (the line count).
+__global__ void first_round_sig_kernel(const u64* __restrict__ row_ptr,
+                                       const u64* __restrict__ row_w,
+                                       const u64* __restrict__ dang_w, u64 R, u64 hot_h,
+                                       u64 dang_h, u64* __restrict__ sig) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    const u64 d = dang_w[i];
+    const u64 w = row_w ? row_w[i] : row_ptr[i + 1] - row_ptr[i];
+    sig[i] = hot_h * (w - d) + dang_h * d;
+}
+
 __global__ void fill_u64(u64* p, u64 n, u64 v) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1313,6 +1331,8 @@ void dgraph_release(rbsc_dgraph* g) {
     if (g->tile_row) (void)hipFree(g->tile_row);
     if (g->carry_row) (void)hipFree(g->carry_row);
     if (g->dup_rep) (void)hipFree(g->dup_rep);
+    if (g->dang_w) (void)hipFree(g->dang_w);
+    if (g->row_w) (void)hipFree(g->row_w);
     if (g->bins) bins_release(g->bins);
     *g = rbsc_dgraph();
 }
@@ -1521,6 +1541,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         for (int b = 0; b < 2; ++b) HIP_TRY(hipEventRecord(ctx->ev_done[b], s));
     }
     bool prefetched[2] = {false, false};
+    bool streamed_once = false;  // a round has streamed the slices into the buffers
     auto issue_copy = [&](u64 sl) -> int {
         const int b = (int)(sl & 1);
         const u64 lo = sl * Bs, hi = std::min<u64>(g->E, lo + Bs);
@@ -1562,11 +1583,25 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     do {
         K = Knew;
         const u64 dang_h = rbsc_randomize(0 + (u64)it);
-        if (binned) {
+        // First round from a uniform start: signatures in closed form from the
+        // per-row weights (first_round_sig_kernel), on every layout; the
+        // colour of the nodes with lines is N+1, hashed as the round-0 state
+        // holds it (raw u32 or u64).
+        const bool closed_first =
+            excluded && it == 0 && g->dang_w && !(prm.flags & RBSC_FLAG_GATHER_FIRST);
+        if (binned && !closed_first) {
             prof_begin(ctx, 6);
             bins_scatter(ctx, g, w.hc, w.cold);
             prof_end(ctx);
         }
+        if (closed_first) {  // timed with the round-0 set-up: it reads no line
+            prof_begin(ctx, 4);
+            const u64 hot_h = rbsc_randomize(c32 ? (u64)(u32)(N + 1) : N + 1);
+            if (g->R)
+                first_round_sig_kernel<<<grid_for(g->R), BLK, 0, s>>>(g->row_ptr, g->row_w, g->dang_w,
+                                                                     g->R, hot_h, dang_h, w.sig);
+            prof_end(ctx);
+        } else {
         prof_begin(ctx, 0);
         if (binned) {
             bins_sig(ctx, g, w.hc, w.cold, (u64)it, dang_h, w.sig);
@@ -1581,7 +1616,8 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         } else if (n_tiles && !streaming) {
             launch_sig(ctx, g, g->dst, g->wt, 0, n_tiles, 0, gather_limit, dang_h, it, c32);
         } else if (n_tiles) {
-            const bool resident = n_slices <= 2 && it > 0;  // both slices stay in the buffers
+            // both slices stay in the buffers once a round has streamed them
+            const bool resident = n_slices <= 2 && streamed_once;
             for (u64 sl = 0; sl < n_slices; ++sl) {
                 const int b = (int)(sl & 1);
                 if (!resident && sl < 2 && !prefetched[b]) {  // later slices were issued
@@ -1599,6 +1635,7 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                 }
             }
             prefetched[0] = prefetched[1] = false;
+            streamed_once = true;
             if (n_slices > 2) {  // next round's first slices overlap this round's node phase
                 for (u64 sl = 0; sl < 2; ++sl) {
                     rc = issue_copy(sl);
@@ -1608,8 +1645,9 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
             }
         }
         prof_end(ctx);
+        }  // the line pass
         prof_begin(ctx, 5);
-        if (n_tiles && !binned)
+        if (n_tiles && !binned && !closed_first)
             sig_fixup_kernel<<<grid_for(n_tiles), BLK, 0, s>>>(g->carry_row, w.carry,
                                                                g->n_parts > 1 ? g->seg_row : nullptr,
                                                                w.sig, n_tiles);

@@ -205,3 +205,38 @@ def test_staged_scatter_sparse_tiles(ctx):
             res = dg.refine(max_rounds=r, partition=False, colors=True)
             assert np.array_equal(res.colors, tr[r - 1]), f"{(scatter, hashed)} round {r}"
     dg.free()
+
+
+@pytest.mark.parametrize("name", ["rmat_s14_c30", "rmat_s12_w3", "rmat_s11_w64"])
+def test_first_round_closed_form_equals_gathered(ctx, name):
+    # From a uniform start the first round's signatures come from per-row
+    # weights (no line read); with RBSC_FLAG_GATHER_FIRST the line kernels
+    # gather them.  Both equal the oracle's colours round by round on the
+    # binned layout (raw and hashed forms), the plain layout, and lines
+    # streamed from host memory in two and in five slices, with repeated
+    # rows, weights up to 2^64 and rows whose lines all go to nodes without
+    # lines.
+    g = CASES[name]()
+    c, K, R, tr = _trace(g)
+    want = oracle.canonical(c)
+    dg = ctx.upload(g)
+
+    def check(label, flags=0, batch=0):
+        for f in (flags, flags | rbsc_amd.FLAG_GATHER_FIRST):
+            for r in (1, 2):
+                res = dg.refine(max_rounds=r, partition=False, colors=True, flags=f,
+                                max_batch_edges=batch)
+                assert np.array_equal(res.colors, tr[r - 1]), f"{label} flags={f} round {r}"
+            full = dg.refine(verify=not batch, flags=f, max_batch_edges=batch)
+            assert (full.nodes, full.rounds) == (K, R), f"{label} flags={f}"
+            assert batch or full.verified == 1
+            assert np.array_equal(full.partition, want), f"{label} flags={f}"
+
+    for hashed in (False, True):
+        dg.bin(0, 1024, 0, hashed=hashed)
+        check(f"binned hashed={hashed}")
+    check("plain", rbsc_amd.FLAG_NOBINS)
+    dg.lines_to_host()
+    for parts in (2, 5):
+        check(f"streamed in {parts}", batch=dg.stored_lines // parts + 1)
+    dg.free()
