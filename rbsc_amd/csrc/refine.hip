@@ -1518,9 +1518,12 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
         if (rc) return rc;
         pc = which;
     }
+    // Signatures of nodes without lines are 0 (rbsc.cu:104-116) and no line
+    // kernel writes them; without initial colours those nodes are not ranked
+    // and every round writes the stored rows' signatures before reading them.
     if (prm.d_sig_fill)
         permute_fill_kernel<<<grid_for(N), BLK, 0, s>>>(prm.d_sig_fill, g->pi, N, w.sig);
-    else
+    else if (!excluded)
         fill_u64<<<grid_for(N), BLK, 0, s>>>(w.sig, N, 0);
     prof_end(ctx);
 
