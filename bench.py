@@ -56,8 +56,15 @@ def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
               f"{R} rsc rounds, {K} blocks")
     host = {"host_threads": os.cpu_count(), "cpu_model": _cpu_model()}
     port = {"value": R * E / dt, "seconds": dt, "blocks": K}
+    # the oracle's exact lumping refiner (the Valmari tool's partition, by
+    # splitters), one core, on the same sample
+    t = time.perf_counter()
+    _, KL = oracle.lump_refine(g.n_nodes, src, g.dst)
+    lump = {"seconds": time.perf_counter() - t, "blocks": KL,
+            "method": "splitter refinement, oracle/lump.c"}
     out = {"value": port["value"], "unit": "edge-visits/s", "cores": 1, "kind": "port",
-           "sample": f"rsc restatement (oracle/rbsc_oracle.c) on {sample}", **host}
+           "sample": f"rsc restatement (oracle/rbsc_oracle.c) on {sample}", **host,
+           "lumping_port": lump}
     if not oracle.have_ref():
         return out
     with tempfile.TemporaryDirectory() as d:
@@ -74,7 +81,7 @@ def cpu_baseline(scale: int, log_draws: int, clone: float, seed: int) -> dict:
                 "sample": f"reference rsc (src/rsc/rsc.cpp, compiled from source, its own timer) on "
                           f"{sample}",
                 "port": port, "valmari": runs["valmari"],
-                "blocks_agree": runs["rsc"]["blocks"] == runs["valmari"]["blocks"] == K})
+                "blocks_agree": runs["rsc"]["blocks"] == runs["valmari"]["blocks"] == K == KL})
     return out
 
 

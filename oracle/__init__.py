@@ -3,8 +3,10 @@
 Only tests, the smoke check and the CPU-baseline leg of the bench may import
 this package; the product path (rbsc_amd, librbsc.so) never does.
 
-* liboracle.so (rbsc_oracle.c): plain-C restatements of rbsc.cu / rsc.cpp
-  and an exact hash-free refiner, each citing the reference file:line.
+* liboracle.so (rbsc_oracle.c, lump.c): plain-C restatements of rbsc.cu /
+  rsc.cpp, an exact hash-free refiner, and an exact lumping refiner by
+  splitters (the Valmari tool's partition), each citing the reference
+  file:line or the published method.
 * _ref/rsc and _ref/part_ref: the reference's own CPU tools compiled from
   src/rsc/rsc.cpp and src/part_ref/MDPmin.cc by oracle/Makefile (only where
   the reference tree is present; never shipped as source).
@@ -61,6 +63,10 @@ def _declare(L):
     L.oracle_exact_refine.restype = C.c_int
     L.oracle_exact_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, u32p, u32p,
                                       u32p, u32p]
+    if hasattr(L, "oracle_lump_refine"):  # copies built from rbsc_oracle.c alone lack it
+        L.oracle_lump_refine.restype = C.c_int
+        L.oracle_lump_refine.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, u32p, u32p,
+                                         u32p]
     L.oracle_rbsc_multibatch.restype = C.c_int
     L.oracle_rbsc_multibatch.argtypes = [C.c_uint32, C.c_uint64, u32p, u64p, u32p, C.c_uint64,
                                          C.c_int, u32p, u32p, u32p, u32p]
@@ -135,6 +141,22 @@ def exact_refine(n, src, dst, weight=None, init=None):
     if rc:
         raise MemoryError("oracle_exact_refine failed")
     return colors[:n], int(K.value), int(R.value)
+
+
+def lump_refine(n, src, dst, weight=None, init=None, _from=None):
+    """Exact coarsest lumping by splitter refinement (lump.c): the partition
+    of the Valmari tool's definition, computed in O(m log n) line scans.
+    Returns (labels in first-occurrence order, K)."""
+    src, dst, w = _arrs(src, dst, weight)
+    init = None if init is None else np.ascontiguousarray(init, np.uint32)
+    labels = np.empty(max(n, 1), np.uint32)
+    K = C.c_uint32()
+    rc = (_from or lib()).oracle_lump_refine(n, len(dst), _p(src, C.c_uint32), _p(w, C.c_uint64),
+                                  _p(dst, C.c_uint32), _p(init, C.c_uint32),
+                                  _p(labels, C.c_uint32), C.byref(K))
+    if rc:
+        raise MemoryError("oracle_lump_refine failed")
+    return labels[:n], int(K.value)
 
 
 def rbsc_multibatch(n, src, dst, weight, max_batch, aggressive=False, partition=False):
