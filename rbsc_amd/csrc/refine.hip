@@ -1136,21 +1136,24 @@ __global__ void map_colors_kernel(const u32* __restrict__ col_int, const u32* __
 // rbsc.cu:431-445): first[c] = least original id of colour c; the first
 // occurrences, flagged in original id order and scanned, number the blocks;
 // a node's label is the number of its colour's first occurrence.
-// (Colours above N are not expected; one sets *bad and is skipped, and the
+// first[] comes from a stable sort of (colour, id) pairs by colour: each
+// colour's run starts with its least id (one random atomic per node, the
+// earlier form, ran at about 3 G nodes/s on graphs of mostly singleton
+// blocks).  (Colours above N are not expected; one sets *bad, and the
 // caller then relabels on the host.)
-__global__ void canon_first_kernel(const u32* __restrict__ col, u64 N, u32* __restrict__ first,
+__global__ void canon_pairs_kernel(const u32* __restrict__ col, u64 N, u64* __restrict__ pairs,
                                    u32* __restrict__ bad) {
     const u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= N) return;
     const u32 c = col[u];
-    // read first: most nodes of a large class find an earlier id recorded
-    // and skip the atomic (the class of nodes without lines holds most ids)
-    if (c <= N) {
-        if ((u32)u < __hip_atomic_load(&first[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMin(&first[c], (u32)u);
-    } else {
-        atomicOr(bad, 1u);
-    }
+    if (c > N) atomicOr(bad, 1u);
+    pairs[u] = ((u64)c << 32) | u;
+}
+__global__ void canon_first_kernel(const u64* __restrict__ sorted, u64 N, u32* __restrict__ first) {
+    const u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const u64 k = sorted[p];
+    if (p == 0 || (sorted[p - 1] >> 32) != (k >> 32)) first[k >> 32] = (u32)k;
 }
 __global__ void canon_flag_kernel(const u32* __restrict__ col, const u32* __restrict__ first, u64 N,
                                   u64* __restrict__ flag) {
@@ -2078,17 +2081,23 @@ int device_partition(rbsc_ctx* ctx, rbsc_dgraph* g, u32* partition_out, u32* col
     if (colors_out)
         HIP_TRY(hipMemcpyAsync(colors_out, d_col, sizeof(u32) * N, hipMemcpyDeviceToHost, s));
     if (partition_out) {
-        u32* first = reinterpret_cast<u32*>(w.P[0]);  // N + 1 entries
-        u64* flag = w.P[1];
+        u32* first = reinterpret_cast<u32*>(w.keys[0]);  // N + 1 entries (d_int is done)
         // labels straight into the caller's page-locked buffer when it is
         // one from rbsc_host_alloc (no staging copy), else through hc2
         u32* direct = static_cast<u32*>(host_device_ptr(partition_out, sizeof(u32) * N));
         u32* d_lab = direct ? direct : reinterpret_cast<u32*>(w.hc2);
-        fill_u32_kernel<<<grid_for(N + 1), BLK, 0, s>>>(first, N + 1, 0xFFFFFFFFu);
         fill_u32_kernel<<<1, 32, 0, s>>>(ctx->d_scalar + 12, 1, 0);
-        canon_first_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, N, first, ctx->d_scalar + 12);
+        u64* pairs[2] = {w.P[0], w.P[1]};
+        canon_pairs_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, N, pairs[0], ctx->d_scalar + 12);
+        int cbits = 8;  // colours are at most N
+        while (cbits < 32 && (N >> cbits)) cbits += 8;
+        int wp = 0;
+        int rc = radix_sort_u64(&ctx->sort, pairs, nullptr, N, 32, 32 + cbits, &wp, s);
+        if (rc) return rc;
+        canon_first_kernel<<<grid_for(N), BLK, 0, s>>>(pairs[wp], N, first);
+        u64* flag = pairs[wp];  // the sorted pairs are done
         canon_flag_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, first, N, flag);
-        int rc = scan_excl_u64(flag, flag, N, nullptr, w.sig, s);
+        rc = scan_excl_u64(flag, flag, N, nullptr, w.sig, s);
         if (rc) return rc;
         canon_label_kernel<<<grid_for(N), BLK, 0, s>>>(d_col, first, flag, N, d_lab);
         if (!direct)
