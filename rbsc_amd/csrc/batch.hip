@@ -514,12 +514,15 @@ struct rbsc_dbatch {
     u64 bytes = 0;
 };
 
+// The batch's buffers come from ctx_alloc (a serving loop takes and gives
+// back one batch per launch while other refinements run).
 static void dbatch_release(rbsc_ctx* ctx, rbsc_dbatch* b) {
     for (rbsc_dgraph* g : b->big_g) rbsc_graph_free(ctx, g);
+    hipStream_t s = ctx ? ctx->stream : nullptr;
     for (void* p : {(void*)b->dev.desc, (void*)b->dev.order, (void*)b->dev.rp, (void*)b->dev.dst,
                     (void*)b->dev.wt, (void*)b->dev.init, (void*)b->d_labels, (void*)b->d_colors,
                     (void*)b->d_blocks, (void*)b->d_rounds, (void*)b->d_queue, (void*)b->d_sizes})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
 }
 
 extern "C" int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* in, rbsc_dbatch** out) {
@@ -645,7 +648,7 @@ extern "C" int rbsc_batch_upload(rbsc_ctx* ctx, const rbsc_batch* in, rbsc_dbatc
     }
     hipStream_t s = ctx->stream;
     auto up = [&](void** dptr, const void* src, size_t bytes) -> int {
-        HIP_TRY(hipMalloc(dptr, std::max<size_t>(bytes, 8)));
+        HIP_TRY(ctx_alloc(dptr, std::max<size_t>(bytes, 8), s));
         if (bytes && src) HIP_TRY(hipMemcpyAsync(*dptr, src, bytes, hipMemcpyHostToDevice, s));
         b->bytes += bytes;
         return RBSC_OK;
@@ -702,7 +705,7 @@ static int batch_run(rbsc_ctx* ctx, rbsc_dbatch* b, const rbsc_opts* opts, u32* 
         ? static_cast<u32*>(host_device_ptr(sizes_out, sizeof(u32) * b->total_nodes))
         : nullptr;
     if (sizes_out && !direct_sizes && !b->d_sizes)
-        HIP_TRY(hipMalloc(&b->d_sizes, sizeof(u32) * std::max<u64>(b->total_nodes, 1)));
+        HIP_TRY(ctx_alloc(&b->d_sizes, sizeof(u32) * std::max<u64>(b->total_nodes, 1), s));
     u32* d_sizes = sizes_out ? (direct_sizes ? direct_sizes : b->d_sizes) : nullptr;
     HIP_TRY(hipEventRecord(ctx->ev0, s));
     const int split_lo = (opts && (opts->flags & RBSC_FLAG_SPLIT16)) ? 48 : 32;

@@ -695,13 +695,13 @@ void bins_sig(rbsc_ctx* ctx, const rbsc_dgraph* g, const void* col, const void* 
     }
 }
 
-void bins_release(rbsc_bins* b) {
+void bins_release(rbsc_bins* b, hipStream_t s) {
     if (!b) return;
     for (void* p : {(void*)b->bin_slot, (void*)b->bin_hot, (void*)b->slot_key, (void*)b->slot_row,
                     (void*)b->slot_wt, (void*)b->dsum, (void*)b->csc_head, (void*)b->csc_slot,
                     (void*)b->tile_line, (void*)b->chunks, (void*)b->zero_bins,
                     (void*)b->cold_pos, (void*)b->csc_perm, (void*)b->warm_items})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
     delete b;
 }
 
@@ -773,15 +773,15 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
         rc = RBSC_ERR_INVALID;
         goto done;
     }
-    B_TRY(hipMalloc(&n_hot, sizeof(u64) * (R + 1)));
-    B_TRY(hipMalloc(&n_cold, sizeof(u64) * (R + 1)));
-    B_TRY(hipMalloc(&b->dsum, sizeof(u64) * std::max<u64>(R, 1)));
-    B_TRY(hipMalloc(&tmp, sizeof(u64) *
-                              scan_tmp_words(std::max<u64>(std::max<u64>(R, n_bins), n_lt) + 1)));
-    B_TRY(hipMalloc(&tot, sizeof(u64) * 2));
-    B_TRY(hipMalloc(&lt_first, sizeof(u32) * (n_lt + 1)));
-    B_TRY(hipMalloc(&lt_hot, sizeof(u64) * (n_lt + 1)));
-    B_TRY(hipMalloc(&lt_cold, sizeof(u64) * (n_lt + 1)));
+    B_TRY(ctx_alloc(&n_hot, sizeof(u64) * (R + 1), s));
+    B_TRY(ctx_alloc(&n_cold, sizeof(u64) * (R + 1), s));
+    B_TRY(ctx_alloc(&b->dsum, sizeof(u64) * std::max<u64>(R, 1), s));
+    B_TRY(ctx_alloc(&tmp, sizeof(u64) *
+                              scan_tmp_words(std::max<u64>(std::max<u64>(R, n_bins), n_lt) + 1), s));
+    B_TRY(ctx_alloc(&tot, sizeof(u64) * 2, s));
+    B_TRY(ctx_alloc(&lt_first, sizeof(u32) * (n_lt + 1), s));
+    B_TRY(ctx_alloc(&lt_hot, sizeof(u64) * (n_lt + 1), s));
+    B_TRY(ctx_alloc(&lt_cold, sizeof(u64) * (n_lt + 1), s));
     B_TRY(hipMemsetAsync(n_hot, 0, sizeof(u64) * (R + 1), s));
     B_TRY(hipMemsetAsync(n_cold, 0, sizeof(u64) * (R + 1), s));
     B_TRY(hipMemsetAsync(b->dsum, 0, sizeof(u64) * std::max<u64>(R, 1), s));
@@ -816,12 +816,12 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
         goto done;
     }
     // per-bin sizes and slot offsets
-    B_TRY(hipMalloc(&hot_start, sizeof(u64) * (n_bins + 1)));
-    B_TRY(hipMalloc(&cold_start, sizeof(u64) * (n_bins + 1)));
-    B_TRY(hipMalloc(&bin_n, sizeof(u64) * (n_bins + 1)));
-    B_TRY(hipMalloc(&hot_all, sizeof(u64) * (n_bins + 1)));
-    B_TRY(hipMalloc(&b->bin_hot, sizeof(u64) * (n_bins + 1)));
-    B_TRY(hipMalloc(&b->bin_slot, sizeof(u64) * (n_bins + 1)));
+    B_TRY(ctx_alloc(&hot_start, sizeof(u64) * (n_bins + 1), s));
+    B_TRY(ctx_alloc(&cold_start, sizeof(u64) * (n_bins + 1), s));
+    B_TRY(ctx_alloc(&bin_n, sizeof(u64) * (n_bins + 1), s));
+    B_TRY(ctx_alloc(&hot_all, sizeof(u64) * (n_bins + 1), s));
+    B_TRY(ctx_alloc(&b->bin_hot, sizeof(u64) * (n_bins + 1), s));
+    B_TRY(ctx_alloc(&b->bin_slot, sizeof(u64) * (n_bins + 1), s));
     bin_sizes_kernel<<<grid_for(n_bins + 1), BLK, 0, s>>>(n_hot, n_cold, R, log_s, n_bins, hot_start,
                                                           cold_start, bin_n, hot_all);
     h_bin_n.resize(n_bins + 1);
@@ -864,30 +864,30 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
     });
     b->n_chunks = h_chunks.size();
     b->n_zero = h_zero.size();
-    B_TRY(hipMalloc(&b->chunks, sizeof(BinChunk) * std::max<u64>(b->n_chunks, 1)));
+    B_TRY(ctx_alloc(&b->chunks, sizeof(BinChunk) * std::max<u64>(b->n_chunks, 1), s));
     B_TRY(hipMemcpyAsync(b->chunks, h_chunks.data(), sizeof(BinChunk) * b->n_chunks,
                          hipMemcpyHostToDevice, s));
     if (b->n_zero) {
-        B_TRY(hipMalloc(&b->zero_bins, sizeof(u32) * b->n_zero));
+        B_TRY(ctx_alloc(&b->zero_bins, sizeof(u32) * b->n_zero, s));
         B_TRY(hipMemcpyAsync(b->zero_bins, h_zero.data(), sizeof(u32) * b->n_zero,
                              hipMemcpyHostToDevice, s));
     }
     // slot arrays
     // + SLOT_RUN zeroed entries: the loads of the last run stay in bounds
-    B_TRY(hipMalloc(&b->slot_key, sizeof(u32) * (b->n_slots + SLOT_RUN)));
-    B_TRY(hipMalloc(&b->slot_row, sizeof(unsigned short) * (b->n_slots + SLOT_RUN)));
+    B_TRY(ctx_alloc(&b->slot_key, sizeof(u32) * (b->n_slots + SLOT_RUN), s));
+    B_TRY(ctx_alloc(&b->slot_row, sizeof(unsigned short) * (b->n_slots + SLOT_RUN), s));
     B_TRY(hipMemsetAsync(b->slot_key, 0, sizeof(u32) * (b->n_slots + SLOT_RUN), s));
     B_TRY(hipMemsetAsync(b->slot_row, 0, sizeof(unsigned short) * (b->n_slots + SLOT_RUN), s));
     if (g->wt) {
-        B_TRY(hipMalloc(&b->slot_wt, sizeof(u64) * (b->n_slots + SLOT_RUN)));
+        B_TRY(ctx_alloc(&b->slot_wt, sizeof(u64) * (b->n_slots + SLOT_RUN), s));
         B_TRY(hipMemsetAsync(b->slot_wt + b->n_slots, 0, sizeof(u64) * SLOT_RUN, s));
     }
     if ((rc = sort_ws_init(&ctx->sort, std::max(b->n_hot, b->n_cold), ctx->d_err, s))) goto done;
     // hot slots: records sorted by (bin, head)
     if (b->n_hot) {
         for (int i = 0; i < 2; ++i) {
-            B_TRY(hipMalloc(&keys[i], sizeof(u64) * b->n_hot));
-            B_TRY(hipMalloc(&vals[i], sizeof(u64) * b->n_hot));
+            B_TRY(ctx_alloc(&keys[i], sizeof(u64) * b->n_hot, s));
+            B_TRY(ctx_alloc(&vals[i], sizeof(u64) * b->n_hot, s));
         }
         bin_fill_tile_kernel<true><<<(unsigned)n_lt, LT_THREADS, 0, s>>>(
             g->row_ptr, g->dst, R, g->E, H, GL, rep, g->U, log_s, hb, lt_first, lt_hot, keys[0],
@@ -906,7 +906,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
         if (D < H) {
             // Warm segments: boundaries at about equal shares of the warm lines.
             const u64 nw = H - D;
-            B_TRY(hipMalloc(&warm_cnt, sizeof(u32) * nw));
+            B_TRY(ctx_alloc(&warm_cnt, sizeof(u32) * nw, s));
             B_TRY(hipMemsetAsync(warm_cnt, 0, sizeof(u32) * nw, s));
             warm_hist_kernel<<<grid_for(b->n_hot), BLK, 0, s>>>(keys[w], b->n_hot, hb, D, H,
                                                                warm_cnt);
@@ -937,10 +937,10 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
                 const int x = (round & 1) ? (int)BIN_GROUPS - 1 - pos : pos;
                 xcd_segs.seg[x][round] = (unsigned char)order[r];
             }
-            B_TRY(hipMalloc(&warm_bound, sizeof(u32) * (WARM_SEGS + 1)));
+            B_TRY(ctx_alloc(&warm_bound, sizeof(u32) * (WARM_SEGS + 1), s));
             B_TRY(hipMemcpyAsync(warm_bound, h_bound, sizeof(u32) * (WARM_SEGS + 1),
                                  hipMemcpyHostToDevice, s));
-            B_TRY(hipMalloc(&warm_off, sizeof(u64) * n_bins * (WARM_SEGS + 1)));
+            B_TRY(ctx_alloc(&warm_off, sizeof(u64) * n_bins * (WARM_SEGS + 1), s));
             warm_off_kernel<<<grid_for(n_bins * (WARM_SEGS + 1)), BLK, 0, s>>>(
                 b->slot_key, b->bin_slot, hot_all, n_bins, warm_bound, warm_off, b->bin_hot);
         } else {
@@ -949,8 +949,8 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
         }
         B_TRY(hipStreamSynchronize(s));
         for (int i = 0; i < 2; ++i) {
-            (void)hipFree(keys[i]);
-            (void)hipFree(vals[i]);
+            ctx_free(keys[i], s);
+            ctx_free(vals[i], s);
             keys[i] = vals[i] = nullptr;
         }
     }
@@ -996,7 +996,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
                 }
             }
             b->warm_first.f[BIN_GROUPS] = (u32)items.size();
-            B_TRY(hipMalloc(&b->warm_items, sizeof(WarmItem) * items.size()));
+            B_TRY(ctx_alloc(&b->warm_items, sizeof(WarmItem) * items.size(), s));
             B_TRY(hipMemcpyAsync(b->warm_items, items.data(), sizeof(WarmItem) * items.size(),
                                  hipMemcpyHostToDevice, s));
             b->bytes_warm = sizeof(WarmItem) * items.size();
@@ -1006,17 +1006,17 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
             rc = RBSC_ERR_INVALID;
             goto done;
         }
-        B_TRY(hipMalloc(&b->cold_pos, sizeof(u64) * std::max<u64>(n_bins, 1)));
+        B_TRY(ctx_alloc(&b->cold_pos, sizeof(u64) * std::max<u64>(n_bins, 1), s));
         B_TRY(hipMemcpyAsync(b->cold_pos, h_pos.data(), sizeof(u64) * n_bins,
                              hipMemcpyHostToDevice, s));
     }
     // cold slots and the scatter records
     if (b->n_cold) {
         for (int i = 0; i < 2; ++i) {
-            B_TRY(hipMalloc(&keys[i], sizeof(u64) * b->n_cold));
-            B_TRY(hipMalloc(&vals[i], sizeof(u64) * b->n_cold));
+            B_TRY(ctx_alloc(&keys[i], sizeof(u64) * b->n_cold, s));
+            B_TRY(ctx_alloc(&vals[i], sizeof(u64) * b->n_cold, s));
         }
-        B_TRY(hipMalloc(&grp_of_bin, sizeof(u32) * (n_bins + 1)));
+        B_TRY(ctx_alloc(&grp_of_bin, sizeof(u32) * (n_bins + 1), s));
         B_TRY(hipMemcpyAsync(grp_of_bin, h_grp.data(), sizeof(u32) * (n_bins + 1),
                              hipMemcpyHostToDevice, s));
         bin_fill_tile_kernel<false><<<(unsigned)n_lt, LT_THREADS, 0, s>>>(
@@ -1049,7 +1049,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
             h_first[k + 1] += h_first[k];
         }
         b->n_tiles = (most + scatter_chunk - 1) / scatter_chunk * BIN_GROUPS;
-        B_TRY(hipMalloc(&grp_first, sizeof(u64) * (BIN_GROUPS + 1)));
+        B_TRY(ctx_alloc(&grp_first, sizeof(u64) * (BIN_GROUPS + 1), s));
         B_TRY(hipMemcpyAsync(grp_first, h_first, sizeof(u64) * (BIN_GROUPS + 1),
                              hipMemcpyHostToDevice, s));
         u64* k3[2] = {k2[w2 ^ 1], k2[w2]};
@@ -1060,11 +1060,11 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
         int w3 = 0;
         bits = (32 + bitlen(b->n_tiles) + 7) / 8 * 8;
         if ((rc = radix_sort_u64(&ctx->sort, k3, v3, b->n_cold, 0, bits, &w3, s))) goto done;
-        B_TRY(hipMalloc(&b->csc_head, sizeof(u32) * (b->n_cold + SLOT_RUN)));
-        B_TRY(hipMalloc(&b->csc_slot, sizeof(u32) * (b->n_cold + SLOT_RUN)));
+        B_TRY(ctx_alloc(&b->csc_head, sizeof(u32) * (b->n_cold + SLOT_RUN), s));
+        B_TRY(ctx_alloc(&b->csc_slot, sizeof(u32) * (b->n_cold + SLOT_RUN), s));
         B_TRY(hipMemsetAsync(b->csc_head, 0, sizeof(u32) * (b->n_cold + SLOT_RUN), s));
         B_TRY(hipMemsetAsync(b->csc_slot, 0, sizeof(u32) * (b->n_cold + SLOT_RUN), s));
-        B_TRY(hipMalloc(&b->tile_line, sizeof(u64) * (b->n_tiles + 1)));
+        B_TRY(ctx_alloc(&b->tile_line, sizeof(u64) * (b->n_tiles + 1), s));
         csc_fill_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(k3[w3], v3[w3], b->n_cold, b->n_tiles,
                                                            b->csc_head, b->csc_slot, b->tile_line);
         if (head_order) {
@@ -1074,7 +1074,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
                                                                 v4[0]);
             int w4 = 0;
             if ((rc = radix_sort_u64(&ctx->sort, k4, v4, b->n_cold, 0, bits, &w4, s))) goto done;
-            B_TRY(hipMalloc(&b->csc_perm, sizeof(unsigned short) * b->n_cold));
+            B_TRY(ctx_alloc(&b->csc_perm, sizeof(unsigned short) * b->n_cold, s));
             stage_fill_kernel<<<grid_for(b->n_cold), BLK, 0, s>>>(
                 k4[w4], v4[w4], b->n_cold, b->tile_line, b->csc_slot, b->csc_perm);
         }
@@ -1087,7 +1087,7 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
                4 * b->n_zero + 8 * n_bins;
     if (g->bins) {
         g->bytes -= g->bins->bytes;
-        bins_release(g->bins);
+        bins_release(g->bins, s);
     }
     g->bins = b;
     g->bytes += b->bytes;
@@ -1095,10 +1095,10 @@ int dgraph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, u32 H, u32 D, u32 log_s, u64 chunk
 done:
     for (u64* p : {n_hot, n_cold, tmp, tot, hot_start, cold_start, bin_n, keys[0], keys[1],
                    vals[0], vals[1], grp_first, hot_all, warm_off, lt_hot, lt_cold})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
     for (u32* p : {grp_of_bin, warm_cnt, warm_bound, lt_first})
-        if (p) (void)hipFree(p);
-    if (b) bins_release(b);
+        ctx_free(p, s);
+    if (b) bins_release(b, s);
     return rc;
 #undef B_TRY
 }

@@ -99,3 +99,19 @@ __device__ __forceinline__ u64 lb_load(u64* p) {
     } while (0)
 
 extern "C" void rbsc_set_error(const char* fmt, ...);
+
+// Device memory that a context takes and gives back while other contexts may
+// have work in flight (its workspace and that workspace's growth, per-launch
+// buffers).  hipFree and hipHostFree wait for the work of every stream on the
+// device, so one serving worker growing its workspace would stall every other
+// refinement; stream-ordered allocation from the device's memory pool orders
+// the free after this stream's work only.  The pool keeps freed memory for
+// reuse (its release threshold is raised when a context is created).  Memory
+// from ctx_alloc is given back with ctx_free only.
+template <class T>
+inline hipError_t ctx_alloc(T** p, size_t bytes, hipStream_t s) {
+    return hipMallocAsync(reinterpret_cast<void**>(p), bytes, s);
+}
+inline void ctx_free(void* p, hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+}

@@ -80,13 +80,13 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
     u64 h_tot[2] = {0, 0};
     auto cleanup = [&]() {
         for (int i = 0; i < 2; ++i)
-            if (keys[i]) (void)hipFree(keys[i]);
-        if (rp) (void)hipFree(rp);
-        if (deg) (void)hipFree(deg);
-        if (flag) (void)hipFree(flag);
-        if (ro) (void)hipFree(ro);
-        if (tmp) (void)hipFree(tmp);
-        if (d_tot) (void)hipFree(d_tot);
+            ctx_free(keys[i], s);
+        ctx_free(rp, s);
+        ctx_free(deg, s);
+        ctx_free(flag, s);
+        ctx_free(ro, s);
+        ctx_free(tmp, s);
+        ctx_free(d_tot, s);
     };
 #define GEN_TRY(expr)                                                                   \
     do {                                                                                \
@@ -94,18 +94,18 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
         if (_e != hipSuccess) {                                                         \
             rbsc_set_error("%s failed: %s", #expr, hipGetErrorString(_e));              \
             cleanup();                                                                  \
-            raw_release(&raw);                                                          \
+            raw_release(&raw, s);                                                          \
             return _e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;           \
         }                                                                               \
     } while (0)
-    GEN_TRY(hipMalloc(&keys[0], sizeof(u64) * std::max<u64>(M, 1)));
-    GEN_TRY(hipMalloc(&keys[1], sizeof(u64) * std::max<u64>(M, 1)));
-    GEN_TRY(hipMalloc(&rp, sizeof(u64) * (N + 1)));
-    GEN_TRY(hipMalloc(&deg, sizeof(u64) * N));
-    GEN_TRY(hipMalloc(&flag, sizeof(u64) * N));
-    GEN_TRY(hipMalloc(&ro, sizeof(u64) * (N + 1)));
-    GEN_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(N + 1)));
-    GEN_TRY(hipMalloc(&d_tot, sizeof(u64) * 2));
+    GEN_TRY(ctx_alloc(&keys[0], sizeof(u64) * std::max<u64>(M, 1), s));
+    GEN_TRY(ctx_alloc(&keys[1], sizeof(u64) * std::max<u64>(M, 1), s));
+    GEN_TRY(ctx_alloc(&rp, sizeof(u64) * (N + 1), s));
+    GEN_TRY(ctx_alloc(&deg, sizeof(u64) * N, s));
+    GEN_TRY(ctx_alloc(&flag, sizeof(u64) * N, s));
+    GEN_TRY(ctx_alloc(&ro, sizeof(u64) * (N + 1), s));
+    GEN_TRY(ctx_alloc(&tmp, sizeof(u64) * scan_tmp_words(N + 1), s));
+    GEN_TRY(ctx_alloc(&d_tot, sizeof(u64) * 2, s));
     if (M) rmat_keys_kernel<<<grid_for(M), BLK, 0, s>>>(spec, keys[0], M);
     rc = sort_ws_init(&ctx->sort, std::max<u64>(M, N), ctx->d_err, s);
     if (rc) { cleanup(); return rc; }
@@ -127,9 +127,9 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
     raw.N = N;
     raw.E = E;
     raw.U = U;
-    GEN_TRY(hipMalloc(&raw.row_ptr, sizeof(u64) * (U + 1)));
-    GEN_TRY(hipMalloc(&raw.row_node, sizeof(u32) * std::max<u64>(U, 1)));
-    GEN_TRY(hipMalloc(&raw.dst, sizeof(u32) * std::max<u64>(E, 1)));
+    GEN_TRY(ctx_alloc(&raw.row_ptr, sizeof(u64) * (U + 1), s));
+    GEN_TRY(ctx_alloc(&raw.row_node, sizeof(u32) * std::max<u64>(U, 1), s));
+    GEN_TRY(ctx_alloc(&raw.dst, sizeof(u32) * std::max<u64>(E, 1), s));
     build_rows_kernel<<<grid_for(N), BLK, 0, s>>>(ro, flag, deg, N, raw.row_ptr, raw.row_node);
     set_u64<<<1, 1, 0, s>>>(raw.row_ptr + U, E);
     copy_lines_kernel<<<grid_for(N), BLK, 0, s>>>(spec, rp, ro, sk, N, raw.dst);
@@ -140,9 +140,9 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
     rp = deg = flag = ro = tmp = d_tot = nullptr;
     g = new rbsc_dgraph();
     rc = dgraph_build(ctx, &raw, g);
-    raw_release(&raw);
+    raw_release(&raw, s);
     if (rc) {
-        dgraph_release(g);
+        dgraph_release(g, s);
         delete g;
         return rc;
     }

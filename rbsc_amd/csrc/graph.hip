@@ -366,12 +366,12 @@ __global__ void tile_setup_kernel(const u64* __restrict__ row_ptr, u32 U, u64 E,
 }
 }  // namespace
 
-void raw_release(RawGraph* r) {
-    if (r->row_ptr) (void)hipFree(r->row_ptr);
-    if (r->row_node) (void)hipFree(r->row_node);
-    if (r->dst) (void)hipFree(r->dst);
-    if (r->wt) (void)hipFree(r->wt);
-    if (r->init) (void)hipFree(r->init);
+void raw_release(RawGraph* r, hipStream_t s) {
+    ctx_free(r->row_ptr, s);
+    ctx_free(r->row_node, s);
+    ctx_free(r->dst, s);
+    ctx_free(r->wt, s);
+    ctx_free(r->init, s);
     *r = RawGraph();
 }
 
@@ -396,18 +396,18 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
             goto done;                                                               \
         }                                                                            \
     } while (0)
-    B_TRY(hipMalloc(&g->pi, sizeof(u32) * std::max<u64>(N, 1)));
-    if (raw->init) B_TRY(hipMalloc(&g->init, sizeof(u32) * std::max<u64>(N, 1)));
-    B_TRY(hipMalloc(&indeg, sizeof(u32) * std::max<u64>(N, 1)));
-    B_TRY(hipMalloc(&dang, sizeof(u64) * (2 * N + 1)));
-    B_TRY(hipMalloc(&deg, sizeof(u64) * (U + 1)));
-    B_TRY(hipMalloc(&src_row, sizeof(u32) * std::max<u64>(U, 1)));
-    B_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(std::max(N, U) + 1)));
-    B_TRY(hipMalloc(&d_tot, sizeof(u64) * 2));
-    B_TRY(hipMalloc(&tfirst, sizeof(u32) * (lt_tiles(E_in) + 1)));
+    B_TRY(ctx_alloc(&g->pi, sizeof(u32) * std::max<u64>(N, 1), s));
+    if (raw->init) B_TRY(ctx_alloc(&g->init, sizeof(u32) * std::max<u64>(N, 1), s));
+    B_TRY(ctx_alloc(&indeg, sizeof(u32) * std::max<u64>(N, 1), s));
+    B_TRY(ctx_alloc(&dang, sizeof(u64) * (2 * N + 1), s));
+    B_TRY(ctx_alloc(&deg, sizeof(u64) * (U + 1), s));
+    B_TRY(ctx_alloc(&src_row, sizeof(u32) * std::max<u64>(U, 1), s));
+    B_TRY(ctx_alloc(&tmp, sizeof(u64) * scan_tmp_words(std::max(N, U) + 1), s));
+    B_TRY(ctx_alloc(&d_tot, sizeof(u64) * 2, s));
+    B_TRY(ctx_alloc(&tfirst, sizeof(u32) * (lt_tiles(E_in) + 1), s));
     for (int i = 0; i < 2; ++i) {
-        B_TRY(hipMalloc(&keys[i], sizeof(u64) * std::max<u64>(U, 1)));
-        B_TRY(hipMalloc(&vals[i], sizeof(u64) * std::max<u64>(U, 1)));
+        B_TRY(ctx_alloc(&keys[i], sizeof(u64) * std::max<u64>(U, 1), s));
+        B_TRY(ctx_alloc(&vals[i], sizeof(u64) * std::max<u64>(U, 1), s));
     }
     if (U) {
         rc = sort_ws_init(&ctx->sort, U, ctx->d_err, s);
@@ -423,8 +423,8 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     // Identical rows: sort rows by fingerprint, compare neighbours exactly,
     // one leader (stored row) per run of identical neighbours.
     if (U > 1 && ctx->dedup) {
-        B_TRY(hipMalloc(&h2, sizeof(u64) * U));
-        B_TRY(hipMalloc(&rep_raw, sizeof(u32) * U));
+        B_TRY(ctx_alloc(&h2, sizeof(u64) * U, s));
+        B_TRY(ctx_alloc(&rep_raw, sizeof(u32) * U, s));
         const u64 nt = lt_tiles(E_in);
         lt_first_row_kernel<<<grid_for(nt + 1), BLK, 0, s>>>(raw->row_ptr, U, E_in, nt, tfirst);
         B_TRY(hipMemsetAsync(keys[0], 0, sizeof(u64) * U, s));
@@ -476,7 +476,7 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     if (rc) goto done;
     place_dangling_kernel<<<grid_for(N), BLK, 0, s>>>(dang, dang + N, N, U, g->pi);
     // Row offsets of the stored rows; their total is the stored line count.
-    B_TRY(hipMalloc(&g->row_ptr, sizeof(u64) * (R + 1)));
+    B_TRY(ctx_alloc(&g->row_ptr, sizeof(u64) * (R + 1), s));
     rc = scan_excl_u64(deg, g->row_ptr, R, d_tot + 1, tmp, s);
     if (rc) goto done;
     if (R < U) {
@@ -487,22 +487,22 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     g->R = (u32)R;
     g->E = E;
     g->n_tiles = (E + SIG_TW - 1) / SIG_TW;
-    B_TRY(hipMalloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32));
-    if (raw->wt) B_TRY(hipMalloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1)));
+    B_TRY(ctx_alloc(&g->dst, sizeof(u32) * std::max<u64>(E, 1) + 32, s));
+    if (raw->wt) B_TRY(ctx_alloc(&g->wt, sizeof(u64) * std::max<u64>(E, 1), s));
     g->end_words = (E + 31) / 32 + 64;  // padding covers whole-tile byte reads
-    B_TRY(hipMalloc(&g->end_bits, sizeof(u32) * g->end_words));
-    B_TRY(hipMalloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1)));
-    B_TRY(hipMalloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1)));
+    B_TRY(ctx_alloc(&g->end_bits, sizeof(u32) * g->end_words, s));
+    B_TRY(ctx_alloc(&g->tile_row, sizeof(u32) * (g->n_tiles + 1), s));
+    B_TRY(ctx_alloc(&g->carry_row, sizeof(u32) * std::max<u64>(g->n_tiles, 1), s));
     if (R < U) {
-        B_TRY(hipMalloc(&g->dup_rep, sizeof(u32) * (U - R)));
+        B_TRY(ctx_alloc(&g->dup_rep, sizeof(u32) * (U - R), s));
         dup_rep_kernel<<<grid_for(U - R), BLK, 0, s>>>(src_row, rep_raw, raw->row_node, g->pi, R, U,
                                                        g->dup_rep);
     }
     if (!raw->init && R) {  // per-row weights of the first round's closed form
-        B_TRY(hipMalloc(&g->dang_w, sizeof(u64) * R));
+        B_TRY(ctx_alloc(&g->dang_w, sizeof(u64) * R, s));
         B_TRY(hipMemsetAsync(g->dang_w, 0, sizeof(u64) * R, s));
         if (raw->wt) {
-            B_TRY(hipMalloc(&g->row_w, sizeof(u64) * R));
+            B_TRY(ctx_alloc(&g->row_w, sizeof(u64) * R, s));
             B_TRY(hipMemsetAsync(g->row_w, 0, sizeof(u64) * R, s));
         }
     }
@@ -534,12 +534,12 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     rc = check_kernel_errors(ctx);
 done:
     for (int i = 0; i < 2; ++i) {
-        if (keys[i]) (void)hipFree(keys[i]);
-        if (vals[i]) (void)hipFree(vals[i]);
+        ctx_free(keys[i], s);
+        ctx_free(vals[i], s);
     }
     for (void* p : {(void*)dang, (void*)deg, (void*)tmp, (void*)indeg, (void*)src_row, (void*)h2,
                     (void*)rep_raw, (void*)d_tot, (void*)tfirst})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
     return rc;
 #undef B_TRY
 }
@@ -644,12 +644,12 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
             goto done;                                                               \
         }                                                                            \
     } while (0)
-    R_TRY(hipMalloc(&counts, sizeof(u64) * PU));
-    R_TRY(hipMalloc(&line_off, sizeof(u64) * PU));
-    R_TRY(hipMalloc(&flags, sizeof(u64) * PU));
-    R_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(PU + 1)));
-    R_TRY(hipMalloc(&tot, sizeof(u64) * 2 * P));
-    R_TRY(hipMalloc(&adj, sizeof(u64) * 2 * P));
+    R_TRY(ctx_alloc(&counts, sizeof(u64) * PU, s));
+    R_TRY(ctx_alloc(&line_off, sizeof(u64) * PU, s));
+    R_TRY(ctx_alloc(&flags, sizeof(u64) * PU, s));
+    R_TRY(ctx_alloc(&tmp, sizeof(u64) * scan_tmp_words(PU + 1), s));
+    R_TRY(ctx_alloc(&tot, sizeof(u64) * 2 * P, s));
+    R_TRY(ctx_alloc(&adj, sizeof(u64) * 2 * P, s));
     part_count_kernel<<<grid_for(U), BLK, 0, s>>>(g->row_ptr, g->dst, U, P, hot, lined, counts);
     flag_nonzero_kernel<<<grid_for(PU), BLK, 0, s>>>(counts, PU, flags);
     // per-partition totals of lines and segments
@@ -679,13 +679,13 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
     }
     n_tiles = E_pad / SIG_TW;
     end_words = (E_pad + 31) / 32 + 64;
-    R_TRY(hipMalloc(&ndst, sizeof(u32) * E_pad + 32));
-    if (g->wt) R_TRY(hipMalloc(&nwt, sizeof(u64) * E_pad));
-    R_TRY(hipMalloc(&seg_ptr, sizeof(u64) * (R + 1)));
-    R_TRY(hipMalloc(&seg_row, sizeof(u32) * R));
-    R_TRY(hipMalloc(&nbits, sizeof(u32) * end_words));
-    R_TRY(hipMalloc(&tile_row, sizeof(u32) * (n_tiles + 1)));
-    R_TRY(hipMalloc(&carry_row, sizeof(u32) * n_tiles));
+    R_TRY(ctx_alloc(&ndst, sizeof(u32) * E_pad + 32, s));
+    if (g->wt) R_TRY(ctx_alloc(&nwt, sizeof(u64) * E_pad, s));
+    R_TRY(ctx_alloc(&seg_ptr, sizeof(u64) * (R + 1), s));
+    R_TRY(ctx_alloc(&seg_row, sizeof(u32) * R, s));
+    R_TRY(ctx_alloc(&nbits, sizeof(u32) * end_words, s));
+    R_TRY(ctx_alloc(&tile_row, sizeof(u32) * (n_tiles + 1), s));
+    R_TRY(ctx_alloc(&carry_row, sizeof(u32) * n_tiles, s));
     R_TRY(hipMemcpyAsync(adj, adj_line.data(), sizeof(u64) * P, hipMemcpyHostToDevice, s));
     R_TRY(hipMemcpyAsync(adj + P, adj_seg.data(), sizeof(u64) * P, hipMemcpyHostToDevice, s));
     zero_u32<<<grid_for(end_words), BLK, 0, s>>>(nbits, end_words);
@@ -713,11 +713,11 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
     R_TRY(hipGetLastError());
     R_TRY(hipStreamSynchronize(s));
     // swap in the partitioned arrays
-    (void)hipFree(g->dst);
-    if (g->wt) (void)hipFree(g->wt);
-    (void)hipFree(g->end_bits);
-    (void)hipFree(g->tile_row);
-    (void)hipFree(g->carry_row);
+    ctx_free(g->dst, s);
+    ctx_free(g->wt, s);
+    ctx_free(g->end_bits, s);
+    ctx_free(g->tile_row, s);
+    ctx_free(g->carry_row, s);
     g->dst = ndst;
     g->wt = nwt;
     g->end_bits = nbits;
@@ -732,7 +732,7 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
     g->part_tile.assign(P + 1, 0);
     for (u32 p = 0; p < P; ++p) g->part_tile[p] = adj_line[p] / SIG_TW;
     g->part_tile[P] = n_tiles;
-    R_TRY(hipMalloc(&g->d_part_tile, sizeof(u64) * (P + 1)));
+    R_TRY(ctx_alloc(&g->d_part_tile, sizeof(u64) * (P + 1), s));
     R_TRY(hipMemcpy(g->d_part_tile, g->part_tile.data(), sizeof(u64) * (P + 1), hipMemcpyHostToDevice));
     g->bytes += 4 * (E_pad - E) + 4 * R;
     ndst = nullptr;
@@ -742,9 +742,9 @@ int dgraph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, u32 P, u32 hot) {
     seg_row = nullptr;
 done:
     for (u64* p : {counts, line_off, flags, tmp, tot, adj, seg_ptr, nwt})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
     for (u32* p : {ndst, seg_row, nbits, tile_row, carry_row})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
     return rc;
 #undef R_TRY
 }

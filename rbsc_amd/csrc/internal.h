@@ -113,7 +113,7 @@ struct rbsc_bins {
     u32* zero_bins = nullptr;       // n_zero: bins with several chunks
     u64 bytes = 0;
 };
-void bins_release(rbsc_bins* b);
+void bins_release(rbsc_bins* b, hipStream_t s);
 // Builds g->bins (hot heads below H, of them dense below D, 2^log_s rows per
 // bin, chunks of at most chunk_slots slots, scatter tiles of scatter_chunk
 // lines).
@@ -268,7 +268,7 @@ int prof_collect(rbsc_ctx* ctx);
 int ws_reserve(rbsc_ctx* ctx, u64 n);
 // Frees the refinement workspace (it is allocated again on demand).
 void ws_release(rbsc_ctx* ctx);
-void raw_release(RawGraph* r);
+void raw_release(RawGraph* r, hipStream_t s);
 // Builds g from raw (raw arrays are left for the caller to release).
 int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g);
 // Regroups lines by head: n_parts interleaved partitions (head mod n_parts),
@@ -297,6 +297,6 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
 int quotient_refine(rbsc_ctx* ctx, const rbsc_graph* gr, const rbsc_opts* opts, rbsc_result* res,
                     uint32_t* partition_out);
 int check_kernel_errors(rbsc_ctx* ctx);
-void dgraph_release(rbsc_dgraph* g);
+void dgraph_release(rbsc_dgraph* g, hipStream_t s);
 // Canonical first-occurrence labels (rbsc.cu:431-445) of a colour vector.
 u32 canonicalize_host(u32 N, const u32* colors, u32* out);

@@ -309,15 +309,16 @@ __global__ void compose_kernel(u32* __restrict__ comp, u64 N0, const u32* __rest
 
 // Device buffers of one quotient run, freed together.
 struct Bufs {
+    hipStream_t s = nullptr;  // frees are ordered on this stream
     std::vector<void*> held;
     ~Bufs() {
         for (void* p : held)
-            if (p) (void)hipFree(p);
+            ctx_free(p, s);
     }
     template <class T>
     hipError_t get(T** out, u64 n) {
         *out = nullptr;
-        const hipError_t e = hipMalloc((void**)out, sizeof(T) * (n ? n : 1));
+        const hipError_t e = ctx_alloc((void**)out, sizeof(T) * (n ? n : 1), s);
         if (e == hipSuccess) held.push_back(*out);
         return e;
     }
@@ -361,6 +362,7 @@ int quotient_refine(rbsc_ctx* ctx, const rbsc_graph* gr, const rbsc_opts* opts, 
     const auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = ctx->stream;
     Bufs B;
+    B.s = s;
     u32 *src[2], *dst[2], *node_group, *gsrc, *init, *d_int, *col, *first, *w, *comp, *rep, *brn,
         *bdst;
     u64 *wt[2], *flag, *pos, *tmp, *first_start, *gs, *fill, *brp, *bwt, *keys[2], *vals[2], *runs,

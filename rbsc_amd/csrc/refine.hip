@@ -1267,21 +1267,16 @@ int check_kernel_errors(rbsc_ctx* ctx) {
 
 void ws_release(rbsc_ctx* ctx) {
     Workspace& w = ctx->ws;
-    if (w.keys[0] || w.sig) (void)hipStreamSynchronize(ctx->stream);
+    hipStream_t s = ctx->stream;
     for (int i = 0; i < 2; ++i) {
-        if (w.keys[i]) (void)hipFree(w.keys[i]);
-        if (w.P[i]) (void)hipFree(w.P[i]);
-        w.keys[i] = w.P[i] = nullptr;
+        ctx_free(w.keys[i], s);
+        ctx_free(w.P[i], s);
     }
-    if (w.sig) (void)hipFree(w.sig);
-    if (w.hc) (void)hipFree(w.hc);
-    if (w.hc2) (void)hipFree(w.hc2);
-    if (w.rank_status) (void)hipFree(w.rank_status);
-    if (w.cold) (void)hipFree(w.cold);
-    for (void* p : {(void*)w.tie_runs, (void*)w.tie_runs_m, (void*)w.tie_runs_l,
+    for (void* p : {(void*)w.sig, (void*)w.hc, (void*)w.hc2, (void*)w.rank_status, w.cold,
+                    (void*)w.tie_runs, (void*)w.tie_runs_m, (void*)w.tie_runs_l,
                     (void*)w.tie_pre, (void*)w.tie_tmp, (void*)w.tie_mark, (void*)w.tie_buf,
                     (void*)w.carry})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
     w = Workspace();
     ctx->sort.extra_status = nullptr;
     ctx->sort.extra_words = 0;
@@ -1290,53 +1285,56 @@ void ws_release(rbsc_ctx* ctx) {
 int ws_reserve(rbsc_ctx* ctx, u64 n) {
     Workspace& w = ctx->ws;
     if (n <= w.cap_n && w.keys[0]) return RBSC_OK;
+    // grows by half at least, so a context serving graphs of growing sizes
+    // reallocates a few times only
+    const u64 cap = std::max<u64>(std::max<u64>(n, 4096), w.cap_n + w.cap_n / 2);
     ws_release(ctx);
-    const u64 cap = std::max<u64>(n, 4096);
+    hipStream_t s = ctx->stream;
     for (int i = 0; i < 2; ++i) {
-        HIP_TRY(hipMalloc(&w.keys[i], sizeof(u64) * cap));
-        HIP_TRY(hipMalloc(&w.P[i], sizeof(u64) * cap));
+        HIP_TRY(ctx_alloc(&w.keys[i], sizeof(u64) * cap, s));
+        HIP_TRY(ctx_alloc(&w.P[i], sizeof(u64) * cap, s));
     }
-    HIP_TRY(hipMalloc(&w.sig, sizeof(u64) * (cap + 1)));  // + spare slot for padding
-    HIP_TRY(hipMalloc(&w.hc, sizeof(u64) * cap));
-    HIP_TRY(hipMalloc(&w.hc2, sizeof(u64) * cap));
+    HIP_TRY(ctx_alloc(&w.sig, sizeof(u64) * (cap + 1), s));  // + spare slot for padding
+    HIP_TRY(ctx_alloc(&w.hc, sizeof(u64) * cap, s));
+    HIP_TRY(ctx_alloc(&w.hc2, sizeof(u64) * cap, s));
     w.tie_cap = cap / 16 + 1024;
-    HIP_TRY(hipMalloc(&w.tie_runs, sizeof(u64) * 2 * w.tie_cap));
-    HIP_TRY(hipMalloc(&w.tie_runs_m, sizeof(u64) * 2 * TIE_MID_CAP));
-    HIP_TRY(hipMalloc(&w.tie_runs_l, sizeof(u64) * 2 * TIE_LONG_CAP));
-    HIP_TRY(hipMalloc(&w.tie_pre, sizeof(u64) * (TIE_LONG_CAP + 1)));
-    HIP_TRY(hipMalloc(&w.tie_tmp, sizeof(u64) * scan_tmp_words(TIE_LONG_CAP)));
-    HIP_TRY(hipMalloc(&w.tie_mark, sizeof(u32) * cap));
-    HIP_TRY(hipMemsetAsync(w.tie_mark, 0, sizeof(u32) * cap, ctx->stream));
+    HIP_TRY(ctx_alloc(&w.tie_runs, sizeof(u64) * 2 * w.tie_cap, s));
+    HIP_TRY(ctx_alloc(&w.tie_runs_m, sizeof(u64) * 2 * TIE_MID_CAP, s));
+    HIP_TRY(ctx_alloc(&w.tie_runs_l, sizeof(u64) * 2 * TIE_LONG_CAP, s));
+    HIP_TRY(ctx_alloc(&w.tie_pre, sizeof(u64) * (TIE_LONG_CAP + 1), s));
+    HIP_TRY(ctx_alloc(&w.tie_tmp, sizeof(u64) * scan_tmp_words(TIE_LONG_CAP), s));
+    HIP_TRY(ctx_alloc(&w.tie_mark, sizeof(u32) * cap, s));
+    HIP_TRY(hipMemsetAsync(w.tie_mark, 0, sizeof(u32) * cap, s));
     w.tie_epoch = 0;
     w.rank_tiles = (cap + RANK_TILE - 1) / RANK_TILE + 1;
-    HIP_TRY(hipMalloc(&w.rank_status, sizeof(u64) * w.rank_tiles));
-    HIP_TRY(hipMemsetAsync(w.rank_status, 0, sizeof(u64) * w.rank_tiles, ctx->stream));
+    HIP_TRY(ctx_alloc(&w.rank_status, sizeof(u64) * w.rank_tiles, s));
+    HIP_TRY(hipMemsetAsync(w.rank_status, 0, sizeof(u64) * w.rank_tiles, s));
     w.cap_n = cap;
     ctx->sort.extra_status = w.rank_status;
     ctx->sort.extra_words = w.rank_tiles;
-    int rc = sort_ws_init(&ctx->sort, cap, ctx->d_err, ctx->stream);
+    int rc = sort_ws_init(&ctx->sort, cap, ctx->d_err, s);
     if (rc) return rc;
     return RBSC_OK;
 }
 
-void dgraph_release(rbsc_dgraph* g) {
+void dgraph_release(rbsc_dgraph* g, hipStream_t s) {
     if (!g) return;
-    if (g->row_ptr) (void)hipFree(g->row_ptr);
-    if (g->pi) (void)hipFree(g->pi);
-    if (g->dst) (void)hipFree(g->dst);
-    if (g->wt) (void)hipFree(g->wt);
-    if (g->init) (void)hipFree(g->init);
-    if (g->end_bits) (void)hipFree(g->end_bits);
-    if (g->seg_row) (void)hipFree(g->seg_row);
-    if (g->d_part_tile) (void)hipFree(g->d_part_tile);
+    ctx_free(g->row_ptr, s);
+    ctx_free(g->pi, s);
+    ctx_free(g->dst, s);
+    ctx_free(g->wt, s);
+    ctx_free(g->init, s);
+    ctx_free(g->end_bits, s);
+    ctx_free(g->seg_row, s);
+    ctx_free(g->d_part_tile, s);
     if (g->dst_host) (void)hipHostFree(g->dst_host);
     if (g->wt_host) (void)hipHostFree(g->wt_host);
-    if (g->tile_row) (void)hipFree(g->tile_row);
-    if (g->carry_row) (void)hipFree(g->carry_row);
-    if (g->dup_rep) (void)hipFree(g->dup_rep);
-    if (g->dang_w) (void)hipFree(g->dang_w);
-    if (g->row_w) (void)hipFree(g->row_w);
-    if (g->bins) bins_release(g->bins);
+    ctx_free(g->tile_row, s);
+    ctx_free(g->carry_row, s);
+    ctx_free(g->dup_rep, s);
+    ctx_free(g->dang_w, s);
+    ctx_free(g->row_w, s);
+    if (g->bins) bins_release(g->bins, s);
     *g = rbsc_dgraph();
 }
 
@@ -1436,15 +1434,15 @@ static void launch_sig(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* dst, const u64*
 static int stream_reserve(rbsc_ctx* ctx, u64 lines, bool weighted) {
     if (ctx->sbuf_lines >= lines && (!weighted || ctx->swbuf[0])) return RBSC_OK;
     for (int b = 0; b < 2; ++b) {
-        if (ctx->sbuf[b]) (void)hipFree(ctx->sbuf[b]);
-        if (ctx->swbuf[b]) (void)hipFree(ctx->swbuf[b]);
+        ctx_free(ctx->sbuf[b], ctx->stream);
+        ctx_free(ctx->swbuf[b], ctx->stream);
         ctx->sbuf[b] = nullptr;
         ctx->swbuf[b] = nullptr;
     }
     ctx->sbuf_lines = 0;
     for (int b = 0; b < 2; ++b) {
-        HIP_TRY(hipMalloc(&ctx->sbuf[b], sizeof(u32) * lines + 32));
-        if (weighted) HIP_TRY(hipMalloc(&ctx->swbuf[b], sizeof(u64) * lines));
+        HIP_TRY(ctx_alloc(&ctx->sbuf[b], sizeof(u32) * lines + 32, ctx->stream));
+        if (weighted) HIP_TRY(ctx_alloc(&ctx->swbuf[b], sizeof(u64) * lines, ctx->stream));
     }
     ctx->sbuf_lines = lines;
     return RBSC_OK;
@@ -1459,10 +1457,10 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     if (rc) return rc;
     Workspace& w = ctx->ws;
     if (g->n_tiles > w.carry_cap || !w.carry) {
-        if (w.carry) HIP_TRY(hipFree(w.carry));
+        ctx_free(w.carry, s);
         w.carry = nullptr;
         w.carry_cap = 0;
-        HIP_TRY(hipMalloc(&w.carry, sizeof(u64) * std::max<u64>(g->n_tiles, 1)));
+        HIP_TRY(ctx_alloc(&w.carry, sizeof(u64) * std::max<u64>(g->n_tiles, 1), s));
         w.carry_cap = std::max<u64>(g->n_tiles, 1);
     }
     // Without initial colours, nodes without lines are always colour 0: their
@@ -1495,10 +1493,10 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
     if (binned && bins_cold_bytes(g->bins) > w.cold_bytes) {
         // the rounds' cold-slot colours belong to this refinement, not to the
         // graph, so concurrent refinements of one graph do not share them
-        if (w.cold) HIP_TRY(hipFree(w.cold));
+        ctx_free(w.cold, s);
         w.cold = nullptr;
         w.cold_bytes = 0;
-        HIP_TRY(hipMalloc(&w.cold, bins_cold_bytes(g->bins)));
+        HIP_TRY(ctx_alloc(&w.cold, bins_cold_bytes(g->bins), s));
         HIP_TRY(hipMemsetAsync(w.cold, 0, bins_cold_bytes(g->bins), s));
         w.cold_bytes = bins_cold_bytes(g->bins);
     }
@@ -1727,11 +1725,11 @@ int refine_core(rbsc_ctx* ctx, rbsc_dgraph* g, const RefineParams& prm, RefineOu
                                    hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             if (total > w.tie_buf_cap) {
-                if (w.tie_buf) HIP_TRY(hipFree(w.tie_buf));
+                ctx_free(w.tie_buf, s);
                 w.tie_buf = nullptr;
                 w.tie_buf_cap = 0;
                 const u64 want = std::max<u64>(total, 4096);
-                HIP_TRY(hipMalloc(&w.tie_buf, sizeof(u64) * 8 * want));
+                HIP_TRY(ctx_alloc(&w.tie_buf, sizeof(u64) * 8 * want, s));
                 w.tie_buf_cap = want;
             }
             const u64 C = w.tie_buf_cap;
@@ -2202,6 +2200,13 @@ extern "C" int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out) 
         const int rc = batch_device_scratch_init(device);
         if (rc) return rc;
     }
+    {
+        // the device's memory pool (ctx_alloc) keeps freed memory for reuse
+        hipMemPool_t pool = nullptr;
+        HIP_TRY(hipDeviceGetDefaultMemPool(&pool, device));
+        uint64_t keep = ~0ull;
+        HIP_TRY(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
     rbsc_ctx* c = new rbsc_ctx();
     c->device = device;
     c->budget = hbm_budget;
@@ -2226,16 +2231,26 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
     if (!c) return RBSC_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->copy_stream);
     ws_release(c);
-    sort_ws_free(&c->sort);
+    sort_ws_free(&c->sort, c->stream);
+    for (int b = 0; b < 2; ++b) {
+        ctx_free(c->sbuf[b], c->stream);
+        ctx_free(c->swbuf[b], c->stream);
+        c->sbuf[b] = nullptr;
+        c->swbuf[b] = nullptr;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    {
+        // hand the pool's unused memory back (memory other contexts hold stays)
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, c->device) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+    }
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->d_scalar) (void)hipFree(c->d_scalar);
     if (c->h_scalar) (void)hipHostFree(c->h_scalar);
     for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
-    (void)hipStreamSynchronize(c->copy_stream);
     for (int b = 0; b < 2; ++b) {
-        if (c->sbuf[b]) (void)hipFree(c->sbuf[b]);
-        if (c->swbuf[b]) (void)hipFree(c->swbuf[b]);
         if (c->ev_copy[b]) (void)hipEventDestroy(c->ev_copy[b]);
         if (c->ev_done[b]) (void)hipEventDestroy(c->ev_done[b]);
     }
@@ -2365,13 +2380,13 @@ static int rows_on_device(rbsc_ctx* ctx, const rbsc_graph* gr, RawGraph* raw) {
             goto done;                                                               \
         }                                                                            \
     } while (0)
-    U_TRY(hipMalloc(&d_err, sizeof(u32)));
-    U_TRY(hipMalloc(&d_tot, sizeof(u64)));
+    U_TRY(ctx_alloc(&d_err, sizeof(u32), s));
+    U_TRY(ctx_alloc(&d_tot, sizeof(u64), s));
     U_TRY(hipMemsetAsync(d_err, 0, sizeof(u32), s));
     if (gr->row_offsets) {
-        U_TRY(hipMalloc(&d_ro, sizeof(u64) * (N + 1)));
-        U_TRY(hipMalloc(&flag, sizeof(u64) * std::max<u64>(N, 1)));
-        U_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(N + 1)));
+        U_TRY(ctx_alloc(&d_ro, sizeof(u64) * (N + 1), s));
+        U_TRY(ctx_alloc(&flag, sizeof(u64) * std::max<u64>(N, 1), s));
+        U_TRY(ctx_alloc(&tmp, sizeof(u64) * scan_tmp_words(N + 1), s));
         U_TRY(hipMemcpyAsync(d_ro, gr->row_offsets, sizeof(u64) * (N + 1), hipMemcpyHostToDevice, s));
         csr_flag_kernel<<<grid_for(std::max<u64>(N, 1)), BLK, 0, s>>>(d_ro, N, E, flag, d_err);
         if ((rc = scan_excl_u64(flag, flag, N, d_tot, tmp, s))) goto done;
@@ -2380,15 +2395,15 @@ static int rows_on_device(rbsc_ctx* ctx, const rbsc_graph* gr, RawGraph* raw) {
         U_TRY(hipStreamSynchronize(s));
         if (h_err) goto done;
         raw->U = h_tot;
-        U_TRY(hipMalloc(&raw->row_ptr, sizeof(u64) * (raw->U + 1)));
-        U_TRY(hipMalloc(&raw->row_node, sizeof(u32) * std::max<u64>(raw->U, 1)));
+        U_TRY(ctx_alloc(&raw->row_ptr, sizeof(u64) * (raw->U + 1), s));
+        U_TRY(ctx_alloc(&raw->row_node, sizeof(u32) * std::max<u64>(raw->U, 1), s));
         if (N) csr_rows_kernel<<<grid_for(N), BLK, 0, s>>>(d_ro, flag, N, raw->row_ptr, raw->row_node);
     } else {
         const u64 nt = lt_tiles(E);
-        U_TRY(hipMalloc(&d_src, sizeof(u32) * std::max<u64>(E, 1)));
-        U_TRY(hipMalloc(&flag, sizeof(u64) * (nt + 1)));
-        U_TRY(hipMalloc(&tmp, sizeof(u64) * scan_tmp_words(nt + 1)));
-        U_TRY(hipMalloc(&seen, sizeof(u32) * std::max<u64>(N, 1)));
+        U_TRY(ctx_alloc(&d_src, sizeof(u32) * std::max<u64>(E, 1), s));
+        U_TRY(ctx_alloc(&flag, sizeof(u64) * (nt + 1), s));
+        U_TRY(ctx_alloc(&tmp, sizeof(u64) * scan_tmp_words(nt + 1), s));
+        U_TRY(ctx_alloc(&seen, sizeof(u32) * std::max<u64>(N, 1), s));
         if (E) U_TRY(hipMemcpyAsync(d_src, gr->src, sizeof(u32) * E, hipMemcpyHostToDevice, s));
         U_TRY(hipMemsetAsync(seen, 0, sizeof(u32) * std::max<u64>(N, 1), s));
         U_TRY(hipMemsetAsync(flag + nt, 0, sizeof(u64), s));
@@ -2399,8 +2414,8 @@ static int rows_on_device(rbsc_ctx* ctx, const rbsc_graph* gr, RawGraph* raw) {
         U_TRY(hipStreamSynchronize(s));
         if (h_err) goto done;
         raw->U = h_tot;
-        U_TRY(hipMalloc(&raw->row_ptr, sizeof(u64) * (raw->U + 1)));
-        U_TRY(hipMalloc(&raw->row_node, sizeof(u32) * std::max<u64>(raw->U, 1)));
+        U_TRY(ctx_alloc(&raw->row_ptr, sizeof(u64) * (raw->U + 1), s));
+        U_TRY(ctx_alloc(&raw->row_node, sizeof(u32) * std::max<u64>(raw->U, 1), s));
         if (nt)
             coo_rows_kernel<<<(unsigned)nt, LT_THREADS, 0, s>>>(d_src, E, N, flag, seen,
                                                                 raw->row_ptr, raw->row_node, d_err);
@@ -2413,7 +2428,7 @@ static int rows_on_device(rbsc_ctx* ctx, const rbsc_graph* gr, RawGraph* raw) {
 done:
     for (void* p : {(void*)d_ro, (void*)flag, (void*)tmp, (void*)d_tot, (void*)d_src, (void*)seen,
                     (void*)d_err})
-        if (p) (void)hipFree(p);
+        ctx_free(p, s);
     if (!rc && h_err) {
         // the host check names the first bad line
         std::vector<u64> row_ptr;
@@ -2465,14 +2480,14 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
     raw.sample_indeg = sample_indeg;
     auto fail = [&](hipError_t e, const char* what) {
         rbsc_set_error("%s failed: %s", what, hipGetErrorString(e));
-        raw_release(&raw);
+        raw_release(&raw, s);
         return e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;
     };
     hipError_t e;
-    if ((e = hipMalloc(&raw.dst, sizeof(u32) * std::max<u64>(E, 1))) != hipSuccess) return fail(e, "hipMalloc");
-    if (gr->weight && (e = hipMalloc(&raw.wt, sizeof(u64) * std::max<u64>(E, 1))) != hipSuccess)
+    if ((e = ctx_alloc(&raw.dst, sizeof(u32) * std::max<u64>(E, 1), s)) != hipSuccess) return fail(e, "hipMalloc");
+    if (gr->weight && (e = ctx_alloc(&raw.wt, sizeof(u64) * std::max<u64>(E, 1), s)) != hipSuccess)
         return fail(e, "hipMalloc");
-    if (gr->init_color && (e = hipMalloc(&raw.init, sizeof(u32) * std::max<u64>(N, 1))) != hipSuccess)
+    if (gr->init_color && (e = ctx_alloc(&raw.init, sizeof(u32) * std::max<u64>(N, 1), s)) != hipSuccess)
         return fail(e, "hipMalloc");
     if ((E && (e = hipMemcpyAsync(raw.dst, gr->dst, 4 * E, hipMemcpyHostToDevice, s))) ||
         (E && gr->weight && (e = hipMemcpyAsync(raw.wt, gr->weight, 8 * E, hipMemcpyHostToDevice, s))) ||
@@ -2480,21 +2495,21 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
         return fail(e, "hipMemcpyAsync");
     rc = rows_on_device(ctx, gr, &raw);
     if (rc) {
-        raw_release(&raw);
+        raw_release(&raw, s);
         return rc;
     }
     rbsc_dgraph* g = new rbsc_dgraph();
     rc = dgraph_build(ctx, &raw, g);
-    raw_release(&raw);
+    raw_release(&raw, s);
     if (rc) {
-        dgraph_release(g);
+        dgraph_release(g, s);
         delete g;
         return rc;
     }
     if (lines_on_host && E) {
         rc = rbsc_graph_lines_to_host(ctx, g);
         if (rc) {
-            dgraph_release(g);
+            dgraph_release(g, s);
             delete g;
             return rc;
         }
@@ -2507,10 +2522,10 @@ extern "C" int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* gr, rbsc_dgrap
     return graph_upload_host(ctx, gr, false, out);
 }
 
-static void drop_bins(rbsc_dgraph* g) {
+static void drop_bins(rbsc_dgraph* g, hipStream_t s) {
     if (!g->bins) return;
     g->bytes -= g->bins->bytes;
-    bins_release(g->bins);
+    bins_release(g->bins, s);
     g->bins = nullptr;
 }
 
@@ -2533,7 +2548,7 @@ extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts
     while ((1u << log_s) < o.rows_per_bin) ++log_s;
     HIP_TRY(hipSetDevice(ctx->device));
     if (g->R == 0) {
-        drop_bins(g);
+        drop_bins(g, ctx->stream);
         return RBSC_OK;
     }
     return dgraph_bin(ctx, g, o.hot_heads, o.dense_heads, log_s, o.chunk_slots, o.scatter_lines,
@@ -2543,7 +2558,7 @@ extern "C" int rbsc_graph_bin(rbsc_ctx* ctx, rbsc_dgraph* g, const rbsc_bin_opts
 extern "C" int rbsc_graph_unbin(rbsc_ctx* ctx, rbsc_dgraph* g) {
     if (!g) return RBSC_OK;
     if (ctx) HIP_TRY(hipSetDevice(ctx->device));
-    drop_bins(g);
+    drop_bins(g, ctx ? ctx->stream : nullptr);
     return RBSC_OK;
 }
 
@@ -2567,7 +2582,7 @@ extern "C" int rbsc_graph_relayout(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t n_par
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    drop_bins(g);
+    drop_bins(g, ctx->stream);
     return dgraph_relayout(ctx, g, n_parts, 0);
 }
 
@@ -2577,7 +2592,7 @@ extern "C" int rbsc_graph_split_hot(rbsc_ctx* ctx, rbsc_dgraph* g, uint32_t hot_
         return RBSC_ERR_INVALID;
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    drop_bins(g);
+    drop_bins(g, ctx->stream);
     return dgraph_relayout(ctx, g, 2, std::min<u32>(hot_heads, g->U ? g->U : 1));
 }
 
@@ -2592,18 +2607,19 @@ extern "C" int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g) {
     }
     if (!g->dst) return RBSC_OK;  // already on the host (or no lines)
     HIP_TRY(hipSetDevice(ctx->device));
-    drop_bins(g);  // streamed rounds read the plain line array
+    hipStream_t s = ctx->stream;
+    drop_bins(g, s);  // streamed rounds read the plain line array
     const u64 E = g->E;
     HIP_TRY(hipHostMalloc(&g->dst_host, sizeof(u32) * std::max<u64>(E, 1), hipHostMallocDefault));
     HIP_TRY(hipMemcpy(g->dst_host, g->dst, sizeof(u32) * E, hipMemcpyDeviceToHost));
     if (g->wt) {
         HIP_TRY(hipHostMalloc(&g->wt_host, sizeof(u64) * std::max<u64>(E, 1), hipHostMallocDefault));
         HIP_TRY(hipMemcpy(g->wt_host, g->wt, sizeof(u64) * E, hipMemcpyDeviceToHost));
-        HIP_TRY(hipFree(g->wt));
+        ctx_free(g->wt, s);
         g->wt = nullptr;
         g->bytes -= sizeof(u64) * E;
     }
-    HIP_TRY(hipFree(g->dst));
+    ctx_free(g->dst, s);
     g->dst = nullptr;
     g->bytes -= sizeof(u32) * E;
     return RBSC_OK;
@@ -2612,7 +2628,9 @@ extern "C" int rbsc_graph_lines_to_host(rbsc_ctx* ctx, rbsc_dgraph* g) {
 extern "C" int rbsc_graph_free(rbsc_ctx* ctx, rbsc_dgraph* g) {
     if (!g) return RBSC_OK;
     if (ctx) (void)hipSetDevice(ctx->device);
-    dgraph_release(g);
+    // without a context the free is ordered on the null stream (the
+    // context's streams do not block on it; the graph is idle by contract)
+    dgraph_release(g, ctx ? ctx->stream : nullptr);
     delete g;
     return RBSC_OK;
 }

@@ -39,7 +39,7 @@ struct SortWorkspace {
 constexpr u64 SORT_TILE = 256 * RBSC_SORT_ITEMS;
 
 int sort_ws_init(SortWorkspace* ws, u64 max_n, u32* err_flag, hipStream_t s);
-void sort_ws_free(SortWorkspace* ws);
+void sort_ws_free(SortWorkspace* ws, hipStream_t s);
 
 // Sort (keys[0], vals[0]) by key bits [bit_begin, bit_end) (multiple of 8 wide),
 // stably, ping-ponging with (keys[1], vals[1]).  vals may be null (keys only).

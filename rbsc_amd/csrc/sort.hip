@@ -246,27 +246,28 @@ u32 lb_next_epoch(SortWorkspace* ws, hipStream_t s) {
 int sort_ws_init(SortWorkspace* ws, u64 max_n, u32* err_flag, hipStream_t s) {
     const u64 tiles = (max_n + TILE - 1) / TILE + 1;
     if (ws->status && ws->status_tiles >= tiles) return RBSC_OK;
-    if (ws->status) HIP_TRY(hipFree(ws->status));
+    ctx_free(ws->status, s);
+    ws->status = nullptr;
     if (!ws->hist) {
-        HIP_TRY(hipMalloc(&ws->hist, sizeof(u32) * 8 * RADIX));
-        HIP_TRY(hipMalloc(&ws->digit_base, sizeof(u64) * 8 * RADIX));
-        HIP_TRY(hipMalloc(&ws->ticket, sizeof(u64)));
+        HIP_TRY(ctx_alloc(&ws->hist, sizeof(u32) * 8 * RADIX, s));
+        HIP_TRY(ctx_alloc(&ws->digit_base, sizeof(u64) * 8 * RADIX, s));
+        HIP_TRY(ctx_alloc(&ws->ticket, sizeof(u64), s));
         HIP_TRY(hipMemsetAsync(ws->hist, 0, sizeof(u32) * 8 * RADIX, s));
         HIP_TRY(hipMemsetAsync(ws->ticket, 0, sizeof(u64), s));
         ws->ticket_total = 0;
     }
-    HIP_TRY(hipMalloc(&ws->status, sizeof(u64) * tiles * RADIX));
+    HIP_TRY(ctx_alloc(&ws->status, sizeof(u64) * tiles * RADIX, s));
     HIP_TRY(hipMemsetAsync(ws->status, 0, sizeof(u64) * tiles * RADIX, s));
     ws->status_tiles = tiles;
     ws->err = err_flag;
     return RBSC_OK;
 }
 
-void sort_ws_free(SortWorkspace* ws) {
-    if (ws->hist) (void)hipFree(ws->hist);
-    if (ws->digit_base) (void)hipFree(ws->digit_base);
-    if (ws->status) (void)hipFree(ws->status);
-    if (ws->ticket) (void)hipFree(ws->ticket);
+void sort_ws_free(SortWorkspace* ws, hipStream_t s) {
+    ctx_free(ws->hist, s);
+    ctx_free(ws->digit_base, s);
+    ctx_free(ws->status, s);
+    ctx_free(ws->ticket, s);
     *ws = SortWorkspace();
 }
 

@@ -234,7 +234,7 @@ int verify_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* d_col, u32 K, u64
     std::vector<void*> owned;
     int rc = RBSC_OK;
     auto alloc = [&](void** p, size_t bytes) -> int {
-        const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+        const hipError_t e = ctx_alloc(p, std::max<size_t>(bytes, 8), s);
         if (e != hipSuccess) {
             rbsc_set_error("exact check: hipMalloc of %zu bytes failed: %s", bytes,
                            hipGetErrorString(e));
@@ -245,7 +245,7 @@ int verify_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u32* d_col, u32 K, u64
     };
     auto done = [&](int code) {
         (void)hipStreamSynchronize(s);
-        for (void* p : owned) (void)hipFree(p);
+        for (void* p : owned) ctx_free(p, s);
         return code;
     };
     u32* rep = nullptr;
@@ -339,12 +339,13 @@ extern "C" int rbsc_graph_check_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u
     const u64 N = g->N;
     *bad_out = 0;
     if (N == 0) return RBSC_OK;
+    hipStream_t s = ctx->stream;
     std::vector<u32> dense(N);
     const u32 K = canonicalize_host((u32)N, colors, dense.data());
     u32 *h = nullptr, *col = nullptr;
-    HIP_TRY(hipMalloc(&h, sizeof(u32) * N));
-    if (hipMalloc(&col, sizeof(u32) * N) != hipSuccess) {
-        (void)hipFree(h);
+    HIP_TRY(ctx_alloc(&h, sizeof(u32) * N, s));
+    if (ctx_alloc(&col, sizeof(u32) * N, s) != hipSuccess) {
+        ctx_free(h, s);
         rbsc_set_error("rbsc_graph_check_partition: out of device memory");
         return RBSC_ERR_NOMEM;
     }
@@ -359,7 +360,7 @@ extern "C" int rbsc_graph_check_partition(rbsc_ctx* ctx, rbsc_dgraph* g, const u
         rc = verify_partition(ctx, g, col, K, bad_out);
     }
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(h);
-    (void)hipFree(col);
+    ctx_free(h, s);
+    ctx_free(col, s);
     return rc;
 }
