@@ -107,10 +107,23 @@ extern "C" void rbsc_set_error(const char* fmt, ...);
 // refinement; stream-ordered allocation from the device's memory pool orders
 // the free after this stream's work only.  The pool keeps freed memory for
 // reuse (its release threshold is raised when a context is created).  Memory
-// from ctx_alloc is given back with ctx_free only.
+// from ctx_alloc is given back with ctx_free only.  When the device is out of
+// memory, the pool's cached blocks (which may all be of other sizes) are
+// handed back once all frees have completed, and the allocation is retried.
 template <class T>
 inline hipError_t ctx_alloc(T** p, size_t bytes, hipStream_t s) {
-    return hipMallocAsync(reinterpret_cast<void**>(p), bytes, s);
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(p), bytes, s);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceSynchronize() == hipSuccess && hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess &&
+            hipMemPoolTrimTo(pool, 0) == hipSuccess)
+            e = hipMallocAsync(reinterpret_cast<void**>(p), bytes, s);
+        if (e != hipSuccess) (void)hipGetLastError();
+    }
+    return e;
 }
 inline void ctx_free(void* p, hipStream_t s) {
     if (p) (void)hipFreeAsync(p, s);

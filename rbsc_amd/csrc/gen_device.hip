@@ -94,7 +94,7 @@ extern "C" int rbsc_gen_rmat_device(rbsc_ctx* ctx, uint32_t scale, uint64_t n_dr
         if (_e != hipSuccess) {                                                         \
             rbsc_set_error("%s failed: %s", #expr, hipGetErrorString(_e));              \
             cleanup();                                                                  \
-            raw_release(&raw, s);                                                          \
+            raw_release(&raw, s);                                                       \
             return _e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;           \
         }                                                                               \
     } while (0)

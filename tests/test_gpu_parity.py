@@ -181,6 +181,21 @@ def test_device_rmat_equals_host_rmat(ctx):
     d.free()
 
 
+def test_device_out_of_memory_is_an_error_and_the_context_recovers(ctx):
+    # 2^39 draws need 4 TiB of sort keys: the first device allocation fails
+    # (after the memory pool's cached blocks are handed back and the
+    # allocation retried) and is reported as RBSC_ERR_NOMEM; nothing is
+    # launched before it.
+    with pytest.raises(rbsc_amd.RbscError) as e:
+        ctx.gen_rmat(12, 1 << 39, seed=5)
+    assert e.value.code == -2
+    h = gen.rmat(10, 1 << 13, clone_frac=0.2, seed=22)
+    c, K, R, _ = _oracle_full(h)
+    r = ctx.refine(h)
+    assert (r.nodes, r.rounds) == (K, R)
+    assert np.array_equal(r.partition, oracle.canonical(c))
+
+
 def test_rmat_2p20_clone30_bit_exact(ctx):
     """The 2^20-node / 2^24-draw R-MAT with 30% clones, checked bit-exact
     against the oracle (final colours and partition)."""
