@@ -178,6 +178,10 @@ int rbsc_refine(rbsc_ctx* ctx, const rbsc_graph* graph, const rbsc_opts* opts,
 /* Device-resident graphs: upload once, refine many times (the bench and the
  * serving loop keep graphs resident).  Upload validates ids and grouping. */
 int rbsc_graph_upload(rbsc_ctx* ctx, const rbsc_graph* graph, rbsc_dgraph** out);
+/* The graph must be idle (no refinement of it in flight on any context).
+ * Its device memory goes back to the device's memory pool, ordered on ctx's
+ * stream (ctx may be null: then on the null stream), without waiting for
+ * other streams' work. */
 int rbsc_graph_free(rbsc_ctx* ctx, rbsc_dgraph* g);
 /* Bytes of device memory held by a resident graph. */
 uint64_t rbsc_graph_bytes(const rbsc_dgraph* g);
