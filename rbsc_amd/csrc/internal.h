@@ -250,6 +250,8 @@ struct rbsc_ctx {
     u64* swbuf[2] = {nullptr, nullptr};
     u64 sbuf_lines = 0;
     hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+    // fork and join of launches split over stream and copy_stream (batch tiers)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // per-phase profiling (rbsc_ctx_profile)
     bool prof = false;
     static constexpr int NPHASE = 7;

@@ -2223,6 +2223,8 @@ extern "C" int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out) 
         HIP_TRY(hipEventCreateWithFlags(&c->ev_copy[b], hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->ev_done[b], hipEventDisableTiming));
     }
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     *out = c;
     return RBSC_OK;
 }
@@ -2254,6 +2256,8 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
         if (c->ev_copy[b]) (void)hipEventDestroy(c->ev_copy[b]);
         if (c->ev_done[b]) (void)hipEventDestroy(c->ev_done[b]);
     }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
