@@ -856,13 +856,14 @@ extern "C" int rbsc_batch_autotune(rbsc_ctx* ctx, rbsc_dbatch* b, uint32_t* thre
     HIP_TRY(hipSetDevice(ctx->device));
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-    // 1. the largest tier's workgroup width and count
+    // 1. the largest tier's workgroup width and count, timed with every tier
+    //    (the tiers share the device: tiers 1 and 3 run beside 0 and 2)
     double best = 1e30;
     int bt = b->threads, bw = b->wgs;
     for (int threads : {256, 512, 1024}) {
         for (int mult : {1, 2, 4}) {
             double ms = 0;
-            const int rc = time_tiers(ctx, b, threads, cus * mult, 1u, &ms);
+            const int rc = time_tiers(ctx, b, threads, cus * mult, 0xFu, &ms);
             if (rc) return rc;
             if (ms < best) {
                 best = ms;
