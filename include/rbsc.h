@@ -164,7 +164,11 @@ int rbsc_host_alloc(uint64_t bytes, void** out);
 int rbsc_host_free(void* p);
 
 /* Context on HIP device `device`; hbm_budget = 0 means no budget.  Replaces
- * the per-GPU setup of rbsc.cu:234-270 (streams, scratch, CUB temp). */
+ * the per-GPU setup of rbsc.cu:234-270 (streams, scratch, CUB temp).  The
+ * library's device memory comes from the device's default stream-ordered
+ * memory pool; creating a context raises that pool's release threshold, so
+ * freed memory stays cached for reuse, and destroying one hands the pool's
+ * unused memory back to the device. */
 int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out);
 int rbsc_ctx_destroy(rbsc_ctx* ctx);
 
