@@ -513,7 +513,7 @@ int main(int argc, char** argv) {
     }
     const uint64_t max_batch = pos.size() >= 1 ? (uint64_t)atoll(pos[0].c_str()) : E;
     const long long gpus = pos.size() >= 2 ? atoll(pos[1].c_str()) : 1;
-    if (gpus > 1) fprintf(stderr, "rbsc: %lld GPUs requested; this build runs on one device\n", gpus);
+    if (gpus > 1) fprintf(stderr, "rbsc: %lld GPUs requested; running on one device\n", gpus);
 
     rbsc_ctx* ctx = nullptr;
     if (rbsc_ctx_create(device, 0, &ctx) != RBSC_OK) {

@@ -15,7 +15,7 @@ rerun resumes where the last one stopped.  Each tool cell is the JSON of
 parse (time-out, crash) records ``time = TIMEOUT, done = 0, nodes = 0``
 (bench.py:41-54).  Tools run as the same shell pipelines (bench.py:7-12):
 ``{input} | timeout T ./rbsc {edges} {gpu}``, ``./rsc``, ``./part_ref``;
-``rbsc`` is this build's command (rbsc_amd/bin/rbsc), ``rsc`` and
+``rbsc`` is the library's command (rbsc_amd/bin/rbsc), ``rsc`` and
 ``part_ref`` the reference's CPU tools, which the caller provides
 (``--rsc``, ``--part-ref``; by default ``./rsc`` and ``./part_ref`` in the
 working directory, as the reference's harness runs them).  The ``cuda``
@@ -24,7 +24,7 @@ columns run at edge batches of 100 %, 50 % and 25 % (bench.py:83-89).
 Inputs: ``tool = 'file'`` reads ``datasets/{name}.txt`` as the reference
 does (bench.py:9, 36-37); ``tool = 'law'`` reads the BVGraph
 ``{name}/{name}.graph`` (where the reference's download.sh puts it) through
-this build's decoder, ``webgraph-to arcs-cuda|arcs-valmari`` in place of
+the library's decoder, ``webgraph-to arcs-cuda|arcs-valmari`` in place of
 ``./webgraph to ...`` (bench.py:8); nothing is downloaded, so a missing
 graph fails its row; ``tool = 'gen'`` names a seeded workload of
 :data:`WORKLOADS`, written as text by the library.  The sigref columns stay empty results, as in the
@@ -258,7 +258,7 @@ def main(argv=None) -> int:
     b.add_argument("--law-dir", default=".", help="directory holding NAME/NAME.graph")
     b.add_argument("--rsc", default=RSC, help="the RSC CPU tool (default ./rsc)")
     b.add_argument("--part-ref", default=PART_REF, help="Valmari's tool (default ./part_ref)")
-    b.add_argument("--rbsc", default=RBSC, help="the rbsc command (default: this build's)")
+    b.add_argument("--rbsc", default=RBSC, help="the rbsc command (default: the library's)")
     args = ap.parse_args(argv)
     if args.cmd == "init":
         init_db(args.db, args.large, tuple(args.only) if args.only else WORKLOADS)

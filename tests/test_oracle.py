@@ -383,7 +383,7 @@ def _build_copy(tmp_path, name, mutation):
 
 
 def _caught(lib):
-    """True iff the per-round check above rejects this build of the oracle on
+    """True iff the per-round check above rejects this compiled copy of the oracle on
     one of the graphs (its final partitions may still all be right)."""
     for n, s, d, w in _step_graphs():
         want = _reference_round_steps(n, s, d, w)
