@@ -272,14 +272,15 @@ class DeviceGraph:
         return HostGraph(n, dst[:e], row_offsets=ro)
 
     def free(self):
+        """Give the graph's device memory back (through the context's stream,
+        or, once the context is closed, the null stream)."""
         if self.handle:
             load().rbsc_graph_free(self.ctx.handle, self.handle)
             self.handle = None
 
     def __del__(self):
         try:
-            if self.handle and self.ctx.handle:
-                self.free()
+            self.free()
         except Exception:
             pass
 
@@ -487,8 +488,7 @@ class DeviceBatch:
 
     def __del__(self):
         try:
-            if self.handle and self.ctx.handle:
-                self.free()
+            self.free()
         except Exception:
             pass
 

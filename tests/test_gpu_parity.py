@@ -196,6 +196,33 @@ def test_device_out_of_memory_is_an_error_and_the_context_recovers(ctx):
     assert np.array_equal(r.partition, oracle.canonical(c))
 
 
+def test_graph_and_batch_outlive_their_context():
+    # Memory freed after its context is closed goes back through the null
+    # stream; later contexts allocate and refine as before.
+    h = gen.rmat(12, 1 << 16, clone_frac=0.2, seed=31)
+    c, K, R, _ = _oracle_full(h)
+    hb = gen.batch_mixed(40, 10, 300, seed=32)
+    c1 = rbsc_amd.Context(0)
+    g = c1.upload(h)
+    db = c1.upload_batch(hb)
+    r1 = g.refine()
+    b1 = db.refine()
+    c1.close()
+    g.free()
+    db.free()
+    with rbsc_amd.Context(0) as c2:
+        g2 = c2.upload(h)
+        r2 = g2.refine()
+        db2 = c2.upload_batch(hb)
+        b2 = db2.refine()
+        for r in (r1, r2):
+            assert (r.nodes, r.rounds) == (K, R)
+            assert np.array_equal(r.partition, oracle.canonical(c))
+        assert np.array_equal(b1.blocks, b2.blocks) and np.array_equal(b1.labels, b2.labels)
+        g2.free()
+        db2.free()
+
+
 def test_rmat_2p20_clone30_bit_exact(ctx):
     """The 2^20-node / 2^24-draw R-MAT with 30% clones, checked bit-exact
     against the oracle (final colours and partition)."""
