@@ -220,16 +220,21 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int, workers: int 
     out = {"workload": f"rmat_pool{pool_size}_E1e3_5e8_poisson", "requests_per_rate": n_req,
            "pool_lines": int(E.sum()), "plain_loop_graphs_per_s": n_req / loop_s, "rates": []}
     budget = (250 << 30) - pool_bytes
+    # one long-lived server takes every rate (its workers and contexts stay up
+    # between rates, as in a serving process); batch counts are per rate, the
+    # admitted-bytes peak is the server's so far
+    srv = rbsc_amd.Server(workers=workers, hbm_budget=budget, keep_partition=True)
+    st = srv.stats()
     for lam in rates:
         gaps = rng.exponential(1.0 / lam, n_req)
         arrivals = np.cumsum(gaps) - gaps[0]
         picks = rng.integers(0, pool_size, n_req)
-        with rbsc_amd.Server(workers=workers, hbm_budget=budget, keep_partition=True) as srv:
-            t = time.perf_counter()
-            ts = [submit(srv, pool[i], float(a)) for i, a in zip(picks, arrivals)]
-            rs = [srv.wait(tk) for tk in ts]
-            wall = time.perf_counter() - t
-            st = srv.stats()
+        st0 = st
+        t = time.perf_counter()
+        ts = [submit(srv, pool[i], float(a)) for i, a in zip(picks, arrivals)]
+        rs = [srv.wait(tk) for tk in ts]
+        wall = time.perf_counter() - t
+        st = srv.stats()
         lat = np.array([r.latency_s for r in rs])
         first = min(r.arrival_s for r in rs)
         last = max(r.done_s for r in rs)
@@ -241,13 +246,15 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int, workers: int 
         rec = {"rate_per_s": lam, "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
                "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
                "graphs_per_s": n_req / (last - first), "wall_s": wall,
-               "batches": st["batches"], "batched_graphs": st["batched_graphs"],
+               "batches": st["batches"] - st0["batches"],
+               "batched_graphs": st["batched_graphs"] - st0["batched_graphs"],
                "peak_admitted_gib": st["peak_admitted_bytes"] / 2**30,
                "partition_mismatches_vs_loop": mism}
         log(f"[bench] serving rate {lam}/s: p50 {rec['p50_latency_ms']:.1f} ms, "
             f"p99 {rec['p99_latency_ms']:.1f} ms, {rec['graphs_per_s']:.0f} graphs/s, "
             f"mismatches {mism}")
         out["rates"].append(rec)
+    srv.close()
     for kind, g in pool:
         if kind == "resident":
             g.free()
