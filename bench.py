@@ -9,8 +9,12 @@ resident in HBM.  A step is one refinement of that graph to its fixpoint
 
 Prints one JSON line (rank 0) with the measured value, a live roofline for
 the round (HIP events on the library's stream) and a single-core CPU
-baseline (the oracle's rsc restatement) on a bounded sample of the same
-R-MAT family.
+baseline on a bounded sample of the same R-MAT family: the reference's rsc
+compiled from its sources (oracle/_ref) when that is built, else the
+oracle's rsc restatement.  Further legs, each in its own JSON block: the
+one-shot drop-in call, the rbsc command on the workload's text, the
+multi-batch quotient mode, the 50,000-graph batch, the serving loop and the
+streamed (pinned host memory) refinement.
 """
 from __future__ import annotations
 
