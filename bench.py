@@ -228,6 +228,20 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int, workers: int 
     # between rates, as in a serving process); batch counts are per rate, the
     # admitted-bytes peak is the server's so far
     srv = rbsc_amd.Server(workers=workers, hbm_budget=budget, keep_partition=True)
+    try:
+        out["rates"] = _serve_rates(srv, pool, rates, n_req, pool_size, rng, loop_part, submit)
+    finally:
+        srv.close()
+    for kind, g in pool:
+        if kind == "resident":
+            g.free()
+    return out
+
+
+def _serve_rates(srv, pool, rates, n_req, pool_size, rng, loop_part, submit) -> list:
+    """One record per arrival rate, all served by `srv` (see serving_leg)."""
+    import numpy as np
+    recs = []
     st = srv.stats()
     for lam in rates:
         gaps = rng.exponential(1.0 / lam, n_req)
@@ -257,12 +271,8 @@ def serving_leg(ctx, rates, n_req: int, pool_size: int, seed: int, workers: int 
         log(f"[bench] serving rate {lam}/s: p50 {rec['p50_latency_ms']:.1f} ms, "
             f"p99 {rec['p99_latency_ms']:.1f} ms, {rec['graphs_per_s']:.0f} graphs/s, "
             f"mismatches {mism}")
-        out["rates"].append(rec)
-    srv.close()
-    for kind, g in pool:
-        if kind == "resident":
-            g.free()
-    return out
+        recs.append(rec)
+    return recs
 
 
 def measured_traffic(workload: str, kernel: str):
