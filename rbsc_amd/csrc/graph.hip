@@ -54,11 +54,21 @@ __global__ void fill_ones_kernel(u64* p, u64 n) {
 // atomics into N words).  Measured on that graph, the sample's order moves
 // 5.3 M lines from the hot to the cold class and costs the binned round
 // 0.54 GB more gather traffic, so resident graphs count every line.
-__global__ void indeg_kernel(const u32* __restrict__ dst, u64 E, u64 stride,
+// Lines k * stride in [lo, hi) (k from k0); heads not below N (malformed
+// input, refused by the build afterwards) are skipped.
+__global__ void indeg_kernel(const u32* __restrict__ dst, u64 k0, u64 hi, u64 stride, u64 N,
                              u32* __restrict__ indeg) {
-    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const u64 k = k0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
     const u64 e = k * stride;
-    if (e < E) atomicAdd(&indeg[dst[e]], 1u);
+    if (e < hi) {
+        const u32 d = dst[e];
+        if (d < N) atomicAdd(&indeg[d], 1u);
+    }
+}
+__global__ void dst_range_check_kernel(const u32* __restrict__ dst, u64 lo, u64 hi, u64 N,
+                                       u32* __restrict__ err) {
+    const u64 e = lo + (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < hi && dst[e] >= N) atomicOr(err, 1u);
 }
 // Sampled lines: about 2^27 of them (every line below 2^28 lines).
 inline u64 indeg_stride(u64 E) {
@@ -157,10 +167,10 @@ template <bool HAS_WT>
 __global__ __launch_bounds__(LT_THREADS) void row_hash_tile_kernel(
     const u64* __restrict__ row_ptr, const u32* __restrict__ dst, const u64* __restrict__ wt,
     u64 U, u64 E, const u32* __restrict__ first, unsigned long long* __restrict__ h1,
-    unsigned long long* __restrict__ h2) {
+    unsigned long long* __restrict__ h2, u64 t0) {
     __shared__ LtLds s;
     __shared__ unsigned long long acc[2][LT_LINES + 1];
-    const u64 t = blockIdx.x;
+    const u64 t = t0 + blockIdx.x;
     u32 row[LT_PER];
     const u32 last = lt_rows(row_ptr, U, E, first, t, &s, row);
     const u32 r0 = first[t], nr = last - r0 + 1;
@@ -372,6 +382,7 @@ void raw_release(RawGraph* r, hipStream_t s) {
     ctx_free(r->dst, s);
     ctx_free(r->wt, s);
     ctx_free(r->init, s);
+    ctx_free(r->dst_err, s);
     *r = RawGraph();
 }
 
@@ -387,6 +398,7 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
     u32 *indeg = nullptr, *src_row = nullptr, *rep_raw = nullptr, *tfirst = nullptr;
     u64 R = U, E = E_in, h_tot[2] = {0, 0};
     int rc = RBSC_OK;
+    bool hashing = false;
 #define B_TRY(expr)                                                                  \
     do {                                                                             \
         hipError_t _e = (expr);                                                      \
@@ -414,15 +426,11 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
         if (rc) goto done;
     }
     zero_u32<<<grid_for(N), BLK, 0, s>>>(indeg, N);
-    if (E_in) {
-        const u64 st = raw->sample_indeg ? indeg_stride(E_in) : 1;
-        indeg_kernel<<<grid_for((E_in + st - 1) / st), BLK, 0, s>>>(raw->dst, E_in, st, indeg);
-    }
     fill_ones_kernel<<<grid_for(N), BLK, 0, s>>>(dang, N);
-
     // Identical rows: sort rows by fingerprint, compare neighbours exactly,
     // one leader (stored row) per run of identical neighbours.
-    if (U > 1 && ctx->dedup) {
+    hashing = U > 1 && ctx->dedup;
+    if (hashing) {
         B_TRY(ctx_alloc(&h2, sizeof(u64) * U, s));
         B_TRY(ctx_alloc(&rep_raw, sizeof(u32) * U, s));
         const u64 nt = lt_tiles(E_in);
@@ -430,14 +438,48 @@ int dgraph_build(rbsc_ctx* ctx, RawGraph* raw, rbsc_dgraph* g) {
         B_TRY(hipMemsetAsync(keys[0], 0, sizeof(u64) * U, s));
         B_TRY(hipMemsetAsync(h2, 0, sizeof(u64) * U, s));
         iota_u64_kernel<<<grid_for(U), BLK, 0, s>>>(vals[0], U);
+    }
+    // The kernels that read every line (head check, in-degrees, row
+    // fingerprints), chunk by chunk as the lines arrive.
+    {
+        const int n_ch = raw->n_chunks ? raw->n_chunks : 1;
+        const u64 st = raw->sample_indeg ? indeg_stride(E_in) : 1;
         auto* h1p = reinterpret_cast<unsigned long long*>(keys[0]);
         auto* h2p = reinterpret_cast<unsigned long long*>(h2);
-        if (raw->wt)
-            row_hash_tile_kernel<true><<<(unsigned)nt, LT_THREADS, 0, s>>>(
-                raw->row_ptr, raw->dst, raw->wt, U, E_in, tfirst, h1p, h2p);
-        else
-            row_hash_tile_kernel<false><<<(unsigned)nt, LT_THREADS, 0, s>>>(
-                raw->row_ptr, raw->dst, nullptr, U, E_in, tfirst, h1p, h2p);
+        for (int c = 0; c < n_ch && E_in; ++c) {
+            const u64 lo = c ? raw->chunk_end[c - 1] : 0;
+            const u64 hi = raw->n_chunks ? raw->chunk_end[c] : E_in;
+            if (raw->n_chunks) B_TRY(hipStreamWaitEvent(s, raw->chunk_ev[c], 0));
+            if (hi <= lo) continue;
+            if (raw->dst_err)
+                dst_range_check_kernel<<<grid_for(hi - lo), BLK, 0, s>>>(raw->dst, lo, hi, N,
+                                                                        raw->dst_err);
+            const u64 k0 = (lo + st - 1) / st, k1 = (hi + st - 1) / st;
+            if (k1 > k0)
+                indeg_kernel<<<grid_for(k1 - k0), BLK, 0, s>>>(raw->dst, k0, hi, st, N, indeg);
+            if (hashing) {
+                const u64 t0 = lo / LT_LINES, t1 = (hi + LT_LINES - 1) / LT_LINES;
+                if (raw->wt)
+                    row_hash_tile_kernel<true><<<(unsigned)(t1 - t0), LT_THREADS, 0, s>>>(
+                        raw->row_ptr, raw->dst, raw->wt, U, E_in, tfirst, h1p, h2p, t0);
+                else
+                    row_hash_tile_kernel<false><<<(unsigned)(t1 - t0), LT_THREADS, 0, s>>>(
+                        raw->row_ptr, raw->dst, nullptr, U, E_in, tfirst, h1p, h2p, t0);
+            }
+        }
+    }
+    if (raw->dst_err) {
+        // a head not below N: refused before any kernel indexes by head
+        u32 bad = 0;
+        B_TRY(hipMemcpyAsync(&bad, raw->dst_err, sizeof(u32), hipMemcpyDeviceToHost, s));
+        B_TRY(hipStreamSynchronize(s));
+        if (bad) {
+            rbsc_set_error("malformed lines: a head is not below n_nodes");
+            rc = RBSC_ERR_INPUT;
+            goto done;
+        }
+    }
+    if (hashing) {
         int w = 0;
         rc = radix_sort_u64(&ctx->sort, keys, vals, U, 0, 64, &w, s);
         if (rc) goto done;

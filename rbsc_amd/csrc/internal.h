@@ -27,6 +27,17 @@ struct RawGraph {
     // costs the binned layout's gathers a little locality every round, so
     // only a graph refined once (the one-shot rbsc_refine) takes it.
     bool sample_indeg = false;
+    // Lines copied from the host in chunks, so that the build's line kernels
+    // run on each chunk while the next one crosses the link: chunk c holds
+    // lines [chunk_end[c - 1], chunk_end[c]) and is on the device once
+    // chunk_ev[c] (recorded on the copying stream) has completed; n_chunks 0
+    // means every line is present.  dst_err (with chunks) receives the flag
+    // of a line whose head is not below N, checked by the build before any
+    // kernel indexes by head.
+    int n_chunks = 0;
+    u64 chunk_end[4] = {0, 0, 0, 0};
+    hipEvent_t chunk_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    u32* dst_err = nullptr;
 };
 
 // Binned signature layout (bins.hip): stored rows cut into bins of S rows
@@ -250,6 +261,7 @@ struct rbsc_ctx {
     u64* swbuf[2] = {nullptr, nullptr};
     u64 sbuf_lines = 0;
     hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+    hipEvent_t ev_chunk[4] = {nullptr, nullptr, nullptr, nullptr};  // chunked line uploads
     // fork and join of launches split over stream and copy_stream (batch tiers)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // per-phase profiling (rbsc_ctx_profile)

@@ -2225,6 +2225,8 @@ extern "C" int rbsc_ctx_create(int device, uint64_t hbm_budget, rbsc_ctx** out) 
     }
     HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    for (int k = 0; k < 4; ++k)
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_chunk[k], hipEventDisableTiming));
     *out = c;
     return RBSC_OK;
 }
@@ -2256,6 +2258,8 @@ extern "C" int rbsc_ctx_destroy(rbsc_ctx* c) {
         if (c->ev_copy[b]) (void)hipEventDestroy(c->ev_copy[b]);
         if (c->ev_done[b]) (void)hipEventDestroy(c->ev_done[b]);
     }
+    for (hipEvent_t e : c->ev_chunk)
+        if (e) (void)hipEventDestroy(e);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -2425,7 +2429,8 @@ static int rows_on_device(rbsc_ctx* ctx, const rbsc_graph* gr, RawGraph* raw) {
                                                                 raw->row_ptr, raw->row_node, d_err);
     }
     set_u64_kernel<<<1, 1, 0, s>>>(raw->row_ptr + raw->U, E);
-    if (E) dst_check_kernel<<<grid_for(E), BLK, 0, s>>>(raw->dst, E, N, d_err);
+    // heads of lines copied in chunks are checked by the build, chunk by chunk
+    if (E && !raw->n_chunks) dst_check_kernel<<<grid_for(E), BLK, 0, s>>>(raw->dst, E, N, d_err);
     U_TRY(hipMemcpyAsync(&h_err, d_err, sizeof(u32), hipMemcpyDeviceToHost, s));
     U_TRY(hipGetLastError());
     U_TRY(hipStreamSynchronize(s));
@@ -2446,6 +2451,9 @@ done:
     return rc;
 #undef U_TRY
 }
+
+// Line arrays from this size on are copied in chunks (see graph_upload_host).
+constexpr u64 CHUNK_MIN_LINES = 1ull << 20;
 
 int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, rbsc_dgraph** out,
                       bool sample_indeg) {
@@ -2484,6 +2492,7 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
     raw.sample_indeg = sample_indeg;
     auto fail = [&](hipError_t e, const char* what) {
         rbsc_set_error("%s failed: %s", what, hipGetErrorString(e));
+        if (raw.n_chunks) (void)hipStreamSynchronize(ctx->copy_stream);  // chunks in flight
         raw_release(&raw, s);
         return e == hipErrorOutOfMemory ? RBSC_ERR_NOMEM : RBSC_ERR_HIP;
     };
@@ -2493,17 +2502,58 @@ int graph_upload_host(rbsc_ctx* ctx, const rbsc_graph* gr, bool lines_on_host, r
         return fail(e, "hipMalloc");
     if (gr->init_color && (e = ctx_alloc(&raw.init, sizeof(u32) * std::max<u64>(N, 1), s)) != hipSuccess)
         return fail(e, "hipMalloc");
-    if ((E && (e = hipMemcpyAsync(raw.dst, gr->dst, 4 * E, hipMemcpyHostToDevice, s))) ||
-        (E && gr->weight && (e = hipMemcpyAsync(raw.wt, gr->weight, 8 * E, hipMemcpyHostToDevice, s))) ||
-        (N && gr->init_color && (e = hipMemcpyAsync(raw.init, gr->init_color, 4 * N, hipMemcpyHostToDevice, s))))
+    // Large line arrays cross the link in four chunks on the second stream;
+    // the build's line kernels take each chunk as it lands (chunk ends on
+    // whole line tiles of the row-fingerprint kernel).
+    const bool chunked = E >= CHUNK_MIN_LINES && ctx->copy_stream;
+    if (chunked) {
+        if ((e = ctx_alloc(&raw.dst_err, sizeof(u32), s)) != hipSuccess) return fail(e, "hipMalloc");
+        fill_u32<<<1, 32, 0, s>>>(raw.dst_err, 1, 0);
+        if ((e = hipEventRecord(ctx->ev_fork, s)) ||
+            (e = hipStreamWaitEvent(ctx->copy_stream, ctx->ev_fork, 0)))
+            return fail(e, "hipEventRecord");
+        raw.n_chunks = 4;
+        for (int c = 0; c < 4; ++c) {
+            const u64 lo = c ? raw.chunk_end[c - 1] : 0;
+            u64 hi = c == 3 ? E : std::min<u64>(E, (E * (c + 1) / 4 + LT_LINES - 1) / LT_LINES * LT_LINES);
+            if (hi < lo) hi = lo;
+            raw.chunk_end[c] = hi;
+            raw.chunk_ev[c] = ctx->ev_chunk[c];
+            if ((hi > lo && (e = hipMemcpyAsync(raw.dst + lo, gr->dst + lo, 4 * (hi - lo),
+                                                hipMemcpyHostToDevice, ctx->copy_stream))) ||
+                (hi > lo && gr->weight &&
+                 (e = hipMemcpyAsync(raw.wt + lo, gr->weight + lo, 8 * (hi - lo),
+                                     hipMemcpyHostToDevice, ctx->copy_stream))) ||
+                (e = hipEventRecord(raw.chunk_ev[c], ctx->copy_stream)))
+                return fail(e, "hipMemcpyAsync");
+        }
+    } else if ((E && (e = hipMemcpyAsync(raw.dst, gr->dst, 4 * E, hipMemcpyHostToDevice, s))) ||
+               (E && gr->weight &&
+                (e = hipMemcpyAsync(raw.wt, gr->weight, 8 * E, hipMemcpyHostToDevice, s)))) {
         return fail(e, "hipMemcpyAsync");
+    }
+    if (N && gr->init_color && (e = hipMemcpyAsync(raw.init, gr->init_color, 4 * N, hipMemcpyHostToDevice, s)))
+        return fail(e, "hipMemcpyAsync");
+    // the lines' memory goes back on this stream only after every chunk landed
+    auto settle = [&]() {
+        if (chunked) (void)hipStreamWaitEvent(s, raw.chunk_ev[3], 0);
+    };
     rc = rows_on_device(ctx, gr, &raw);
     if (rc) {
+        settle();
         raw_release(&raw, s);
         return rc;
     }
     rbsc_dgraph* g = new rbsc_dgraph();
     rc = dgraph_build(ctx, &raw, g);
+    settle();
+    if (rc == RBSC_ERR_INPUT && chunked) {
+        // the host check names the first bad line
+        std::vector<u64> row_ptr;
+        std::vector<u32> row_node;
+        if (compress_host(gr, row_ptr, row_node) == RBSC_OK)
+            rbsc_set_error("rbsc_graph_upload: malformed lines");
+    }
     raw_release(&raw, s);
     if (rc) {
         dgraph_release(g, s);

@@ -167,6 +167,40 @@ def test_input_errors(ctx):
     assert e.value.code == -5  # id out of range
 
 
+@pytest.mark.parametrize("where", ["first", "last"])
+def test_large_upload_with_a_bad_head_is_refused(ctx, where):
+    # line arrays from 2^20 lines cross the link in chunks whose heads the
+    # build checks as they land; a bad head in any chunk is refused before
+    # any kernel indexes by head, and the context keeps working
+    g = gen.rmat(16, 1 << 21, seed=41)
+    assert g.n_edges >= 1 << 20
+    dst = g.dst.copy()
+    dst[3 if where == "first" else g.n_edges - 2] = g.n_nodes + 5
+    bad = HostGraph(g.n_nodes, dst, row_offsets=g.row_offsets) if g.row_offsets is not None \
+        else HostGraph(g.n_nodes, dst, src=g.src)
+    with pytest.raises(rbsc_amd.RbscError) as e:
+        ctx.refine(bad)
+    assert e.value.code == -5
+    with pytest.raises(rbsc_amd.RbscError) as e:
+        ctx.upload(bad)
+    assert e.value.code == -5
+    c, K, R, _ = _oracle_full(g)
+    r = ctx.refine(g)
+    assert (r.nodes, r.rounds) == (K, R) and np.array_equal(r.partition, oracle.canonical(c))
+
+
+def test_large_weighted_upload_in_chunks(ctx):
+    g = _weighted(gen.rmat(16, 1 << 21, clone_frac=0.2, seed=42), 4)
+    assert g.n_edges >= 1 << 20
+    c, K, R, _ = _oracle_full(g)
+    r = ctx.refine(g)
+    assert (r.nodes, r.rounds) == (K, R) and np.array_equal(r.partition, oracle.canonical(c))
+    dg = ctx.upload(g)
+    r2 = dg.refine()
+    assert np.array_equal(r2.partition, r.partition)
+    dg.free()
+
+
 def test_device_rmat_equals_host_rmat(ctx):
     h = gen.rmat(13, 1 << 17, clone_frac=0.3, seed=21)
     d = ctx.gen_rmat(13, 1 << 17, clone_frac=0.3, seed=21)
